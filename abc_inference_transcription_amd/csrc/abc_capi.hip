@@ -1,0 +1,568 @@
+// abc_capi.hip -- host side of libabc.so: the C ABI declared in include/abc.h.
+//
+// Orchestrates, per call, on one HIP stream:
+//   prior_kernel -> rates_kernel -> radix sort of particles by expected cost
+//   (heaviest first) -> ssa_kernel (persistent, lane-level work stealing) ->
+//   stats_kernel -> error / acceptance kernels.
+// Device buffers are owned by the context and grow on demand.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/abc.h"
+#include "abc_kernels.h"
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct abc_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int num_cu = 0;
+  int ssa_blocks = 0;        // 0 = occupancy default
+  int ssa_blocks_default = 0;
+  DesignDev hd{};
+  DesignDev* dd = nullptr;
+  double* d_betas = nullptr;
+  int64_t* d_goff = nullptr;
+  DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, cu, cl, cb, steps, react,
+      queue, stats, moments, superset, edata, ese, err, cand, ccount;
+  hipEvent_t ev[5]{};
+  bool timed[4]{};
+  std::string msg;
+};
+
+namespace {
+
+int fail(abc_ctx* c, int code, const std::string& m) {
+  if (c) c->msg = m;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail(ctx, ABC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int ensure(abc_ctx* c, DevBuf& b, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  if (b.bytes >= bytes) return ABC_OK;
+  if (b.p) {
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return fail(c, ABC_E_HIP, hipGetErrorString(e));
+    hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes + bytes / 4;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    b.p = nullptr;
+    return fail(c, ABC_E_OOM, "device allocation of " + std::to_string(want) + " bytes failed");
+  }
+  b.bytes = want;
+  return ABC_OK;
+}
+
+template <typename T>
+T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+uint32_t key0(uint64_t seed) { return (uint32_t)seed; }
+uint32_t key1(uint64_t seed, int model) {
+  return (uint32_t)(seed >> 32) ^ (0x9E3779B9u * (uint32_t)model);
+}
+
+int check_range(abc_ctx* c, int model, int64_t first, int64_t n) {
+  if (model < 1 || model > 5) return fail(c, ABC_E_MODEL, "model must be 1..5");
+  if (n < 0 || first < 0 || first + n > 0xFFFFFFFFll)
+    return fail(c, ABC_E_ARG, "particle range must lie in [0, 2^32)");
+  return ABC_OK;
+}
+
+int grid1(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+// prior (optional) + rates + cost ordering; theta_dev must hold n x n_par
+int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
+  int rc;
+  if ((rc = ensure(c, c->rates, sizeof(PRates) * n))) return rc;
+  if ((rc = ensure(c, c->cost, sizeof(float) * n))) return rc;
+  if ((rc = ensure(c, c->cost_sorted, sizeof(float) * n))) return rc;
+  if ((rc = ensure(c, c->ident, sizeof(int32_t) * n))) return rc;
+  if ((rc = ensure(c, c->order, sizeof(int32_t) * n))) return rc;
+  hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
+                     theta_dev, P<PRates>(c->rates), P<float>(c->cost), P<int32_t>(c->ident));
+  HIPCHK(c, hipGetLastError());
+  size_t tmp = 0;
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(
+                nullptr, tmp, P<float>(c->cost), P<float>(c->cost_sorted), P<int32_t>(c->ident),
+                P<int32_t>(c->order), (int)n, 0, 32, c->stream));
+  if ((rc = ensure(c, c->sort_tmp, tmp))) return rc;
+  tmp = c->sort_tmp.bytes;
+  HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(
+                c->sort_tmp.p, tmp, P<float>(c->cost), P<float>(c->cost_sorted),
+                P<int32_t>(c->ident), P<int32_t>(c->order), (int)n, 0, 32, c->stream));
+  return ABC_OK;
+}
+
+int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
+                 const double* theta_dev, double* stats_dev, double* moments_dev,
+                 double* superset_dev, int64_t* steps_dev) {
+  int rc;
+  if ((rc = check_range(c, model, first, n))) return rc;
+  if (n == 0) return ABC_OK;
+  const size_t cells = (size_t)n * (size_t)c->hd.C;
+  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
+  if ((rc = ensure(c, c->cu, sizeof(uint32_t) * cells))) return rc;
+  if ((rc = ensure(c, c->cl, sizeof(uint32_t) * cells))) return rc;
+  if ((rc = ensure(c, c->cb, cells))) return rc;
+  if ((rc = ensure(c, c->react, sizeof(uint64_t) * n))) return rc;
+  if ((rc = ensure(c, c->queue, 64))) return rc;
+  unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
+  if (!steps) {
+    if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
+    steps = P<unsigned long long>(c->steps);
+  }
+  // counters cleared by a kernel (no memset nodes, so the sequence can be captured)
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, steps, n);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
+                     P<unsigned long long>(c->react), n);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                     P<unsigned long long>(c->queue), (int64_t)8);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
+  hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
+                     P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
+                     key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
+                     P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps,
+                     P<unsigned long long>(c->react));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
+                     P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
+                     moments_dev, superset_dev);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  c->timed[0] = c->timed[1] = c->timed[2] = true;
+  return ABC_OK;
+}
+
+int make_spec(abc_ctx* c, int K, const int32_t* parts, int n_parts, double clamp, ErrSpec* E) {
+  if (K <= 0 || K > ABC_MAX_STATS) return fail(c, ABC_E_ARG, "K must be 1..256");
+  if (n_parts <= 0 || n_parts > ABC_MAX_PARTS || !parts)
+    return fail(c, ABC_E_ARG, "n_parts must be 1..16");
+  int s = 0;
+  for (int q = 0; q < n_parts; ++q) {
+    if (parts[q] < 0) return fail(c, ABC_E_ARG, "negative part size");
+    E->parts[q] = parts[q];
+    s += parts[q];
+  }
+  if (s != K) return fail(c, ABC_E_ARG, "part sizes must sum to K");
+  E->K = K;
+  E->n_parts = n_parts;
+  E->clamp = clamp;
+  return ABC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
+  if (!out || !d) return ABC_E_ARG;
+  *out = nullptr;
+  abc_ctx* c = new abc_ctx();
+  auto bad = [&](const char* m) {
+    c->msg = m;
+    return ABC_E_ARG;
+  };
+  // validation keeps every kernel index in bounds
+  if (!(d->cycle_h > 0.0)) { int r = bad("cycle_h must be > 0"); delete c; return r; }
+  if (d->n_bins < 1 || d->n_bins > ABC_MAX_BINS) { delete c; return ABC_E_ARG; }
+  if (d->n_conds < 1 || d->n_conds > ABC_MAX_CONDS) { delete c; return ABC_E_ARG; }
+  if (d->cells_per_particle < 1 || d->cells_per_particle > (1 << 24)) { delete c; return ABC_E_ARG; }
+  if (!d->pulse_h || !d->chase_h || !d->is_chase || !d->age_weights) { delete c; return ABC_E_ARG; }
+  if (d->age_mode != 0 && d->age_mode != 1) { delete c; return ABC_E_ARG; }
+  if (d->capture_mode < 0 || d->capture_mode > 2) { delete c; return ABC_E_ARG; }
+  if (d->capture_mode == 1 &&
+      (d->beta_a < 1 || d->beta_a > 4 || d->beta_b < 1 || d->beta_a + d->beta_b - 1 > 32)) {
+    delete c; return ABC_E_ARG;
+  }
+  if (d->prior_mode != 0 && d->prior_mode != 1) { delete c; return ABC_E_ARG; }
+  if (!(d->burn_eps > 0.0) || d->burn_max < 1 || d->burn_max > 64) { delete c; return ABC_E_ARG; }
+  DesignDev& h = c->hd;
+  h.T = d->cycle_h;
+  h.nb = d->n_bins;
+  h.J = d->n_conds;
+  h.C = d->cells_per_particle;
+  h.age_mode = d->age_mode;
+  h.cap_mode = d->capture_mode;
+  h.beta_a = d->beta_a;
+  h.beta_b = d->beta_b;
+  h.prior_mode = d->prior_mode;
+  h.burn_eps = d->burn_eps;
+  h.splice_log10 = d->splice_log10;
+  double wmax = 0.0;
+  for (int j = 0; j < h.J; ++j) {
+    if (!(d->pulse_h[j] > 0.0) || d->chase_h[j] < 0.0 || (d->is_chase[j] != 0 && d->is_chase[j] != 1)) {
+      delete c; return ABC_E_ARG;
+    }
+    h.pulse[j] = d->pulse_h[j];
+    h.chase[j] = d->chase_h[j];
+    h.is_chase[j] = d->is_chase[j];
+    wmax = std::max(wmax, d->pulse_h[j] + d->chase_h[j]);
+  }
+  h.bmin = std::max(1, (int)std::ceil(wmax / h.T));
+  h.bmax = std::max(h.bmin, (int)d->burn_max);
+  for (int j = 0; j < h.nb; ++j) h.bound[j] = (h.T * (double)(h.nb + j + 1)) / (double)h.nb;
+  for (int q = 0; q < h.nb * h.J; ++q) h.weights[q] = d->age_weights[q];
+  // capture: grouped empirical list (group = bin*2 + is_chase)
+  std::vector<double> grouped;
+  std::vector<int64_t> goff(2 * h.nb + 1, 0);
+  if (h.cap_mode == 2) {
+    if (!d->betas || !d->beta_bin || !d->beta_is_chase || d->n_betas <= 0) { delete c; return ABC_E_ARG; }
+    std::vector<std::vector<double>> g(2 * h.nb);
+    for (int64_t q = 0; q < d->n_betas; ++q) {
+      int b = d->beta_bin[q], ch = d->beta_is_chase[q];
+      if (b < 0 || b >= h.nb || (ch != 0 && ch != 1)) { delete c; return ABC_E_ARG; }
+      g[b * 2 + ch].push_back(d->betas[q]);
+    }
+    double msum = 0.0;
+    for (int q = 0; q < 2 * h.nb; ++q) {
+      // every (bin, group) used by a condition needs at least one efficiency
+      bool used = false;
+      for (int j = 0; j < h.J; ++j) if (h.is_chase[j] == (q & 1)) used = true;
+      if (used && g[q].empty()) {
+        c->msg = "capture list has no efficiency for an age bin / group";
+        delete c; return ABC_E_ARG;
+      }
+      goff[q + 1] = goff[q] + (int64_t)g[q].size();
+      grouped.insert(grouped.end(), g[q].begin(), g[q].end());
+      for (double x : g[q]) msum += x;
+    }
+    h.capture_mean = msum / (double)grouped.size();
+  } else if (h.cap_mode == 1) {
+    h.capture_mean = (double)h.beta_a / (double)(h.beta_a + h.beta_b);
+  } else {
+    h.capture_mean = 1.0;
+  }
+  if (grouped.empty()) grouped.push_back(0.0);
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) { delete c; return ABC_E_HIP; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return ABC_E_HIP; }
+  c->num_cu = prop.multiProcessorCount;
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ssa_kernel, SSA_WG, 0) != hipSuccess ||
+      per_cu < 1)
+    per_cu = 1;
+  c->ssa_blocks_default = c->num_cu * per_cu;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ABC_E_HIP; }
+  for (auto& ev : c->ev)
+    if (hipEventCreate(&ev) != hipSuccess) { abc_destroy(c); return ABC_E_HIP; }
+  if (hipMalloc(&c->dd, sizeof(DesignDev)) != hipSuccess ||
+      hipMalloc(&c->d_betas, sizeof(double) * grouped.size()) != hipSuccess ||
+      hipMalloc(&c->d_goff, sizeof(int64_t) * goff.size()) != hipSuccess) {
+    abc_destroy(c);
+    return ABC_E_OOM;
+  }
+  if (hipMemcpy(c->dd, &h, sizeof(DesignDev), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_betas, grouped.data(), sizeof(double) * grouped.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_goff, goff.data(), sizeof(int64_t) * goff.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    abc_destroy(c);
+    return ABC_E_HIP;
+  }
+  *out = c;
+  return ABC_OK;
+}
+
+int32_t abc_destroy(abc_ctx* c) {
+  if (!c) return ABC_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->theta, &c->rates, &c->cost, &c->cost_sorted, &c->ident, &c->order,
+                    &c->sort_tmp, &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
+                    &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
+                    &c->cand, &c->ccount};
+  for (DevBuf* b : bufs)
+    if (b->p) hipFree(b->p);
+  if (c->dd) hipFree(c->dd);
+  if (c->d_betas) hipFree(c->d_betas);
+  if (c->d_goff) hipFree(c->d_goff);
+  for (auto& ev : c->ev)
+    if (ev) hipEventDestroy(ev);
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
+  delete c;
+  return ABC_OK;
+}
+
+const char* abc_last_error(const abc_ctx* c) { return c ? c->msg.c_str() : "null context"; }
+
+int32_t abc_n_params(const abc_ctx* c, int32_t model) {
+  if (!c || model < 1 || model > 5) return ABC_E_ARG;
+  return abc_layout_n_params(model, c->hd.nb);
+}
+
+int32_t abc_n_stats(const abc_ctx* c) { return c ? 4 * c->hd.nb + 3 * c->hd.J : ABC_E_ARG; }
+
+int32_t abc_set_stream(abc_ctx* c, void* s) {
+  if (!c) return ABC_E_ARG;
+  hipStreamSynchronize(c->stream);
+  if (c->own_stream) hipStreamDestroy(c->stream);
+  if (s) {
+    c->stream = (hipStream_t)s;
+    c->own_stream = false;
+  } else {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return ABC_E_HIP;
+    c->own_stream = true;
+  }
+  return ABC_OK;
+}
+
+int32_t abc_synchronize(abc_ctx* c) {
+  if (!c) return ABC_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_set_ssa_blocks(abc_ctx* c, int32_t blocks) {
+  if (!c || blocks < 0) return ABC_E_ARG;
+  c->ssa_blocks = blocks;
+  return ABC_OK;
+}
+
+int32_t abc_get_ssa_blocks(const abc_ctx* c) {
+  if (!c) return ABC_E_ARG;
+  return c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
+}
+
+int32_t abc_sample_prior_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                             double* theta_dev) {
+  if (!c) return ABC_E_ARG;
+  int rc = check_range(c, model, first, n);
+  if (rc) return rc;
+  if (n == 0) return ABC_OK;
+  hipLaunchKernelGGL(prior_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model,
+                     key0(seed), key1(seed, model), (uint32_t)first, n, theta_dev);
+  HIPCHK(c, hipGetLastError());
+  return ABC_OK;
+}
+
+int32_t abc_sample_prior(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                         double* theta) {
+  if (!c || (!theta && n > 0)) return ABC_E_ARG;
+  int rc = check_range(c, model, first, n);
+  if (rc) return rc;
+  if (n == 0) return ABC_OK;
+  const int np = abc_layout_n_params(model, c->hd.nb);
+  if ((rc = ensure(c, c->theta, sizeof(double) * n * np))) return rc;
+  if ((rc = abc_sample_prior_dev(c, model, seed, first, n, P<double>(c->theta)))) return rc;
+  HIPCHK(c, hipMemcpyAsync(theta, c->theta.p, sizeof(double) * n * np, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_simulate_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                         const double* theta_dev, double* stats_dev, double* moments_dev,
+                         double* superset_dev, int64_t* steps_dev) {
+  if (!c || (n > 0 && (!theta_dev || !stats_dev))) return ABC_E_ARG;
+  return simulate_dev(c, model, seed, first, n, theta_dev, stats_dev, moments_dev, superset_dev,
+                      steps_dev);
+}
+
+int32_t abc_simulate(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                     const double* theta, double* stats, double* moments, double* superset,
+                     int64_t* steps, uint32_t* cell_u, uint32_t* cell_l, uint8_t* cell_bin) {
+  if (!c || (n > 0 && (!theta || !stats))) return ABC_E_ARG;
+  int rc = check_range(c, model, first, n);
+  if (rc) return rc;
+  if (n == 0) return ABC_OK;
+  const int np = abc_layout_n_params(model, c->hd.nb);
+  const int K = 4 * c->hd.nb + 3 * c->hd.J, NBJ = c->hd.nb * c->hd.J;
+  if ((rc = ensure(c, c->theta, sizeof(double) * n * np))) return rc;
+  if ((rc = ensure(c, c->stats, sizeof(double) * n * K))) return rc;
+  if (moments && (rc = ensure(c, c->moments, sizeof(double) * n * NBJ * 6))) return rc;
+  if (superset && (rc = ensure(c, c->superset, sizeof(double) * n * NBJ * 2))) return rc;
+  if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->theta.p, theta, sizeof(double) * n * np, hipMemcpyHostToDevice, c->stream));
+  rc = simulate_dev(c, model, seed, first, n, P<double>(c->theta), P<double>(c->stats),
+                    moments ? P<double>(c->moments) : nullptr,
+                    superset ? P<double>(c->superset) : nullptr, P<int64_t>(c->steps));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(stats, c->stats.p, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->stream));
+  if (moments)
+    HIPCHK(c, hipMemcpyAsync(moments, c->moments.p, sizeof(double) * n * NBJ * 6, hipMemcpyDeviceToHost, c->stream));
+  if (superset)
+    HIPCHK(c, hipMemcpyAsync(superset, c->superset.p, sizeof(double) * n * NBJ * 2, hipMemcpyDeviceToHost, c->stream));
+  if (steps)
+    HIPCHK(c, hipMemcpyAsync(steps, c->steps.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  const size_t cells = (size_t)n * (size_t)c->hd.C;
+  if (cell_u) HIPCHK(c, hipMemcpyAsync(cell_u, c->cu.p, 4 * cells, hipMemcpyDeviceToHost, c->stream));
+  if (cell_l) HIPCHK(c, hipMemcpyAsync(cell_l, c->cl.p, 4 * cells, hipMemcpyDeviceToHost, c->stream));
+  if (cell_bin) HIPCHK(c, hipMemcpyAsync(cell_bin, c->cb.p, cells, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_errors_dev(abc_ctx* c, int64_t n, int32_t K, const double* stats_dev, int64_t G,
+                       const double* data_dev, const double* se_dev, const int32_t* parts,
+                       int32_t n_parts, double clamp, double* err_dev) {
+  if (!c || n < 0 || G < 0) return ABC_E_ARG;
+  ErrSpec E;
+  int rc = make_spec(c, K, parts, n_parts, clamp, &E);
+  if (rc) return rc;
+  if (n == 0 || G == 0) return ABC_OK;
+  if (G > 65535 * 64) return fail(c, ABC_E_ARG, "too many genes for one call");
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  // grid.y is limited to 65535: split genes into slices
+  for (int64_t g0 = 0; g0 < G; g0 += 65535) {
+    const int64_t gn = std::min<int64_t>(65535, G - g0);
+    hipLaunchKernelGGL(err_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0, c->stream,
+                       stats_dev, n, data_dev, se_dev, G, g0, E, err_dev + n * g0);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+  c->timed[3] = true;
+  return ABC_OK;
+}
+
+int32_t abc_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_t G,
+                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
+                   double clamp, double* err) {
+  if (!c || n < 0 || G < 0 || (n > 0 && G > 0 && (!stats || !data || !se || !err))) return ABC_E_ARG;
+  ErrSpec E;
+  int rc = make_spec(c, K, parts, n_parts, clamp, &E);
+  if (rc) return rc;
+  if (n == 0 || G == 0) return ABC_OK;
+  if ((rc = ensure(c, c->stats, sizeof(double) * n * K))) return rc;
+  if ((rc = ensure(c, c->edata, sizeof(double) * G * K))) return rc;
+  if ((rc = ensure(c, c->ese, sizeof(double) * G * K))) return rc;
+  if ((rc = ensure(c, c->err, sizeof(double) * n * G))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->edata.p, data, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ese.p, se, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
+  for (int64_t g0 = 0; g0 < G; g0 += 65535) {
+    const int64_t gn = std::min<int64_t>(65535, G - g0);
+    hipLaunchKernelGGL(err_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0, c->stream,
+                       P<double>(c->stats), n, P<double>(c->edata), P<double>(c->ese), G, g0, E,
+                       P<double>(c->err) + n * g0);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(err, c->err.p, sizeof(double) * n * G, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_t G,
+                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
+                   double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
+                   int64_t* counts, int64_t* idx, double* err_out, int64_t cap) {
+  if (!c || n < 0 || G < 0 || (G > 0 && !counts) || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+  if (mode != 0 && mode != 1) return fail(c, ABC_E_ARG, "mode must be 0 (fixed eps) or 1 (top-k)");
+  if (mode == 1 && k < 0) return fail(c, ABC_E_ARG, "k must be >= 0");
+  ErrSpec E;
+  int rc = make_spec(c, K, parts, n_parts, clamp, &E);
+  if (rc) return rc;
+  for (int64_t g = 0; g < G; ++g) counts[g] = 0;
+  if (n == 0 || G == 0) return ABC_OK;
+  if ((rc = ensure(c, c->stats, sizeof(double) * n * K))) return rc;
+  if ((rc = ensure(c, c->edata, sizeof(double) * G * K))) return rc;
+  if ((rc = ensure(c, c->ese, sizeof(double) * G * K))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->edata.p, data, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->ese.p, se, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
+  std::vector<std::vector<std::pair<double, int64_t>>> per(G);
+  if (mode == 0) {
+    unsigned long long ccap = 1ull << 22;
+    for (;;) {
+      if ((rc = ensure(c, c->cand, sizeof(Candidate) * ccap))) return rc;
+      if ((rc = ensure(c, c->ccount, 64))) return rc;
+      hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                         P<unsigned long long>(c->ccount), (int64_t)1);
+      for (int64_t g0 = 0; g0 < G; g0 += 65535) {
+        const int64_t gn = std::min<int64_t>(65535, G - g0);
+        hipLaunchKernelGGL(accept_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0,
+                           c->stream, P<double>(c->stats), n, P<double>(c->edata),
+                           P<double>(c->ese), G, g0, E, eps, first_index, P<Candidate>(c->cand),
+                           P<unsigned long long>(c->ccount), ccap);
+        HIPCHK(c, hipGetLastError());
+      }
+      unsigned long long cnt = 0;
+      HIPCHK(c, hipMemcpyAsync(&cnt, c->ccount.p, 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      if (cnt > ccap) { ccap = cnt; continue; }
+      std::vector<Candidate> h(cnt);
+      if (cnt) HIPCHK(c, hipMemcpy(h.data(), c->cand.p, sizeof(Candidate) * cnt, hipMemcpyDeviceToHost));
+      for (const Candidate& x : h) per[x.gene].emplace_back(x.err, x.particle);
+      break;
+    }
+  } else {
+    // exact top-k from FP64 errors, genes in slices that fit the error buffer
+    const int64_t gslice = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)((1ull << 28) / (uint64_t)std::max<int64_t>(n, 1))));
+    std::vector<double> herr;
+    for (int64_t g0 = 0; g0 < G; g0 += gslice) {
+      const int64_t gn = std::min<int64_t>(gslice, G - g0);
+      if ((rc = ensure(c, c->err, sizeof(double) * n * gn))) return rc;
+      hipLaunchKernelGGL(err_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0, c->stream,
+                         P<double>(c->stats), n, P<double>(c->edata), P<double>(c->ese), G, g0, E,
+                         P<double>(c->err));
+      HIPCHK(c, hipGetLastError());
+      herr.resize((size_t)n * gn);
+      HIPCHK(c, hipMemcpyAsync(herr.data(), c->err.p, sizeof(double) * n * gn, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int64_t gg = 0; gg < gn; ++gg) {
+        auto& v = per[g0 + gg];
+        v.resize(n);
+        for (int64_t i = 0; i < n; ++i) v[i] = {herr[i + n * gg], first_index + i};
+        const int64_t kk = std::min<int64_t>(k, n);
+        std::partial_sort(v.begin(), v.begin() + kk, v.end());
+        v.resize(kk);
+      }
+    }
+  }
+  int64_t total = 0;
+  for (int64_t g = 0; g < G; ++g) {
+    auto& v = per[g];
+    std::sort(v.begin(), v.end());
+    counts[g] = (int64_t)v.size();
+    for (auto& pr : v) {
+      if (total < cap) {
+        idx[total] = pr.second + 1;
+        if (err_out) err_out[total] = pr.first;
+      }
+      total++;
+    }
+  }
+  if (total > cap) return fail(c, ABC_E_CAPACITY, "accepted entries exceed cap");
+  return ABC_OK;
+}
+
+int32_t abc_last_timing(abc_ctx* c, double* ms) {
+  if (!c || !ms) return ABC_E_ARG;
+  for (int q = 0; q < 4; ++q) {
+    ms[q] = 0.0;
+    if (!c->timed[q]) continue;
+    float v = 0.f;
+    HIPCHK(c, hipEventSynchronize(c->ev[q + 1]));
+    HIPCHK(c, hipEventElapsedTime(&v, c->ev[q], c->ev[q + 1]));
+    ms[q] = v;
+  }
+  return ABC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,134 @@
+// abc_device.h -- device-side random numbers and elementary functions for
+// the gfx950 kernels.
+//
+// Philox4x32-10 (Salmon et al., SC'11): one independent stream per
+// (particle, cell), counter = (draw_lo, draw_hi, cell, particle), key =
+// (seed_lo, seed_hi ^ model * 0x9E3779B9).  The reference draws from Julia's
+// unseeded global RNG (scripts/abc_simulation.jl:4-9); counter-based streams
+// make every particle reproducible and independent of the batch split.
+//
+// log/exp follow Cody & Waite (1980): power-of-two reduction + 2*atanh series
+// for log, ln2 two-word reduction + degree-13 Taylor polynomial for exp.  Only
+// IEEE-rounded +,-,*,/ are used and the library is compiled with
+// -ffp-contract=off, so results are identical on every IEEE machine that
+// performs the same steps; this is what makes GPU runs replayable on a CPU.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define ABC_DEV __device__ __forceinline__
+
+namespace abcd {
+
+struct u4 { uint32_t x, y, z, w; };
+
+ABC_DEV u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c0;
+    const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2;
+    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0;
+    const uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+  }
+  return u4{c0, c1, c2, c3};
+}
+
+ABC_DEV double u01_32(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+ABC_DEV double u01_53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 0x1p-53;
+}
+
+ABC_DEV uint32_t lemire(uint32_t w, uint32_t n) {
+  return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
+}
+
+// natural log for finite x > 0 (the callers never pass anything else)
+ABC_DEV double ln(double x) {
+  int64_t bits = __double_as_longlong(x);
+  int e_adj = 0;
+  if (x < 0x1p-1022) {
+    bits = __double_as_longlong(x * 0x1p54);
+    e_adj = -54;
+  }
+  int e = (int)((bits >> 52) & 0x7ff) - 1023 + e_adj;
+  double m = __longlong_as_double((bits & 0x000fffffffffffffll) | 0x3ff0000000000000ll);
+  if (m > 0x1.6a09e667f3bcdp+0) { m = m * 0.5; e = e + 1; }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double z = s * s;
+  double p = 0x1.8618618618618p-4;
+  p = p * z + 0x1.af286bca1af28p-4;
+  p = p * z + 0x1.e1e1e1e1e1e1ep-4;
+  p = p * z + 0x1.1111111111111p-3;
+  p = p * z + 0x1.3b13b13b13b14p-3;
+  p = p * z + 0x1.745d1745d1746p-3;
+  p = p * z + 0x1.c71c71c71c71cp-3;
+  p = p * z + 0x1.2492492492492p-2;
+  p = p * z + 0x1.999999999999ap-2;
+  p = p * z + 0x1.5555555555555p-1;
+  const double logm = (s + s) + (s * z) * p;
+  const double de = (double)e;
+  return de * 0x1.62e42fee00000p-1 + (logm + de * 0x1.a39ef35793c76p-33);
+}
+
+ABC_DEV double ex(double x) {
+  if (x != x) return x;
+  if (x > 709.78) return __longlong_as_double(0x7ff0000000000000ll);
+  if (x < -745.2) return 0.0;
+  const double kd = floor(x * 0x1.71547652b82fep+0 + 0.5);
+  const int k = (int)kd;
+  const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
+  double p = 0x1.6124613a86d09p-33;
+  p = p * r + 0x1.1eed8eff8d898p-29;
+  p = p * r + 0x1.ae64567f544e4p-26;
+  p = p * r + 0x1.27e4fb7789f5cp-22;
+  p = p * r + 0x1.71de3a556c734p-19;
+  p = p * r + 0x1.a01a01a01a01ap-16;
+  p = p * r + 0x1.a01a01a01a01ap-13;
+  p = p * r + 0x1.6c16c16c16c17p-10;
+  p = p * r + 0x1.1111111111111p-7;
+  p = p * r + 0x1.5555555555555p-5;
+  p = p * r + 0x1.5555555555555p-3;
+  p = p * r + 0.5;
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  if (k < -1000) return (p * __longlong_as_double((int64_t)(k + 1000 + 1023) << 52)) * 0x1p-1000;
+  if (k > 1000) return (p * __longlong_as_double((int64_t)(k - 1000 + 1023) << 52)) * 0x1p1000;
+  return p * __longlong_as_double((int64_t)(k + 1023) << 52);
+}
+
+ABC_DEV double ex10(double x) { return ex(x * 0x1.26bb1bbb55516p+1); }
+
+// correctly rounded (nearest-even) conversion of the 128-bit integer hi:lo
+ABC_DEV double u128_to_f64(uint64_t hi, uint64_t lo) {
+  if (hi == 0 && (lo >> 53) == 0) return (double)lo;  // exact
+  const int p = hi ? 127 - __clzll(hi) : 63 - __clzll(lo);
+  const int sh = p - 52;  // 1..75: bits shifted out below the 53-bit mantissa
+  uint64_t mant, rem_hi, rem_lo;
+  if (sh >= 64) {
+    mant = hi >> (sh - 64);
+    rem_hi = hi & ((1ull << (sh - 64)) - 1ull);
+    rem_lo = lo;
+  } else {
+    mant = (lo >> sh) | (hi << (64 - sh));
+    rem_hi = 0ull;
+    rem_lo = lo & ((1ull << sh) - 1ull);
+  }
+  const int hb = sh - 1;  // weight of one half ulp
+  const uint64_t half_hi = hb >= 64 ? (1ull << (hb - 64)) : 0ull;
+  const uint64_t half_lo = hb >= 64 ? 0ull : (1ull << hb);
+  const bool gt = rem_hi > half_hi || (rem_hi == half_hi && rem_lo > half_lo);
+  const bool eq = rem_hi == half_hi && rem_lo == half_lo;
+  int e = sh;
+  if (gt || (eq && (mant & 1ull))) {
+    mant += 1ull;
+    if (mant >> 53) { mant >>= 1; e += 1; }
+  }
+  return ldexp((double)mant, e);
+}
+
+}  // namespace abcd

@@ -1,0 +1,74 @@
+// abc_kernels.h -- device data layouts and kernel declarations shared by the
+// kernels and the C-ABI host layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "abc_device.h"
+
+#define ABC_MAX_BINS 8
+#define ABC_MAX_CONDS 16
+#define ABC_MAX_STATS 256
+#define ABC_MAX_PARTS 16
+#define SSA_WG 256
+
+// Experimental design in device memory (read through uniform scalar loads).
+struct DesignDev {
+  double T;                        // cycle length (h)
+  int nb, J, C;                    // age bins, conditions, cells per particle
+  int age_mode, cap_mode;          // 0 bin centres / 1 uniform; capture 0/1/2
+  int beta_a, beta_b;              // integer Beta(a, b) for cap_mode 1
+  int bmin, bmax;                  // burn-in cycle bounds
+  int prior_mode, pad0;
+  double burn_eps;                 // residual survival target of the burn-in
+  double splice_log10;             // splicing rate for the reference prior
+  double capture_mean;             // mean capture efficiency (cost estimate)
+  double pulse[ABC_MAX_CONDS], chase[ABC_MAX_CONDS];
+  int is_chase[ABC_MAX_CONDS];
+  double bound[ABC_MAX_BINS];      // T * (nb + j + 1) / nb (thinning bound, h)
+  double weights[ABC_MAX_BINS * ABC_MAX_CONDS];  // age weights, col-major nb x J
+};
+
+// Per-particle rates (1/h) for each age bin.
+struct __align__(16) PRates {
+  double kon[ABC_MAX_BINS], koff[ABC_MAX_BINS], amax[ABC_MAX_BINS], gam[ABC_MAX_BINS];
+  double bs;                       // splicing rate
+  uint64_t thr_lam;                // floor(lambda * 2^32): labelling threshold
+  uint64_t thr_pi;                 // floor(P(gene on at cycle start) * 2^32)
+  int32_t B, step, scaling, pad;   // burn-in cycles, grid stride, size scaling
+};
+
+struct ErrSpec {
+  int K, n_parts;
+  int parts[ABC_MAX_PARTS];
+  double clamp;
+};
+
+struct Candidate {
+  double err;
+  int64_t particle;  // global 0-based particle index
+  int32_t gene;
+  int32_t pad;
+};
+
+int abc_layout_n_params(int model, int nb);
+
+__global__ void prior_kernel(const DesignDev* D, int model, uint32_t k0, uint32_t k1,
+                             uint32_t first, int64_t n, double* theta);
+__global__ void rates_kernel(const DesignDev* D, int model, int64_t n, const double* theta,
+                             PRates* R, float* cost, int32_t* ident);
+__global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
+                           uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
+                           const int64_t* goff, unsigned long long* queue, uint32_t* out_u,
+                           uint32_t* out_l, uint8_t* out_bin, unsigned long long* steps_out,
+                           unsigned long long* react_out);
+__global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
+                             const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
+                             double* superset);
+__global__ void err_fp64_kernel(const double* stats, int64_t n, const double* data,
+                                const double* se, int64_t G, int64_t g0, ErrSpec E, double* err);
+__global__ void accept_fp64_kernel(const double* stats, int64_t n, const double* data,
+                                   const double* se, int64_t G, int64_t g0, ErrSpec E,
+                                   double eps, int64_t first, Candidate* cand, unsigned long long* count,
+                                   unsigned long long cap);
+__global__ void zero_u64_kernel(unsigned long long* p, int64_t n);

@@ -1,0 +1,653 @@
+// abc_kernels.hip -- gfx950 kernels of the ABC simulation hot path.
+//
+//   prior_kernel ...... fix_params (scripts/abc_simulation.jl:3-11) and the
+//                       config priors; one thread per particle
+//   rates_kernel ...... log10 parameters -> per-particle rates, thinning
+//                       bounds, periodic gene law, burn-in length, cost key
+//                       (scripts/model.jl:1-43)
+//   ssa_kernel ........ persistent Gillespie simulation, one cell per lane,
+//                       lanes refill from a global cell queue the moment their
+//                       cell ends (lane-level work stealing); replaces the
+//                       moment ODE solves of scripts/model.jl:89-206
+//   stats_kernel ...... per-particle statistics from observed cell counts with
+//                       exact integer accumulation in LDS
+//                       (scripts/abc_simulation.jl:23-46 semantics on cells,
+//                       estimators of scripts/data_summary_statistics.jl)
+//   err_fp64_kernel ... reference-order FP64 error matrix
+//                       (scripts/compute_errors.jl:30-70)
+//   accept_fp64_kernel  fixed-epsilon candidates from the same FP64 errors
+//                       (scripts/accepted_particles.jl:20-22)
+#include "abc_kernels.h"
+
+using namespace abcd;
+
+// ------------------------------------------------------------------ layout
+struct Layout {
+  int n_kon, n_al, n_ga, o_kon, o_koff, o_al, o_ga, o_lam, o_spl, n_par;
+};
+
+__host__ __device__ inline Layout layout_of(int model, int nb) {
+  Layout L;
+  L.n_kon = model == 3 ? nb : 1;
+  L.n_al = model == 4 ? nb : 1;
+  L.n_ga = model == 5 ? nb : 1;
+  L.o_kon = 0;
+  L.o_koff = L.n_kon;
+  L.o_al = L.o_koff + 1;
+  L.o_ga = L.o_al + L.n_al;
+  L.o_lam = L.o_ga + L.n_ga;
+  L.o_spl = L.o_lam + 1;
+  L.n_par = L.o_spl + 1;
+  return L;
+}
+
+int abc_layout_n_params(int model, int nb) { return layout_of(model, nb).n_par; }
+
+// ------------------------------------------------------------------- prior
+struct PriorStream {
+  uint32_t k0, k1, pid;
+  int q;
+  u4 w;
+  ABC_DEV double next() {
+    const int call = q >> 1, half = q & 1;
+    if (half == 0) w = philox((uint32_t)call, 0u, 0xFFFFFFFFu, pid, k0, k1);
+    q++;
+    return half == 0 ? u01_53(w.x, w.y) : u01_53(w.z, w.w);
+  }
+  ABC_DEV double unif(double lo, double hi) {
+    const double u = next();
+    return lo + (hi - lo) * u;
+  }
+};
+
+#define FC_LO (-0.698970004336018804786)
+#define FC_HI (0.698970004336018804786)
+
+__global__ __launch_bounds__(256) void prior_kernel(const DesignDev* __restrict__ D, int model,
+                                                    uint32_t k0, uint32_t k1, uint32_t first,
+                                                    int64_t n, double* __restrict__ theta) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nb = D->nb;
+  const Layout L = layout_of(model, nb);
+  PriorStream s{k0, k1, first + (uint32_t)i, 0, u4{0, 0, 0, 0}};
+  double* th = theta + i;  // column c at th[n * c]
+  if (D->prior_mode == 0) {
+    for (int c = 0; c < L.n_kon; ++c) th[n * (L.o_kon + c)] = s.unif(-3.0, 3.0);
+    th[n * L.o_koff] = s.unif(-3.0, 3.0);
+    for (int c = 0; c < L.n_al; ++c) th[n * (L.o_al + c)] = s.unif(-3.0, 3.0);
+    for (int c = 0; c < L.n_ga; ++c) th[n * (L.o_ga + c)] = s.unif(-3.0, 2.0);
+    th[n * L.o_lam] = s.unif(-0.7, 0.0);
+    th[n * L.o_spl] = D->splice_log10;
+  } else {
+    const double kb = s.unif(-2.0, 2.0);
+    if (L.n_kon > 1) {
+      for (int c = 0; c < nb; ++c) th[n * (L.o_kon + c)] = kb + s.unif(FC_LO, FC_HI);
+    } else {
+      th[n * L.o_kon] = kb;
+    }
+    const double koff = s.unif(-3.0, 3.0);
+    th[n * L.o_koff] = koff;
+    const double bb = s.unif(-1.0, 3.0);
+    if (L.n_al > 1) {
+      for (int c = 0; c < nb; ++c) th[n * (L.o_al + c)] = (bb + s.unif(FC_LO, FC_HI)) + koff;
+    } else {
+      th[n * L.o_al] = bb + koff;
+    }
+    const double gb = s.unif(-2.0, 1.0);
+    if (L.n_ga > 1) {
+      for (int c = 0; c < nb; ++c) th[n * (L.o_ga + c)] = gb + s.unif(FC_LO, FC_HI);
+    } else {
+      th[n * L.o_ga] = gb;
+    }
+    th[n * L.o_lam] = s.unif(-0.7, 0.0);
+    th[n * L.o_spl] = s.unif(-1.0, 2.0);
+  }
+}
+
+// ------------------------------------------------------------------- rates
+__global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict__ D, int model,
+                                                    int64_t n, const double* __restrict__ theta,
+                                                    PRates* __restrict__ R,
+                                                    float* __restrict__ cost,
+                                                    int32_t* __restrict__ ident) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nb = D->nb;
+  const Layout L = layout_of(model, nb);
+  const double T = D->T;
+  const double h = T / (double)nb;
+  PRates r;
+  r.scaling = model != 2;
+  r.step = model == 2 ? nb : 1;
+  for (int j = 0; j < nb; ++j) {
+    r.kon[j] = ex10(theta[i + n * (L.o_kon + (L.n_kon > 1 ? j : 0))]);
+    r.koff[j] = ex10(theta[i + n * L.o_koff]);
+    const double al = ex10(theta[i + n * (L.o_al + (L.n_al > 1 ? j : 0))]);
+    r.amax[j] = r.scaling ? (al * (double)(nb + j + 1)) / (double)nb : al;
+    r.gam[j] = ex10(theta[i + n * (L.o_ga + (L.n_ga > 1 ? j : 0))]);
+  }
+  r.bs = ex10(theta[i + n * L.o_spl]);
+  const double lam = ex10(theta[i + n * L.o_lam]);
+  r.thr_lam = (uint64_t)(lam * 4294967296.0);
+  double pi;
+  if (L.n_kon > 1) {
+    double p = 0.0, E = 1.0;
+    for (int j = 0; j < nb; ++j) {
+      const double rr = r.kon[j] + r.koff[j];
+      const double pj = r.kon[j] / rr;
+      const double e = ex(-(rr * h));
+      p = pj + (p - pj) * e;
+      E = E * e;
+    }
+    pi = p / (1.0 - E);
+  } else {
+    pi = r.kon[0] / (r.kon[0] + r.koff[0]);
+  }
+  if (pi > 1.0) pi = 1.0;
+  if (pi < 0.0) pi = 0.0;
+  r.thr_pi = (uint64_t)(pi * 4294967296.0);
+  double gsum = r.gam[0];
+  for (int j = 1; j < nb; ++j) gsum = gsum + r.gam[j];
+  const double s = 0.5 * ex(-(gsum * h));
+  double S = 1.0;
+  for (int q = 0; q < D->bmin; ++q) S = S * s;
+  int B = D->bmin;
+  while (S > D->burn_eps && B < D->bmax) { S = S * s; B++; }
+  r.B = B;
+  R[i] = r;
+  // Expected Philox draws per cell (a scheduling key only, never a result):
+  // gene switches + births (times 1 + splicing + loss) over the simulated span.
+  double sw = 0.0, births = 0.0, loss = 0.0;
+  for (int j = 0; j < nb; ++j) {
+    const double pon = r.kon[j] / (r.kon[j] + r.koff[j]);
+    sw += 2.0 * r.kon[j] * (1.0 - pon);
+    const double b = r.amax[j] * pon * D->capture_mean;
+    births += b;
+    loss += b * (r.bs / (r.bs + r.gam[j]) + 1.0);
+  }
+  const double per_hour = (sw + births + loss) / (double)nb;
+  cost[i] = (float)(per_hour * T * ((double)B + 0.5) + 4.0);
+  ident[i] = (int32_t)i;
+}
+
+// ---------------------------------------------------------------- the SSA
+// Lane state machine.  Each loop iteration draws exactly one Philox block and
+// spends it on one event proposal, one boundary crossing or 128 division
+// coin flips, so all lanes of a wave execute the same instruction stream
+// except for the short reaction branches.
+__device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double T, int nb) {
+  return ((double)(k - nbB) * T) / (double)nb;
+}
+
+__global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict__ D,
+                                                     const PRates* __restrict__ R,
+                                                     const int32_t* __restrict__ order,
+                                                     int64_t n, uint32_t first, uint32_t k0,
+                                                     uint32_t k1,
+                                                     const double* __restrict__ betas,
+                                                     const int64_t* __restrict__ goff,
+                                                     unsigned long long* __restrict__ queue,
+                                                     uint32_t* __restrict__ out_u,
+                                                     uint32_t* __restrict__ out_l,
+                                                     uint8_t* __restrict__ out_bin,
+                                                     unsigned long long* __restrict__ steps_out,
+                                                     unsigned long long* __restrict__ react_out) {
+  const double T = D->T;
+  const int nb = D->nb, J = D->J, C = D->C;
+  const int age_mode = D->age_mode, cap_mode = D->cap_mode;
+  const uint64_t total = (uint64_t)n * (uint64_t)C;
+
+  // per-lane cell state
+  bool have = false;
+  int p = 0, c = 0, bin = 0, j = 0, step = 1, scaling = 0;
+  uint32_t pid = 0, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  uint64_t ctr = 0, thr_lam = 0;
+  int64_t k = 0, knext = 0, nbB = 0;
+  double t = 0.0, tb = 0.0, gk = 0.0, age = 0.0, ws = 0.0, we = 0.0, cs = 0.0;
+  double kon = 0.0, koff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0;
+  bool g = false, ws_done = false, in_win = false;
+  bool div = false;
+  int div_s = 0;
+  uint32_t div_rem = 0, div_keep = 0;
+  unsigned long long st = 0, re = 0;
+
+  for (;;) {
+    if (!have) {
+      const unsigned long long q = atomicAdd(queue, 1ull);
+      if (q >= total) break;
+      const int64_t rnk = (int64_t)(q / (uint64_t)C);
+      c = (int)(q - (uint64_t)rnk * (uint64_t)C);
+      p = order[rnk];
+      pid = first + (uint32_t)p;
+      const PRates* rp = R + p;
+      ctr = 0;
+      u4 w = philox(0u, 0u, (uint32_t)c, pid, k0, k1);
+      ctr = 1;
+      if (age_mode == 1) {
+        const double th = u01_53(w.x, w.y);
+        age = th * T;
+        bin = (int)(th * (double)nb);
+        if (bin >= nb) bin = nb - 1;
+      } else {
+        bin = (c / J) % nb;
+        age = ((double)(2 * bin + 1) / (double)(2 * nb)) * T;
+      }
+      g = (uint64_t)w.z < rp->thr_pi;
+      const int cond = c % J;
+      if (cap_mode == 2) {
+        const int grp = bin * 2 + D->is_chase[cond];
+        const int64_t o = goff[grp];
+        const uint32_t cnt = (uint32_t)(goff[grp + 1] - o);
+        bc = betas[o + (int64_t)lemire(w.w, cnt)];
+      } else if (cap_mode == 1) {
+        const int a = D->beta_a, nu = D->beta_a + D->beta_b - 1;
+        uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
+        u4 v{0, 0, 0, 0};
+        for (int qq = 0; qq < nu; ++qq) {
+          if ((qq & 3) == 0) {
+            v = philox((uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)c, pid, k0, k1);
+            ctr++;
+          }
+          const int sel = qq & 3;
+          const uint32_t x = sel == 0 ? v.x : (sel == 1 ? v.y : (sel == 2 ? v.z : v.w));
+          if (x < s3) {
+            s3 = x;
+            if (s3 < s2) { const uint32_t tt = s2; s2 = s3; s3 = tt; }
+            if (s2 < s1) { const uint32_t tt = s1; s1 = s2; s2 = tt; }
+            if (s1 < s0) { const uint32_t tt = s0; s0 = s1; s1 = tt; }
+          }
+        }
+        const uint32_t xa = a == 1 ? s0 : (a == 2 ? s1 : (a == 3 ? s2 : s3));
+        bc = u01_32(xa);
+      } else {
+        bc = 1.0;
+      }
+      ws = age - D->pulse[cond] - D->chase[cond];
+      we = age - D->chase[cond];
+      step = rp->step;
+      scaling = rp->scaling;
+      thr_lam = rp->thr_lam;
+      bs = rp->bs;
+      nbB = (int64_t)rp->B * nb;
+      k = 0;
+      knext = step;
+      j = 0;
+      t = grid_time(0, nbB, T, nb);
+      cs = t;
+      kon = rp->kon[0];
+      koff = rp->koff[0];
+      atx = rp->amax[0] * bc;
+      gam = rp->gam[0];
+      bound = D->bound[0];
+      n0 = n1 = n2 = n3 = 0;
+      ws_done = false;
+      in_win = false;
+      div = false;
+      gk = grid_time(knext, nbB, T, nb);
+      tb = age;
+      if (gk < tb) tb = gk;
+      if (ws < tb) tb = ws;
+      st = 0;
+      re = 0;
+      have = true;
+    }
+
+    const u4 w = philox((uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)c, pid, k0, k1);
+    ctr++;
+    st++;
+
+    if (div) {
+      // binomial(count, 1/2) partition of species div_s, 128 coin flips per draw
+      const uint32_t m = div_rem < 128u ? div_rem : 128u;
+      uint32_t cnt = __popc(m >= 32u ? w.x : (w.x & ((1u << m) - 1u)));
+      if (m > 32u) cnt += __popc(m >= 64u ? w.y : (w.y & ((1u << (m - 32u)) - 1u)));
+      if (m > 64u) cnt += __popc(m >= 96u ? w.z : (w.z & ((1u << (m - 64u)) - 1u)));
+      if (m > 96u) cnt += __popc(m >= 128u ? w.w : (w.w & ((1u << (m - 96u)) - 1u)));
+      div_keep += cnt;
+      div_rem -= m;
+      if (div_rem == 0u) {
+        if (div_s == 0) n0 = div_keep;
+        else if (div_s == 1) n1 = div_keep;
+        else if (div_s == 2) n2 = div_keep;
+        else n3 = div_keep;
+        div_s++;
+        while (div_s < 4 && (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3)) == 0u) div_s++;
+        if (div_s < 4) {
+          div_rem = div_s == 1 ? n1 : (div_s == 2 ? n2 : n3);
+          div_keep = 0;
+        } else {
+          div = false;
+        }
+      }
+      continue;
+    }
+
+    const double a_on = g ? 0.0 : kon;
+    const double a_off = g ? koff : 0.0;
+    const double a_tx = g ? atx : 0.0;
+    const uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
+    const double a_spl = bs * (double)nun;
+    const double a_dec = gam * (double)ntot;
+    const double c1 = a_on, c2 = c1 + a_off, c3 = c2 + a_tx, c4 = c3 + a_spl;
+    const double a0 = c4 + a_dec;
+    const double tau = -ln(u01_32(w.x)) / a0;
+    const double tn = t + tau;
+    if (tn >= tb) {
+      // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
+      t = tb;
+      if (tb == age) {
+        const size_t o = (size_t)p * (size_t)C + (size_t)c;
+        out_u[o] = n0 + n1;
+        out_l[o] = n2 + n3;
+        out_bin[o] = (uint8_t)bin;
+        atomicAdd(steps_out + p, st);
+        atomicAdd(react_out + p, re);
+        have = false;
+        continue;
+      }
+      if (tb == gk) {
+        k = knext;
+        knext += step;
+        j = (int)(k % nb);
+        if (j == 0) {
+          cs = tb;
+          div_s = 0;
+          while (div_s < 4 && (div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3))) == 0u) div_s++;
+          if (div_s < 4) {
+            div = true;
+            div_rem = div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3));
+            div_keep = 0;
+          }
+        }
+        if (step == 1) {
+          const PRates* rp = R + p;
+          kon = rp->kon[j];
+          koff = rp->koff[j];
+          atx = rp->amax[j] * bc;
+          gam = rp->gam[j];
+          bound = D->bound[j];
+        }
+        gk = grid_time(knext, nbB, T, nb);
+      }
+      if (!ws_done && tb == ws) { ws_done = true; in_win = true; }
+      if (in_win && tb == we) in_win = false;
+      tb = age;
+      if (gk < tb) tb = gk;
+      if (!ws_done && ws < tb) tb = ws;
+      if (in_win && we < tb) tb = we;
+      continue;
+    }
+    t = tn;
+    const double rr = u01_32(w.y) * a0;
+    if (rr < c1) {
+      g = true; re++;
+    } else if (rr < c2) {
+      g = false; re++;
+    } else if (rr < c3) {
+      bool acc = true;
+      if (scaling) acc = u01_32(w.z) * bound < T + (t - cs);
+      if (acc) {
+        re++;
+        if (in_win && (uint64_t)w.w < thr_lam) n2++; else n0++;
+      }
+    } else if (rr < c4) {
+      re++;
+      if (lemire(w.z, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+    } else {
+      re++;
+      const uint32_t x = lemire(w.z, ntot);
+      if (x < n0) n0--;
+      else if (x < n0 + n1) n1--;
+      else if (x < n0 + n1 + n2) n2--;
+      else n3--;
+    }
+  }
+}
+
+// -------------------------------------------------------------- statistics
+__device__ __forceinline__ void add128(unsigned long long* lo, unsigned long long* hi,
+                                       unsigned long long x) {
+  const unsigned long long old = atomicAdd(lo, x);
+  if (old + x < old) atomicAdd(hi, 1ull);
+}
+
+struct U128 { uint64_t hi, lo; };
+
+__device__ __forceinline__ U128 mul64(uint64_t a, uint64_t b) {
+  return U128{__umul64hi(a, b), a * b};
+}
+__device__ __forceinline__ U128 add(U128 a, U128 b) {
+  U128 r{a.hi + b.hi, a.lo + b.lo};
+  if (r.lo < a.lo) r.hi++;
+  return r;
+}
+__device__ __forceinline__ U128 sub(U128 a, U128 b) {  // requires a >= b
+  U128 r{a.hi - b.hi, a.lo - b.lo};
+  if (a.lo < b.lo) r.hi--;
+  return r;
+}
+__device__ __forceinline__ bool ge(U128 a, U128 b) {
+  return a.hi > b.hi || (a.hi == b.hi && a.lo >= b.lo);
+}
+// n * (hi:lo) with n < 2^32 and the product < 2^128
+__device__ __forceinline__ U128 mul_small(uint64_t n, U128 v) {
+  U128 lo = mul64(n, v.lo);
+  lo.hi += n * v.hi;
+  return lo;
+}
+
+__device__ __forceinline__ double var_of(uint64_t n, uint64_t S, U128 SS) {
+  const U128 num = sub(mul_small(n, SS), mul64(S, S));
+  return u128_to_f64(num.hi, num.lo) / ((double)n * (double)(n - 1));
+}
+__device__ __forceinline__ double cov_of(uint64_t n, uint64_t Sa, uint64_t Sb, U128 Sab) {
+  const U128 a = mul_small(n, Sab), b = mul64(Sa, Sb);
+  const double den = (double)n * (double)(n - 1);
+  if (ge(a, b)) { const U128 d = sub(a, b); return u128_to_f64(d.hi, d.lo) / den; }
+  const U128 d = sub(b, a);
+  return -(u128_to_f64(d.hi, d.lo) / den);
+}
+
+__device__ __forceinline__ double wcov(const double* x, const double* y, const double* w, int nb) {
+  double mx = w[0] * x[0], my = w[0] * y[0];
+  for (int b = 1; b < nb; ++b) mx = mx + w[b] * x[b];
+  for (int b = 1; b < nb; ++b) my = my + w[b] * y[b];
+  double acc = w[0] * ((x[0] - mx) * (y[0] - my));
+  for (int b = 1; b < nb; ++b) acc = acc + w[b] * ((x[b] - mx) * (y[b] - my));
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void stats_kernel(const DesignDev* __restrict__ D,
+                                                    const uint32_t* __restrict__ cu,
+                                                    const uint32_t* __restrict__ cl,
+                                                    const uint8_t* __restrict__ cb, int64_t n,
+                                                    double* __restrict__ stats_ref,
+                                                    double* __restrict__ moments,
+                                                    double* __restrict__ superset) {
+  const int nb = D->nb, J = D->J, C = D->C, NBJ = nb * J;
+  const int64_t i = blockIdx.x;
+  __shared__ unsigned int s_n[ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ unsigned long long s_su[ABC_MAX_BINS * ABC_MAX_CONDS], s_sl[ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ unsigned long long s_uu[2][ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ unsigned long long s_ll[2][ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ unsigned long long s_ul[2][ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ double s_m1[ABC_MAX_BINS * ABC_MAX_CONDS], s_m2[ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ double s_v1[ABC_MAX_BINS * ABC_MAX_CONDS], s_v2[ABC_MAX_BINS * ABC_MAX_CONDS];
+  __shared__ double s_c12[ABC_MAX_BINS * ABC_MAX_CONDS];
+  for (int q = threadIdx.x; q < NBJ; q += blockDim.x) {
+    s_n[q] = 0; s_su[q] = 0; s_sl[q] = 0;
+    s_uu[0][q] = s_uu[1][q] = 0; s_ll[0][q] = s_ll[1][q] = 0; s_ul[0][q] = s_ul[1][q] = 0;
+  }
+  __syncthreads();
+  const uint32_t* u = cu + (size_t)i * C;
+  const uint32_t* l = cl + (size_t)i * C;
+  const uint8_t* bn = cb + (size_t)i * C;
+  for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    const int idx = bn[c] + nb * (c % J);
+    const unsigned long long a = u[c], b = l[c];
+    atomicAdd(&s_n[idx], 1u);
+    atomicAdd(&s_su[idx], a);
+    atomicAdd(&s_sl[idx], b);
+    add128(&s_uu[0][idx], &s_uu[1][idx], a * a);
+    add128(&s_ll[0][idx], &s_ll[1][idx], b * b);
+    add128(&s_ul[0][idx], &s_ul[1][idx], a * b);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < NBJ; idx += blockDim.x) {
+    const uint64_t nn = s_n[idx];
+    const U128 uu{s_uu[1][idx], s_uu[0][idx]}, ll{s_ll[1][idx], s_ll[0][idx]},
+        ul{s_ul[1][idx], s_ul[0][idx]};
+    double mt = 0.0, fft = 0.0, m1 = 0.0, m2 = 0.0, v1 = 0.0, v2 = 0.0, c12 = 0.0;
+    if (nn > 0) {
+      const double q = (double)nn;
+      m1 = (double)s_su[idx] / q;
+      m2 = (double)s_sl[idx] / q;
+      v1 = var_of(nn, s_su[idx], uu);
+      v2 = var_of(nn, s_sl[idx], ll);
+      c12 = cov_of(nn, s_su[idx], s_sl[idx], ul);
+      const uint64_t St = s_su[idx] + s_sl[idx];
+      const U128 Stt = add(add(uu, add(ul, ul)), ll);
+      mt = (double)St / q;
+      const double vt = var_of(nn, St, Stt);
+      fft = vt / (mt + (mt == 0.0 ? 0.0001 : 0.0));
+    }
+    s_m1[idx] = m1; s_m2[idx] = m2; s_v1[idx] = v1; s_v2[idx] = v2; s_c12[idx] = c12;
+    if (moments) {
+      double* mo = moments + i;
+      const int64_t base = 6 * (int64_t)idx;
+      mo[n * (base + 0)] = (double)nn;
+      mo[n * (base + 1)] = m1;
+      mo[n * (base + 2)] = m2;
+      mo[n * (base + 3)] = v1;
+      mo[n * (base + 4)] = c12;
+      mo[n * (base + 5)] = v2;
+      if (nn == 0) {
+        for (int qq = 0; qq < 6; ++qq) mo[n * (base + qq)] = 0.0;
+      }
+    }
+    if (superset) {
+      superset[i + n * idx] = mt;
+      superset[i + n * (NBJ + idx)] = fft;
+    }
+  }
+  __syncthreads();
+  if (!stats_ref) return;
+  const int tid = threadIdx.x;
+  if (tid < 2 * nb) {
+    const int grp = tid / nb, b = tid % nb;
+    uint64_t nn = 0, St = 0;
+    U128 Stt{0, 0};
+    for (int jj = 0; jj < J; ++jj) {
+      if (D->is_chase[jj] != grp) continue;
+      const int idx = b + nb * jj;
+      nn += s_n[idx];
+      St += s_su[idx] + s_sl[idx];
+      const U128 uu{s_uu[1][idx], s_uu[0][idx]}, ll{s_ll[1][idx], s_ll[0][idx]},
+          ul{s_ul[1][idx], s_ul[0][idx]};
+      Stt = add(Stt, add(add(uu, add(ul, ul)), ll));
+    }
+    double mean = 0.0, ff = 0.0;
+    if (nn > 0) {
+      mean = (double)St / (double)nn;
+      const double v = var_of(nn, St, Stt);
+      ff = v / (mean + (mean == 0.0 ? 0.0001 : 0.0));
+    }
+    stats_ref[i + n * ((2 * grp) * nb + b)] = mean;
+    stats_ref[i + n * ((2 * grp + 1) * nb + b)] = ff;
+  } else if (tid >= 64 && tid < 64 + J) {
+    const int jj = tid - 64;
+    uint64_t nn = 0, su = 0, sl = 0;
+    bool anynz = false;
+    for (int b = 0; b < nb; ++b) {
+      const int idx = b + nb * jj;
+      nn += s_n[idx]; su += s_su[idx]; sl += s_sl[idx];
+      if (s_c12[idx] != 0.0) anynz = true;
+    }
+    double ratio = 0.0;
+    if (nn > 0) {
+      const double a1 = (double)su / (double)nn, a2 = (double)sl / (double)nn;
+      if (a1 + a2 > 0.0) ratio = a2 / (a1 + a2);
+    }
+    const double* w = D->weights + nb * jj;
+    const double* m1 = s_m1 + nb * jj;
+    const double* m2 = s_m2 + nb * jj;
+    const double* v1 = s_v1 + nb * jj;
+    const double* v2 = s_v2 + nb * jj;
+    const double* c12 = s_c12 + nb * jj;
+    double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
+    for (int b = 1; b < nb; ++b) s1 = s1 + w[b] * v1[b];
+    for (int b = 1; b < nb; ++b) s2 = s2 + w[b] * v2[b];
+    for (int b = 1; b < nb; ++b) s12 = s12 + w[b] * c12[b];
+    const double tv1 = s1 + wcov(m1, m1, w, nb);
+    const double tv2 = s2 + wcov(m2, m2, w, nb);
+    double mc = 0.0, cm = 0.0;
+    if (anynz && tv1 != 0.0 && tv2 != 0.0) mc = s12 / sqrt(fabs(tv1 * tv2));
+    if (tv1 != 0.0 && tv2 != 0.0) cm = wcov(m1, m2, w, nb) / sqrt(fabs(tv1 * tv2));
+    stats_ref[i + n * (4 * nb + jj)] = ratio;
+    stats_ref[i + n * (4 * nb + J + jj)] = mc;
+    stats_ref[i + n * (4 * nb + 2 * J + jj)] = cm;
+  }
+}
+
+// ------------------------------------------------------------------ errors
+// One FP64 error in the order of compute_errors.jl:30-64.
+__device__ __forceinline__ double err_pair(const double* __restrict__ data,
+                                           const double* __restrict__ se, int64_t G, int64_t g,
+                                           const double* __restrict__ stats, int64_t n,
+                                           int64_t i, const ErrSpec& E) {
+  const double s2 = 0.1 * 0.1;
+  double err = 0.0;
+  int k = 0;
+  for (int pp = 0; pp < E.n_parts; ++pp) {
+    double e = 0.0;
+    const int end = k + E.parts[pp];
+    for (; k < end; ++k) {
+      const double d = data[g + G * k];
+      const double sg = se[g + G * k];
+      const double s = stats[i + n * k];
+      const double eps = (sg + d != 0.0) ? 0.0 : 0.0001;
+      const double diff = d - s;
+      e += (diff * diff) / ((sg * sg + s2 * (d * d)) + eps);
+    }
+    err += e / (double)E.K;
+  }
+  if (err > E.clamp) err = E.clamp;
+  return err;
+}
+
+// err holds the columns of genes g0 .. g0 + gridDim.y - 1
+__global__ __launch_bounds__(256) void err_fp64_kernel(const double* __restrict__ stats,
+                                                       int64_t n, const double* __restrict__ data,
+                                                       const double* __restrict__ se, int64_t G,
+                                                       int64_t g0, ErrSpec E,
+                                                       double* __restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = g0 + blockIdx.y;
+  if (i >= n) return;
+  err[i + n * (int64_t)blockIdx.y] = err_pair(data, se, G, g, stats, n, i, E);
+}
+
+__global__ __launch_bounds__(256) void accept_fp64_kernel(const double* __restrict__ stats,
+                                                          int64_t n,
+                                                          const double* __restrict__ data,
+                                                          const double* __restrict__ se,
+                                                          int64_t G, int64_t g0, ErrSpec E,
+                                                          double eps, int64_t first,
+                                                          Candidate* __restrict__ cand,
+                                                          unsigned long long* __restrict__ count,
+                                                          unsigned long long cap) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t g = g0 + blockIdx.y;
+  if (i >= n) return;
+  const double e = err_pair(data, se, G, g, stats, n, i, E);
+  if (e <= eps) {
+    const unsigned long long slot = atomicAdd(count, 1ull);
+    if (slot < cap) cand[slot] = Candidate{e, first + i, (int32_t)g, 0};
+  }
+}
+
+__global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}

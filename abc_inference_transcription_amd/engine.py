@@ -1,0 +1,163 @@
+"""Engine: one libabc context bound to a design, with numpy in/out.
+
+Every computation runs in libabc.so on the GPU; this module only marshals
+arrays (column-major, as the Julia host hands them over).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .design import stats_layout
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+class Engine:
+    def __init__(self, design: dict, device: int = 0):
+        self.L = _lib.load()
+        self.design = design
+        d = design
+        nb, J = int(d["n_bins"]), int(d["n_conds"])
+        self.nb, self.J, self.C = nb, J, int(d["cells"])
+        self._keep = []
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt).reshape(-1, order="F"))
+            self._keep.append(a)
+            return a
+
+        pulse, chase = arr(d["pulse"], np.float64), arr(d["chase"], np.float64)
+        is_chase = arr(d["is_chase"], np.int32)
+        w = arr(np.asarray(d["weights"], np.float64).reshape((nb, J), order="F"), np.float64)
+        betas = bbin = bch = None
+        nbeta = 0
+        if d.get("betas") is not None:
+            betas = arr(d["betas"], np.float64)
+            bbin = arr(d["beta_bin"], np.int32)
+            bch = arr(d["beta_is_chase"], np.int32)
+            nbeta = len(betas)
+        s = _lib.AbcDesign(
+            float(d["cycle"]), nb, J, pulse.ctypes.data, chase.ctypes.data, is_chase.ctypes.data,
+            self.C, int(d["age_mode"]), int(d["capture_mode"]), int(d.get("beta_a", 2)),
+            int(d.get("beta_b", 8)), None if betas is None else betas.ctypes.data,
+            None if bbin is None else bbin.ctypes.data, None if bch is None else bch.ctypes.data,
+            nbeta, w.ctypes.data, float(d.get("burn_eps", 5e-3)), int(d.get("burn_max", 16)),
+            int(d["prior_mode"]), float(d.get("splice_log10", 0.0)))
+        ctx = C.c_void_p()
+        rc = self.L.abc_create(C.byref(ctx), device, C.byref(s))
+        if rc != 0:
+            raise RuntimeError(f"abc_create failed ({rc}): {_lib.ERRORS.get(rc, '?')}")
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.abc_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        _lib.check(rc, self.ctx, what)
+
+    # ------------------------------------------------------------------ API
+    def n_params(self, model: int) -> int:
+        return self.L.abc_n_params(self.ctx, model)
+
+    @property
+    def n_stats(self) -> int:
+        return self.L.abc_n_stats(self.ctx)
+
+    @property
+    def parts(self):
+        return stats_layout(self.nb, self.J)
+
+    def sample_prior(self, model: int, n: int, seed: int = 0, first: int = 0) -> np.ndarray:
+        th = np.zeros((n, self.n_params(model)), order="F")
+        self._chk(self.L.abc_sample_prior(self.ctx, model, seed, first, n, _p(th)), "sample_prior")
+        return th
+
+    def simulate(self, model: int, theta, seed: int = 0, first: int = 0, cells=False,
+                 moments=False, superset=False) -> dict:
+        theta = np.asfortranarray(theta, dtype=np.float64)
+        n = theta.shape[0]
+        if theta.shape[1] != self.n_params(model):
+            raise ValueError(f"theta needs {self.n_params(model)} columns for model {model}")
+        NBJ = self.nb * self.J
+        out = {"stats": np.zeros((n, self.n_stats), order="F"),
+               "steps": np.zeros(n, np.int64)}
+        if moments:
+            out["moments"] = np.zeros((n, NBJ * 6), order="F")
+        if superset:
+            out["superset"] = np.zeros((n, NBJ * 2), order="F")
+        if cells:
+            out["cell_u"] = np.zeros((n, self.C), np.uint32)
+            out["cell_l"] = np.zeros((n, self.C), np.uint32)
+            out["cell_bin"] = np.zeros((n, self.C), np.uint8)
+        rc = self.L.abc_simulate(self.ctx, model, seed, first, n, _p(theta), _p(out["stats"]),
+                                 _p(out.get("moments")), _p(out.get("superset")),
+                                 _p(out["steps"]), _p(out.get("cell_u")), _p(out.get("cell_l")),
+                                 _p(out.get("cell_bin")))
+        self._chk(rc, "simulate")
+        return out
+
+    def errors(self, stats, data, se, parts=None, clamp: float = 10.0) -> np.ndarray:
+        stats = np.asfortranarray(stats, np.float64)
+        data = np.asfortranarray(data, np.float64)
+        se = np.asfortranarray(se, np.float64)
+        n, K = stats.shape
+        G = data.shape[0]
+        parts = np.ascontiguousarray(self.parts if parts is None else parts, np.int32)
+        err = np.zeros((n, G), order="F")
+        rc = self.L.abc_errors(self.ctx, n, K, _p(stats), G, _p(data), _p(se), _p(parts),
+                               len(parts), clamp, _p(err))
+        self._chk(rc, "errors")
+        return err
+
+    def accept(self, stats, data, se, parts=None, clamp: float = 10.0, mode: int = 0,
+               eps: float = 4.8, k: int = 1000, first_index: int = 0, cap: int | None = None):
+        """Per gene: (1-based particle indices, FP64 errors), sorted by (err, index)."""
+        stats = np.asfortranarray(stats, np.float64)
+        data = np.asfortranarray(data, np.float64)
+        se = np.asfortranarray(se, np.float64)
+        n, K = stats.shape
+        G = data.shape[0]
+        parts = np.ascontiguousarray(self.parts if parts is None else parts, np.int32)
+        if cap is None:
+            cap = G * (k if mode == 1 else min(n, 4096)) + 16
+        while True:
+            counts = np.zeros(G, np.int64)
+            idx = np.zeros(cap, np.int64)
+            eo = np.zeros(cap)
+            rc = self.L.abc_accept(self.ctx, n, K, _p(stats), G, _p(data), _p(se), _p(parts),
+                                   len(parts), clamp, mode, eps, k, first_index, _p(counts),
+                                   _p(idx), _p(eo), cap)
+            if rc == -5:
+                cap = int(counts.sum()) + 16
+                continue
+            self._chk(rc, "accept")
+            break
+        out, o = [], 0
+        for g in range(G):
+            out.append((idx[o:o + counts[g]].copy(), eo[o:o + counts[g]].copy()))
+            o += counts[g]
+        return out
+
+    def last_timing(self):
+        ms = np.zeros(4)
+        self._chk(self.L.abc_last_timing(self.ctx, _p(ms)), "last_timing")
+        return ms
+
+    def set_ssa_blocks(self, blocks: int):
+        self._chk(self.L.abc_set_ssa_blocks(self.ctx, blocks), "set_ssa_blocks")
+
+    def ssa_blocks(self) -> int:
+        return self.L.abc_get_ssa_blocks(self.ctx)

@@ -1,0 +1,204 @@
+"""Benchmark: ABC particles/sec for the plain path (SSA + statistics + errors).
+
+Workload: all five models, the cell design of the SSA workload (2,500 cells
+per particle, theta ~ U[0,1), 21 h cycle, 5 age bins, 4 pulses + 5 chases,
+Beta(2,8) capture, config priors, Philox seed 0); every particle's statistics
+(the 90-value mean/Fano superset) are scored against 2,000 synthetic genes
+whose data statistics come from ground-truth particles of the same priors with
+SE = |stat| * U[0.05, 0.3].  One step = one batch of P particles per model
+through prior -> rates -> SSA -> statistics -> FP64 errors, all in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles P]
+
+Prints one JSON line.  Only rank 0 / one GPU is supported.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "ABC particles/sec (SSA+stats+errors); accepted/sec surrogate vs plain GPU"
+HBM_PEAK = 8000.0       # GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--particles", type=int, default=512, help="particles per model per step")
+    ap.add_argument("--genes", type=int, default=2000)
+    ap.add_argument("--cells", type=int, default=2500)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if a.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+              "(particles shard by contiguous ranges with no communication; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from abc_inference_transcription_amd import _lib
+    from abc_inference_transcription_amd.design import config_design
+    from abc_inference_transcription_amd.engine import Engine
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    design = config_design(cells=a.cells)
+    eng = Engine(design, device=0)
+    L = eng.L
+    stream = torch.cuda.current_stream()
+    _lib.check(L.abc_set_stream(eng.ctx, C.c_void_p(stream.cuda_stream)), eng.ctx, "set_stream")
+    nb, J = eng.nb, eng.J
+    KS = 2 * nb * J
+    parts = np.array([nb * J, nb * J], np.int32)
+    models = [1, 2, 3, 4, 5]
+    P = a.particles
+
+    def ptr(t):
+        return C.c_void_p(t.data_ptr())
+
+    # ---- synthetic genes: ground-truth particles of the same priors (untimed)
+    G = a.genes
+    gt_first = 3_000_000_000  # stream range disjoint from the benchmark particles
+    per = (G + len(models) - 1) // len(models)
+    data_rows = []
+    for m in models:
+        npar = eng.n_params(m)
+        th = torch.empty((npar, per), dtype=torch.float64, device=dev)
+        st = torch.empty((4 * nb + 3 * J, per), dtype=torch.float64, device=dev)
+        sup = torch.empty((KS, per), dtype=torch.float64, device=dev)
+        _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, gt_first + m * per, per, ptr(th)), eng.ctx, "prior")
+        _lib.check(L.abc_simulate_dev(eng.ctx, m, 0, gt_first + m * per, per, ptr(th), ptr(st), None,
+                                      ptr(sup), None), eng.ctx, "simulate")
+        data_rows.append(sup.t().cpu().numpy())
+    torch.cuda.synchronize()
+    data = np.vstack(data_rows)[:G]
+    rng = np.random.default_rng(0)
+    se = np.abs(data) * rng.uniform(0.05, 0.3, data.shape)
+    data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64, device=dev)
+    se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
+    parts_c = np.ascontiguousarray(parts)
+
+    # ---- device buffers for one step
+    bufs = {}
+    for m in models:
+        npar = eng.n_params(m)
+        bufs[m] = dict(
+            theta=torch.empty(npar * P, dtype=torch.float64, device=dev),
+            stats=torch.empty((4 * nb + 3 * J) * P, dtype=torch.float64, device=dev),
+            sup=torch.empty(KS * P, dtype=torch.float64, device=dev),
+            steps=torch.zeros(P, dtype=torch.int64, device=dev),
+            err=torch.empty(P * G, dtype=torch.float64, device=dev),
+        )
+    ssa_ms, total_steps, err_ms = [], 0, []
+
+    def step(it, timed):
+        nonlocal total_steps
+        first = it * P
+        for m in models:
+            b = bufs[m]
+            _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, first, P, ptr(b["theta"])), eng.ctx, "prior")
+            _lib.check(L.abc_simulate_dev(eng.ctx, m, 0, first, P, ptr(b["theta"]), ptr(b["stats"]),
+                                          None, ptr(b["sup"]), ptr(b["steps"])), eng.ctx, "simulate")
+            _lib.check(L.abc_errors_dev(eng.ctx, P, KS, ptr(b["sup"]), G, ptr(data_d), ptr(se_d),
+                                        parts_c.ctypes.data_as(C.c_void_p), 2, 10.0, ptr(b["err"])),
+                       eng.ctx, "errors")
+            if timed:
+                ms = eng.last_timing()
+                ssa_ms.append(ms[1])
+                err_ms.append(ms[3])
+                total_steps += int(b["steps"].sum().item())
+
+    for w in range(a.warmup):
+        step(10_000 + w, False)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for it in range(a.steps):
+        step(it, True)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    n_particles = a.steps * P * len(models)
+    value = n_particles / (gpu_ms / 1e3)
+
+    # ---- roofline of the dominant kernel (SSA): algorithmic VALU lane-ops
+    from abc_inference_transcription_amd.roofline import SSA_OPS_PER_STEP, VALU_PEAK_TOPS
+    ssa_s = sum(ssa_ms) / 1e3
+    steps_per_s = total_steps / ssa_s
+    achieved = steps_per_s * SSA_OPS_PER_STEP / 1e12
+    roof = {"kernel": "ssa_kernel", "bound": "valu", "achieved": round(achieved, 3),
+            "peak": VALU_PEAK_TOPS, "unit": "Top/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+            "traffic": None, "steps_per_s": steps_per_s, "ops_per_step": SSA_OPS_PER_STEP,
+            "avg_launch_ms": sum(ssa_ms) / len(ssa_ms),
+            "ssa_share_of_step": ssa_s / (gpu_ms / 1e3)}
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "particles/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(gpu_ms / a.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (prior draws, Philox seed 0; genes from ground-truth prior particles)",
+        "config": {"workload": "plain ABC: 5 models x %d particles/step, 2500 cells, SSA+stats+FP64 errors vs %d genes" % (P, G),
+                   "models": models, "particles_per_model_per_step": P, "cells_per_particle": a.cells,
+                   "genes": G, "stats": KS, "cycle_h": 21.0, "capture": "Beta(2,8)",
+                   "mean_steps_per_particle": total_steps / n_particles},
+        "roofline": roof,
+        "wall_s": round(wall, 3),
+    }
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(design, models, P, total_steps / n_particles, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):
+    """The oracle's scalar replay of the same SSA (bit-identical work) on all
+    host cores, over a bounded sample of the benchmark's particles, scaled to
+    particles/s through the workload's mean steps per particle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    od = O.OracleDesign(design)
+    done_steps, done_p, t_used = 0, 0, 0.0
+    it = 0
+    while t_used < budget_s and it < 64:
+        m = models[it % len(models)]
+        first = (it // len(models)) * 8
+        th = O.sample_prior(od, m, 0, first, 8)
+        t = time.perf_counter()
+        r = O.simulate(od, m, 0, first, th, n_threads=threads, cells=False)
+        t_used += time.perf_counter() - t
+        done_steps += int(r["steps"].sum())
+        done_p += 8
+        it += 1
+    sps = done_steps / t_used
+    return {"value": round(sps / gpu_steps_per_particle, 4), "unit": "particles/s",
+            "cores": threads, "kind": "port",
+            "sample": f"{done_p} particles (8 per model, round-robin, first particles of each model's "
+                      f"stream), {done_steps} SSA steps in {t_used:.1f}s on {threads} threads = "
+                      f"{sps:.3e} steps/s; scaled by the benchmark's mean "
+                      f"{gpu_steps_per_particle:.4g} steps/particle",
+            "steps_per_s": sps}
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,145 @@
+/*
+ * abc.h -- C ABI of the MI355X ABC simulation engine (libabc.so).
+ *
+ * The reference has no FFI: its boundary is the Julia script contract
+ * (wrapper.jl:59-78): set the globals m, n_trials, submit and include()
+ * scripts/abc_simulation.jl, then scripts/compute_errors.jl, then
+ * scripts/accepted_particles.jl.  Each entry point below replaces the body of
+ * one of those stages; a Julia shim keeps the globals and the output files and
+ * calls these functions through ccall.
+ *
+ * Conventions
+ *  - Host arrays are column-major (Julia Array{Float64}): element (i, k) of an
+ *    n x K matrix is at [i + n*k].  The library never keeps a pointer past the
+ *    call that received it.
+ *  - Particle i of a call uses the random stream of global particle
+ *    first + i (Philox4x32-10 counter), so results do not depend on how a run is
+ *    split into calls or submits.
+ *  - Functions return ABC_OK (0) or a negative code; abc_last_error() gives
+ *    the message.  The library never aborts the caller.
+ *  - Host-pointer functions are synchronous.  *_dev functions take device
+ *    pointers, enqueue on the context stream and return without waiting.
+ *  - One context per host thread; a context is not re-entrant.
+ */
+#ifndef ABC_H
+#define ABC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ABC_OK 0
+#define ABC_E_ARG (-1)      /* invalid argument                       */
+#define ABC_E_HIP (-2)      /* HIP runtime error                      */
+#define ABC_E_OOM (-3)      /* device allocation failed               */
+#define ABC_E_MODEL (-4)    /* model index outside 1..5               */
+#define ABC_E_CAPACITY (-5) /* output buffer too small (counts valid) */
+
+typedef struct abc_ctx abc_ctx;
+
+/* Experimental design.  Replaces the globals read by abc_sim
+ * (scripts/abc_simulation.jl:13-14, 64-85) and produced by
+ * scripts/load_process_data.jl:50-83 (age bins, pulse/chase split,
+ * age_id_distribution) and wrapper.jl:36 (betas). */
+typedef struct {
+  double cycle_h;               /* cell-cycle length (abc_simulation.jl:72)         */
+  int32_t n_bins;               /* age bins = rate bins (n_clusters = 5)            */
+  int32_t n_conds;              /* labelling conditions (abc_simulation.jl:65)      */
+  const double* pulse_h;        /* [n_conds] pulse lengths                          */
+  const double* chase_h;        /* [n_conds] chase times                            */
+  const int32_t* is_chase;      /* [n_conds] 0 = pulse cells, 1 = chase cells       */
+  int32_t cells_per_particle;   /* simulated cells per parameter set                */
+  int32_t age_mode;             /* 0 = bin centres (reference), 1 = theta~U[0,1)    */
+  int32_t capture_mode;         /* 0 none, 1 Beta(beta_a,beta_b), 2 empirical list  */
+  int32_t beta_a, beta_b;       /* integer Beta parameters, beta_a <= 4             */
+  const double* betas;          /* [n_betas] capture efficiencies (mode 2)          */
+  const int32_t* beta_bin;      /* [n_betas] 0-based age bin of each efficiency     */
+  const int32_t* beta_is_chase; /* [n_betas] 0 pulse cell / 1 chase cell            */
+  int64_t n_betas;
+  const double* age_weights;    /* [n_bins*n_conds] col-major (age_id_distribution) */
+  double burn_eps;              /* burn-in residual target, e.g. 5e-3               */
+  int32_t burn_max;             /* burn-in cycle cap, e.g. 16                        */
+  int32_t prior_mode;           /* 0 = fix_params ranges, 1 = config priors          */
+  double splice_log10;          /* splicing rate for prior_mode 0                   */
+} abc_design;
+
+/* Context: owns the HIP stream, device buffers and launch shapes.
+ * New entry point (the Julia host has no context; the shim creates one per
+ * run from the globals above). */
+int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* design);
+int32_t abc_destroy(abc_ctx* ctx);
+const char* abc_last_error(const abc_ctx* ctx);
+
+/* Columns of theta for a model: reference columns (5 or 9) + log10 splicing. */
+int32_t abc_n_params(const abc_ctx* ctx, int32_t model);
+/* Length of the reference statistics vector: 4*n_bins + 3*n_conds (53). */
+int32_t abc_n_stats(const abc_ctx* ctx);
+
+/* Replaces fix_params (scripts/abc_simulation.jl:3-11) for n particles.
+ * theta: n x abc_n_params(model), log10 units, reference column order
+ * [kon..., koff, alpha..., gamma..., lambda, splice]. */
+int32_t abc_sample_prior(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                         double* theta);
+
+/* Replaces the per-particle body of scripts/abc_simulation.jl:88-97
+ * (abc_sim, scripts/abc_simulation.jl:13-62, and the model.jl solves under it).
+ *   stats    : n x abc_n_stats col-major, reference order
+ *              [pulse mean(n_bins), pulse FF(n_bins), chase mean, chase FF,
+ *               ratio(n_conds), mean_corr(n_conds), corr_mean(n_conds)]
+ *   moments  : optional n x (n_bins*n_conds*6): per (bin,cond)
+ *              [n, mean_u, mean_l, var_u, cov_ul, var_l]
+ *   superset : optional n x (2*n_bins*n_conds): mean and Fano factor of u+l
+ *   steps    : optional n, Philox draws spent per particle (work counter)
+ *   cell_u, cell_l, cell_bin : optional n*cells observed counts per cell */
+int32_t abc_simulate(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                     const double* theta, double* stats, double* moments, double* superset,
+                     int64_t* steps, uint32_t* cell_u, uint32_t* cell_l, uint8_t* cell_bin);
+
+/* Replaces compute_trunc_errors (scripts/compute_errors.jl:45-70): the FP64
+ * error of every (particle, gene) pair in the reference operation order.
+ *   stats: n x K, data/se: G x K, parts: sizes of the summed blocks
+ *   ({5,5,5,5,11,11,11} for the reference), err: n x G, clamped at clamp. */
+int32_t abc_errors(abc_ctx* ctx, int64_t n, int32_t K, const double* stats, int64_t G,
+                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
+                   double clamp, double* err);
+
+/* Replaces the loop of scripts/accepted_particles.jl:12-32 for one model.
+ * mode 0: per gene, particles with err <= eps, sorted by (err, index).
+ * mode 1: per gene, the k particles with smallest (err, index).
+ * Errors are screened with split-BF16 MFMA GEMMs and every particle within a
+ * proven margin of the threshold is rescored in FP64, so the sets equal the
+ * FP64 ones.  counts: G; idx: 1-based first_index+i+1, gene-major, cap
+ * entries; err_out: FP64 errors (optional).  ABC_E_CAPACITY if the total
+ * exceeds cap (counts are still written). */
+int32_t abc_accept(abc_ctx* ctx, int64_t n, int32_t K, const double* stats, int64_t G,
+                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
+                   double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
+                   int64_t* counts, int64_t* idx, double* err_out, int64_t cap);
+
+/* ---- device-resident variants (benchmarks, pipelines that keep data in HBM) */
+int32_t abc_set_stream(abc_ctx* ctx, void* hip_stream);   /* NULL = own stream */
+int32_t abc_synchronize(abc_ctx* ctx);
+int32_t abc_sample_prior_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first,
+                             int64_t n, double* theta_dev);
+int32_t abc_simulate_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first, int64_t n,
+                         const double* theta_dev, double* stats_dev, double* moments_dev,
+                         double* superset_dev, int64_t* steps_dev);
+int32_t abc_errors_dev(abc_ctx* ctx, int64_t n, int32_t K, const double* stats_dev, int64_t G,
+                       const double* data_dev, const double* se_dev, const int32_t* parts,
+                       int32_t n_parts, double clamp, double* err_dev);
+
+/* Milliseconds of the last simulate/errors call, measured with HIP events on
+ * the context stream: [0] prior+rates+order, [1] SSA kernel, [2] statistics
+ * kernel, [3] error kernel.  Waits for those events. */
+int32_t abc_last_timing(abc_ctx* ctx, double* ms4);
+
+/* Launch shape of the persistent SSA kernel: workgroups (0 = occupancy
+ * default).  Set by the autotuner. */
+int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
+int32_t abc_get_ssa_blocks(const abc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,136 @@
+/*
+ * oracle.h -- CPU oracle for the ABC hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * This library is the checker, never the product: only tests/, the smoke
+ * check in __graft_entry__.py and the cpu_baseline leg of bench.py may load it.
+ *
+ * It holds two independent restatements:
+ *   1. The reference moment model (scripts/model.jl, scripts/abc_simulation.jl,
+ *      scripts/compute_errors.jl, scripts/accepted_particles.jl) in plain C:
+ *      moment ODE + periodic boundary + transient phase + trajectories +
+ *      downsample + the 53 summary statistics + reference-order FP64 errors +
+ *      fixed-epsilon acceptance.
+ *   2. A scalar CPU replay of the engine's stochastic simulation (Gillespie SSA
+ *      over four mRNA species, Philox4x32-10 streams, integer-exact cell
+ *      statistics).  The GPU kernels must reproduce its outputs bit for bit.
+ *
+ * Every function cites the reference file:line it restates.  The SSA replay
+ * has no reference counterpart (the reference integrates moments); it is
+ * pinned statistically against restatement 1 and against closed forms.
+ */
+#ifndef ABC_ORACLE_H
+#define ABC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_BINS 8
+#define ORC_MAX_CONDS 16
+
+/* ------------------------------------------------------------------ RNG */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_u01_32(uint32_t w);
+double orc_u01_53(uint32_t a, uint32_t b);
+double orc_log(double x);
+double orc_exp(double x);
+double orc_exp10(double x);
+double orc_u128_to_f64(uint64_t hi, uint64_t lo);
+
+/* ------------------------------------------------------------- design */
+typedef struct {
+  double cycle;             /* cell-cycle length T in hours                 */
+  int32_t n_bins;           /* age bins (5 in the reference)                */
+  int32_t n_conds;          /* labelling conditions                         */
+  const double* pulse;      /* [n_conds] pulse length (h)                   */
+  const double* chase;      /* [n_conds] chase length (h)                   */
+  const int32_t* is_chase;  /* [n_conds] 0 = pulse group, 1 = chase group   */
+  int32_t cells;            /* cells per particle                           */
+  int32_t age_mode;         /* 0 = bin centres, 1 = theta ~ U[0,1)          */
+  int32_t capture_mode;     /* 0 = none, 1 = Beta(a,b), 2 = empirical list  */
+  int32_t beta_a, beta_b;   /* integer Beta parameters (capture_mode 1)     */
+  const double* betas;      /* grouped empirical list (capture_mode 2)      */
+  const int64_t* group_off; /* [2*n_bins+1], group = bin*2 + is_chase      */
+  const double* weights;    /* [n_bins*n_conds] col-major age weights       */
+  double burn_eps;          /* residual survival target for burn-in         */
+  int32_t burn_max;         /* cap on burn-in cycles                        */
+  int32_t prior_mode;       /* 0 = reference fix_params, 1 = config priors  */
+  double splice_log10;      /* splicing rate used by prior_mode 0           */
+} orc_design;
+
+int orc_n_params(int model);   /* reference columns + 1 (splicing) */
+
+/* Prior draws, theta is n x n_params column-major, log10 units. */
+void orc_sample_prior(const orc_design* d, int model, uint64_t seed, int64_t first,
+                      int64_t n, double* theta);
+
+/* Stochastic simulation replay.  Outputs (all optional except stats):
+ *   cell_u, cell_l, cell_bin : n*cells observed counts and age bin per cell
+ *   stats_ref   : n x (4*n_bins + 3*n_conds) col-major, reference layout
+ *   moments     : n x (n_bins*n_conds*6) col-major: per (bin,cond)
+ *                 [n, mean_u, mean_l, var_u, cov_ul, var_l]
+ *   superset    : n x (2*n_bins*n_conds): [mean_t(b,j)...] then [ff_t(b,j)...]
+ *   steps, reactions : n per-particle work counters
+ * Returns 0 or a negative error code. */
+int orc_simulate(const orc_design* d, int model, uint64_t seed, int64_t first, int64_t n,
+                 const double* theta, int n_threads, uint32_t* cell_u, uint32_t* cell_l,
+                 uint8_t* cell_bin, double* stats_ref, double* moments, double* superset,
+                 int64_t* steps, int64_t* reactions);
+
+/* Statistics of one particle from its cells (exposed for tests). */
+void orc_cell_stats(const orc_design* d, const uint32_t* u, const uint32_t* l,
+                    const uint8_t* bin, double* stats_ref, double* moments, double* superset);
+
+/* ---------------------------------------------------- moment model (ref) */
+typedef struct {
+  double cycle; int32_t n_steps;            /* T and number of rate bins    */
+  int32_t n_ages; const double* ages;       /* sampled ages (h)             */
+  int32_t n_conds; const double* pulse; const double* chase;
+  double t0;                                /* start of trajectories        */
+  double iv[9];                             /* initial moments              */
+  int32_t transient_mode;                   /* 0 = reference 1% rule,
+                                               1 = iterate to fixed point   */
+  double rtol, atol;                        /* integrator tolerances        */
+} orc_moment_cfg;
+
+/* y' = f(y) at time t (model.jl:67-86). */
+void orc_moment_rhs(int model, const double* theta, double cycle, int n_steps,
+                    double texp, double pulse, double t, const double* y, double* dy);
+/* model(): integrate [tmin,tmax] (model.jl:89-96). Returns steps taken. */
+int64_t orc_moment_solve(int model, const double* theta, double cycle, int n_steps,
+                         double texp, double pulse, double tmin, double tmax, double* y,
+                         double rtol, double atol);
+void orc_periodic_boundary(const double* in, double* out);
+/* transient_phase (model.jl:114-142); returns iterations. */
+int orc_transient(int model, const double* theta, const orc_moment_cfg* c, double* ss);
+/* run_part_sim without downsampling (recover_statistics.jl:2-11):
+ * out[cond][age][5] = [mean_u, mean_l, var_u, cov_ul, var_l]. */
+void orc_recover(int model, const double* theta, const orc_moment_cfg* c, double* out);
+/* downsample (model.jl:221-239): data n_ages x 5 row-major in place. */
+void orc_downsample(double* data, int n_ages, const double* mean_b, const double* m2_b,
+                    const double* var_b);
+/* abc_sim statistics (abc_simulation.jl:23-46) from per-condition moments
+ * s[cond][age][5]; w is n_ages x n_conds col-major; pulse_cond/chase_cond are
+ * the 0-based conditions feeding s_pulse / s_chase. out has 4*n_ages+3*n_conds. */
+void orc_moment_stats(const double* s, int n_ages, int n_conds, const double* w,
+                      int pulse_cond, int chase_cond, double* out);
+
+/* ------------------------------------------------- errors and acceptance */
+/* nlsqerror_part + compute_trunc_errors for one (particle, gene) pair in the
+ * reference operation order (compute_errors.jl:30-64). */
+double orc_error_pair(const double* d, const double* se, const double* s, int K,
+                      const int32_t* parts, int n_parts, double clamp);
+/* Full error matrix, err is n x G column-major. */
+void orc_errors(int64_t n, const double* stats, int K, int64_t G, const double* data,
+                const double* se, const int32_t* parts, int n_parts, double clamp,
+                double* err, int n_threads);
+/* accepted_particles.jl:20-22 for one gene: indices (1-based) with err <= eps
+ * sorted by (err, index).  Returns count; idx/err_out sized n. */
+int64_t orc_accept_gene(const double* err_col, int64_t n, double eps, int64_t* idx,
+                        double* err_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,138 @@
+/*
+ * orc_rng.c -- oracle RNG and elementary functions (TEST INFRASTRUCTURE ONLY).
+ *
+ * Philox4x32-10: Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
+ * as 1, 2, 3", SC'11.  Pinned by the Random123 known-answer vectors in
+ * tests/test_oracle_rng.py.
+ *
+ * log/exp: Cody & Waite, "Software Manual for the Elementary Functions"
+ * (1980): argument reduction by powers of two (log) and by multiples of ln 2
+ * with a two-word constant (exp), followed by a truncated series
+ * (log: 2*atanh(s) series in s=(m-1)/(m+1); exp: Taylor series to degree 13).
+ * Written with plain IEEE +,-,*,/ and compiled with -ffp-contract=off, so any
+ * other implementation of the same steps gets the same bits.  Pinned against
+ * libm/mpmath to a few ulp in tests.
+ *
+ * The reference draws its randomness from Julia's unseeded global RNG
+ * (scripts/abc_simulation.jl:4-9); there is no bit-level counterpart.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <string.h>
+
+#define PHILOX_M0 0xD2511F53u
+#define PHILOX_M1 0xCD9E8D57u
+#define PHILOX_W0 0x9E3779B9u
+#define PHILOX_W1 0xBB67AE85u
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int r = 0; r < 10; ++r) {
+    uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+    uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+    uint32_t n0 = hi1 ^ c1 ^ k0;
+    uint32_t n2 = hi0 ^ c3 ^ k1;
+    c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    k0 += PHILOX_W0; k1 += PHILOX_W1;
+  }
+  out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* (w + 1/2) * 2^-32: exactly representable, strictly inside (0,1). */
+double orc_u01_32(uint32_t w) { return ((double)w + 0.5) * 0x1p-32; }
+
+/* 53-bit uniform on [0,1) from two words (27 + 26 bits). */
+double orc_u01_53(uint32_t a, uint32_t b) {
+  return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * 0x1p-53;
+}
+
+static const double LN2_HI = 0x1.62e42fee00000p-1;
+static const double LN2_LO = 0x1.a39ef35793c76p-33;
+static const double INV_LN2 = 0x1.71547652b82fep+0;
+static const double LN10 = 0x1.26bb1bbb55516p+1;
+static const double SQRT2 = 0x1.6a09e667f3bcdp+0;
+
+static inline uint64_t dbits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
+static inline double bitsd(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
+
+double orc_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  int e_adj = 0;
+  if (x < 0x1p-1022) { x = x * 0x1p54; e_adj = -54; }   /* subnormal */
+  uint64_t u = dbits(x);
+  int e = (int)((u >> 52) & 0x7ff) - 1023 + e_adj;
+  double m = bitsd((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  /* [1,2) */
+  if (m > SQRT2) { m = m * 0.5; e = e + 1; }
+  /* s in [-0.1716, 0.1716]; log(m) = 2 atanh(s) */
+  double s = (m - 1.0) / (m + 1.0);
+  double z = s * s;
+  double p = 0x1.8618618618618p-4;           /* 2/21 */
+  p = p * z + 0x1.af286bca1af28p-4;          /* 2/19 */
+  p = p * z + 0x1.e1e1e1e1e1e1ep-4;          /* 2/17 */
+  p = p * z + 0x1.1111111111111p-3;          /* 2/15 */
+  p = p * z + 0x1.3b13b13b13b14p-3;          /* 2/13 */
+  p = p * z + 0x1.745d1745d1746p-3;          /* 2/11 */
+  p = p * z + 0x1.c71c71c71c71cp-3;          /* 2/9  */
+  p = p * z + 0x1.2492492492492p-2;          /* 2/7  */
+  p = p * z + 0x1.999999999999ap-2;          /* 2/5  */
+  p = p * z + 0x1.5555555555555p-1;          /* 2/3  */
+  double logm = (s + s) + (s * z) * p;
+  double de = (double)e;
+  return de * LN2_HI + (logm + de * LN2_LO);
+}
+
+double orc_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.78) return INFINITY;
+  if (x < -745.2) return 0.0;
+  double kd = floor(x * INV_LN2 + 0.5);
+  int k = (int)kd;
+  double r = (x - kd * LN2_HI) - kd * LN2_LO;
+  double p = 0x1.6124613a86d09p-33;          /* 1/13! */
+  p = p * r + 0x1.1eed8eff8d898p-29;         /* 1/12! */
+  p = p * r + 0x1.ae64567f544e4p-26;         /* 1/11! */
+  p = p * r + 0x1.27e4fb7789f5cp-22;         /* 1/10! */
+  p = p * r + 0x1.71de3a556c734p-19;         /* 1/9!  */
+  p = p * r + 0x1.a01a01a01a01ap-16;         /* 1/8!  */
+  p = p * r + 0x1.a01a01a01a01ap-13;         /* 1/7!  */
+  p = p * r + 0x1.6c16c16c16c17p-10;         /* 1/6!  */
+  p = p * r + 0x1.1111111111111p-7;          /* 1/5!  */
+  p = p * r + 0x1.5555555555555p-5;          /* 1/4!  */
+  p = p * r + 0x1.5555555555555p-3;          /* 1/3!  */
+  p = p * r + 0.5;                           /* 1/2!  */
+  p = p * r + 1.0;
+  p = p * r + 1.0;
+  /* scale by 2^k in two exact steps so subnormal results stay exact-ish */
+  if (k < -1000) {
+    double s1 = bitsd((uint64_t)(k + 1000 + 1023) << 52);
+    return (p * s1) * 0x1p-1000;
+  }
+  if (k > 1000) {
+    double s1 = bitsd((uint64_t)(k - 1000 + 1023) << 52);
+    return (p * s1) * 0x1p1000;
+  }
+  return p * bitsd((uint64_t)(k + 1023) << 52);
+}
+
+double orc_exp10(double x) { return orc_exp(x * LN10); }
+
+/* Correctly rounded (nearest-even) conversion of hi*2^64+lo to double. */
+double orc_u128_to_f64(uint64_t hi, uint64_t lo) {
+  unsigned __int128 v = ((unsigned __int128)hi << 64) | lo;
+  if (v == 0) return 0.0;
+  int p = hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll(lo);
+  if (p < 53) return (double)(uint64_t)v;          /* exact */
+  int sh = p - 52;
+  unsigned __int128 mant = v >> sh;
+  unsigned __int128 rem = v & ((((unsigned __int128)1) << sh) - 1);
+  unsigned __int128 half = ((unsigned __int128)1) << (sh - 1);
+  if (rem > half || (rem == half && (mant & 1))) {
+    mant += 1;
+    if (mant >> 53) { mant >>= 1; sh += 1; }
+  }
+  return ldexp((double)(uint64_t)mant, sh);
+}

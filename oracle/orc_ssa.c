@@ -1,0 +1,540 @@
+/*
+ * orc_ssa.c -- scalar CPU replay of the engine's stochastic simulation
+ * (TEST INFRASTRUCTURE ONLY: the checker for the GPU kernels, and the CPU
+ * baseline leg of bench.py).
+ *
+ * Semantics restated from the reference:
+ *   prior draws ........ scripts/abc_simulation.jl:3-11 (fix_params)
+ *   parameter layout ... scripts/model.jl:30-43 (get_vary_map), abc_simulation.jl:82-85
+ *   rates at time t .... scripts/model.jl:1-27 (get_rate, size_scaling)
+ *   labelling window ... scripts/model.jl:58-64, 183
+ *   reaction network ... scripts/model.jl:74-86 (the moment equations of the
+ *                        telegraph model with labelled/unlabelled mRNA)
+ *   division ........... scripts/model.jl:98-111 (binomial partitioning)
+ *   capture ............ scripts/model.jl:221-239 (downsample: binomial
+ *                        thinning with a per-cell efficiency)
+ *   cell statistics .... scripts/data_summary_statistics.jl:2-38, 70-77,
+ *                        110-147, 179-181 (estimators on cells)
+ *
+ * The reference integrates the exact first and second moments of this
+ * network; here the network itself is simulated cell by cell.  Two mRNA
+ * species of the reference (u, l) are split into unspliced/spliced; both decay
+ * at gamma so u = Uu+Us and l = Lu+Ls follow the reference exactly.
+ * Capture thinning is applied to births (a birth is captured with
+ * probability beta_c), which has the same law as Binomial(N, beta_c) at
+ * sampling because molecules never feed back on the gene.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef unsigned __int128 u128;
+
+typedef struct {
+  int n_kon, n_al, n_ga;
+  int o_kon, o_koff, o_al, o_ga, o_lam, o_spl, n_par;
+} layout_t;
+
+static layout_t layout_of(int model, int nb) {
+  layout_t L;
+  L.n_kon = model == 3 ? nb : 1;
+  L.n_al = model == 4 ? nb : 1;
+  L.n_ga = model == 5 ? nb : 1;
+  L.o_kon = 0;
+  L.o_koff = L.n_kon;
+  L.o_al = L.o_koff + 1;
+  L.o_ga = L.o_al + L.n_al;
+  L.o_lam = L.o_ga + L.n_ga;
+  L.o_spl = L.o_lam + 1;
+  L.n_par = L.o_spl + 1;
+  return L;
+}
+
+int orc_n_params(int model) { return layout_of(model, 5).n_par; }
+
+static void key_of(uint64_t seed, int model, uint32_t key[2]) {
+  key[0] = (uint32_t)seed;
+  key[1] = (uint32_t)(seed >> 32) ^ (0x9E3779B9u * (uint32_t)model);
+}
+
+/* ---------------------------------------------------------------- prior */
+typedef struct {
+  uint32_t key[2]; uint32_t pidx; int q; uint32_t w[4];
+} prior_stream;
+
+static double prior_u(prior_stream* s) {
+  int call = s->q >> 1, half = s->q & 1;
+  if (half == 0) {
+    uint32_t ctr[4] = {(uint32_t)call, 0u, 0xFFFFFFFFu, s->pidx};
+    orc_philox4x32_10(ctr, s->key, s->w);
+  }
+  s->q++;
+  return orc_u01_53(s->w[2 * half], s->w[2 * half + 1]);
+}
+
+static double unif(prior_stream* s, double lo, double hi) {
+  double u = prior_u(s);
+  return lo + (hi - lo) * u;
+}
+
+#define FC_LO (-0.698970004336018804786)
+#define FC_HI (0.698970004336018804786)
+
+void orc_sample_prior(const orc_design* d, int model, uint64_t seed, int64_t first,
+                      int64_t n, double* theta) {
+  int nb = d->n_bins;
+  layout_t L = layout_of(model, nb);
+  for (int64_t i = 0; i < n; ++i) {
+    prior_stream s;
+    key_of(seed, model, s.key);
+    s.pidx = (uint32_t)(first + i);
+    s.q = 0;
+    double th[2 * ORC_MAX_BINS + 8];
+    if (d->prior_mode == 0) {
+      /* fix_params: abc_simulation.jl:4-10, column order kon, koff, alpha, gamma, lambda */
+      for (int c = 0; c < L.n_kon; ++c) th[L.o_kon + c] = unif(&s, -3.0, 3.0);
+      th[L.o_koff] = unif(&s, -3.0, 3.0);
+      for (int c = 0; c < L.n_al; ++c) th[L.o_al + c] = unif(&s, -3.0, 3.0);
+      for (int c = 0; c < L.n_ga; ++c) th[L.o_ga + c] = unif(&s, -3.0, 2.0);
+      th[L.o_lam] = unif(&s, -0.7, 0.0);
+      th[L.o_spl] = d->splice_log10;
+    } else {
+      /* burst frequency, koff, burst size (alpha = size*koff), decay with
+       * per-bin log-uniform fold changes 0.2..5 for the varying rate. */
+      double kb = unif(&s, -2.0, 2.0);
+      if (L.n_kon > 1)
+        for (int c = 0; c < nb; ++c) th[L.o_kon + c] = kb + unif(&s, FC_LO, FC_HI);
+      else
+        th[L.o_kon] = kb;
+      double koff = unif(&s, -3.0, 3.0);
+      th[L.o_koff] = koff;
+      double bb = unif(&s, -1.0, 3.0);
+      if (L.n_al > 1)
+        for (int c = 0; c < nb; ++c) th[L.o_al + c] = (bb + unif(&s, FC_LO, FC_HI)) + koff;
+      else
+        th[L.o_al] = bb + koff;
+      double gb = unif(&s, -2.0, 1.0);
+      if (L.n_ga > 1)
+        for (int c = 0; c < nb; ++c) th[L.o_ga + c] = gb + unif(&s, FC_LO, FC_HI);
+      else
+        th[L.o_ga] = gb;
+      th[L.o_lam] = unif(&s, -0.7, 0.0);
+      th[L.o_spl] = unif(&s, -1.0, 2.0);
+    }
+    for (int c = 0; c < L.n_par; ++c) theta[i + n * c] = th[c];
+  }
+}
+
+/* ------------------------------------------------------ particle rates */
+typedef struct {
+  double kon[ORC_MAX_BINS], koff[ORC_MAX_BINS], amax[ORC_MAX_BINS], gam[ORC_MAX_BINS];
+  double bound[ORC_MAX_BINS];  /* T * (1 + (j+1)/nb): thinning bound in hours */
+  double bs;
+  uint64_t thr_lam, thr_pi;
+  int B, step, scaling;
+} prates;
+
+static int burn_min(const orc_design* d) {
+  double wmax = 0.0;
+  for (int j = 0; j < d->n_conds; ++j) {
+    double w = d->pulse[j] + d->chase[j];
+    if (w > wmax) wmax = w;
+  }
+  int b = (int)ceil(wmax / d->cycle);
+  return b < 1 ? 1 : b;
+}
+
+static void make_rates(const orc_design* d, int model, const double* th, prates* r) {
+  int nb = d->n_bins;
+  layout_t L = layout_of(model, nb);
+  double h = d->cycle / (double)nb;
+  r->scaling = model != 2;
+  r->step = model == 2 ? nb : 1;
+  for (int j = 0; j < nb; ++j) {
+    r->kon[j] = orc_exp10(th[L.o_kon + (L.n_kon > 1 ? j : 0)]);
+    r->koff[j] = orc_exp10(th[L.o_koff]);
+    double al = orc_exp10(th[L.o_al + (L.n_al > 1 ? j : 0)]);
+    /* size scaling (model.jl:5,25-27): alpha(t) = alpha * (1 + a/T) within a
+     * cycle; bin j is thinned from its end-of-bin bound alpha*(nb+j+1)/nb */
+    r->amax[j] = r->scaling ? (al * (double)(nb + j + 1)) / (double)nb : al;
+    r->bound[j] = (d->cycle * (double)(nb + j + 1)) / (double)nb;
+    r->gam[j] = orc_exp10(th[L.o_ga + (L.n_ga > 1 ? j : 0)]);
+  }
+  r->bs = orc_exp10(th[L.o_spl]);
+  double lam = orc_exp10(th[L.o_lam]);
+  r->thr_lam = (uint64_t)(lam * 4294967296.0);
+  double pi;
+  if (L.n_kon > 1) {
+    /* periodic stationary on-probability at cycle start: the one-bin map
+     * p -> pi_j + (p - pi_j) e_j composed over the cycle, then its fixed point */
+    double p = 0.0, E = 1.0;
+    for (int j = 0; j < nb; ++j) {
+      double rr = r->kon[j] + r->koff[j];
+      double pj = r->kon[j] / rr;
+      double e = orc_exp(-(rr * h));
+      p = pj + (p - pj) * e;
+      E = E * e;
+    }
+    pi = p / (1.0 - E);
+  } else {
+    pi = r->kon[0] / (r->kon[0] + r->koff[0]);
+  }
+  if (pi > 1.0) pi = 1.0;
+  if (pi < 0.0) pi = 0.0;
+  r->thr_pi = (uint64_t)(pi * 4294967296.0);
+  double gsum = r->gam[0];
+  for (int j = 1; j < nb; ++j) gsum = gsum + r->gam[j];
+  double s = 0.5 * orc_exp(-(gsum * h));
+  int bmin = burn_min(d);
+  double S = 1.0;
+  for (int i = 0; i < bmin; ++i) S = S * s;
+  int B = bmin;
+  while (S > d->burn_eps && B < d->burn_max) { S = S * s; B++; }
+  r->B = B;
+}
+
+/* --------------------------------------------------------------- cells */
+static inline uint32_t lemire(uint32_t w, uint32_t n) {
+  return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
+}
+
+typedef struct {
+  uint32_t key[2]; uint32_t cell, pidx; uint64_t ctr;
+} cell_stream;
+
+static void draw(cell_stream* s, uint32_t w[4]) {
+  uint32_t c[4] = {(uint32_t)s->ctr, (uint32_t)(s->ctr >> 32), s->cell, s->pidx};
+  orc_philox4x32_10(c, s->key, w);
+  s->ctr++;
+}
+
+static inline double grid_t(int64_t k, int64_t nbB, double T, int nb) {
+  return ((double)(k - nbB) * T) / (double)nb;
+}
+
+static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
+                    uint32_t pidx, int c, uint32_t* u_out, uint32_t* l_out,
+                    uint8_t* bin_out, int64_t* steps, int64_t* reactions) {
+  const int nb = d->n_bins, J = d->n_conds;
+  const double T = d->cycle;
+  cell_stream s = {{key[0], key[1]}, (uint32_t)c, pidx, 0};
+  uint32_t w[4];
+  draw(&s, w);
+  int bin;
+  double age;
+  if (d->age_mode == 1) {
+    double th = orc_u01_53(w[0], w[1]);
+    age = th * T;
+    bin = (int)(th * (double)nb);
+    if (bin >= nb) bin = nb - 1;
+  } else {
+    bin = (c / J) % nb;
+    age = ((double)(2 * bin + 1) / (double)(2 * nb)) * T;
+  }
+  int g = (uint64_t)w[2] < r->thr_pi;
+  int cond = c % J;
+  double bc = 1.0;
+  if (d->capture_mode == 2) {
+    int grp = bin * 2 + d->is_chase[cond];
+    int64_t cnt = d->group_off[grp + 1] - d->group_off[grp];
+    if (cnt <= 0) return -3;
+    bc = d->betas[d->group_off[grp] + (int64_t)lemire(w[3], (uint32_t)cnt)];
+  } else if (d->capture_mode == 1) {
+    int a = d->beta_a, nu = d->beta_a + d->beta_b - 1;
+    uint32_t lo[32];
+    int have = 0;
+    uint32_t v[4];
+    for (int q = 0; q < nu; ++q) {
+      if ((q & 3) == 0) draw(&s, v);
+      uint32_t x = v[q & 3];
+      /* keep the a smallest words, sorted ascending */
+      int pos = have < a ? have : a;
+      while (pos > 0 && lo[pos - 1] > x) {
+        if (pos < a) lo[pos] = lo[pos - 1];
+        pos--;
+      }
+      if (pos < a) lo[pos] = x;
+      if (have < a) have++;
+    }
+    bc = orc_u01_32(lo[a - 1]);
+  }
+  const double pulse = d->pulse[cond], chase = d->chase[cond];
+  const double ws = age - pulse - chase, we = age - chase;
+  const int64_t nbB = (int64_t)r->B * nb;
+  int64_t k = 0, knext = r->step;
+  double t = grid_t(0, nbB, T, nb);
+  int win_started = 0, in_win = 0;
+  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  int64_t st = 0, re = 0;
+  for (;;) {
+    double tb = age;
+    double gk = grid_t(knext, nbB, T, nb);
+    if (gk < tb) tb = gk;
+    if (!win_started && ws < tb) tb = ws;
+    if (in_win && we < tb) tb = we;
+    int j = (int)(k % nb);
+    draw(&s, w);
+    st++;
+    double a_on = g ? 0.0 : r->kon[j];
+    double a_off = g ? r->koff[j] : 0.0;
+    double a_tx = g ? r->amax[j] * bc : 0.0;
+    uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
+    double a_spl = r->bs * (double)nun;
+    double a_dec = r->gam[j] * (double)ntot;
+    double c1 = a_on, c2 = c1 + a_off, c3 = c2 + a_tx, c4 = c3 + a_spl, a0 = c4 + a_dec;
+    double tau = -orc_log(orc_u01_32(w[0])) / a0;
+    double tn = t + tau;
+    if (tn >= tb) {
+      t = tb;
+      if (tb == age) break;
+      if (tb == gk) {
+        k = knext;
+        knext += r->step;
+        if (k % nb == 0) {
+          uint32_t* sp[4] = {&n0, &n1, &n2, &n3};
+          for (int q = 0; q < 4; ++q) {
+            uint32_t cnt = *sp[q];
+            if (cnt == 0) continue;
+            uint32_t keep = 0, rem = cnt;
+            while (rem > 0) {
+              uint32_t v[4];
+              draw(&s, v);
+              st++;
+              uint32_t m = rem < 128u ? rem : 128u;
+              for (int b = 0; b < 4; ++b) {
+                int bits = (int)m - 32 * b;
+                if (bits <= 0) break;
+                uint32_t x = bits >= 32 ? v[b] : (v[b] & ((1u << bits) - 1u));
+                keep += (uint32_t)__builtin_popcount(x);
+              }
+              rem -= m;
+            }
+            *sp[q] = keep;
+          }
+        }
+      }
+      if (!win_started && tb == ws) { win_started = 1; in_win = 1; }
+      if (in_win && tb == we) in_win = 0;
+      continue;
+    }
+    t = tn;
+    double rr = orc_u01_32(w[1]) * a0;
+    if (rr < c1) {
+      g = 1; re++;
+    } else if (rr < c2) {
+      g = 0; re++;
+    } else if (rr < c3) {
+      int acc = 1;
+      if (r->scaling) {
+        /* accept with probability (T + a) / bound_j, a = time since cycle start */
+        double cs = grid_t(k - j, nbB, T, nb);
+        acc = orc_u01_32(w[2]) * r->bound[j] < T + (t - cs);
+      }
+      if (acc) {
+        re++;
+        if (in_win && (uint64_t)w[3] < r->thr_lam) n2++; else n0++;
+      }
+    } else if (rr < c4) {
+      re++;
+      if (lemire(w[2], nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+    } else {
+      re++;
+      uint32_t x = lemire(w[2], ntot);
+      if (x < n0) n0--;
+      else if (x < n0 + n1) n1--;
+      else if (x < n0 + n1 + n2) n2--;
+      else n3--;
+    }
+  }
+  *u_out = n0 + n1;
+  *l_out = n2 + n3;
+  *bin_out = (uint8_t)bin;
+  *steps += st;
+  *reactions += re;
+  return 0;
+}
+
+/* --------------------------------------------------------- statistics */
+static double f128(u128 v) { return orc_u128_to_f64((uint64_t)(v >> 64), (uint64_t)v); }
+
+static double var_of(uint64_t n, uint64_t S, u128 SS) {
+  u128 num = (u128)n * SS - (u128)S * S;
+  return f128(num) / ((double)n * (double)(n - 1));
+}
+
+static double cov_of(uint64_t n, uint64_t Sa, uint64_t Sb, u128 Sab) {
+  u128 a = (u128)n * Sab, b = (u128)Sa * Sb;
+  double den = (double)n * (double)(n - 1);
+  if (a >= b) return f128(a - b) / den;
+  return -(f128(b - a) / den);
+}
+
+/* weighted_cov, data_summary_statistics.jl:179-181 */
+static double wcov(const double* x, const double* y, const double* w, int n) {
+  double mx = w[0] * x[0], my = w[0] * y[0];
+  for (int b = 1; b < n; ++b) { mx = mx + w[b] * x[b]; }
+  for (int b = 1; b < n; ++b) { my = my + w[b] * y[b]; }
+  double acc = w[0] * ((x[0] - mx) * (y[0] - my));
+  for (int b = 1; b < n; ++b) acc = acc + w[b] * ((x[b] - mx) * (y[b] - my));
+  return acc;
+}
+
+void orc_cell_stats(const orc_design* d, const uint32_t* u, const uint32_t* l,
+                    const uint8_t* bin, double* stats_ref, double* moments, double* superset) {
+  const int nb = d->n_bins, J = d->n_conds, C = d->cells;
+  uint64_t n[ORC_MAX_BINS * ORC_MAX_CONDS] = {0}, Su[ORC_MAX_BINS * ORC_MAX_CONDS] = {0},
+           Sl[ORC_MAX_BINS * ORC_MAX_CONDS] = {0};
+  u128 Suu[ORC_MAX_BINS * ORC_MAX_CONDS] = {0}, Sll[ORC_MAX_BINS * ORC_MAX_CONDS] = {0},
+       Sul[ORC_MAX_BINS * ORC_MAX_CONDS] = {0};
+  for (int c = 0; c < C; ++c) {
+    int idx = bin[c] + nb * (c % J);
+    uint64_t a = u[c], b = l[c];
+    n[idx]++; Su[idx] += a; Sl[idx] += b;
+    Suu[idx] += (u128)a * a; Sll[idx] += (u128)b * b; Sul[idx] += (u128)a * b;
+  }
+  /* per (bin, cond) moments and the mean / Fano superset */
+  for (int j = 0; j < J; ++j)
+    for (int b = 0; b < nb; ++b) {
+      int idx = b + nb * j;
+      double* m = moments ? moments + 6 * idx : NULL;
+      double mt = 0.0, fft = 0.0;
+      if (n[idx] == 0) {
+        if (m) for (int q = 0; q < 6; ++q) m[q] = 0.0;
+      } else {
+        double nn = (double)n[idx];
+        if (m) {
+          m[0] = nn;
+          m[1] = (double)Su[idx] / nn;
+          m[2] = (double)Sl[idx] / nn;
+          m[3] = var_of(n[idx], Su[idx], Suu[idx]);
+          m[4] = cov_of(n[idx], Su[idx], Sl[idx], Sul[idx]);
+          m[5] = var_of(n[idx], Sl[idx], Sll[idx]);
+        }
+        uint64_t St = Su[idx] + Sl[idx];
+        u128 Stt = Suu[idx] + 2 * Sul[idx] + Sll[idx];
+        mt = (double)St / nn;
+        double vt = var_of(n[idx], St, Stt);
+        fft = vt / (mt + (mt == 0.0 ? 0.0001 : 0.0));
+      }
+      if (superset) { superset[idx] = mt; superset[nb * J + idx] = fft; }
+    }
+  if (!stats_ref) return;
+  /* pooled pulse / chase Fano per bin (data_summary_statistics.jl:2-32, 315-317) */
+  for (int grp = 0; grp < 2; ++grp)
+    for (int b = 0; b < nb; ++b) {
+      uint64_t nn = 0, St = 0;
+      u128 Stt = 0;
+      for (int j = 0; j < J; ++j) {
+        if (d->is_chase[j] != grp) continue;
+        int idx = b + nb * j;
+        nn += n[idx]; St += Su[idx] + Sl[idx];
+        Stt += Suu[idx] + 2 * Sul[idx] + Sll[idx];
+      }
+      double mean = 0.0, ff = 0.0;
+      if (nn > 0) {
+        mean = (double)St / (double)nn;
+        double v = var_of(nn, St, Stt);
+        ff = v / (mean + (mean == 0.0 ? 0.0001 : 0.0));
+      }
+      stats_ref[(2 * grp) * nb + b] = mean;
+      stats_ref[(2 * grp + 1) * nb + b] = ff;
+    }
+  /* ratios (data_summary_statistics.jl:61-77) and correlations (110-147) */
+  for (int j = 0; j < J; ++j) {
+    uint64_t nn = 0, su = 0, sl = 0;
+    double m1[ORC_MAX_BINS], m2[ORC_MAX_BINS], v1[ORC_MAX_BINS], v2[ORC_MAX_BINS],
+        c12[ORC_MAX_BINS];
+    int anynz = 0;
+    for (int b = 0; b < nb; ++b) {
+      int idx = b + nb * j;
+      nn += n[idx]; su += Su[idx]; sl += Sl[idx];
+      if (n[idx] == 0) {
+        m1[b] = m2[b] = v1[b] = v2[b] = c12[b] = 0.0;
+      } else {
+        double q = (double)n[idx];
+        m1[b] = (double)Su[idx] / q;
+        m2[b] = (double)Sl[idx] / q;
+        v1[b] = var_of(n[idx], Su[idx], Suu[idx]);
+        v2[b] = var_of(n[idx], Sl[idx], Sll[idx]);
+        c12[b] = cov_of(n[idx], Su[idx], Sl[idx], Sul[idx]);
+      }
+      if (c12[b] != 0.0) anynz = 1;
+    }
+    double ratio = 0.0;
+    if (nn > 0) {
+      double a1 = (double)su / (double)nn, a2 = (double)sl / (double)nn;
+      if (a1 + a2 > 0.0) ratio = a2 / (a1 + a2);
+    }
+    const double* w = d->weights + nb * j;
+    double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
+    for (int b = 1; b < nb; ++b) s1 = s1 + w[b] * v1[b];
+    for (int b = 1; b < nb; ++b) s2 = s2 + w[b] * v2[b];
+    for (int b = 1; b < nb; ++b) s12 = s12 + w[b] * c12[b];
+    double tv1 = s1 + wcov(m1, m1, w, nb);
+    double tv2 = s2 + wcov(m2, m2, w, nb);
+    double mc = 0.0, cm = 0.0;
+    if (anynz && tv1 != 0.0 && tv2 != 0.0) mc = s12 / sqrt(fabs(tv1 * tv2));
+    if (tv1 != 0.0 && tv2 != 0.0) cm = wcov(m1, m2, w, nb) / sqrt(fabs(tv1 * tv2));
+    stats_ref[4 * nb + j] = ratio;
+    stats_ref[4 * nb + J + j] = mc;
+    stats_ref[4 * nb + 2 * J + j] = cm;
+  }
+}
+
+/* ------------------------------------------------------------ driver */
+int orc_simulate(const orc_design* d, int model, uint64_t seed, int64_t first, int64_t n,
+                 const double* theta, int n_threads, uint32_t* cell_u, uint32_t* cell_l,
+                 uint8_t* cell_bin, double* stats_ref, double* moments, double* superset,
+                 int64_t* steps, int64_t* reactions) {
+  if (model < 1 || model > 5) return -1;
+  if (d->n_bins > ORC_MAX_BINS || d->n_conds > ORC_MAX_CONDS || d->cells <= 0) return -1;
+  if (first < 0 || first + n > 0xFFFFFFFFll || d->cells > 0x7FFFFFFF) return -1;
+  const int C = d->cells, nb = d->n_bins, J = d->n_conds;
+  const int K = 4 * nb + 3 * J, M = nb * J * 6, KS = 2 * nb * J;
+  const int np = layout_of(model, nb).n_par;
+  uint32_t key[2];
+  key_of(seed, model, key);
+  int err = 0;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    double th[2 * ORC_MAX_BINS + 8];
+    for (int c = 0; c < np; ++c) th[c] = theta[i + n * c];
+    prates r;
+    make_rates(d, model, th, &r);
+    uint32_t* u = (uint32_t*)malloc(sizeof(uint32_t) * C);
+    uint32_t* l = (uint32_t*)malloc(sizeof(uint32_t) * C);
+    uint8_t* bn = (uint8_t*)malloc(C);
+    int64_t st = 0, re = 0;
+    int e = 0;
+    for (int c = 0; c < C && !e; ++c)
+      e = sim_cell(d, &r, key, (uint32_t)(first + i), c, u + c, l + c, bn + c, &st, &re);
+    if (e) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+      err = e;
+    } else {
+      double sr[4 * ORC_MAX_BINS + 3 * ORC_MAX_CONDS];
+      double mo[ORC_MAX_BINS * ORC_MAX_CONDS * 6];
+      double ss[2 * ORC_MAX_BINS * ORC_MAX_CONDS];
+      orc_cell_stats(d, u, l, bn, sr, mo, ss);
+      if (stats_ref) for (int k = 0; k < K; ++k) stats_ref[i + n * k] = sr[k];
+      if (moments) for (int k = 0; k < M; ++k) moments[i + n * k] = mo[k];
+      if (superset) for (int k = 0; k < KS; ++k) superset[i + n * k] = ss[k];
+      if (cell_u) memcpy(cell_u + i * C, u, sizeof(uint32_t) * C);
+      if (cell_l) memcpy(cell_l + i * C, l, sizeof(uint32_t) * C);
+      if (cell_bin) memcpy(cell_bin + i * C, bn, C);
+      if (steps) steps[i] = st;
+      if (reactions) reactions[i] = re;
+    }
+    free(u); free(l); free(bn);
+  }
+  return err;
+}
