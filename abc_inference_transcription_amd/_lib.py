@@ -24,6 +24,7 @@ class AbcDesign(C.Structure):
         ("betas", C.c_void_p), ("beta_bin", C.c_void_p), ("beta_is_chase", C.c_void_p),
         ("n_betas", C.c_int64), ("age_weights", C.c_void_p), ("burn_eps", C.c_double),
         ("burn_max", C.c_int32), ("prior_mode", C.c_int32), ("splice_log10", C.c_double),
+        ("split_target", C.c_double), ("split_max", C.c_int32),
     ]
 
 
@@ -63,6 +64,7 @@ def _declare(L):
         "abc_simulate_dev": (I32, [V, I32, U64, I64, I64, V, V, V, V, V]),
         "abc_errors_dev": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V]),
         "abc_last_timing": (I32, [V, V]),
+        "abc_ssa_time": (I32, [V, I32, V, V]),
         "abc_set_ssa_blocks": (I32, [V, I32]),
         "abc_get_ssa_blocks": (I32, [V]),
     }

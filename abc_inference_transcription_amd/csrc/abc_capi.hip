@@ -37,10 +37,16 @@ struct abc_ctx {
   DesignDev* dd = nullptr;
   double* d_betas = nullptr;
   int64_t* d_goff = nullptr;
-  DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, cu, cl, cb, steps, react,
-      queue, stats, moments, superset, edata, ese, err, cand, ccount;
+  DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, items, off, scan_tmp, cu, cl,
+      cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount;
   hipEvent_t ev[5]{};
   bool timed[4]{};
+  // ring of (start, end) event pairs around every ssa_kernel launch
+  static constexpr int kRing = 256;
+  hipEvent_t ring[kRing][2]{};
+  int ring_used = 0;
+  double ssa_ms_acc = 0.0;
+  int64_t ssa_launches = 0;
   std::string msg;
 };
 
@@ -115,6 +121,33 @@ int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
   HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(
                 c->sort_tmp.p, tmp, P<float>(c->cost), P<float>(c->cost_sorted),
                 P<int32_t>(c->ident), P<int32_t>(c->order), (int)n, 0, 32, c->stream));
+  // work-item offsets in queue order: off[0] = 0, off[r+1] = off[r] + C * M
+  if ((rc = ensure(c, c->items, sizeof(int64_t) * n))) return rc;
+  if ((rc = ensure(c, c->off, sizeof(int64_t) * (n + 1)))) return rc;
+  hipLaunchKernelGGL(items_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd,
+                     P<PRates>(c->rates), P<int32_t>(c->order), n, P<int64_t>(c->items),
+                     P<int64_t>(c->off));
+  HIPCHK(c, hipGetLastError());
+  size_t stmp = 0;
+  HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, stmp, P<int64_t>(c->items),
+                                             P<int64_t>(c->off) + 1, (int)n, c->stream));
+  if ((rc = ensure(c, c->scan_tmp, stmp))) return rc;
+  stmp = c->scan_tmp.bytes;
+  HIPCHK(c, hipcub::DeviceScan::InclusiveSum(c->scan_tmp.p, stmp, P<int64_t>(c->items),
+                                             P<int64_t>(c->off) + 1, (int)n, c->stream));
+  return ABC_OK;
+}
+
+// fold the pending ssa_kernel event pairs into the running total
+int drain_ring(abc_ctx* c) {
+  for (int q = 0; q < c->ring_used; ++q) {
+    float v = 0.f;
+    HIPCHK(c, hipEventSynchronize(c->ring[q][1]));
+    HIPCHK(c, hipEventElapsedTime(&v, c->ring[q][0], c->ring[q][1]));
+    c->ssa_ms_acc += v;
+    c->ssa_launches++;
+  }
+  c->ring_used = 0;
   return ABC_OK;
 }
 
@@ -127,8 +160,13 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   const size_t cells = (size_t)n * (size_t)c->hd.C;
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
-  if ((rc = ensure(c, c->cu, sizeof(uint32_t) * cells))) return rc;
-  if ((rc = ensure(c, c->cl, sizeof(uint32_t) * cells))) return rc;
+  const int64_t cell_words = (int64_t)((cells + 1) / 2);  // u32 counts cleared as u64 words
+  if ((rc = ensure(c, c->cu, sizeof(uint64_t) * cell_words))) return rc;
+  if ((rc = ensure(c, c->cl, sizeof(uint64_t) * cell_words))) return rc;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(cell_words, 256)), dim3(256), 0, c->stream,
+                     P<unsigned long long>(c->cu), cell_words);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(cell_words, 256)), dim3(256), 0, c->stream,
+                     P<unsigned long long>(c->cl), cell_words);
   if ((rc = ensure(c, c->cb, cells))) return rc;
   if ((rc = ensure(c, c->react, sizeof(uint64_t) * n))) return rc;
   if ((rc = ensure(c, c->queue, 64))) return rc;
@@ -145,13 +183,20 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                      P<unsigned long long>(c->queue), (int64_t)8);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
+  if (c->ring_used == abc_ctx::kRing) {
+    if ((rc = drain_ring(c))) return rc;
+  }
+  hipEvent_t* pair = c->ring[c->ring_used++];
+  HIPCHK(c, hipEventRecord(pair[0], c->stream));
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
-                     P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
+                     P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
+                     (uint32_t)first, key0(seed),
                      key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps,
                      P<unsigned long long>(c->react));
   HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(pair[1], c->stream));
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
@@ -217,6 +262,11 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
   h.prior_mode = d->prior_mode;
   h.burn_eps = d->burn_eps;
   h.splice_log10 = d->splice_log10;
+  if (d->split_target < 0.0 || d->split_max < 1 || d->split_max > 4096) { delete c; return ABC_E_ARG; }
+  h.split_target = d->split_target;
+  h.split_max = d->split_max;
+  h.split_scale = d->capture_mode == 1 ? (double)d->beta_a / (double)(d->beta_a + d->beta_b)
+                                       : (d->capture_mode == 2 ? 0.25 : 1.0);
   double wmax = 0.0;
   for (int j = 0; j < h.J; ++j) {
     if (!(d->pulse_h[j] > 0.0) || d->chase_h[j] < 0.0 || (d->is_chase[j] != 0 && d->is_chase[j] != 1)) {
@@ -276,6 +326,9 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ABC_E_HIP; }
   for (auto& ev : c->ev)
     if (hipEventCreate(&ev) != hipSuccess) { abc_destroy(c); return ABC_E_HIP; }
+  for (auto& pr : c->ring)
+    for (auto& ev : pr)
+      if (hipEventCreate(&ev) != hipSuccess) { abc_destroy(c); return ABC_E_HIP; }
   if (hipMalloc(&c->dd, sizeof(DesignDev)) != hipSuccess ||
       hipMalloc(&c->d_betas, sizeof(double) * grouped.size()) != hipSuccess ||
       hipMalloc(&c->d_goff, sizeof(int64_t) * goff.size()) != hipSuccess) {
@@ -296,7 +349,8 @@ int32_t abc_destroy(abc_ctx* c) {
   if (!c) return ABC_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->theta, &c->rates, &c->cost, &c->cost_sorted, &c->ident, &c->order,
-                    &c->sort_tmp, &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
+                    &c->sort_tmp, &c->items, &c->off, &c->scan_tmp,
+                    &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount};
   for (DevBuf* b : bufs)
@@ -306,6 +360,9 @@ int32_t abc_destroy(abc_ctx* c) {
   if (c->d_goff) hipFree(c->d_goff);
   for (auto& ev : c->ev)
     if (ev) hipEventDestroy(ev);
+  for (auto& pr : c->ring)
+    for (auto& ev : pr)
+      if (ev) hipEventDestroy(ev);
   if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
   delete c;
   return ABC_OK;
@@ -549,6 +606,19 @@ int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_
     }
   }
   if (total > cap) return fail(c, ABC_E_CAPACITY, "accepted entries exceed cap");
+  return ABC_OK;
+}
+
+int32_t abc_ssa_time(abc_ctx* c, int32_t reset, double* ms_total, int64_t* launches) {
+  if (!c || !ms_total || !launches) return ABC_E_ARG;
+  int rc = drain_ring(c);
+  if (rc) return rc;
+  *ms_total = c->ssa_ms_acc;
+  *launches = c->ssa_launches;
+  if (reset) {
+    c->ssa_ms_acc = 0.0;
+    c->ssa_launches = 0;
+  }
   return ABC_OK;
 }
 

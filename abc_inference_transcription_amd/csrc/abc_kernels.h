@@ -23,6 +23,9 @@ struct DesignDev {
   double burn_eps;                 // residual survival target of the burn-in
   double splice_log10;             // splicing rate for the reference prior
   double capture_mean;             // mean capture efficiency (cost estimate)
+  double split_target;             // molecule events per sub-lane (0 = no split)
+  double split_scale;              // capture scale used by the split rule
+  int split_max, pad1;
   double pulse[ABC_MAX_CONDS], chase[ABC_MAX_CONDS];
   int is_chase[ABC_MAX_CONDS];
   double bound[ABC_MAX_BINS];      // T * (nb + j + 1) / nb (thinning bound, h)
@@ -35,7 +38,9 @@ struct __align__(16) PRates {
   double bs;                       // splicing rate
   uint64_t thr_lam;                // floor(lambda * 2^32): labelling threshold
   uint64_t thr_pi;                 // floor(P(gene on at cycle start) * 2^32)
-  int32_t B, step, scaling, pad;   // burn-in cycles, grid stride, size scaling
+  int32_t B, step, scaling;        // burn-in cycles, grid stride, size scaling
+  int32_t gene_varies;             // kon differs between bins (burst-frequency model)
+  int32_t M, pad[3];               // sub-lanes per cell
 };
 
 struct ErrSpec {
@@ -57,7 +62,10 @@ __global__ void prior_kernel(const DesignDev* D, int model, uint32_t k0, uint32_
                              uint32_t first, int64_t n, double* theta);
 __global__ void rates_kernel(const DesignDev* D, int model, int64_t n, const double* theta,
                              PRates* R, float* cost, int32_t* ident);
-__global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
+__global__ void items_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
+                             int64_t* items, int64_t* off0);
+__global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
+                           const int64_t* off, int64_t n,
                            uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
                            const int64_t* goff, unsigned long long* queue, uint32_t* out_u,
                            uint32_t* out_l, uint8_t* out_bin, unsigned long long* steps_out,

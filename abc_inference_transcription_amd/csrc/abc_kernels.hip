@@ -155,9 +155,21 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
   int B = D->bmin;
   while (S > D->burn_eps && B < D->bmax) { S = S * s; B++; }
   r.B = B;
+  r.gene_varies = L.n_kon > 1;
+  // Sub-lanes per cell: smallest power of two M with (expected molecule
+  // events per cell) <= split_target * M.  This value selects random streams,
+  // so it is computed with the same operations as the CPU replay.
+  int M = 1;
+  if (D->split_target > 0.0) {
+    double births = 0.0;
+    for (int j = 0; j < nb; ++j) births = births + r.amax[j] * (r.kon[j] / (r.kon[j] + r.koff[j]));
+    const double est = (((births / (double)nb) * D->split_scale) * (T * ((double)B + 1.0))) * 3.0;
+    while (M < D->split_max && est > D->split_target * (double)M) M *= 2;
+  }
+  r.M = M;
   R[i] = r;
-  // Expected Philox draws per cell (a scheduling key only, never a result):
-  // gene switches + births (times 1 + splicing + loss) over the simulated span.
+  // Expected Philox draws per work item (a scheduling key only, never a
+  // result): gene switches + births (times 1 + splicing + loss) / M.
   double sw = 0.0, births = 0.0, loss = 0.0;
   for (int j = 0; j < nb; ++j) {
     const double pon = r.kon[j] / (r.kon[j] + r.koff[j]);
@@ -166,25 +178,49 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
     births += b;
     loss += b * (r.bs / (r.bs + r.gam[j]) + 1.0);
   }
-  const double per_hour = (sw + births + loss) / (double)nb;
+  const double per_hour = (sw + (births + loss) / (double)M) / (double)nb;
   cost[i] = (float)(per_hour * T * ((double)B + 0.5) + 4.0);
   ident[i] = (int32_t)i;
 }
 
+// Work items of particle rank r (heaviest first): cells x sub-lanes.
+__global__ __launch_bounds__(256) void items_kernel(const DesignDev* __restrict__ D,
+                                                    const PRates* __restrict__ R,
+                                                    const int32_t* __restrict__ order, int64_t n,
+                                                    int64_t* __restrict__ items,
+                                                    int64_t* __restrict__ off0) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r == 0) off0[0] = 0;
+  if (r >= n) return;
+  items[r] = (int64_t)D->C * (int64_t)R[order[r]].M;
+}
+
 // ---------------------------------------------------------------- the SSA
-// Lane state machine.  Each loop iteration draws exactly one Philox block and
-// spends it on one event proposal, one boundary crossing or 128 division
-// coin flips, so all lanes of a wave execute the same instruction stream
-// except for the short reaction branches.
+// Work item = (particle, cell, sub-lane).  A cell is split into M sub-lanes:
+// every sub-lane replays the cell's gene path from the shared stream 0
+// (first-reaction clock) and simulates its own 1/M share of the births and
+// those molecules' fates from stream 1 + sub (direct method).  Molecules never
+// act on the gene, so the sum over sub-lanes has the law of the whole cell.
+// Lanes refill from a global item queue the moment their item ends, so heavy
+// and light particles share the machine without idle lanes.
 __device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double T, int nb) {
   return ((double)(k - nbB) * T) / (double)nb;
+}
+
+__device__ __forceinline__ u4 draw(uint64_t ctr, uint32_t stream, uint32_t cell, uint32_t pid,
+                                   uint32_t k0, uint32_t k1) {
+  return philox((uint32_t)ctr, ((uint32_t)(ctr >> 32) & 0xFFFFu) | (stream << 16), cell, pid, k0, k1);
+}
+
+__device__ __forceinline__ double wait_time(uint32_t w, double a) {
+  return -ln(u01_32(w)) / a;
 }
 
 __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict__ D,
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
-                                                     int64_t n, uint32_t first, uint32_t k0,
-                                                     uint32_t k1,
+                                                     const int64_t* __restrict__ off, int64_t n,
+                                                     uint32_t first, uint32_t k0, uint32_t k1,
                                                      const double* __restrict__ betas,
                                                      const int64_t* __restrict__ goff,
                                                      unsigned long long* __restrict__ queue,
@@ -196,34 +232,44 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
   const int age_mode = D->age_mode, cap_mode = D->cap_mode;
-  const uint64_t total = (uint64_t)n * (uint64_t)C;
+  const unsigned long long total = (unsigned long long)off[n];
 
-  // per-lane cell state
+  // per-lane item state
   bool have = false;
-  int p = 0, c = 0, bin = 0, j = 0, step = 1, scaling = 0;
-  uint32_t pid = 0, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  uint64_t ctr = 0, thr_lam = 0;
+  int p = 0, c = 0, bin = 0, j = 0, step = 1, scaling = 0, gvar = 0;
+  uint32_t pid = 0, sstream = 1, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  uint64_t gctr = 0, mctr = 0, thr_lam = 0;
   int64_t k = 0, knext = 0, nbB = 0;
-  double t = 0.0, tb = 0.0, gk = 0.0, age = 0.0, ws = 0.0, we = 0.0, cs = 0.0;
-  double kon = 0.0, koff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0;
-  bool g = false, ws_done = false, in_win = false;
-  bool div = false;
+  double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, ws = 0.0, we = 0.0, cs = 0.0;
+  double kon = 0.0, koff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0, share = 1.0;
+  bool g = false, ws_done = false, in_win = false, div = false;
   int div_s = 0;
   uint32_t div_rem = 0, div_keep = 0;
   unsigned long long st = 0, re = 0;
+  int sub = 0;
 
   for (;;) {
     if (!have) {
       const unsigned long long q = atomicAdd(queue, 1ull);
       if (q >= total) break;
-      const int64_t rnk = (int64_t)(q / (uint64_t)C);
-      c = (int)(q - (uint64_t)rnk * (uint64_t)C);
-      p = order[rnk];
-      pid = first + (uint32_t)p;
+      // particle rank: largest r with off[r] <= q
+      int64_t lo = 0, hi = n;
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((unsigned long long)off[mid] <= q) lo = mid; else hi = mid;
+      }
+      p = order[lo];
       const PRates* rp = R + p;
-      ctr = 0;
-      u4 w = philox(0u, 0u, (uint32_t)c, pid, k0, k1);
-      ctr = 1;
+      const int M = rp->M;
+      const int64_t rem = (int64_t)(q - (unsigned long long)off[lo]);
+      c = (int)(rem / M);
+      sub = (int)(rem - (int64_t)c * M);
+      pid = first + (uint32_t)p;
+      sstream = 1u + (uint32_t)sub;
+      share = (double)M;
+      gctr = 0;
+      mctr = 0;
+      u4 w = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
       if (age_mode == 1) {
         const double th = u01_53(w.x, w.y);
         age = th * T;
@@ -241,14 +287,12 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
         const uint32_t cnt = (uint32_t)(goff[grp + 1] - o);
         bc = betas[o + (int64_t)lemire(w.w, cnt)];
       } else if (cap_mode == 1) {
+        // Beta(a, b) = a-th smallest of a+b-1 uniforms
         const int a = D->beta_a, nu = D->beta_a + D->beta_b - 1;
         uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
         u4 v{0, 0, 0, 0};
         for (int qq = 0; qq < nu; ++qq) {
-          if ((qq & 3) == 0) {
-            v = philox((uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)c, pid, k0, k1);
-            ctr++;
-          }
+          if ((qq & 3) == 0) v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
           const int sel = qq & 3;
           const uint32_t x = sel == 0 ? v.x : (sel == 1 ? v.y : (sel == 2 ? v.z : v.w));
           if (x < s3) {
@@ -258,8 +302,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
             if (s1 < s0) { const uint32_t tt = s0; s0 = s1; s1 = tt; }
           }
         }
-        const uint32_t xa = a == 1 ? s0 : (a == 2 ? s1 : (a == 3 ? s2 : s3));
-        bc = u01_32(xa);
+        bc = u01_32(a == 1 ? s0 : (a == 2 ? s1 : (a == 3 ? s2 : s3)));
       } else {
         bc = 1.0;
       }
@@ -267,6 +310,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       we = age - D->chase[cond];
       step = rp->step;
       scaling = rp->scaling;
+      gvar = rp->gene_varies;
       thr_lam = rp->thr_lam;
       bs = rp->bs;
       nbB = (int64_t)rp->B * nb;
@@ -277,7 +321,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       cs = t;
       kon = rp->kon[0];
       koff = rp->koff[0];
-      atx = rp->amax[0] * bc;
+      atx = (rp->amax[0] * bc) / share;
       gam = rp->gam[0];
       bound = D->bound[0];
       n0 = n1 = n2 = n3 = 0;
@@ -290,15 +334,18 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       if (ws < tb) tb = ws;
       st = 0;
       re = 0;
+      {
+        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+        st++;
+        tg = t + wait_time(v.x, g ? koff : kon);
+      }
       have = true;
     }
 
-    const u4 w = philox((uint32_t)ctr, (uint32_t)(ctr >> 32), (uint32_t)c, pid, k0, k1);
-    ctr++;
-    st++;
-
     if (div) {
       // binomial(count, 1/2) partition of species div_s, 128 coin flips per draw
+      const u4 w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
+      st++;
       const uint32_t m = div_rem < 128u ? div_rem : 128u;
       uint32_t cnt = __popc(m >= 32u ? w.x : (w.x & ((1u << m) - 1u)));
       if (m > 32u) cnt += __popc(m >= 64u ? w.y : (w.y & ((1u << (m - 32u)) - 1u)));
@@ -323,85 +370,99 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       continue;
     }
 
-    const double a_on = g ? 0.0 : kon;
-    const double a_off = g ? koff : 0.0;
     const double a_tx = g ? atx : 0.0;
     const uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
     const double a_spl = bs * (double)nun;
     const double a_dec = gam * (double)ntot;
-    const double c1 = a_on, c2 = c1 + a_off, c3 = c2 + a_tx, c4 = c3 + a_spl;
-    const double a0 = c4 + a_dec;
-    const double tau = -ln(u01_32(w.x)) / a0;
-    const double tn = t + tau;
-    if (tn >= tb) {
-      // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
-      t = tb;
-      if (tb == age) {
-        const size_t o = (size_t)p * (size_t)C + (size_t)c;
-        out_u[o] = n0 + n1;
-        out_l[o] = n2 + n3;
-        out_bin[o] = (uint8_t)bin;
-        atomicAdd(steps_out + p, st);
-        atomicAdd(react_out + p, re);
-        have = false;
-        continue;
-      }
-      if (tb == gk) {
-        k = knext;
-        knext += step;
-        j = (int)(k % nb);
-        if (j == 0) {
-          cs = tb;
-          div_s = 0;
-          while (div_s < 4 && (div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3))) == 0u) div_s++;
-          if (div_s < 4) {
-            div = true;
-            div_rem = div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3));
-            div_keep = 0;
-          }
+    const double c1 = a_tx, c2 = c1 + a_spl;
+    const double a0 = c2 + a_dec;
+    double tm = __longlong_as_double(0x7ff0000000000000ll);
+    u4 w{0, 0, 0, 0};
+    if (a0 > 0.0) {
+      w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
+      st++;
+      tm = t + wait_time(w.x, a0);
+    }
+    if (tm < tg && tm < tb) {
+      t = tm;
+      const double rr = u01_32(w.y) * a0;
+      if (rr < c1) {
+        bool acc = true;
+        if (scaling) acc = u01_32(w.z) * bound < T + (t - cs);
+        if (acc) {
+          re++;
+          if (in_win && (uint64_t)w.w < thr_lam) n2++; else n0++;
         }
-        if (step == 1) {
-          const PRates* rp = R + p;
-          kon = rp->kon[j];
-          koff = rp->koff[j];
-          atx = rp->amax[j] * bc;
-          gam = rp->gam[j];
-          bound = D->bound[j];
-        }
-        gk = grid_time(knext, nbB, T, nb);
+      } else if (rr < c2) {
+        re++;
+        if (lemire(w.z, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+      } else {
+        re++;
+        const uint32_t x = lemire(w.z, ntot);
+        if (x < n0) n0--;
+        else if (x < n0 + n1) n1--;
+        else if (x < n0 + n1 + n2) n2--;
+        else n3--;
       }
-      if (!ws_done && tb == ws) { ws_done = true; in_win = true; }
-      if (in_win && tb == we) in_win = false;
-      tb = age;
-      if (gk < tb) tb = gk;
-      if (!ws_done && ws < tb) tb = ws;
-      if (in_win && we < tb) tb = we;
       continue;
     }
-    t = tn;
-    const double rr = u01_32(w.y) * a0;
-    if (rr < c1) {
-      g = true; re++;
-    } else if (rr < c2) {
-      g = false; re++;
-    } else if (rr < c3) {
-      bool acc = true;
-      if (scaling) acc = u01_32(w.z) * bound < T + (t - cs);
-      if (acc) {
-        re++;
-        if (in_win && (uint64_t)w.w < thr_lam) n2++; else n0++;
-      }
-    } else if (rr < c4) {
+    if (tg < tb) {
+      t = tg;
+      g = !g;
       re++;
-      if (lemire(w.z, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
-    } else {
-      re++;
-      const uint32_t x = lemire(w.z, ntot);
-      if (x < n0) n0--;
-      else if (x < n0 + n1) n1--;
-      else if (x < n0 + n1 + n2) n2--;
-      else n3--;
+      const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+      st++;
+      tg = t + wait_time(v.x, g ? koff : kon);
+      continue;
     }
+    // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
+    t = tb;
+    if (tb == age) {
+      const size_t o = (size_t)p * (size_t)C + (size_t)c;
+      atomicAdd(out_u + o, n0 + n1);
+      atomicAdd(out_l + o, n2 + n3);
+      if (sub == 0) out_bin[o] = (uint8_t)bin;
+      atomicAdd(steps_out + p, st);
+      atomicAdd(react_out + p, re);
+      have = false;
+      continue;
+    }
+    if (tb == gk) {
+      k = knext;
+      knext += step;
+      j = (int)(k % nb);
+      if (j == 0) {
+        cs = tb;
+        div_s = 0;
+        while (div_s < 4 && (div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3))) == 0u) div_s++;
+        if (div_s < 4) {
+          div = true;
+          div_rem = div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3));
+          div_keep = 0;
+        }
+      }
+      if (step == 1) {
+        const PRates* rp = R + p;
+        kon = rp->kon[j];
+        koff = rp->koff[j];
+        atx = (rp->amax[j] * bc) / share;
+        gam = rp->gam[j];
+        bound = D->bound[j];
+      }
+      if (gvar) {
+        // the gene's rate changed: restart its clock (memoryless)
+        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+        st++;
+        tg = t + wait_time(v.x, g ? koff : kon);
+      }
+      gk = grid_time(knext, nbB, T, nb);
+    }
+    if (!ws_done && tb == ws) { ws_done = true; in_win = true; }
+    if (in_win && tb == we) in_win = false;
+    tb = age;
+    if (gk < tb) tb = gk;
+    if (!ws_done && ws < tb) tb = ws;
+    if (in_win && we < tb) tb = we;
   }
 }
 
@@ -484,6 +545,7 @@ __global__ __launch_bounds__(256) void stats_kernel(const DesignDev* __restrict_
   const uint32_t* l = cl + (size_t)i * C;
   const uint8_t* bn = cb + (size_t)i * C;
   for (int c = threadIdx.x; c < C; c += blockDim.x) {
+    if (bn[c] >= nb) continue;  // never produced by the SSA; keeps LDS indexing in range
     const int idx = bn[c] + nb * (c % J);
     const unsigned long long a = u[c], b = l[c];
     atomicAdd(&s_n[idx], 1u);

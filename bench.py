@@ -61,7 +61,10 @@ def main():
     design = config_design(cells=a.cells)
     eng = Engine(design, device=0)
     L = eng.L
-    stream = torch.cuda.current_stream()
+    # one explicit (non-null, default-priority) stream shared by torch and libabc, so the
+    # timing events below bracket exactly the library's kernels
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     _lib.check(L.abc_set_stream(eng.ctx, C.c_void_p(stream.cuda_stream)), eng.ctx, "set_stream")
     nb, J = eng.nb, eng.J
     KS = 2 * nb * J
@@ -105,10 +108,9 @@ def main():
             steps=torch.zeros(P, dtype=torch.int64, device=dev),
             err=torch.empty(P * G, dtype=torch.float64, device=dev),
         )
-    ssa_ms, total_steps, err_ms = [], 0, []
+    steps_acc = torch.zeros((), dtype=torch.int64, device=dev)
 
     def step(it, timed):
-        nonlocal total_steps
         first = it * P
         for m in models:
             b = bufs[m]
@@ -119,36 +121,42 @@ def main():
                                         parts_c.ctypes.data_as(C.c_void_p), 2, 10.0, ptr(b["err"])),
                        eng.ctx, "errors")
             if timed:
-                ms = eng.last_timing()
-                ssa_ms.append(ms[1])
-                err_ms.append(ms[3])
-                total_steps += int(b["steps"].sum().item())
+                steps_acc.add_(b["steps"].sum())  # same stream: no host sync
+
+    def ssa_time(reset):
+        ms, nl = C.c_double(), C.c_int64()
+        _lib.check(L.abc_ssa_time(eng.ctx, reset, C.byref(ms), C.byref(nl)), eng.ctx, "ssa_time")
+        return ms.value, nl.value
 
     for w in range(a.warmup):
         step(10_000 + w, False)
     torch.cuda.synchronize()
+    ssa_time(1)
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev0.record(stream)
     for it in range(a.steps):
         step(it, True)
+        print(f"[bench] step {it} enqueued", file=sys.stderr, flush=True)
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
     n_particles = a.steps * P * len(models)
     value = n_particles / (gpu_ms / 1e3)
+    ssa_ms_total, launches = ssa_time(1)
+    total_steps = int(steps_acc.item())
 
     # ---- roofline of the dominant kernel (SSA): algorithmic VALU lane-ops
     from abc_inference_transcription_amd.roofline import SSA_OPS_PER_STEP, VALU_PEAK_TOPS
-    ssa_s = sum(ssa_ms) / 1e3
+    ssa_s = ssa_ms_total / 1e3
     steps_per_s = total_steps / ssa_s
     achieved = steps_per_s * SSA_OPS_PER_STEP / 1e12
     roof = {"kernel": "ssa_kernel", "bound": "valu", "achieved": round(achieved, 3),
             "peak": VALU_PEAK_TOPS, "unit": "Top/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
             "traffic": None, "steps_per_s": steps_per_s, "ops_per_step": SSA_OPS_PER_STEP,
-            "avg_launch_ms": sum(ssa_ms) / len(ssa_ms),
+            "avg_launch_ms": ssa_ms_total / max(launches, 1), "launches": launches,
             "ssa_share_of_step": ssa_s / (gpu_ms / 1e3)}
 
     out = {
@@ -171,32 +179,35 @@ def main():
 
 
 def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):
-    """The oracle's scalar replay of the same SSA (bit-identical work) on all
-    host cores, over a bounded sample of the benchmark's particles, scaled to
-    particles/s through the workload's mean steps per particle."""
+    """The oracle's scalar replay of the same SSA (the identical Philox steps)
+    on the host cores, over a bounded sample of the benchmark's particles:
+    the same particles and cell design but 100 cells each, so the sample stays
+    within the time budget.  steps/s is then scaled to particles/s through the
+    benchmark's measured mean steps per 2,500-cell particle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = min(os.cpu_count() or 1, 16)
-    od = O.OracleDesign(design)
+    small = dict(design, cells=100)
+    od = O.OracleDesign(small)
     done_steps, done_p, t_used = 0, 0, 0.0
     it = 0
-    while t_used < budget_s and it < 64:
+    while t_used < budget_s and it < 200:
         m = models[it % len(models)]
-        first = (it // len(models)) * 8
-        th = O.sample_prior(od, m, 0, first, 8)
+        first = (it // len(models)) * 16
+        th = O.sample_prior(od, m, 0, first, 16)
         t = time.perf_counter()
         r = O.simulate(od, m, 0, first, th, n_threads=threads, cells=False)
         t_used += time.perf_counter() - t
         done_steps += int(r["steps"].sum())
-        done_p += 8
+        done_p += 16
         it += 1
     sps = done_steps / t_used
     return {"value": round(sps / gpu_steps_per_particle, 4), "unit": "particles/s",
             "cores": threads, "kind": "port",
-            "sample": f"{done_p} particles (8 per model, round-robin, first particles of each model's "
-                      f"stream), {done_steps} SSA steps in {t_used:.1f}s on {threads} threads = "
-                      f"{sps:.3e} steps/s; scaled by the benchmark's mean "
-                      f"{gpu_steps_per_particle:.4g} steps/particle",
+            "sample": f"{done_p} benchmark particles (16 per model round-robin, first of each "
+                      f"model's stream) with 100 cells each: {done_steps} SSA steps in "
+                      f"{t_used:.1f}s on {threads} threads = {sps:.3e} steps/s; scaled by the "
+                      f"benchmark's mean {gpu_steps_per_particle:.4g} steps per 2500-cell particle",
             "steps_per_s": sps}
 
 

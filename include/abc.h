@@ -62,6 +62,9 @@ typedef struct {
   int32_t burn_max;             /* burn-in cycle cap, e.g. 16                        */
   int32_t prior_mode;           /* 0 = fix_params ranges, 1 = config priors          */
   double splice_log10;          /* splicing rate for prior_mode 0                   */
+  double split_target;          /* expected molecule events per sub-lane (0 = one
+                                   lane per cell), e.g. 16384                       */
+  int32_t split_max;            /* sub-lanes per cell cap, 1..4096, e.g. 1024       */
 } abc_design;
 
 /* Context: owns the HIP stream, device buffers and launch shapes.
@@ -133,6 +136,11 @@ int32_t abc_errors_dev(abc_ctx* ctx, int64_t n, int32_t K, const double* stats_d
  * the context stream: [0] prior+rates+order, [1] SSA kernel, [2] statistics
  * kernel, [3] error kernel.  Waits for those events. */
 int32_t abc_last_timing(abc_ctx* ctx, double* ms4);
+
+/* Total milliseconds spent in ssa_kernel over every simulate call since the
+ * last reset (HIP events around each launch on the context stream), and the
+ * number of launches.  Waits for the pending events; reset != 0 clears. */
+int32_t abc_ssa_time(abc_ctx* ctx, int32_t reset, double* ms_total, int64_t* launches);
 
 /* Launch shape of the persistent SSA kernel: workgroups (0 = occupancy
  * default).  Set by the autotuner. */

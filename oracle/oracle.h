@@ -57,6 +57,8 @@ typedef struct {
   int32_t burn_max;         /* cap on burn-in cycles                        */
   int32_t prior_mode;       /* 0 = reference fix_params, 1 = config priors  */
   double splice_log10;      /* splicing rate used by prior_mode 0           */
+  double split_target;      /* molecule events per sub-lane (0 = no split)  */
+  int32_t split_max;        /* largest number of sub-lanes per cell         */
 } orc_design;
 
 int orc_n_params(int model);   /* reference columns + 1 (splicing) */

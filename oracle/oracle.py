@@ -42,7 +42,7 @@ class Design(C.Structure):
         ("beta_a", C.c_int32), ("beta_b", C.c_int32),
         ("betas", C.c_void_p), ("group_off", C.c_void_p), ("weights", C.c_void_p),
         ("burn_eps", C.c_double), ("burn_max", C.c_int32), ("prior_mode", C.c_int32),
-        ("splice_log10", C.c_double),
+        ("splice_log10", C.c_double), ("split_target", C.c_double), ("split_max", C.c_int32),
     ]
 
 
@@ -152,7 +152,8 @@ class OracleDesign:
             _p(self.is_chase), int(d["cells"]), int(d["age_mode"]), int(d["capture_mode"]),
             int(d.get("beta_a", 2)), int(d.get("beta_b", 8)), _p(self.betas),
             _p(self.group_off), _p(self.weights), float(d.get("burn_eps", 5e-3)),
-            int(d.get("burn_max", 16)), int(d["prior_mode"]), float(d.get("splice_log10", 0.0)))
+            int(d.get("burn_max", 16)), int(d["prior_mode"]), float(d.get("splice_log10", 0.0)),
+            float(d.get("split_target", 16384.0)), int(d.get("split_max", 1024)))
 
     @property
     def ref(self):

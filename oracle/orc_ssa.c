@@ -135,8 +135,26 @@ typedef struct {
   double bound[ORC_MAX_BINS];  /* T * (1 + (j+1)/nb): thinning bound in hours */
   double bs;
   uint64_t thr_lam, thr_pi;
-  int B, step, scaling;
+  int B, step, scaling, gene_varies;
+  int M; /* sub-lanes per cell: each carries a 1/M share of the births */
 } prates;
+
+/* Split factor: the smallest power of two M with (expected molecule events
+ * per cell) <= split_target * M, capped at split_max.  Molecules never act on
+ * the gene, so M independent 1/M-thinned birth streams over one shared gene
+ * path have the law of the full birth stream. */
+static int split_factor(const orc_design* d, const prates* r) {
+  if (!(d->split_target > 0.0)) return 1;
+  const int nb = d->n_bins;
+  double births = 0.0;
+  for (int j = 0; j < nb; ++j) births = births + r->amax[j] * (r->kon[j] / (r->kon[j] + r->koff[j]));
+  double scale = d->capture_mode == 1 ? (double)d->beta_a / (double)(d->beta_a + d->beta_b)
+                                      : (d->capture_mode == 2 ? 0.25 : 1.0);
+  double est = (((births / (double)nb) * scale) * (d->cycle * ((double)r->B + 1.0))) * 3.0;
+  int M = 1;
+  while (M < d->split_max && est > d->split_target * (double)M) M *= 2;
+  return M;
+}
 
 static int burn_min(const orc_design* d) {
   double wmax = 0.0;
@@ -154,6 +172,7 @@ static void make_rates(const orc_design* d, int model, const double* th, prates*
   double h = d->cycle / (double)nb;
   r->scaling = model != 2;
   r->step = model == 2 ? nb : 1;
+  r->gene_varies = L.n_kon > 1;
   for (int j = 0; j < nb; ++j) {
     r->kon[j] = orc_exp10(th[L.o_kon + (L.n_kon > 1 ? j : 0)]);
     r->koff[j] = orc_exp10(th[L.o_koff]);
@@ -195,6 +214,7 @@ static void make_rates(const orc_design* d, int model, const double* th, prates*
   int B = bmin;
   while (S > d->burn_eps && B < d->burn_max) { S = S * s; B++; }
   r->B = B;
+  r->M = split_factor(d, r);
 }
 
 /* --------------------------------------------------------------- cells */
@@ -202,12 +222,17 @@ static inline uint32_t lemire(uint32_t w, uint32_t n) {
   return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
 }
 
+/* Stream s of a cell: counter (draw_lo, draw_hi[15:0] | s << 16, cell,
+ * particle).  Stream 0 holds the cell set-up draws and the gene clock and is
+ * replayed identically by every sub-lane; stream 1 + sub holds sub-lane sub's
+ * molecule events and division coin flips. */
 typedef struct {
-  uint32_t key[2]; uint32_t cell, pidx; uint64_t ctr;
+  uint32_t key[2]; uint32_t cell, pidx, stream; uint64_t ctr;
 } cell_stream;
 
 static void draw(cell_stream* s, uint32_t w[4]) {
-  uint32_t c[4] = {(uint32_t)s->ctr, (uint32_t)(s->ctr >> 32), s->cell, s->pidx};
+  uint32_t c[4] = {(uint32_t)s->ctr, ((uint32_t)(s->ctr >> 32) & 0xFFFFu) | (s->stream << 16),
+                   s->cell, s->pidx};
   orc_philox4x32_10(c, s->key, w);
   s->ctr++;
 }
@@ -216,14 +241,20 @@ static inline double grid_t(int64_t k, int64_t nbB, double T, int nb) {
   return ((double)(k - nbB) * T) / (double)nb;
 }
 
-static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
-                    uint32_t pidx, int c, uint32_t* u_out, uint32_t* l_out,
-                    uint8_t* bin_out, int64_t* steps, int64_t* reactions) {
+/* Exponential waiting time with rate a from one word. */
+static inline double wait(uint32_t w, double a) { return -orc_log(orc_u01_32(w)) / a; }
+
+/* One sub-lane of cell c: the cell's gene path (stream 0) with a 1/M share
+ * of its transcription (stream 1 + sub).  Returns its observed u and l. */
+static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], uint32_t pidx,
+                   int c, int sub, uint32_t* u_out, uint32_t* l_out, int* bin_out,
+                   int64_t* steps, int64_t* reactions) {
   const int nb = d->n_bins, J = d->n_conds;
   const double T = d->cycle;
-  cell_stream s = {{key[0], key[1]}, (uint32_t)c, pidx, 0};
-  uint32_t w[4];
-  draw(&s, w);
+  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c, pidx, 0u, 0};
+  cell_stream sm = {{key[0], key[1]}, (uint32_t)c, pidx, 1u + (uint32_t)sub, 0};
+  uint32_t w[4], v[4];
+  draw(&s0, w);
   int bin;
   double age;
   if (d->age_mode == 1) {
@@ -244,14 +275,13 @@ static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
     if (cnt <= 0) return -3;
     bc = d->betas[d->group_off[grp] + (int64_t)lemire(w[3], (uint32_t)cnt)];
   } else if (d->capture_mode == 1) {
+    /* Beta(a, b) = a-th smallest of a+b-1 uniforms */
     int a = d->beta_a, nu = d->beta_a + d->beta_b - 1;
     uint32_t lo[32];
     int have = 0;
-    uint32_t v[4];
     for (int q = 0; q < nu; ++q) {
-      if ((q & 3) == 0) draw(&s, v);
+      if ((q & 3) == 0) draw(&s0, v);
       uint32_t x = v[q & 3];
-      /* keep the a smallest words, sorted ascending */
       int pos = have < a ? have : a;
       while (pos > 0 && lo[pos - 1] > x) {
         if (pos < a) lo[pos] = lo[pos - 1];
@@ -265,96 +295,132 @@ static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
   const double pulse = d->pulse[cond], chase = d->chase[cond];
   const double ws = age - pulse - chase, we = age - chase;
   const int64_t nbB = (int64_t)r->B * nb;
+  const double inv_share = (double)r->M;
   int64_t k = 0, knext = r->step;
+  int j = 0;
   double t = grid_t(0, nbB, T, nb);
   int win_started = 0, in_win = 0;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
   int64_t st = 0, re = 0;
+  /* gene clock (first-reaction method on the shared stream) */
+  draw(&s0, v);
+  st++;
+  double tg = t + wait(v[0], g ? r->koff[0] : r->kon[0]);
   for (;;) {
     double tb = age;
     double gk = grid_t(knext, nbB, T, nb);
     if (gk < tb) tb = gk;
     if (!win_started && ws < tb) tb = ws;
     if (in_win && we < tb) tb = we;
-    int j = (int)(k % nb);
-    draw(&s, w);
-    st++;
-    double a_on = g ? 0.0 : r->kon[j];
-    double a_off = g ? r->koff[j] : 0.0;
-    double a_tx = g ? r->amax[j] * bc : 0.0;
+    /* molecule channels of this sub-lane (direct method) */
+    double a_tx = g ? (r->amax[j] * bc) / inv_share : 0.0;
     uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
     double a_spl = r->bs * (double)nun;
     double a_dec = r->gam[j] * (double)ntot;
-    double c1 = a_on, c2 = c1 + a_off, c3 = c2 + a_tx, c4 = c3 + a_spl, a0 = c4 + a_dec;
-    double tau = -orc_log(orc_u01_32(w[0])) / a0;
-    double tn = t + tau;
-    if (tn >= tb) {
-      t = tb;
-      if (tb == age) break;
-      if (tb == gk) {
-        k = knext;
-        knext += r->step;
-        if (k % nb == 0) {
-          uint32_t* sp[4] = {&n0, &n1, &n2, &n3};
-          for (int q = 0; q < 4; ++q) {
-            uint32_t cnt = *sp[q];
-            if (cnt == 0) continue;
-            uint32_t keep = 0, rem = cnt;
-            while (rem > 0) {
-              uint32_t v[4];
-              draw(&s, v);
-              st++;
-              uint32_t m = rem < 128u ? rem : 128u;
-              for (int b = 0; b < 4; ++b) {
-                int bits = (int)m - 32 * b;
-                if (bits <= 0) break;
-                uint32_t x = bits >= 32 ? v[b] : (v[b] & ((1u << bits) - 1u));
-                keep += (uint32_t)__builtin_popcount(x);
-              }
-              rem -= m;
-            }
-            *sp[q] = keep;
-          }
+    double c1 = a_tx, c2 = c1 + a_spl, a0 = c2 + a_dec;
+    double tm = INFINITY;
+    if (a0 > 0.0) {
+      draw(&sm, w);
+      st++;
+      tm = t + wait(w[0], a0);
+    }
+    if (tm < tg && tm < tb) {
+      t = tm;
+      double rr = orc_u01_32(w[1]) * a0;
+      if (rr < c1) {
+        int acc = 1;
+        if (r->scaling) {
+          /* accept with probability (T + a) / bound_j, a = time since cycle start */
+          double cs = grid_t(k - j, nbB, T, nb);
+          acc = orc_u01_32(w[2]) * r->bound[j] < T + (t - cs);
         }
+        if (acc) {
+          re++;
+          if (in_win && (uint64_t)w[3] < r->thr_lam) n2++; else n0++;
+        }
+      } else if (rr < c2) {
+        re++;
+        if (lemire(w[2], nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+      } else {
+        re++;
+        uint32_t x = lemire(w[2], ntot);
+        if (x < n0) n0--;
+        else if (x < n0 + n1) n1--;
+        else if (x < n0 + n1 + n2) n2--;
+        else n3--;
       }
-      if (!win_started && tb == ws) { win_started = 1; in_win = 1; }
-      if (in_win && tb == we) in_win = 0;
       continue;
     }
-    t = tn;
-    double rr = orc_u01_32(w[1]) * a0;
-    if (rr < c1) {
-      g = 1; re++;
-    } else if (rr < c2) {
-      g = 0; re++;
-    } else if (rr < c3) {
-      int acc = 1;
-      if (r->scaling) {
-        /* accept with probability (T + a) / bound_j, a = time since cycle start */
-        double cs = grid_t(k - j, nbB, T, nb);
-        acc = orc_u01_32(w[2]) * r->bound[j] < T + (t - cs);
-      }
-      if (acc) {
-        re++;
-        if (in_win && (uint64_t)w[3] < r->thr_lam) n2++; else n0++;
-      }
-    } else if (rr < c4) {
+    if (tg < tb) {
+      t = tg;
+      g = !g;
       re++;
-      if (lemire(w[2], nun) < n0) { n0--; n1++; } else { n2--; n3++; }
-    } else {
-      re++;
-      uint32_t x = lemire(w[2], ntot);
-      if (x < n0) n0--;
-      else if (x < n0 + n1) n1--;
-      else if (x < n0 + n1 + n2) n2--;
-      else n3--;
+      draw(&s0, v);
+      st++;
+      tg = t + wait(v[0], g ? r->koff[j] : r->kon[j]);
+      continue;
     }
+    /* boundary: sampling age, rate-bin / cycle edge, or labelling window edge */
+    t = tb;
+    if (tb == age) break;
+    if (tb == gk) {
+      k = knext;
+      knext += r->step;
+      j = (int)(k % nb);
+      if (j == 0) {
+        uint32_t* sp[4] = {&n0, &n1, &n2, &n3};
+        for (int q = 0; q < 4; ++q) {
+          uint32_t cnt = *sp[q];
+          if (cnt == 0) continue;
+          uint32_t keep = 0, rem = cnt;
+          while (rem > 0) {
+            draw(&sm, v);
+            st++;
+            uint32_t m = rem < 128u ? rem : 128u;
+            for (int b = 0; b < 4; ++b) {
+              int bits = (int)m - 32 * b;
+              if (bits <= 0) break;
+              uint32_t x = bits >= 32 ? v[b] : (v[b] & ((1u << bits) - 1u));
+              keep += (uint32_t)__builtin_popcount(x);
+            }
+            rem -= m;
+          }
+          *sp[q] = keep;
+        }
+      }
+      if (r->gene_varies) {
+        /* the gene's rate changed: restart its clock (memoryless) */
+        draw(&s0, v);
+        st++;
+        tg = t + wait(v[0], g ? r->koff[j] : r->kon[j]);
+      }
+    }
+    if (!win_started && tb == ws) { win_started = 1; in_win = 1; }
+    if (in_win && tb == we) in_win = 0;
   }
   *u_out = n0 + n1;
   *l_out = n2 + n3;
-  *bin_out = (uint8_t)bin;
+  *bin_out = bin;
   *steps += st;
   *reactions += re;
+  return 0;
+}
+
+static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
+                    uint32_t pidx, int c, uint32_t* u_out, uint32_t* l_out,
+                    uint8_t* bin_out, int64_t* steps, int64_t* reactions) {
+  uint32_t u = 0, l = 0;
+  int bin = 0;
+  for (int sub = 0; sub < r->M; ++sub) {
+    uint32_t su, sl;
+    int e = sim_sub(d, r, key, pidx, c, sub, &su, &sl, &bin, steps, reactions);
+    if (e) return e;
+    u += su;
+    l += sl;
+  }
+  *u_out = u;
+  *l_out = l;
+  *bin_out = (uint8_t)bin;
   return 0;
 }
 
@@ -499,42 +565,58 @@ int orc_simulate(const orc_design* d, int model, uint64_t seed, int64_t first, i
   uint32_t key[2];
   key_of(seed, model, key);
   int err = 0;
+  prates* R = (prates*)malloc(sizeof(prates) * (n > 0 ? n : 1));
+  uint32_t* U = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1) * C);
+  uint32_t* Lc = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1) * C);
+  uint8_t* BN = (uint8_t*)malloc((size_t)(n > 0 ? n : 1) * C);
+  int64_t* ST = (int64_t*)calloc((size_t)(n > 0 ? n : 1) * C, sizeof(int64_t));
+  int64_t* RE = (int64_t*)calloc((size_t)(n > 0 ? n : 1) * C, sizeof(int64_t));
 #ifdef _OPENMP
   if (n_threads > 0) omp_set_num_threads(n_threads);
-#pragma omp parallel for schedule(dynamic, 1)
 #endif
   for (int64_t i = 0; i < n; ++i) {
     double th[2 * ORC_MAX_BINS + 8];
     for (int c = 0; c < np; ++c) th[c] = theta[i + n * c];
-    prates r;
-    make_rates(d, model, th, &r);
-    uint32_t* u = (uint32_t*)malloc(sizeof(uint32_t) * C);
-    uint32_t* l = (uint32_t*)malloc(sizeof(uint32_t) * C);
-    uint8_t* bn = (uint8_t*)malloc(C);
-    int64_t st = 0, re = 0;
-    int e = 0;
-    for (int c = 0; c < C && !e; ++c)
-      e = sim_cell(d, &r, key, (uint32_t)(first + i), c, u + c, l + c, bn + c, &st, &re);
+    make_rates(d, model, th, &R[i]);
+  }
+  /* every (particle, cell) is independent: schedule them across threads */
+  const int64_t total = n * (int64_t)C;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4)
+#endif
+  for (int64_t q = 0; q < total; ++q) {
+    const int64_t i = q / C;
+    const int c = (int)(q - i * C);
+    int e = sim_cell(d, &R[i], key, (uint32_t)(first + i), c, U + q, Lc + q, BN + q, ST + q, RE + q);
     if (e) {
 #ifdef _OPENMP
 #pragma omp critical
 #endif
       err = e;
-    } else {
+    }
+  }
+  if (!err) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int64_t i = 0; i < n; ++i) {
       double sr[4 * ORC_MAX_BINS + 3 * ORC_MAX_CONDS];
       double mo[ORC_MAX_BINS * ORC_MAX_CONDS * 6];
       double ss[2 * ORC_MAX_BINS * ORC_MAX_CONDS];
-      orc_cell_stats(d, u, l, bn, sr, mo, ss);
+      const size_t o = (size_t)i * C;
+      orc_cell_stats(d, U + o, Lc + o, BN + o, sr, mo, ss);
       if (stats_ref) for (int k = 0; k < K; ++k) stats_ref[i + n * k] = sr[k];
       if (moments) for (int k = 0; k < M; ++k) moments[i + n * k] = mo[k];
       if (superset) for (int k = 0; k < KS; ++k) superset[i + n * k] = ss[k];
-      if (cell_u) memcpy(cell_u + i * C, u, sizeof(uint32_t) * C);
-      if (cell_l) memcpy(cell_l + i * C, l, sizeof(uint32_t) * C);
-      if (cell_bin) memcpy(cell_bin + i * C, bn, C);
+      if (cell_u) memcpy(cell_u + o, U + o, sizeof(uint32_t) * C);
+      if (cell_l) memcpy(cell_l + o, Lc + o, sizeof(uint32_t) * C);
+      if (cell_bin) memcpy(cell_bin + o, BN + o, C);
+      int64_t st = 0, re = 0;
+      for (int c = 0; c < C; ++c) { st += ST[o + c]; re += RE[o + c]; }
       if (steps) steps[i] = st;
       if (reactions) reactions[i] = re;
     }
-    free(u); free(l); free(bn);
   }
+  free(R); free(U); free(Lc); free(BN); free(ST); free(RE);
   return err;
 }

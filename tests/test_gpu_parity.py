@@ -24,10 +24,17 @@ def empirical_design(golden_capture, cells=110):
                             beta_is_chase=rng.integers(0, 2, len(betas)))
 
 
+def split_design(cells=120):
+    d = config_design(cells=cells)
+    d["split_target"], d["split_max"] = 200.0, 64  # force many sub-lanes per cell
+    return d
+
+
 DESIGNS = {
     "ref": lambda gc: reference_design(cells=110),
     "ref_empirical": lambda gc: empirical_design(gc),
     "config_beta": lambda gc: config_design(cells=180),
+    "config_split": lambda gc: split_design(),
 }
 
 

@@ -53,6 +53,25 @@ def test_ssa_matches_moments(oracle, golden_recovered, model, row):
     assert 0.6 < z.std() < 1.4
 
 
+@pytest.mark.parametrize("model,row", [(2, 3), (4, 2)])
+def test_split_cells_match_moments(oracle, golden_recovered, model, row):
+    """Cells split into many sub-lanes (shared gene path, 1/M of the births
+    each) keep the law of the unsplit cell."""
+    th_ref = golden_recovered[f"{MODEL_NAMES[model - 1]}__theta"][row]
+    theta = np.append(th_ref, 0.5)[None, :]
+    per = 160
+    d = reference_design(cells=55 * per)
+    d["split_target"], d["split_max"] = 20.0, 64
+    od = oracle.OracleDesign(d)
+    out = oracle.simulate(od, model, 9, 0, theta)
+    cfg = oracle.moment_cfg(AGES, REF_PULSE, REF_CHASE, 20.0, transient_mode=1)
+    mom = oracle.recover(model, th_ref, cfg)
+    z = z_scores(out, mom, per)
+    assert abs(z.mean()) < 0.35, z.mean()
+    assert np.abs(z).max() < 5.0
+    assert 0.6 < z.std() < 1.4
+
+
 def test_ssa_capture_matches_downsample(oracle, golden_recovered):
     """Beta(2,8) capture thinning reproduces downsample (model.jl:221-239)
     with the Beta moments E=0.2, E[b^2]=6/110 (population variance)."""
