@@ -79,9 +79,11 @@ def _declare_optional(L):
     """Entry points of the screening / surrogate units (declared when present)."""
     V, I32, I64, U64, D, F = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_float
     sig = {
-        "abc_accept_dev": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, I32, D, I64, I64, V, V, V,
-                                 I64, V]),
-        "abc_gemm_screen_dev": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V, V, V]),
+        "abc_accept_begin": (I32, [V, I32, I64, V, V, V, I32, D, I32, D, I64, D]),
+        "abc_accept_add": (I32, [V, I64, V, I64]),
+        "abc_accept_add_dev": (I32, [V, I64, V, I64]),
+        "abc_accept_end": (I32, [V, V, V, V, I64, V]),
+        "abc_screen_errors": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V, V]),
         "abc_mlp_create": (I32, [C.POINTER(V), I32, I32, I32, I32, I32, U64]),
         "abc_mlp_destroy": (I32, [V]),
         "abc_mlp_n_params": (I64, [V]),

@@ -24,6 +24,21 @@ struct DevBuf {
   size_t bytes = 0;
 };
 
+// State of an acceptance session (abc_accept_begin .. abc_accept_end).
+struct AcceptState {
+  bool active = false;
+  ErrSpec E{};
+  int Kp = 0;               // K padded to the MFMA k-chunk (32)
+  int64_t G = 0, Gpad = 0;  // genes, padded to the 64-gene tile
+  int mode = 0;
+  double eps = 0.0, margin_rel = 0.0;
+  int64_t k = 0;
+  int64_t n_candidates = 0, n_screened = 0;
+  unsigned long long cand_cap = 0;
+  std::vector<double> thr_host;                                // per-gene threshold
+  std::vector<std::vector<std::pair<double, int64_t>>> kept;   // per-gene (err, index)
+};
+
 }  // namespace
 
 struct abc_ctx {
@@ -38,7 +53,9 @@ struct abc_ctx {
   double* d_betas = nullptr;
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, items, off, scan_tmp, cu, cl,
-      cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount;
+      cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
+      pflag, bh, bl, cg, gflag, thr;
+  AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
   // ring of (start, end) event pairs around every ssa_kernel launch
@@ -352,7 +369,8 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->sort_tmp, &c->items, &c->off, &c->scan_tmp,
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
-                    &c->cand, &c->ccount};
+                    &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
+                    &c->cg, &c->gflag, &c->thr};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   if (c->dd) hipFree(c->dd);
@@ -525,76 +543,168 @@ int32_t abc_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_
   return ABC_OK;
 }
 
-int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_t G,
-                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
-                   double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
-                   int64_t* counts, int64_t* idx, double* err_out, int64_t cap) {
-  if (!c || n < 0 || G < 0 || (G > 0 && !counts) || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+}  // extern "C"
+
+// ------------------------------------------------------------ accept session
+namespace {
+
+struct GeneHit {
+  double err;
+  int64_t idx;  // global 0-based particle index
+  bool operator<(const GeneHit& o) const { return err < o.err || (err == o.err && idx < o.idx); }
+};
+
+int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index) {
+  AcceptState& A = c->acc;
+  int rc;
+  const int64_t npad = pad_to(n, 64);
+  if ((rc = ensure(c, c->ah, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->al, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->pflag, npad))) return rc;
+  if ((rc = ensure(c, c->ccount, 64))) return rc;
+  hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
+                     stats_dev, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
+                     P<uint8_t>(c->pflag));
+  HIPCHK(c, hipGetLastError());
+  unsigned long long cap = std::max<unsigned long long>(A.cand_cap, 1ull << 16);
+  for (;;) {
+    if ((rc = ensure(c, c->cand, sizeof(Candidate) * cap))) return rc;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                       P<unsigned long long>(c->ccount), (int64_t)1);
+    hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
+                       dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), n, A.Kp,
+                       P<__bf16>(c->bh), P<__bf16>(c->bl), A.G, P<double>(c->cg),
+                       P<uint8_t>(c->gflag), P<uint8_t>(c->pflag), P<double>(c->thr),
+                       A.margin_rel, A.E.clamp, P<Candidate>(c->cand),
+                       P<unsigned long long>(c->ccount), cap, (float*)nullptr, (float*)nullptr);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(rescore_kernel, dim3(grid1((int64_t)cap, 256)), dim3(256), 0, c->stream,
+                       stats_dev, n, P<double>(c->edata), P<double>(c->ese), A.G, A.E,
+                       P<Candidate>(c->cand), P<unsigned long long>(c->ccount), cap);
+    HIPCHK(c, hipGetLastError());
+    unsigned long long cnt = 0;
+    HIPCHK(c, hipMemcpyAsync(&cnt, c->ccount.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (cnt > cap) {  // grow and screen the batch again (the GEMM is cheap)
+      cap = cnt + cnt / 8;
+      A.cand_cap = cap;
+      continue;
+    }
+    A.n_candidates += (int64_t)cnt;
+    std::vector<Candidate> h(cnt);
+    if (cnt)
+      HIPCHK(c, hipMemcpy(h.data(), c->cand.p, sizeof(Candidate) * cnt, hipMemcpyDeviceToHost));
+    std::vector<char> touched(A.mode == 1 ? A.G : 0, 0);
+    for (const Candidate& x : h) {
+      if (!(x.err == x.err)) continue;  // NaN errors are never accepted
+      if (A.mode == 0) {
+        if (x.err <= A.eps) A.kept[x.gene].push_back({x.err, first_index + x.particle});
+      } else {
+        A.kept[x.gene].push_back({x.err, first_index + x.particle});
+        touched[x.gene] = 1;
+      }
+    }
+    if (A.mode == 1) {
+      bool changed = false;
+      for (int64_t g = 0; g < A.G; ++g) {
+        if (!touched[g]) continue;
+        auto& v = A.kept[g];
+        if ((int64_t)v.size() > A.k) {
+          std::nth_element(v.begin(), v.begin() + A.k, v.end(),
+                           [](const auto& a, const auto& b) { return GeneHit{a.first, a.second} < GeneHit{b.first, b.second}; });
+          v.resize(A.k);
+        }
+        if ((int64_t)v.size() == A.k && A.k > 0) {
+          double t = v[0].first;
+          for (auto& e : v) t = std::max(t, e.first);
+          if (t != A.thr_host[g]) { A.thr_host[g] = t; changed = true; }
+        }
+      }
+      if (changed)
+        HIPCHK(c, hipMemcpyAsync(c->thr.p, A.thr_host.data(), sizeof(double) * A.G,
+                                 hipMemcpyHostToDevice, c->stream));
+    }
+    break;
+  }
+  A.n_screened += n;
+  return ABC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, const double* se,
+                         const int32_t* parts, int32_t n_parts, double clamp, int32_t mode,
+                         double eps, int64_t k, double margin_rel) {
+  if (!c || G < 0 || (G > 0 && (!data || !se))) return ABC_E_ARG;
   if (mode != 0 && mode != 1) return fail(c, ABC_E_ARG, "mode must be 0 (fixed eps) or 1 (top-k)");
   if (mode == 1 && k < 0) return fail(c, ABC_E_ARG, "k must be >= 0");
-  ErrSpec E;
-  int rc = make_spec(c, K, parts, n_parts, clamp, &E);
+  if (!(margin_rel >= 0.0)) return fail(c, ABC_E_ARG, "margin_rel must be >= 0");
+  AcceptState& A = c->acc;
+  int rc = make_spec(c, K, parts, n_parts, clamp, &A.E);
   if (rc) return rc;
-  for (int64_t g = 0; g < G; ++g) counts[g] = 0;
-  if (n == 0 || G == 0) return ABC_OK;
-  if ((rc = ensure(c, c->stats, sizeof(double) * n * K))) return rc;
+  if (K > 128) return fail(c, ABC_E_ARG, "screening supports K <= 128");
+  A.G = G;
+  A.Kp = (int)pad_to(K, 32);
+  A.Gpad = pad_to(std::max<int64_t>(G, 1), 64);
+  A.mode = mode;
+  A.eps = eps;
+  A.k = k;
+  A.margin_rel = margin_rel;
+  A.n_candidates = 0;
+  A.n_screened = 0;
+  A.kept.assign(G, {});
+  A.thr_host.assign(std::max<int64_t>(G, 1), mode == 0 ? eps : HUGE_VAL);
+  A.active = true;
+  if (G == 0) return ABC_OK;
   if ((rc = ensure(c, c->edata, sizeof(double) * G * K))) return rc;
   if ((rc = ensure(c, c->ese, sizeof(double) * G * K))) return rc;
-  HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
+  if ((rc = ensure(c, c->bh, sizeof(uint16_t) * A.Gpad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->bl, sizeof(uint16_t) * A.Gpad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->cg, sizeof(double) * G))) return rc;
+  if ((rc = ensure(c, c->gflag, G))) return rc;
+  if ((rc = ensure(c, c->thr, sizeof(double) * G))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->edata.p, data, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->ese.p, se, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
-  std::vector<std::vector<std::pair<double, int64_t>>> per(G);
-  if (mode == 0) {
-    unsigned long long ccap = 1ull << 22;
-    for (;;) {
-      if ((rc = ensure(c, c->cand, sizeof(Candidate) * ccap))) return rc;
-      if ((rc = ensure(c, c->ccount, 64))) return rc;
-      hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
-                         P<unsigned long long>(c->ccount), (int64_t)1);
-      for (int64_t g0 = 0; g0 < G; g0 += 65535) {
-        const int64_t gn = std::min<int64_t>(65535, G - g0);
-        hipLaunchKernelGGL(accept_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0,
-                           c->stream, P<double>(c->stats), n, P<double>(c->edata),
-                           P<double>(c->ese), G, g0, E, eps, first_index, P<Candidate>(c->cand),
-                           P<unsigned long long>(c->ccount), ccap);
-        HIPCHK(c, hipGetLastError());
-      }
-      unsigned long long cnt = 0;
-      HIPCHK(c, hipMemcpyAsync(&cnt, c->ccount.p, 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      if (cnt > ccap) { ccap = cnt; continue; }
-      std::vector<Candidate> h(cnt);
-      if (cnt) HIPCHK(c, hipMemcpy(h.data(), c->cand.p, sizeof(Candidate) * cnt, hipMemcpyDeviceToHost));
-      for (const Candidate& x : h) per[x.gene].emplace_back(x.err, x.particle);
-      break;
-    }
-  } else {
-    // exact top-k from FP64 errors, genes in slices that fit the error buffer
-    const int64_t gslice = std::max<int64_t>(1, std::min<int64_t>(G, (int64_t)((1ull << 28) / (uint64_t)std::max<int64_t>(n, 1))));
-    std::vector<double> herr;
-    for (int64_t g0 = 0; g0 < G; g0 += gslice) {
-      const int64_t gn = std::min<int64_t>(gslice, G - g0);
-      if ((rc = ensure(c, c->err, sizeof(double) * n * gn))) return rc;
-      hipLaunchKernelGGL(err_fp64_kernel, dim3(grid1(n, 256), (unsigned)gn), dim3(256), 0, c->stream,
-                         P<double>(c->stats), n, P<double>(c->edata), P<double>(c->ese), G, g0, E,
-                         P<double>(c->err));
-      HIPCHK(c, hipGetLastError());
-      herr.resize((size_t)n * gn);
-      HIPCHK(c, hipMemcpyAsync(herr.data(), c->err.p, sizeof(double) * n * gn, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      for (int64_t gg = 0; gg < gn; ++gg) {
-        auto& v = per[g0 + gg];
-        v.resize(n);
-        for (int64_t i = 0; i < n; ++i) v[i] = {herr[i + n * gg], first_index + i};
-        const int64_t kk = std::min<int64_t>(k, n);
-        std::partial_sort(v.begin(), v.begin() + kk, v.end());
-        v.resize(kk);
-      }
-    }
-  }
+  HIPCHK(c, hipMemcpyAsync(c->thr.p, A.thr_host.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(gene_prep_kernel, dim3(grid1(A.Gpad, 256)), dim3(256), 0, c->stream,
+                     P<double>(c->edata), P<double>(c->ese), G, K, A.Kp, A.Gpad,
+                     P<__bf16>(c->bh), P<__bf16>(c->bl), P<double>(c->cg), P<uint8_t>(c->gflag));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_accept_add_dev(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index) {
+  if (!c || n < 0 || (n > 0 && !stats_dev) || first_index < 0) return ABC_E_ARG;
+  if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  if (n == 0 || c->acc.G == 0) return ABC_OK;
+  return screen_batch(c, n, stats_dev, first_index);
+}
+
+int32_t abc_accept_add(abc_ctx* c, int64_t n, const double* stats, int64_t first_index) {
+  if (!c || n < 0 || (n > 0 && !stats) || first_index < 0) return ABC_E_ARG;
+  if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  if (n == 0 || c->acc.G == 0) return ABC_OK;
+  int rc;
+  if ((rc = ensure(c, c->stats, sizeof(double) * n * c->acc.E.K))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * c->acc.E.K,
+                           hipMemcpyHostToDevice, c->stream));
+  return screen_batch(c, n, P<double>(c->stats), first_index);
+}
+
+int32_t abc_accept_end(abc_ctx* c, int64_t* counts, int64_t* idx, double* err_out, int64_t cap,
+                       int64_t* n_candidates) {
+  if (!c || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+  AcceptState& A = c->acc;
+  if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  if (A.G > 0 && !counts) return ABC_E_ARG;
   int64_t total = 0;
-  for (int64_t g = 0; g < G; ++g) {
-    auto& v = per[g];
+  for (int64_t g = 0; g < A.G; ++g) {
+    auto& v = A.kept[g];
     std::sort(v.begin(), v.end());
     counts[g] = (int64_t)v.size();
     for (auto& pr : v) {
@@ -605,7 +715,55 @@ int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_
       total++;
     }
   }
+  if (n_candidates) *n_candidates = A.n_candidates;
   if (total > cap) return fail(c, ABC_E_CAPACITY, "accepted entries exceed cap");
+  A.active = false;
+  A.kept.clear();
+  return ABC_OK;
+}
+
+int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_t G,
+                   const double* data, const double* se, const int32_t* parts, int32_t n_parts,
+                   double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
+                   int64_t* counts, int64_t* idx, double* err_out, int64_t cap) {
+  if (!c || n < 0 || G < 0 || (G > 0 && !counts) || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+  int rc = abc_accept_begin(c, K, G, data, se, parts, n_parts, clamp, mode, eps, k, 0.02);
+  if (rc) return rc;
+  if ((rc = abc_accept_add(c, n, stats, first_index))) { c->acc.active = false; return rc; }
+  rc = abc_accept_end(c, counts, idx, err_out, cap, nullptr);
+  c->acc.active = false;
+  return rc;
+}
+
+int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_t G,
+                          const double* data, const double* se, const int32_t* parts,
+                          int32_t n_parts, double clamp, float* approx, float* bound) {
+  if (!c || n <= 0 || G <= 0 || !stats || !approx || !bound) return ABC_E_ARG;
+  int rc = abc_accept_begin(c, K, G, data, se, parts, n_parts, clamp, 0, 0.0, 0, 0.0);
+  if (rc) return rc;
+  AcceptState& A = c->acc;
+  const int64_t npad = pad_to(n, 64);
+  if ((rc = ensure(c, c->stats, sizeof(double) * n * K))) return rc;
+  if ((rc = ensure(c, c->ah, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->al, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+  if ((rc = ensure(c, c->pflag, npad))) return rc;
+  if ((rc = ensure(c, c->err, sizeof(float) * 2 * n * G))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
+                     P<double>(c->stats), n, K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
+                     P<uint8_t>(c->pflag));
+  float* ap = P<float>(c->err);
+  float* bp = ap + n * G;
+  hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
+                     dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), n, A.Kp,
+                     P<__bf16>(c->bh), P<__bf16>(c->bl), G, P<double>(c->cg), P<uint8_t>(c->gflag),
+                     P<uint8_t>(c->pflag), P<double>(c->thr), 0.0, clamp, (Candidate*)nullptr,
+                     (unsigned long long*)nullptr, 0ull, ap, bp);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(approx, ap, sizeof(float) * n * G, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(bound, bp, sizeof(float) * n * G, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  A.active = false;
   return ABC_OK;
 }
 

@@ -80,3 +80,18 @@ __global__ void accept_fp64_kernel(const double* stats, int64_t n, const double*
                                    double eps, int64_t first, Candidate* cand, unsigned long long* count,
                                    unsigned long long cap);
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n);
+
+// screening (abc_gemm.hip)
+__global__ void gene_prep_kernel(const double* data, const double* se, int64_t G, int K, int Kp,
+                                 int64_t Gpad, __bf16* Bh, __bf16* Bl, double* cg, uint8_t* gflag);
+__global__ void particle_prep_kernel(const double* stats, int64_t n, int K, int Kp, int64_t npad,
+                                     __bf16* Ah, __bf16* Al, uint8_t* pflag);
+__global__ void gemm_screen_kernel(const __bf16* Ah, const __bf16* Al, int64_t n, int Kp,
+                                   const __bf16* Bh, const __bf16* Bl, int64_t G, const double* cg,
+                                   const uint8_t* gflag, const uint8_t* pflag, const double* thr,
+                                   double margin_rel, double clamp, Candidate* out,
+                                   unsigned long long* count, unsigned long long cap,
+                                   float* approx_out, float* bound_out);
+__global__ void rescore_kernel(const double* stats, int64_t n, const double* data,
+                               const double* se, int64_t G, ErrSpec E, Candidate* cand,
+                               const unsigned long long* count, unsigned long long cap);

@@ -152,6 +152,48 @@ class Engine:
             o += counts[g]
         return out
 
+    def screen_errors(self, stats, data, se, parts=None, clamp: float = 10.0):
+        """Approximate errors of the split-BF16 GEMM and their proven bounds."""
+        stats = np.asfortranarray(stats, np.float64)
+        data = np.asfortranarray(data, np.float64)
+        se = np.asfortranarray(se, np.float64)
+        n, K = stats.shape
+        G = data.shape[0]
+        parts = np.ascontiguousarray(self.parts if parts is None else parts, np.int32)
+        approx = np.zeros((n, G), np.float32, order="F")
+        bound = np.zeros((n, G), np.float32, order="F")
+        rc = self.L.abc_screen_errors(self.ctx, n, K, _p(stats), G, _p(data), _p(se), _p(parts),
+                                      len(parts), clamp, _p(approx), _p(bound))
+        self._chk(rc, "screen_errors")
+        return approx, bound
+
+    def accept_stream(self, batches, data, se, parts=None, clamp: float = 10.0, mode: int = 0,
+                      eps: float = 4.8, k: int = 1000, margin_rel: float = 0.02):
+        """Streaming acceptance over an iterable of (stats, first_index) batches.
+        Returns (per-gene lists, number of FP64 rescorings)."""
+        data = np.asfortranarray(data, np.float64)
+        se = np.asfortranarray(se, np.float64)
+        G, K = data.shape
+        parts = np.ascontiguousarray(self.parts if parts is None else parts, np.int32)
+        self._chk(self.L.abc_accept_begin(self.ctx, K, G, _p(data), _p(se), _p(parts), len(parts),
+                                          clamp, mode, eps, k, margin_rel), "accept_begin")
+        for stats, first in batches:
+            stats = np.asfortranarray(stats, np.float64)
+            self._chk(self.L.abc_accept_add(self.ctx, stats.shape[0], _p(stats), first),
+                      "accept_add")
+        cap = G * max(k, 1) + 16 if mode == 1 else 1 << 20
+        counts = np.zeros(G, np.int64)
+        idx = np.zeros(cap, np.int64)
+        eo = np.zeros(cap)
+        ncand = C.c_int64()
+        self._chk(self.L.abc_accept_end(self.ctx, _p(counts), _p(idx), _p(eo), cap,
+                                        C.byref(ncand)), "accept_end")
+        out, o = [], 0
+        for g in range(G):
+            out.append((idx[o:o + counts[g]].copy(), eo[o:o + counts[g]].copy()))
+            o += counts[g]
+        return out, ncand.value
+
     def last_timing(self):
         ms = np.zeros(4)
         self._chk(self.L.abc_last_timing(self.ctx, _p(ms)), "last_timing")

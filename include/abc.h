@@ -120,6 +120,29 @@ int32_t abc_accept(abc_ctx* ctx, int64_t n, int32_t K, const double* stats, int6
                    double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
                    int64_t* counts, int64_t* idx, double* err_out, int64_t cap);
 
+/* Streaming acceptance over many batches (the loop of accepted_particles.jl
+ * run as particles are simulated, so the particle x gene matrix is never
+ * stored).  begin uploads the genes' data/se (G x K host arrays) and sets the
+ * rule; each add screens one batch (stats n x K; device or host pointer) with
+ * the split-BF16 GEMM, rescores its candidates in FP64 and merges them;
+ * end writes the per-gene lists like abc_accept and the number of FP64
+ * rescorings (optional).  margin_rel widens the screening band by
+ * margin_rel * |threshold| on top of the proven bound (0.02 in abc_accept). */
+int32_t abc_accept_begin(abc_ctx* ctx, int32_t K, int64_t G, const double* data,
+                         const double* se, const int32_t* parts, int32_t n_parts, double clamp,
+                         int32_t mode, double eps, int64_t k, double margin_rel);
+int32_t abc_accept_add(abc_ctx* ctx, int64_t n, const double* stats, int64_t first_index);
+int32_t abc_accept_add_dev(abc_ctx* ctx, int64_t n, const double* stats_dev,
+                           int64_t first_index);
+int32_t abc_accept_end(abc_ctx* ctx, int64_t* counts, int64_t* idx, double* err_out,
+                       int64_t cap, int64_t* n_candidates);
+
+/* The screening GEMM alone: approximate error and its proven bound for every
+ * (particle, gene) pair (n x G col-major, float).  For validation. */
+int32_t abc_screen_errors(abc_ctx* ctx, int64_t n, int32_t K, const double* stats, int64_t G,
+                          const double* data, const double* se, const int32_t* parts,
+                          int32_t n_parts, double clamp, float* approx, float* bound);
+
 /* ---- device-resident variants (benchmarks, pipelines that keep data in HBM) */
 int32_t abc_set_stream(abc_ctx* ctx, void* hip_stream);   /* NULL = own stream */
 int32_t abc_synchronize(abc_ctx* ctx);

@@ -146,3 +146,50 @@ def test_accept_matches_fp64_oracle(oracle, engine_factory, mode):
         assert same_bits(got[g][1], e)
         total += len(idx)
     assert total > 0
+
+
+def test_screen_bound_holds(oracle, engine_factory):
+    """|approx - exact| <= bound for every pair, over stats and data spanning
+    many decades (also validates the MFMA lane layout)."""
+    d = config_design(cells=120)
+    eng = engine_factory(d)
+    rng = np.random.default_rng(7)
+    n, G, K = 300, 150, 47
+    parts = [5, 5, 5, 5, 9, 9, 9]
+    scale = 10.0 ** rng.uniform(-3, 6, K)
+    data = rng.uniform(0.2, 2.0, (G, K)) * scale
+    se = np.abs(data) * rng.uniform(0.05, 0.3, (G, K))
+    stats = data[rng.integers(0, G, n)] * rng.uniform(0.5, 1.5, (n, K))
+    stats[:7] = data[:7]                       # exact hits: err = 0
+    approx, bound = eng.screen_errors(stats, data, se, parts, clamp=1e300)
+    exact = oracle.errors(stats, data, se, parts, clamp=1e300)
+    gap = np.abs(approx.astype(np.float64) - exact)
+    assert np.all(gap <= bound.astype(np.float64) * (1 + 1e-6) + 1e-6 * np.abs(exact)), \
+        float(np.max(gap - bound))
+    assert np.median(bound / np.maximum(exact, 1e-9)) < 0.1    # the bound is not vacuous
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_accept_stream_matches_fp64(oracle, engine_factory, mode):
+    """Batches screened one after another give the FP64 sets of the whole run;
+    the GEMM screen keeps the FP64 rescoring to a fraction of all pairs."""
+    d = reference_design(cells=110)
+    eng = engine_factory(d)
+    th = eng.sample_prior(2, 900, seed=12)
+    stats = eng.simulate(2, th, seed=12)["stats"]
+    rng = np.random.default_rng(4)
+    data, se = _data_from(eng, stats, rng, 31)
+    err = oracle.errors(stats, data, se, eng.parts)
+    eps, k = 1.5, 25
+    batches = [(stats[a:a + 300], 7000 + a) for a in range(0, 900, 300)]
+    got, ncand = eng.accept_stream(batches, data, se, mode=mode, eps=eps, k=k)
+    for g in range(data.shape[0]):
+        if mode == 0:
+            idx, e = oracle.accept_gene(err[:, g], eps)
+        else:
+            order = np.lexsort((np.arange(len(err)), err[:, g]))[:k]
+            idx, e = order + 1, err[order, g]
+        assert np.array_equal(got[g][0], idx + 7000), g
+        assert same_bits(got[g][1], e)
+    if mode == 0:  # (top-k here sits at the clamp value 10, where every pair is a candidate)
+        assert ncand < 0.5 * stats.shape[0] * data.shape[0]
