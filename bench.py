@@ -40,11 +40,28 @@ def parse():
     ap.add_argument("--cells", type=int, default=2500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three device copies of 256 MiB (profiler byte-counter scale)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+    n = 256 * 1024 * 1024 // 4
+    src = torch.ones(n, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 4 * n, "copy_bytes_written_per_launch": 4 * n}))
 
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
