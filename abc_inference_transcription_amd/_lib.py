@@ -84,15 +84,21 @@ def _declare_optional(L):
         "abc_accept_add_dev": (I32, [V, I64, V, I64]),
         "abc_accept_end": (I32, [V, V, V, V, I64, V]),
         "abc_screen_errors": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V, V]),
-        "abc_mlp_create": (I32, [C.POINTER(V), I32, I32, I32, I32, I32, U64]),
+        "abc_mlp_create": (I32, [C.POINTER(V), I32, I32, I32, I32, I32, I32, U64]),
         "abc_mlp_destroy": (I32, [V]),
+        "abc_mlp_last_error": (C.c_char_p, [V]),
         "abc_mlp_n_params": (I64, [V]),
-        "abc_mlp_set_normalizer": (I32, [V, V, V, V, V]),
-        "abc_mlp_forward": (I32, [V, I64, V, V]),
-        "abc_mlp_train_step": (I32, [V, I64, V, V, F, F, F, F, F, V]),
+        "abc_mlp_n_stats": (I32, [V]),
+        "abc_mlp_set_stream": (I32, [V, V]),
+        "abc_mlp_synchronize": (I32, [V]),
         "abc_mlp_get_params": (I32, [V, V]),
         "abc_mlp_set_params": (I32, [V, V]),
-        "abc_screen_dev": (I32, [V, I64, V, I64, V, V, V, I32, D, D, V, V]),
+        "abc_mlp_forward_dev": (I32, [V, I64, V, V]),
+        "abc_mlp_grad_dev": (I32, [V, I64, V, V, V, V]),
+        "abc_mlp_train_step_dev": (I32, [V, I64, V, V, F, F, F, F, F, V]),
+        "abc_features_dev": (I32, [I32, I64, V, I32, V, V]),
+        "abc_targets_dev": (I32, [I64, V, I32, V, V, V, V]),
+        "abc_screen_dev": (I32, [V, V, I64, V, V, V, D, D, V, V]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

@@ -54,7 +54,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, items, off, scan_tmp, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr;
+      pflag, bh, bl, cg, gflag, thr, wf, sy;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -370,7 +370,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr};
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   if (c->dd) hipFree(c->dd);
@@ -667,14 +667,46 @@ int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, c
   if ((rc = ensure(c, c->cg, sizeof(double) * G))) return rc;
   if ((rc = ensure(c, c->gflag, G))) return rc;
   if ((rc = ensure(c, c->thr, sizeof(double) * G))) return rc;
+  if ((rc = ensure(c, c->wf, sizeof(float) * G * K))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->edata.p, data, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->ese.p, se, sizeof(double) * G * K, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->thr.p, A.thr_host.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(gene_prep_kernel, dim3(grid1(A.Gpad, 256)), dim3(256), 0, c->stream,
                      P<double>(c->edata), P<double>(c->ese), G, K, A.Kp, A.Gpad,
-                     P<__bf16>(c->bh), P<__bf16>(c->bl), P<double>(c->cg), P<uint8_t>(c->gflag));
+                     P<__bf16>(c->bh), P<__bf16>(c->bl), P<double>(c->cg), P<uint8_t>(c->gflag),
+                     P<float>(c->wf));
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
+                       const float* mean_dev, const float* sd_dev, double z, double margin,
+                       uint8_t* pass_dev, int64_t* n_pass) {
+  if (!c || !mlp || n < 0 || (n > 0 && (!x_dev || !mean_dev || !sd_dev || !pass_dev)))
+    return ABC_E_ARG;
+  AcceptState& A = c->acc;
+  if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  if (abc_mlp_n_stats(mlp) != A.E.K) return fail(c, ABC_E_ARG, "surrogate and genes disagree on K");
+  if (n == 0) { if (n_pass) *n_pass = 0; return ABC_OK; }
+  int rc;
+  if ((rc = ensure(c, c->sy, sizeof(float) * n * 2 * A.E.K))) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  rc = abc_mlp_forward_dev(mlp, n, x_dev, P<float>(c->sy));
+  if (rc) return fail(c, rc, "surrogate forward failed");
+  if ((rc = abc_mlp_synchronize(mlp))) return fail(c, rc, "surrogate forward failed");
+  hipLaunchKernelGGL(screen_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c->stream,
+                     P<float>(c->sy), n, A.E.K, mean_dev, sd_dev, (float)z, P<float>(c->wf),
+                     P<double>(c->edata), A.G, P<double>(c->thr), (float)margin, pass_dev);
+  HIPCHK(c, hipGetLastError());
+  if (n_pass) {
+    std::vector<uint8_t> h(n);
+    HIPCHK(c, hipMemcpyAsync(h.data(), pass_dev, n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    int64_t s = 0;
+    for (uint8_t b : h) s += b;
+    *n_pass = s;
+  }
   return ABC_OK;
 }
 

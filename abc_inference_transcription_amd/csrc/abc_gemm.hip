@@ -41,7 +41,8 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(const double* __restrict
                                                         bf16* __restrict__ Bh,
                                                         bf16* __restrict__ Bl,
                                                         double* __restrict__ cg,
-                                                        uint8_t* __restrict__ gflag) {
+                                                        uint8_t* __restrict__ gflag,
+                                                        float* __restrict__ Wf) {
   const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= Gpad) return;
   bf16* bh = Bh + g * 2 * Kp;
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(const double* __restrict
       const double WD = W * d;
       c += WD * d;
       if (!(fabs(W) <= SCREEN_LIMIT) || !(fabs(WD) <= SCREEN_LIMIT)) bad = true;
+      Wf[g + G * k] = (float)W;
       split_bf16(-2.0 * WD, bh + k, bl + k);
       split_bf16(W, bh + Kp + k, bl + Kp + k);
     } else {

@@ -83,7 +83,16 @@ __global__ void zero_u64_kernel(unsigned long long* p, int64_t n);
 
 // screening (abc_gemm.hip)
 __global__ void gene_prep_kernel(const double* data, const double* se, int64_t G, int K, int Kp,
-                                 int64_t Gpad, __bf16* Bh, __bf16* Bl, double* cg, uint8_t* gflag);
+                                 int64_t Gpad, __bf16* Bh, __bf16* Bl, double* cg, uint8_t* gflag,
+                                 float* Wf);
+// surrogate screen (abc_mlp.hip)
+__global__ void screen_kernel(const float* y, int64_t n, int K, const float* mean, const float* sd,
+                              float z, const float* Wf, const double* data, int64_t G,
+                              const double* thr, float margin, uint8_t* pass);
+struct abc_mlp;
+extern "C" int32_t abc_mlp_forward_dev(abc_mlp* p, int64_t n, const float* x_dev, float* y_dev);
+extern "C" int32_t abc_mlp_synchronize(abc_mlp* p);
+extern "C" int32_t abc_mlp_n_stats(const abc_mlp* p);
 __global__ void particle_prep_kernel(const double* stats, int64_t n, int K, int Kp, int64_t npad,
                                      __bf16* Ah, __bf16* Al, uint8_t* pflag);
 __global__ void gemm_screen_kernel(const __bf16* Ah, const __bf16* Al, int64_t n, int Kp,

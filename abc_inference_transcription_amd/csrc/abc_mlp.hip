@@ -1,0 +1,638 @@
+// abc_mlp.hip -- the surrogate: an MLP from (model, log10 parameters) to the
+// mean and log-variance of every summary statistic, trained on the simulator's
+// own output, and the screen that lets the exact simulation run only on
+// particles that can still be accepted.  No reference counterpart (the
+// reference simulates every prior particle, scripts/abc_simulation.jl:88-97).
+//
+// Network: x (n_in, zero-padded to 64) -> [width, SiLU] x depth -> n_out
+// (padded to a multiple of 64).  Every product is a bf16 "NT" GEMM
+// C[m][n] = sum_k A[m][k] * B[n][k] on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation: forward Z = H W^T, backward dH = dZ W (W kept transposed as a
+// second bf16 copy), dW = dZ^T H (epilogues store the transposed activations
+// these need).  Master weights, gradients and Adam moments are FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/abc.h"
+#include "abc_kernels.h"
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2 };
+
+constexpr int BM = 128, BK = 32, LDSK = BK + 8;  // +8 bf16 pad against bank conflicts
+
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// 256 threads, workgroup tile 128 x BN, wave tile 64 x BN/2 (2 x 2 waves).
+template <int BN, int EPI>
+__global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A,
+                                                      const bf16* __restrict__ B, int M, int N,
+                                                      int K, const float* __restrict__ bias,
+                                                      float* __restrict__ Cf,
+                                                      bf16* __restrict__ Cb,
+                                                      bf16* __restrict__ CbT,
+                                                      const float* __restrict__ Zprev,
+                                                      float* __restrict__ Zout) {
+  constexpr int WN = BN / 2, NCB = WN / 16;
+  __shared__ bf16 As[BM][LDSK];
+  __shared__ bf16 Bs[BN][LDSK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  f32x4 acc[4][NCB];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < NCB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = tid + q * 256;
+      const int r = c >> 2, kk = (c & 3) * 8;
+      *reinterpret_cast<bf16x8*>(&As[r][kk]) =
+          *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + r) * K + k0 + kk);
+    }
+#pragma unroll
+    for (int q = 0; q < BN / 64; ++q) {
+      const int c = tid + q * 256;
+      const int r = c >> 2, kk = (c & 3) * 8;
+      *reinterpret_cast<bf16x8*>(&Bs[r][kk]) =
+          *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + r) * K + k0 + kk);
+    }
+    __syncthreads();
+    bf16x8 af[4], bfr[NCB];
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+      af[rb] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + rb * 16 + (lane & 15)][(lane >> 4) * 8]);
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+      bfr[cb] = *reinterpret_cast<const bf16x8*>(&Bs[wn * WN + cb * 16 + (lane & 15)][(lane >> 4) * 8]);
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < NCB; ++cb)
+        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bfr[cb], acc[rb][cb], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + wm * 64 + rb * 16 + (lane >> 4) * 4 + q;
+        const int n = n0 + wn * WN + cb * 16 + (lane & 15);
+        float v = acc[rb][cb][q];
+        const size_t o = (size_t)m * N + n;
+        if (EPI == EPI_F32) {
+          if (bias) v += bias[n];
+          Cf[o] = v;
+        } else if (EPI == EPI_SILU) {
+          const float z = v + bias[n];
+          Zout[o] = z;
+          const bf16 h = (bf16)(z * sigm(z));
+          Cb[o] = h;
+          CbT[(size_t)n * M + m] = h;
+        } else {
+          const float z = Zprev[o];
+          const float s = sigm(z);
+          const bf16 d = (bf16)(v * (s * (1.0f + z * (1.0f - s))));
+          Cb[o] = d;
+          CbT[(size_t)n * M + m] = d;
+        }
+      }
+}
+
+// x (B x n_in FP32, row-major) -> X (B x in_pad bf16) and X^T (in_pad x B).
+__global__ void cast_input_kernel(const float* __restrict__ x, int Bn, int Bpad, int n_in,
+                                  int in_pad, bf16* __restrict__ X, bf16* __restrict__ XT) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)Bpad * in_pad) return;
+  const int b = (int)(q / in_pad), k = (int)(q % in_pad);
+  const float v = (b < Bn && k < n_in) ? x[(size_t)b * n_in + k] : 0.0f;
+  X[q] = (bf16)v;
+  XT[(size_t)k * Bpad + b] = (bf16)v;
+}
+
+// Gaussian negative log-likelihood per element, mean over B*ns.
+// O row: [mu (ns) | logvar (ns) | pad]; dO and dO^T in bf16.
+__global__ void nll_kernel(const float* __restrict__ O, const float* __restrict__ T, int Bn,
+                           int Bpad, int ns, int out_pad, bf16* __restrict__ dO,
+                           bf16* __restrict__ dOT, float* __restrict__ loss) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (q < (int64_t)Bpad * out_pad) {
+    const int b = (int)(q / out_pad), k = (int)(q % out_pad);
+    float g = 0.0f;
+    if (b < Bn && k < 2 * ns) {
+      const int s = k < ns ? k : k - ns;
+      const float mu = O[(size_t)b * out_pad + s];
+      const float lv_raw = O[(size_t)b * out_pad + ns + s];
+      const bool clipped = lv_raw < -12.0f || lv_raw > 12.0f;
+      const float lv = fminf(fmaxf(lv_raw, -12.0f), 12.0f);
+      const float r = T[(size_t)b * ns + s] - mu;
+      const float inv = __expf(-lv);
+      const float scale = 1.0f / ((float)Bn * (float)ns);
+      if (k < ns) {
+        g = -r * inv * scale;
+        l = 0.5f * (lv + r * r * inv) * scale;
+      } else {
+        g = clipped ? 0.0f : 0.5f * (1.0f - r * r * inv) * scale;
+      }
+    }
+    dO[q] = (bf16)g;
+    dOT[(size_t)k * Bpad + b] = (bf16)g;
+  }
+  // block reduction of the loss
+  __shared__ float red[256];
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss, red[0]);
+}
+
+// bias gradient: row sums of dZ^T (rows x Bpad bf16), one workgroup per row
+__global__ void rowsum_kernel(const bf16* __restrict__ dZT, int Bpad, float* __restrict__ db) {
+  const int r = blockIdx.x;
+  float s = 0.0f;
+  for (int b = threadIdx.x; b < Bpad; b += blockDim.x) s += (float)dZT[(size_t)r * Bpad + b];
+  __shared__ float red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int q = 128; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[r] = red[0];
+}
+
+struct LayerDesc {
+  int in, out;      // padded dims
+  int64_t w_off;    // offset of W (out x in) in the flat parameter array
+  int64_t b_off;    // offset of b (out)
+};
+
+// Adam on the flat FP32 parameters; refreshes the bf16 copies W (out x in)
+// and W^T (in x out) of every weight.
+__global__ void adam_kernel(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, int64_t n, float lr, float b1, float b2,
+                            float eps, float wd, float c1, float c2, const LayerDesc* __restrict__ L,
+                            int nl, const int64_t* __restrict__ bf_off, bf16* __restrict__ Wb,
+                            bf16* __restrict__ WbT) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  float gg = g[p];
+  const float mm = b1 * m[p] + (1.0f - b1) * gg;
+  const float vv = b2 * v[p] + (1.0f - b2) * gg * gg;
+  m[p] = mm;
+  v[p] = vv;
+  float ww = w[p];
+  ww -= lr * ((mm / c1) / (sqrtf(vv / c2) + eps) + wd * ww);
+  w[p] = ww;
+  for (int l = 0; l < nl; ++l) {
+    const int64_t sz = (int64_t)L[l].in * L[l].out;
+    if (p >= L[l].w_off && p < L[l].w_off + sz) {
+      const int64_t e = p - L[l].w_off;
+      const int o = (int)(e / L[l].in), i = (int)(e % L[l].in);
+      Wb[bf_off[l] + e] = (bf16)ww;
+      WbT[bf_off[l] + (int64_t)i * L[l].out + o] = (bf16)ww;
+    }
+  }
+}
+
+__global__ void refresh_bf16_kernel(const float* __restrict__ w, const LayerDesc* __restrict__ L,
+                                    int nl, const int64_t* __restrict__ bf_off,
+                                    bf16* __restrict__ Wb, bf16* __restrict__ WbT, int64_t n) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  for (int l = 0; l < nl; ++l) {
+    const int64_t sz = (int64_t)L[l].in * L[l].out;
+    if (p >= L[l].w_off && p < L[l].w_off + sz) {
+      const int64_t e = p - L[l].w_off;
+      const int o = (int)(e / L[l].in), i = (int)(e % L[l].in);
+      Wb[bf_off[l] + e] = (bf16)w[p];
+      WbT[bf_off[l] + (int64_t)i * L[l].out + o] = (bf16)w[p];
+    }
+  }
+}
+
+__global__ void zero_f32_kernel(float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
+// copy the (mu, logvar) block of O (Bpad x out_pad) into y (B x 2 ns)
+__global__ void gather_out_kernel(const float* __restrict__ O, int Bn, int ns, int out_pad,
+                                  float* __restrict__ y) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)Bn * 2 * ns) return;
+  const int b = (int)(q / (2 * ns)), k = (int)(q % (2 * ns));
+  y[q] = O[(size_t)b * out_pad + k];
+}
+
+int g1(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- surrogate I/O
+// Canonical features of a particle: one-hot(model) and its rates per bin,
+// [kon(nb), koff, alpha(nb), gamma(nb), lambda, splice] in log10 / 3, with
+// constant rates repeated over the bins (layout of scripts/model.jl:30-43).
+__global__ void features_kernel(int model, int64_t n, const double* __restrict__ theta, int nb,
+                                float* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nk = model == 3 ? nb : 1, na = model == 4 ? nb : 1, ng = model == 5 ? nb : 1;
+  const int ok = 0, ooff = nk, oa = nk + 1, og = oa + na, ol = og + ng, os = ol + 1;
+  const int nin = 5 + 3 * nb + 3;
+  float* r = x + i * nin;
+  for (int q = 0; q < 5; ++q) r[q] = (q == model - 1) ? 1.0f : 0.0f;
+  int c = 5;
+  for (int j = 0; j < nb; ++j) r[c++] = (float)(theta[i + n * (ok + (nk > 1 ? j : 0))] / 3.0);
+  r[c++] = (float)(theta[i + n * ooff] / 3.0);
+  for (int j = 0; j < nb; ++j) r[c++] = (float)(theta[i + n * (oa + (na > 1 ? j : 0))] / 3.0);
+  for (int j = 0; j < nb; ++j) r[c++] = (float)(theta[i + n * (og + (ng > 1 ? j : 0))] / 3.0);
+  r[c++] = (float)(theta[i + n * ol] / 3.0);
+  r[c++] = (float)(theta[i + n * os] / 3.0);
+}
+
+// Targets: standardised asinh of every statistic (stats n x K column-major).
+__global__ void targets_kernel(int64_t n, const double* __restrict__ stats, int K,
+                               const float* __restrict__ mean, const float* __restrict__ sd,
+                               float* __restrict__ t) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * K) return;
+  const int64_t i = q / K;
+  const int k = (int)(q % K);
+  const double s = stats[i + n * k];
+  float v = 0.0f;
+  if (s == s && fabs(s) < 1e300) v = (float)((asinh(s) - (double)mean[k]) / (double)sd[k]);
+  t[q] = v;
+}
+
+// Screen: a particle passes when, for some gene, the error of the statistics
+// at the edges of the surrogate's z-sigma intervals can reach the gene's
+// threshold: LB_g = sum_k W_gk dist(d_gk, [lo_k, hi_k])^2 <= thr_g (1 + margin).
+// 256 threads x 16 particles per workgroup; genes strided over the threads.
+__global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y, int64_t n, int K,
+                                                     const float* __restrict__ mean,
+                                                     const float* __restrict__ sd, float z,
+                                                     const float* __restrict__ Wf,
+                                                     const double* __restrict__ data, int64_t G,
+                                                     const double* __restrict__ thr, float margin,
+                                                     uint8_t* __restrict__ pass) {
+  constexpr int PT = 16;
+  __shared__ float lo[PT][ABC_MAX_STATS / 2], hi[PT][ABC_MAX_STATS / 2];
+  __shared__ int hit[PT];
+  const int64_t i0 = (int64_t)blockIdx.x * PT;
+  for (int q = threadIdx.x; q < PT * K; q += blockDim.x) {
+    const int p = q / K, k = q % K;
+    const int64_t i = i0 + p;
+    float a = 0.f, b = 0.f;
+    if (i < n) {
+      const float mu = y[i * 2 * K + k];
+      const float lv = fminf(fmaxf(y[i * 2 * K + K + k], -12.f), 12.f);
+      const float s = __expf(0.5f * lv);
+      a = sinhf(mean[k] + sd[k] * (mu - z * s));
+      b = sinhf(mean[k] + sd[k] * (mu + z * s));
+    }
+    lo[p][k] = a;
+    hi[p][k] = b;
+  }
+  if (threadIdx.x < PT) hit[threadIdx.x] = (i0 + threadIdx.x < n) ? 0 : 1;
+  __syncthreads();
+  for (int64_t g = threadIdx.x; g < G; g += blockDim.x) {
+    const float t = (float)(thr[g] * (1.0 + (double)margin));
+    float lb[PT];
+#pragma unroll
+    for (int p = 0; p < PT; ++p) lb[p] = 0.f;
+    for (int k = 0; k < K; ++k) {
+      const float d = (float)data[g + G * k];
+      const float w = Wf[g + G * k];
+#pragma unroll
+      for (int p = 0; p < PT; ++p) {
+        const float e = fmaxf(fmaxf(lo[p][k] - d, d - hi[p][k]), 0.f);
+        lb[p] += w * e * e;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < PT; ++p)
+      if (!(lb[p] > t)) hit[p] = 1;  // benign race: every writer stores 1
+  }
+  __syncthreads();
+  if (threadIdx.x < PT && i0 + threadIdx.x < n) pass[i0 + threadIdx.x] = (uint8_t)hit[threadIdx.x];
+}
+
+struct abc_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = true;
+  int n_in = 0, in_pad = 0, width = 0, depth = 0, n_out = 0, out_pad = 0, ns = 0, max_batch = 0;
+  int64_t n_par = 0, n_bf = 0;
+  std::vector<LayerDesc> layers;
+  std::vector<int64_t> bf_off;
+  LayerDesc* d_layers = nullptr;
+  int64_t* d_bf_off = nullptr;
+  float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr;
+  bf16 *Wb = nullptr, *WbT = nullptr;
+  // activations (max_batch rows, padded to 128)
+  bf16 *X = nullptr, *XT = nullptr;
+  std::vector<float*> Z;
+  std::vector<bf16*> H, HT;
+  float* O = nullptr;
+  bf16 *dA = nullptr, *dAT = nullptr, *dB = nullptr, *dBT = nullptr;
+  float* loss = nullptr;
+  int64_t step = 0;
+  std::string msg;
+};
+
+namespace {
+
+int mfail(abc_mlp* p, int code, const std::string& s) {
+  if (p) p->msg = s;
+  return code;
+}
+
+#define MCHK(p, expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return mfail(p, ABC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <int EPI>
+void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const float* bias,
+          float* Cf, bf16* Cb, bf16* CbT, const float* Zprev, float* Zout) {
+  if (N % 128 == 0) {
+    hipLaunchKernelGGL((gemm_nt_kernel<128, EPI>), dim3(M / BM, N / 128), dim3(256), 0, p->stream,
+                       A, B, M, N, K, bias, Cf, Cb, CbT, Zprev, Zout);
+  } else {
+    hipLaunchKernelGGL((gemm_nt_kernel<64, EPI>), dim3(M / BM, N / 64), dim3(256), 0, p->stream,
+                       A, B, M, N, K, bias, Cf, Cb, CbT, Zprev, Zout);
+  }
+}
+
+int forward(abc_mlp* p, int64_t Bn, const float* x_dev) {
+  const int Bpad = (int)((Bn + 127) / 128 * 128);
+  hipLaunchKernelGGL(cast_input_kernel, dim3(g1((int64_t)Bpad * p->in_pad, 256)), dim3(256), 0,
+                     p->stream, x_dev, (int)Bn, Bpad, p->n_in, p->in_pad, p->X, p->XT);
+  const bf16* in = p->X;
+  for (int l = 0; l < p->depth; ++l) {
+    const LayerDesc& L = p->layers[l];
+    gemm<EPI_SILU>(p, in, p->Wb + p->bf_off[l], Bpad, L.out, L.in, p->w + L.b_off, nullptr,
+                   p->H[l], p->HT[l], nullptr, p->Z[l]);
+    in = p->H[l];
+  }
+  const LayerDesc& L = p->layers[p->depth];
+  gemm<EPI_F32>(p, in, p->Wb + p->bf_off[p->depth], Bpad, L.out, L.in, p->w + L.b_off, p->O,
+                nullptr, nullptr, nullptr, nullptr);
+  MCHK(p, hipGetLastError());
+  return ABC_OK;
+}
+
+// fwd + loss + bwd into p->g; loss (mean NLL) accumulated in p->loss
+int grad(abc_mlp* p, int64_t Bn, const float* x_dev, const float* t_dev) {
+  int rc = forward(p, Bn, x_dev);
+  if (rc) return rc;
+  const int Bpad = (int)((Bn + 127) / 128 * 128);
+  hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, p->stream, p->loss, (int64_t)1);
+  hipLaunchKernelGGL(nll_kernel, dim3(g1((int64_t)Bpad * p->out_pad, 256)), dim3(256), 0,
+                     p->stream, p->O, t_dev, (int)Bn, Bpad, p->ns, p->out_pad, p->dA, p->dAT,
+                     p->loss);
+  // dA / dAT hold dZ of the current layer (output layer first)
+  bf16 *cur = p->dA, *curT = p->dAT, *nxt = p->dB, *nxtT = p->dBT;
+  for (int l = p->depth; l >= 0; --l) {
+    const LayerDesc& L = p->layers[l];
+    const bf16* Hin_T = l == 0 ? p->XT : p->HT[l - 1];
+    // dW_l = dZ_l^T H_{l-1}: (out x Bpad) . (in x Bpad)^T
+    gemm<EPI_F32>(p, curT, Hin_T, L.out, L.in, Bpad, nullptr, p->g + L.w_off, nullptr, nullptr,
+                  nullptr, nullptr);
+    hipLaunchKernelGGL(rowsum_kernel, dim3(L.out), dim3(256), 0, p->stream, curT, Bpad,
+                       p->g + L.b_off);
+    if (l > 0) {
+      // dZ_{l-1} = (dZ_l W_l) * silu'(Z_{l-1}); W_l^T copy is (in x out)
+      gemm<EPI_DSILU>(p, cur, p->WbT + p->bf_off[l], Bpad, L.in, L.out, nullptr, nullptr, nxt,
+                      nxtT, p->Z[l - 1], nullptr);
+      std::swap(cur, nxt);
+      std::swap(curT, nxtT);
+    }
+  }
+  MCHK(p, hipGetLastError());
+  return ABC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t abc_mlp_create(abc_mlp** out, int32_t device, int32_t n_in, int32_t width, int32_t depth,
+                       int32_t n_stats, int32_t max_batch, uint64_t seed) {
+  if (!out || n_in < 1 || n_in > 256 || width < 64 || width % 128 || depth < 1 || depth > 16 ||
+      n_stats < 1 || n_stats > 1024 || max_batch < 1)
+    return ABC_E_ARG;
+  *out = nullptr;
+  abc_mlp* p = new abc_mlp();
+  p->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete p; return ABC_E_HIP; }
+  p->n_in = n_in;
+  p->in_pad = (n_in + 63) / 64 * 64;
+  p->width = width;
+  p->depth = depth;
+  p->ns = n_stats;
+  p->n_out = 2 * n_stats;
+  p->out_pad = (p->n_out + 127) / 128 * 128;  // rows of dW_out are tiled by 128
+  p->max_batch = (max_batch + 127) / 128 * 128;
+  int64_t off = 0, boff = 0;
+  for (int l = 0; l <= depth; ++l) {
+    LayerDesc L;
+    L.in = l == 0 ? p->in_pad : width;
+    L.out = l == depth ? p->out_pad : width;
+    L.w_off = off;
+    off += (int64_t)L.in * L.out;
+    L.b_off = off;
+    off += L.out;
+    p->layers.push_back(L);
+    p->bf_off.push_back(boff);
+    boff += (int64_t)L.in * L.out;
+  }
+  p->n_par = off;
+  p->n_bf = boff;
+  // He-uniform initial weights on the real (unpadded) block, zeros elsewhere
+  std::vector<float> w0(off, 0.0f);
+  std::mt19937_64 rng(seed);
+  for (int l = 0; l <= depth; ++l) {
+    const LayerDesc& L = p->layers[l];
+    const int rin = l == 0 ? n_in : width, rout = l == depth ? p->n_out : width;
+    const float a = std::sqrt(3.0f / (float)rin);
+    std::uniform_real_distribution<float> U(-a, a);
+    for (int o = 0; o < rout; ++o)
+      for (int i = 0; i < rin; ++i) w0[L.w_off + (int64_t)o * L.in + i] = U(rng);
+  }
+  const int64_t Bp = p->max_batch;
+  bool ok = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) == hipSuccess;
+  auto A = [&](auto** ptr, size_t bytes) {
+    if (ok && hipMalloc((void**)ptr, bytes) != hipSuccess) ok = false;
+  };
+  A(&p->w, 4 * off); A(&p->m, 4 * off); A(&p->v, 4 * off); A(&p->g, 4 * off);
+  A(&p->Wb, 2 * boff); A(&p->WbT, 2 * boff);
+  A(&p->d_layers, sizeof(LayerDesc) * p->layers.size());
+  A(&p->d_bf_off, sizeof(int64_t) * p->bf_off.size());
+  A(&p->X, 2 * Bp * p->in_pad); A(&p->XT, 2 * Bp * p->in_pad);
+  p->Z.assign(depth, nullptr); p->H.assign(depth, nullptr); p->HT.assign(depth, nullptr);
+  for (int l = 0; l < depth; ++l) {
+    A(&p->Z[l], 4 * Bp * width); A(&p->H[l], 2 * Bp * width); A(&p->HT[l], 2 * Bp * width);
+  }
+  const int64_t dmax = std::max<int64_t>(width, p->out_pad);
+  A(&p->O, 4 * Bp * p->out_pad);
+  A(&p->dA, 2 * Bp * dmax); A(&p->dAT, 2 * Bp * dmax);
+  A(&p->dB, 2 * Bp * dmax); A(&p->dBT, 2 * Bp * dmax);
+  A(&p->loss, 64);
+  if (!ok) { abc_mlp_destroy(p); return ABC_E_OOM; }
+  std::vector<float> zeros(off, 0.0f);
+  if (hipMemcpy(p->w, w0.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->m, zeros.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->v, zeros.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->d_layers, p->layers.data(), sizeof(LayerDesc) * p->layers.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(p->d_bf_off, p->bf_off.data(), sizeof(int64_t) * p->bf_off.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    abc_mlp_destroy(p);
+    return ABC_E_HIP;
+  }
+  hipLaunchKernelGGL(refresh_bf16_kernel, dim3(g1(off, 256)), dim3(256), 0, p->stream, p->w,
+                     p->d_layers, (int)p->layers.size(), p->d_bf_off, p->Wb, p->WbT, off);
+  if (hipStreamSynchronize(p->stream) != hipSuccess) { abc_mlp_destroy(p); return ABC_E_HIP; }
+  *out = p;
+  return ABC_OK;
+}
+
+int32_t abc_mlp_destroy(abc_mlp* p) {
+  if (!p) return ABC_OK;
+  if (p->stream) hipStreamSynchronize(p->stream);
+  void* bufs[] = {p->w, p->m, p->v, p->g, p->Wb, p->WbT, p->d_layers, p->d_bf_off, p->X, p->XT,
+                  p->O, p->dA, p->dAT, p->dB, p->dBT, p->loss};
+  for (void* b : bufs)
+    if (b) hipFree(b);
+  for (auto* z : p->Z) if (z) hipFree(z);
+  for (auto* h : p->H) if (h) hipFree(h);
+  for (auto* h : p->HT) if (h) hipFree(h);
+  if (p->stream && p->own_stream) hipStreamDestroy(p->stream);
+  delete p;
+  return ABC_OK;
+}
+
+const char* abc_mlp_last_error(const abc_mlp* p) { return p ? p->msg.c_str() : "null model"; }
+
+int64_t abc_mlp_n_params(const abc_mlp* p) { return p ? p->n_par : ABC_E_ARG; }
+
+int32_t abc_mlp_set_stream(abc_mlp* p, void* s) {
+  if (!p) return ABC_E_ARG;
+  hipStreamSynchronize(p->stream);
+  if (p->own_stream) hipStreamDestroy(p->stream);
+  if (s) {
+    p->stream = (hipStream_t)s;
+    p->own_stream = false;
+  } else {
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return ABC_E_HIP;
+    p->own_stream = true;
+  }
+  return ABC_OK;
+}
+
+int32_t abc_mlp_get_params(abc_mlp* p, float* host) {
+  if (!p || !host) return ABC_E_ARG;
+  MCHK(p, hipStreamSynchronize(p->stream));
+  MCHK(p, hipMemcpy(host, p->w, 4 * p->n_par, hipMemcpyDeviceToHost));
+  return ABC_OK;
+}
+
+int32_t abc_mlp_set_params(abc_mlp* p, const float* host) {
+  if (!p || !host) return ABC_E_ARG;
+  MCHK(p, hipStreamSynchronize(p->stream));
+  MCHK(p, hipMemcpy(p->w, host, 4 * p->n_par, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(refresh_bf16_kernel, dim3(g1(p->n_par, 256)), dim3(256), 0, p->stream, p->w,
+                     p->d_layers, (int)p->layers.size(), p->d_bf_off, p->Wb, p->WbT, p->n_par);
+  MCHK(p, hipStreamSynchronize(p->stream));
+  return ABC_OK;
+}
+
+int32_t abc_mlp_forward_dev(abc_mlp* p, int64_t n, const float* x_dev, float* y_dev) {
+  if (!p || n < 0 || (n > 0 && (!x_dev || !y_dev))) return ABC_E_ARG;
+  for (int64_t o = 0; o < n; o += p->max_batch) {
+    const int64_t b = std::min<int64_t>(p->max_batch, n - o);
+    int rc = forward(p, b, x_dev + o * p->n_in);
+    if (rc) return rc;
+    hipLaunchKernelGGL(gather_out_kernel, dim3(g1(b * p->n_out, 256)), dim3(256), 0, p->stream,
+                       p->O, (int)b, p->ns, p->out_pad, y_dev + o * p->n_out);
+  }
+  MCHK(p, hipGetLastError());
+  return ABC_OK;
+}
+
+int32_t abc_mlp_grad_dev(abc_mlp* p, int64_t n, const float* x_dev, const float* t_dev,
+                         float* grad_host, float* loss_host) {
+  if (!p || n < 1 || n > p->max_batch || !x_dev || !t_dev) return ABC_E_ARG;
+  int rc = grad(p, n, x_dev, t_dev);
+  if (rc) return rc;
+  MCHK(p, hipStreamSynchronize(p->stream));
+  if (grad_host) MCHK(p, hipMemcpy(grad_host, p->g, 4 * p->n_par, hipMemcpyDeviceToHost));
+  if (loss_host) MCHK(p, hipMemcpy(loss_host, p->loss, 4, hipMemcpyDeviceToHost));
+  return ABC_OK;
+}
+
+int32_t abc_mlp_train_step_dev(abc_mlp* p, int64_t n, const float* x_dev, const float* t_dev,
+                               float lr, float beta1, float beta2, float eps, float weight_decay,
+                               float* loss_dev_or_null) {
+  if (!p || n < 1 || n > p->max_batch || !x_dev || !t_dev) return ABC_E_ARG;
+  int rc = grad(p, n, x_dev, t_dev);
+  if (rc) return rc;
+  p->step++;
+  const float c1 = 1.0f - std::pow(beta1, (float)p->step);
+  const float c2 = 1.0f - std::pow(beta2, (float)p->step);
+  hipLaunchKernelGGL(adam_kernel, dim3(g1(p->n_par, 256)), dim3(256), 0, p->stream, p->w, p->m,
+                     p->v, p->g, p->n_par, lr, beta1, beta2, eps, weight_decay, c1, c2,
+                     p->d_layers, (int)p->layers.size(), p->d_bf_off, p->Wb, p->WbT);
+  if (loss_dev_or_null)
+    MCHK(p, hipMemcpyAsync(loss_dev_or_null, p->loss, 4, hipMemcpyDeviceToDevice, p->stream));
+  MCHK(p, hipGetLastError());
+  return ABC_OK;
+}
+
+int32_t abc_mlp_synchronize(abc_mlp* p) {
+  if (!p) return ABC_E_ARG;
+  MCHK(p, hipStreamSynchronize(p->stream));
+  return ABC_OK;
+}
+
+int32_t abc_mlp_n_stats(const abc_mlp* p) { return p ? p->ns : ABC_E_ARG; }
+
+int32_t abc_features_dev(int32_t model, int64_t n, const double* theta_dev, int32_t n_bins,
+                         float* x_dev, void* stream) {
+  if (model < 1 || model > 5 || n < 0 || n_bins < 1 || n_bins > ABC_MAX_BINS ||
+      (n > 0 && (!theta_dev || !x_dev)))
+    return ABC_E_ARG;
+  if (n == 0) return ABC_OK;
+  hipLaunchKernelGGL(features_kernel, dim3(g1(n, 256)), dim3(256), 0, (hipStream_t)stream, model,
+                     n, theta_dev, n_bins, x_dev);
+  return hipGetLastError() == hipSuccess ? ABC_OK : ABC_E_HIP;
+}
+
+int32_t abc_targets_dev(int64_t n, const double* stats_dev, int32_t K, const float* mean_dev,
+                        const float* sd_dev, float* t_dev, void* stream) {
+  if (n < 0 || K < 1 || (n > 0 && (!stats_dev || !mean_dev || !sd_dev || !t_dev))) return ABC_E_ARG;
+  if (n == 0) return ABC_OK;
+  hipLaunchKernelGGL(targets_kernel, dim3(g1(n * K, 256)), dim3(256), 0, (hipStream_t)stream, n,
+                     stats_dev, K, mean_dev, sd_dev, t_dev);
+  return hipGetLastError() == hipSuccess ? ABC_OK : ABC_E_HIP;
+}
+
+}  // extern "C"

@@ -165,6 +165,50 @@ int32_t abc_last_timing(abc_ctx* ctx, double* ms4);
  * number of launches.  Waits for the pending events; reset != 0 clears. */
 int32_t abc_ssa_time(abc_ctx* ctx, int32_t reset, double* ms_total, int64_t* launches);
 
+/* ---- surrogate (no reference counterpart: the reference simulates every
+ * prior particle, scripts/abc_simulation.jl:88-97).  An MLP maps a
+ * particle's features to the mean and log-variance of each statistic (in the
+ * standardised asinh space); it is trained on simulator output with BF16
+ * MFMA forward/backward kernels, FP32 master weights, Adam and a Gaussian
+ * negative log-likelihood, and screens prior particles so that the exact
+ * simulation runs only on those that can still be accepted. */
+typedef struct abc_mlp abc_mlp;
+/* n_in features -> depth x [width, SiLU] -> 2*n_stats outputs (width % 128 == 0). */
+int32_t abc_mlp_create(abc_mlp** out, int32_t device, int32_t n_in, int32_t width,
+                       int32_t depth, int32_t n_stats, int32_t max_batch, uint64_t seed);
+int32_t abc_mlp_destroy(abc_mlp* mlp);
+const char* abc_mlp_last_error(const abc_mlp* mlp);
+int64_t abc_mlp_n_params(const abc_mlp* mlp);
+int32_t abc_mlp_n_stats(const abc_mlp* mlp);
+int32_t abc_mlp_set_stream(abc_mlp* mlp, void* hip_stream);
+int32_t abc_mlp_synchronize(abc_mlp* mlp);
+/* Flat FP32 parameters: per layer W (out_pad x in_pad, row-major) then b (out_pad). */
+int32_t abc_mlp_get_params(abc_mlp* mlp, float* host);
+int32_t abc_mlp_set_params(abc_mlp* mlp, const float* host);
+/* x: n x n_in FP32 row-major (device); y: n x 2*n_stats [mean | log-variance]. */
+int32_t abc_mlp_forward_dev(abc_mlp* mlp, int64_t n, const float* x_dev, float* y_dev);
+/* Mean NLL and its gradient (flat like the parameters) for one batch, no update. */
+int32_t abc_mlp_grad_dev(abc_mlp* mlp, int64_t n, const float* x_dev, const float* t_dev,
+                         float* grad_host, float* loss_host);
+/* One Adam step on a batch (n <= max_batch); t: n x n_stats targets.  The
+ * batch loss is copied to loss_dev when it is not NULL. */
+int32_t abc_mlp_train_step_dev(abc_mlp* mlp, int64_t n, const float* x_dev, const float* t_dev,
+                               float lr, float beta1, float beta2, float eps, float weight_decay,
+                               float* loss_dev);
+/* Features of n particles of a model (theta n x abc_n_params, device):
+ * one-hot(model) + per-bin log10 rates / 3 -> x (n x (5 + 3*n_bins + 3)). */
+int32_t abc_features_dev(int32_t model, int64_t n, const double* theta_dev, int32_t n_bins,
+                         float* x_dev, void* hip_stream);
+/* Targets: (asinh(stat) - mean) / sd per statistic; stats n x K col-major. */
+int32_t abc_targets_dev(int64_t n, const double* stats_dev, int32_t K, const float* mean_dev,
+                        const float* sd_dev, float* t_dev, void* hip_stream);
+/* Screen n particles against the genes and thresholds of the open acceptance
+ * session: pass[i] = 1 when some gene's interval lower bound (z-sigma
+ * surrogate intervals) is within (1 + margin) of its threshold. */
+int32_t abc_screen_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float* x_dev,
+                       const float* mean_dev, const float* sd_dev, double z, double margin,
+                       uint8_t* pass_dev, int64_t* n_pass);
+
 /* Launch shape of the persistent SSA kernel: workgroups (0 = occupancy
  * default).  Set by the autotuner. */
 int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
