@@ -170,7 +170,7 @@ int drain_ring(abc_ctx* c) {
 
 int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                  const double* theta_dev, double* stats_dev, double* moments_dev,
-                 double* superset_dev, int64_t* steps_dev) {
+                 double* superset_dev, int64_t* steps_dev, const int64_t* pidx_dev = nullptr) {
   int rc;
   if ((rc = check_range(c, model, first, n))) return rc;
   if (n == 0) return ABC_OK;
@@ -211,7 +211,7 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                      (uint32_t)first, key0(seed),
                      key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps,
-                     P<unsigned long long>(c->react));
+                     P<unsigned long long>(c->react), pidx_dev);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(pair[1], c->stream));
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -460,6 +460,14 @@ int32_t abc_simulate_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first
                       steps_dev);
 }
 
+int32_t abc_simulate_idx_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t n,
+                             const int64_t* pidx_dev, const double* theta_dev, double* stats_dev,
+                             double* moments_dev, double* superset_dev, int64_t* steps_dev) {
+  if (!c || (n > 0 && (!theta_dev || !stats_dev || !pidx_dev))) return ABC_E_ARG;
+  return simulate_dev(c, model, seed, 0, n, theta_dev, stats_dev, moments_dev, superset_dev,
+                      steps_dev, pidx_dev);
+}
+
 int32_t abc_simulate(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
                      const double* theta, double* stats, double* moments, double* superset,
                      int64_t* steps, uint32_t* cell_u, uint32_t* cell_l, uint8_t* cell_bin) {
@@ -556,7 +564,8 @@ struct GeneHit {
 
 int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
-int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index) {
+int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index,
+                 const int64_t* pidx_host = nullptr) {
   AcceptState& A = c->acc;
   int rc;
   const int64_t npad = pad_to(n, 64);
@@ -600,9 +609,9 @@ int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_i
     for (const Candidate& x : h) {
       if (!(x.err == x.err)) continue;  // NaN errors are never accepted
       if (A.mode == 0) {
-        if (x.err <= A.eps) A.kept[x.gene].push_back({x.err, first_index + x.particle});
+        if (x.err <= A.eps) A.kept[x.gene].push_back({x.err, pidx_host ? pidx_host[x.particle] : first_index + x.particle});
       } else {
-        A.kept[x.gene].push_back({x.err, first_index + x.particle});
+        A.kept[x.gene].push_back({x.err, pidx_host ? pidx_host[x.particle] : first_index + x.particle});
         touched[x.gene] = 1;
       }
     }
@@ -697,7 +706,8 @@ int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
   if ((rc = abc_mlp_synchronize(mlp))) return fail(c, rc, "surrogate forward failed");
   hipLaunchKernelGGL(screen_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c->stream,
                      P<float>(c->sy), n, A.E.K, mean_dev, sd_dev, (float)z, P<float>(c->wf),
-                     P<double>(c->edata), A.G, P<double>(c->thr), (float)margin, pass_dev);
+                     P<double>(c->edata), A.G, P<double>(c->thr), (float)margin, A.E.clamp,
+                     pass_dev);
   HIPCHK(c, hipGetLastError());
   if (n_pass) {
     std::vector<uint8_t> h(n);
@@ -715,6 +725,26 @@ int32_t abc_accept_add_dev(abc_ctx* c, int64_t n, const double* stats_dev, int64
   if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (n == 0 || c->acc.G == 0) return ABC_OK;
   return screen_batch(c, n, stats_dev, first_index);
+}
+
+int32_t abc_accept_add_idx_dev(abc_ctx* c, int64_t n, const double* stats_dev,
+                               const int64_t* pidx_host) {
+  if (!c || n < 0 || (n > 0 && (!stats_dev || !pidx_host))) return ABC_E_ARG;
+  if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  if (n == 0 || c->acc.G == 0) return ABC_OK;
+  return screen_batch(c, n, stats_dev, 0, pidx_host);
+}
+
+int32_t abc_accept_set_thresholds(abc_ctx* c, const double* thr) {
+  if (!c || !thr) return ABC_E_ARG;
+  AcceptState& A = c->acc;
+  if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
+  for (int64_t g = 0; g < A.G; ++g) A.thr_host[g] = std::min(A.thr_host[g], thr[g]);
+  if (A.G > 0)
+    HIPCHK(c, hipMemcpyAsync(c->thr.p, A.thr_host.data(), sizeof(double) * A.G,
+                             hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
 }
 
 int32_t abc_accept_add(abc_ctx* c, int64_t n, const double* stats, int64_t first_index) {

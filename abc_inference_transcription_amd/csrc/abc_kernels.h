@@ -69,7 +69,7 @@ __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* o
                            uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
                            const int64_t* goff, unsigned long long* queue, uint32_t* out_u,
                            uint32_t* out_l, uint8_t* out_bin, unsigned long long* steps_out,
-                           unsigned long long* react_out);
+                           unsigned long long* react_out, const int64_t* pidx);
 __global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
                              const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
                              double* superset);
@@ -88,7 +88,7 @@ __global__ void gene_prep_kernel(const double* data, const double* se, int64_t G
 // surrogate screen (abc_mlp.hip)
 __global__ void screen_kernel(const float* y, int64_t n, int K, const float* mean, const float* sd,
                               float z, const float* Wf, const double* data, int64_t G,
-                              const double* thr, float margin, uint8_t* pass);
+                              const double* thr, float margin, double clamp, uint8_t* pass);
 struct abc_mlp;
 extern "C" int32_t abc_mlp_forward_dev(abc_mlp* p, int64_t n, const float* x_dev, float* y_dev);
 extern "C" int32_t abc_mlp_synchronize(abc_mlp* p);

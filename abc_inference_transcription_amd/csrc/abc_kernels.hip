@@ -228,7 +228,8 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
                                                      uint32_t* __restrict__ out_l,
                                                      uint8_t* __restrict__ out_bin,
                                                      unsigned long long* __restrict__ steps_out,
-                                                     unsigned long long* __restrict__ react_out) {
+                                                     unsigned long long* __restrict__ react_out,
+                                                     const int64_t* __restrict__ pidx) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
   const int age_mode = D->age_mode, cap_mode = D->cap_mode;
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       const int64_t rem = (int64_t)(q - (unsigned long long)off[lo]);
       c = (int)(rem / M);
       sub = (int)(rem - (int64_t)c * M);
-      pid = first + (uint32_t)p;
+      pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       sstream = 1u + (uint32_t)sub;
       share = (double)M;
       gctr = 0;

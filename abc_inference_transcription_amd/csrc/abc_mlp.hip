@@ -282,54 +282,86 @@ __global__ void targets_kernel(int64_t n, const double* __restrict__ stats, int 
   t[q] = v;
 }
 
-// Screen: a particle passes when, for some gene, the error of the statistics
-// at the edges of the surrogate's z-sigma intervals can reach the gene's
-// threshold: LB_g = sum_k W_gk dist(d_gk, [lo_k, hi_k])^2 <= thr_g (1 + margin).
-// 256 threads x 16 particles per workgroup; genes strided over the threads.
+// Screen: the surrogate gives, per statistic, y ~ N(mu, sigma^2) in the
+// standardised asinh space, s = sinh(mean_k + sd_k y).  Its central moments
+// (E s, c2, c3, c4) come from 7-point Gauss-Hermite quadrature.  The
+// statistics of different (bin, condition) groups come from disjoint cells,
+// so the error err_g = sum_k W_gk (d_gk - s_k)^2 concentrates around
+//   E err = sum W (delta^2 + c2),  Var err = sum W^2 (4 delta^2 c2 - 4 delta c3 + c4 - c2^2)
+// with delta = d - E s.  A particle passes when E err - z sd err can reach a
+// gene's threshold (or when its prediction is not finite).  Genes whose data
+// are not finite have NaN errors and accept nobody.
 __global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y, int64_t n, int K,
                                                      const float* __restrict__ mean,
                                                      const float* __restrict__ sd, float z,
                                                      const float* __restrict__ Wf,
                                                      const double* __restrict__ data, int64_t G,
                                                      const double* __restrict__ thr, float margin,
-                                                     uint8_t* __restrict__ pass) {
+                                                     double clamp, uint8_t* __restrict__ pass) {
   constexpr int PT = 16;
-  __shared__ float lo[PT][ABC_MAX_STATS / 2], hi[PT][ABC_MAX_STATS / 2];
+  __shared__ float e1[PT][ABC_MAX_STATS / 2], c2[PT][ABC_MAX_STATS / 2];
+  __shared__ float c3[PT][ABC_MAX_STATS / 2], c4[PT][ABC_MAX_STATS / 2];
   __shared__ int hit[PT];
+  const double gx[7] = {-3.7504397177257425, -2.366759410734541, -1.1544053947399682, 0.0,
+                        1.1544053947399682, 2.366759410734541, 3.7504397177257425};
+  const double gw[7] = {0.000548268855972217, 0.03075712396758652, 0.2401231786050127,
+                        0.45714285714285724, 0.2401231786050127, 0.03075712396758652,
+                        0.000548268855972217};
   const int64_t i0 = (int64_t)blockIdx.x * PT;
+  if (threadIdx.x < PT) hit[threadIdx.x] = (i0 + threadIdx.x < n) ? 0 : 1;
+  __syncthreads();
   for (int q = threadIdx.x; q < PT * K; q += blockDim.x) {
     const int p = q / K, k = q % K;
     const int64_t i = i0 + p;
-    float a = 0.f, b = 0.f;
+    double m1 = 0.0, v2 = 0.0, v3 = 0.0, v4 = 0.0;
     if (i < n) {
-      const float mu = y[i * 2 * K + k];
-      const float lv = fminf(fmaxf(y[i * 2 * K + K + k], -12.f), 12.f);
-      const float s = __expf(0.5f * lv);
-      a = sinhf(mean[k] + sd[k] * (mu - z * s));
-      b = sinhf(mean[k] + sd[k] * (mu + z * s));
+      const double mu = y[i * 2 * K + k];
+      const double lv = fmin(fmax((double)y[i * 2 * K + K + k], -12.0), 12.0);
+      const double s = exp(0.5 * lv);
+      double sv[7];
+      for (int r = 0; r < 7; ++r) {
+        sv[r] = sinh((double)mean[k] + (double)sd[k] * (mu + s * gx[r]));
+        m1 += gw[r] * sv[r];
+      }
+      for (int r = 0; r < 7; ++r) {
+        const double e = sv[r] - m1, e2 = e * e;
+        v2 += gw[r] * e2;
+        v3 += gw[r] * e2 * e;
+        v4 += gw[r] * e2 * e2;
+      }
+      if (!(fabs(m1) < 1e30) || !(v4 < 1e300)) hit[p] = 1;
     }
-    lo[p][k] = a;
-    hi[p][k] = b;
+    e1[p][k] = (float)m1;
+    c2[p][k] = (float)v2;
+    c3[p][k] = (float)v3;
+    c4[p][k] = (float)v4;
   }
-  if (threadIdx.x < PT) hit[threadIdx.x] = (i0 + threadIdx.x < n) ? 0 : 1;
   __syncthreads();
   for (int64_t g = threadIdx.x; g < G; g += blockDim.x) {
-    const float t = (float)(thr[g] * (1.0 + (double)margin));
-    float lb[PT];
+    const double tg = thr[g];
+    if (tg >= clamp) {  // every clamped error ties at the threshold: nobody may be skipped
+      for (int p = 0; p < PT; ++p) hit[p] = 1;
+      continue;
+    }
+    const float t = (float)(tg * (1.0 + (double)margin));
+    float em[PT], ev[PT];
 #pragma unroll
-    for (int p = 0; p < PT; ++p) lb[p] = 0.f;
+    for (int p = 0; p < PT; ++p) { em[p] = 0.f; ev[p] = 0.f; }
     for (int k = 0; k < K; ++k) {
       const float d = (float)data[g + G * k];
       const float w = Wf[g + G * k];
+      const float w2 = w * w;
 #pragma unroll
       for (int p = 0; p < PT; ++p) {
-        const float e = fmaxf(fmaxf(lo[p][k] - d, d - hi[p][k]), 0.f);
-        lb[p] += w * e * e;
+        const float dl = d - e1[p][k];
+        const float a = c2[p][k];
+        em[p] += w * (dl * dl + a);
+        ev[p] += w2 * fmaxf(4.f * dl * dl * a - 4.f * dl * c3[p][k] + c4[p][k] - a * a, 0.f);
       }
     }
 #pragma unroll
     for (int p = 0; p < PT; ++p)
-      if (!(lb[p] > t)) hit[p] = 1;  // benign race: every writer stores 1
+      if (em[p] - z * sqrtf(ev[p]) <= t) hit[p] = 1;  // benign race: every writer stores 1
   }
   __syncthreads();
   if (threadIdx.x < PT && i0 + threadIdx.x < n) pass[i0 + threadIdx.x] = (uint8_t)hit[threadIdx.x];

@@ -199,6 +199,12 @@ class Engine:
         self._chk(self.L.abc_last_timing(self.ctx, _p(ms)), "last_timing")
         return ms
 
+    def bind_stream(self, torch_stream):
+        """Launch every later call on this (non-null) torch stream."""
+        self._chk(self.L.abc_set_stream(self.ctx, C.c_void_p(torch_stream.cuda_stream)),
+                  "set_stream")
+        self._stream = torch_stream
+
     def set_ssa_blocks(self, blocks: int):
         self._chk(self.L.abc_set_ssa_blocks(self.ctx, blocks), "set_ssa_blocks")
 

@@ -1,0 +1,287 @@
+"""Surrogate-screened ABC against the exhaustive plain path.
+
+1. Synthetic genes: data statistics of ground-truth particles drawn from the
+   same priors, SE = |stat| * U[0.05, 0.3].
+2. Training set: exact simulations of prior particles (all five models); the
+   surrogate learns mean and log-variance of each statistic (Gaussian NLL).
+3. Held-out evaluation set, simulated exhaustively (the plain path): per gene
+   the k particles with the smallest FP64 errors -> accepted pairs A_plain and
+   thresholds T_g.
+4. Screened path on the same particles: surrogate + interval screen against
+   T_g, exact simulation and FP64 acceptance only for particles that pass.
+Reports recall |A_plain & A_screened| / |A_plain|, the fraction of exact
+simulations avoided, and accepted pairs per second of both paths.
+All compute runs in libabc.so; torch only holds device buffers.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import _lib
+
+MODELS = (1, 2, 3, 4, 5)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def simulate_block(eng, model, n, first, seed=0, with_steps=False):
+    """Prior draws and exact simulation of particles first..first+n-1 on the
+    device: (theta, superset n x K column-major, steps)."""
+    import torch
+    dev = torch.device("cuda:0")
+    L = eng.L
+    npar = eng.n_params(model)
+    KS = 2 * eng.nb * eng.J
+    th = torch.empty(npar * n, dtype=torch.float64, device=dev)
+    st = torch.empty((4 * eng.nb + 3 * eng.J) * n, dtype=torch.float64, device=dev)
+    sup = torch.empty(KS * n, dtype=torch.float64, device=dev)
+    steps = torch.zeros(n, dtype=torch.int64, device=dev)
+    _lib.check(L.abc_sample_prior_dev(eng.ctx, model, seed, first, n, _ptr(th)), eng.ctx, "prior")
+    _lib.check(L.abc_simulate_dev(eng.ctx, model, seed, first, n, _ptr(th), _ptr(st), None,
+                                  _ptr(sup), _ptr(steps)), eng.ctx, "simulate")
+    return th, sup, steps
+
+
+def make_genes(eng, G, seed=0, first=3_000_000_000):
+    """Data statistics of G ground-truth prior particles that pass the
+    reference's expression filter (scripts/gene_selection.jl:13-25: on average
+    at least one count per cell in some age bin); rows with non-finite
+    statistics (an age/condition group with fewer than two cells) are
+    skipped."""
+    nb, J = eng.nb, eng.J
+    rows, got, block = [], 0, 4 * G
+    while got < G:
+        per = (block + len(MODELS) - 1) // len(MODELS)
+        for m in MODELS:
+            _, sup, _ = simulate_block(eng, m, per, first + m * per, seed)
+            KS = sup.numel() // per
+            r = sup.view(KS, per).t().cpu().numpy()
+            means = r[:, :nb * J].reshape(-1, J, nb)          # mean_t(b, j) at b + nb*j
+            per_bin = means.mean(axis=1)                      # average over conditions
+            ok = np.all(np.isfinite(r), axis=1) & (per_bin.max(axis=1) >= 1.0)
+            rows.append(r[ok])
+            got += int(ok.sum())
+        first += len(MODELS) * per + per
+        if first > 4_200_000_000:
+            raise RuntimeError("not enough expressed ground-truth particles for the genes")
+    data = np.vstack(rows)
+    order = np.random.default_rng(seed).permutation(len(data))
+    data = data[order][:G]
+    if len(data) < G:
+        raise RuntimeError("not enough finite ground-truth particles for the genes")
+    rng = np.random.default_rng(seed)
+    se = np.abs(data) * rng.uniform(0.05, 0.3, data.shape)
+    return np.asfortranarray(data), np.asfortranarray(se)
+
+
+def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
+        z=4.0, margin=0.0,
+        train_steps=1500, batch=8192, lr=1e-3, seed=0, log=print):
+    """Returns a dict of metrics (see module docstring)."""
+    import torch
+    dev = torch.device("cuda:0")
+    # one stream for libabc, the surrogate and torch's own buffer work, so every
+    # kernel sees the outputs of the ones enqueued before it
+    stream = torch.cuda.current_stream()
+    if stream.cuda_stream == 0:
+        stream = torch.cuda.Stream(device=dev)
+        torch.cuda.set_stream(stream)
+    eng.bind_stream(stream)
+    sur.bind_stream(stream)
+    L = eng.L
+    nb, J = eng.nb, eng.J
+    KS = 2 * nb * J
+    parts = np.array([nb * J, nb * J], np.int32)
+    mode = 1 if k > 0 else 0   # top-k per gene, or the reference's fixed epsilon
+    out = {}
+    t0 = time.perf_counter()
+    data, se = make_genes(eng, G, seed)
+    log(f"[screened] genes ready ({time.perf_counter() - t0:.1f}s)")
+
+    # ---- training set (streams 1e9 + ...), disjoint from the evaluation set
+    xs, ts_raw = [], []
+    t0 = time.perf_counter()
+    for m in MODELS:
+        th, sup, _ = simulate_block(eng, m, n_train, 1_000_000_000 + m * n_train, seed)
+        xs.append(sur.features(m, th))
+        ts_raw.append(sup)
+    torch.cuda.synchronize()
+    out["train_simulate_s"] = time.perf_counter() - t0
+    stats_np = np.vstack([s.view(KS, n_train).t().cpu().numpy() for s in ts_raw])
+    sur.fit_normalizer(stats_np)
+    x_train = torch.cat(xs)
+    t_train = torch.cat([sur.targets(s, n_train) for s in ts_raw])
+    t0 = time.perf_counter()
+    losses = sur.train(x_train, t_train, steps=train_steps, batch=batch, lr=lr, seed=seed)
+    out["train_s"] = time.perf_counter() - t0
+    out["train_loss_first"] = float(losses[:10].mean())
+    out["train_loss_last"] = float(losses[-50:].mean())
+    log(f"[screened] trained {train_steps} steps: loss {out['train_loss_first']:.3f} -> "
+        f"{out['train_loss_last']:.3f} ({out['train_s']:.1f}s)")
+
+    # ---- acceptance threshold at the target selectivity: the FP64 errors of
+    # the (already simulated) training particles against the genes give the
+    # epsilon at which a fraction `accept_rate` of (particle, gene) pairs pass
+    # (config 2 accepts 1,000 of 2.5e7 particles per gene: 4e-5)
+    if accept_rate is not None and mode == 0:
+        errs = []
+        data_d = torch.tensor(data.ravel(order="F"), dtype=torch.float64, device=dev)
+        se_d = torch.tensor(se.ravel(order="F"), dtype=torch.float64, device=dev)
+        for s in ts_raw:
+            e = torch.empty(n_train * G, dtype=torch.float64, device=dev)
+            _lib.check(L.abc_errors_dev(eng.ctx, n_train, KS, _ptr(s), G, _ptr(data_d), _ptr(se_d),
+                                        parts.ctypes.data_as(C.c_void_p), 2, 10.0, _ptr(e)),
+                       eng.ctx, "errors")
+            errs.append(e)
+        e = torch.cat(errs)
+        e = e[torch.isfinite(e)]
+        kth = max(1, int(round(accept_rate * e.numel())))
+        eps = float(torch.kthvalue(e, kth).values.item())
+        out["accept_rate_target"] = accept_rate
+        log(f"[screened] epsilon {eps:.4g} accepts {accept_rate:g} of training pairs")
+    out["eps"] = eps if mode == 0 else None
+
+    # ---- exhaustive plain path on the evaluation set (timed)
+    def eval_first(m):
+        return 2_000_000_000 + m * n_eval
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.abc_accept_begin(eng.ctx, KS, G, data.ctypes.data_as(C.c_void_p),
+                                  se.ctypes.data_as(C.c_void_p),
+                                  parts.ctypes.data_as(C.c_void_p), 2, 10.0, mode, eps, k, 0.02),
+               eng.ctx, "accept_begin")
+    eval_sets, eval_steps = {}, {}
+    for m in MODELS:
+        th, sup, steps = simulate_block(eng, m, n_eval, eval_first(m), seed)
+        eval_sets[m] = (th, sup)
+        eval_steps[m] = steps
+        _lib.check(L.abc_accept_add_dev(eng.ctx, n_eval, _ptr(sup), eval_first(m)), eng.ctx, "add")
+    A_plain = _end(eng, G, G * max(k, 16))
+    t_plain = time.perf_counter() - t0
+    n_all = n_eval * len(MODELS)
+    thr = (np.array([max(e) if len(e) else 10.0 for (_, e) in A_plain]) if mode == 1
+           else np.full(G, eps))
+    plain_pairs = {(g, int(i)) for g, (idx, _) in enumerate(A_plain) for i in idx}
+    out.update(plain_time_s=t_plain, plain_pairs=len(plain_pairs), eval_particles=n_all,
+               plain_accepted_per_s=len(plain_pairs) / t_plain)
+    log(f"[screened] plain: {len(plain_pairs)} accepted pairs in {t_plain:.2f}s")
+
+    # ---- held-out accuracy of the surrogate against the simulator (per statistic,
+    # in standardised asinh units; the targets carry the sampling noise of the
+    # ~55 cells behind each (bin, condition) statistic, which the predicted
+    # variance is meant to capture)
+    err_rms, cover, nll, zsq = [], [], [], []
+    for m in MODELS:
+        th, sup = eval_sets[m]
+        x = sur.features(m, th)
+        y = sur.predict(x)
+        t = sur.targets(sup, n_eval)
+        lv = y[:, KS:].clamp(-12, 12)
+        mu, sd = y[:, :KS], torch.exp(0.5 * lv)
+        err_rms.append(((mu - t) ** 2).mean(0))
+        cover.append(((mu - t).abs() <= 2 * sd).float().mean(0))
+        nll.append((0.5 * (lv + (t - mu) ** 2 * torch.exp(-lv))).mean(0))
+        zsq.append((((t - mu) / sd) ** 2).mean(0))
+    err_rms = torch.sqrt(torch.stack(err_rms).mean(0)).cpu().numpy()
+    cover = torch.stack(cover).mean(0).cpu().numpy()
+    out["heldout_rmse_std_units_median"] = float(np.median(err_rms))
+    out["heldout_rmse_std_units_max"] = float(err_rms.max())
+    out["heldout_coverage_2sigma"] = float(cover.mean())
+    out["heldout_nll"] = float(torch.stack(nll).mean().item())
+    out["heldout_z_rms"] = float(np.sqrt(torch.stack(zsq).mean().item()))
+    out["heldout_nll_of_mean_predictor"] = 0.5  # N(0,1) on standardised targets
+
+    # ---- screened path (timed) for each z: surrogate screen, then exact work
+    # on the particles that pass only
+    zs = list(z) if isinstance(z, (list, tuple)) else [z]
+    by_z = []
+    for zz in zs:
+        by_z.append(_screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, zz, margin,
+                              eval_sets, eval_steps, eval_first, n_eval, n_all, plain_pairs,
+                              t_plain, seed, dev, log))
+    out["by_z"] = by_z
+    best = [r for r in by_z if r["recall"] >= 0.99]
+    pick = max(best, key=lambda r: r["speedup_accepted_per_s"]) if best else by_z[0]
+    out.update(pick)
+    out.update(genes=G, train_particles=n_train * len(MODELS), k=k)
+    return out
+
+
+def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin, eval_sets,
+              eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev, log):
+    import torch
+    nb, J = eng.nb, eng.J
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _lib.check(L.abc_accept_begin(eng.ctx, KS, G, data.ctypes.data_as(C.c_void_p),
+                                  se.ctypes.data_as(C.c_void_p),
+                                  parts.ctypes.data_as(C.c_void_p), 2, 10.0, mode, eps, k, 0.02),
+               eng.ctx, "accept_begin")
+    _lib.check(L.abc_accept_set_thresholds(eng.ctx, thr.ctypes.data_as(C.c_void_p)), eng.ctx, "thr")
+    n_pass_total = 0
+    t_screen = 0.0
+    work_all, work_pass = 0, 0
+    for m in MODELS:
+        th, _ = eval_sets[m]
+        ts = time.perf_counter()
+        x = sur.features(m, th)
+        pas, npass = sur.screen(eng, x, z=z, margin=margin)
+        t_screen += time.perf_counter() - ts
+        n_pass_total += npass
+        work_all += int(eval_steps[m].sum().item())
+        work_pass += int(eval_steps[m][pas.bool()].sum().item())
+        if npass == 0:
+            continue
+        keep = torch.nonzero(pas, as_tuple=False).flatten()
+        npar = eng.n_params(m)
+        th_keep = th.view(npar, n_eval)[:, keep].contiguous().view(-1)
+        pidx = (keep + eval_first(m)).to(torch.int64)
+        st = torch.empty((4 * nb + 3 * J) * npass, dtype=torch.float64, device=dev)
+        sup = torch.empty(KS * npass, dtype=torch.float64, device=dev)
+        _lib.check(L.abc_simulate_idx_dev(eng.ctx, m, seed, npass, _ptr(pidx), _ptr(th_keep),
+                                          _ptr(st), None, _ptr(sup), None), eng.ctx, "simulate_idx")
+        pidx_h = np.ascontiguousarray(pidx.cpu().numpy())
+        _lib.check(L.abc_accept_add_idx_dev(eng.ctx, npass, _ptr(sup),
+                                            pidx_h.ctypes.data_as(C.c_void_p)), eng.ctx, "add_idx")
+    A_scr = _end(eng, G, G * max(k, 16))
+    t_scr = time.perf_counter() - t0
+    scr_pairs = {(g, int(i)) for g, (idx, e) in enumerate(A_scr) for i, ee in zip(idx, e)
+                 if ee <= thr[g]}
+    hit = len(plain_pairs & scr_pairs)
+    res = dict(z=z, screened_time_s=t_scr, screen_time_s=t_screen, screened_pass=n_pass_total,
+               ssa_fraction_avoided=1.0 - n_pass_total / n_all,
+               ssa_work_fraction_avoided=1.0 - work_pass / max(work_all, 1),
+               recall=hit / max(len(plain_pairs), 1),
+               screened_accepted_per_s=hit / t_scr,
+               speedup_accepted_per_s=(hit / t_scr) / (len(plain_pairs) / t_plain))
+    log(f"[screened] z={z}: pass {n_pass_total}/{n_all}, recall {res['recall']:.4f}, "
+        f"time {t_scr:.2f}s vs plain {t_plain:.2f}s")
+    return res
+
+
+def _end(eng, G, cap):
+    """Close the acceptance session: per gene (0-based particle indices, errors)."""
+    while True:
+        counts = np.zeros(G, np.int64)
+        idx = np.zeros(max(cap, 1), np.int64)
+        eo = np.zeros(max(cap, 1))
+        nc = C.c_int64()
+        rc = eng.L.abc_accept_end(eng.ctx, counts.ctypes.data_as(C.c_void_p),
+                                  idx.ctypes.data_as(C.c_void_p), eo.ctypes.data_as(C.c_void_p),
+                                  cap, C.byref(nc))
+        if rc == -5:  # capacity: counts are valid, the session stays open
+            cap = int(counts.sum())
+            continue
+        _lib.check(rc, eng.ctx, "accept_end")
+        break
+    res, o = [], 0
+    for g in range(G):
+        res.append((idx[o:o + counts[g]] - 1, eo[o:o + counts[g]]))
+        o += counts[g]
+    return res

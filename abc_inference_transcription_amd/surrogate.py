@@ -44,6 +44,12 @@ class Surrogate:
         self.mean = torch.zeros(n_stats, dtype=torch.float32, device=self.dev)
         self.sd = torch.ones(n_stats, dtype=torch.float32, device=self.dev)
 
+    def bind_stream(self, torch_stream):
+        """Launch every later call on this (non-null) torch stream."""
+        self._chk(self.L.abc_mlp_set_stream(self.h, C.c_void_p(torch_stream.cuda_stream)),
+                  "set_stream")
+        self.stream = torch_stream
+
     def _chk(self, rc, what):
         if rc != 0:
             m = self.L.abc_mlp_last_error(self.h)

@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cells", type=int, default=2500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sur-train", type=int, default=12000, help="training particles per model")
+    ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
+    ap.add_argument("--sur-steps", type=int, default=3000, help="Adam steps (batch 8192)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (profiler byte-counter scale)")
     return ap.parse_args()
@@ -191,8 +195,28 @@ def main():
 
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(design, models, P, total_steps / n_particles, a.cpu_seconds)
+    if not a.no_surrogate:
+        out["surrogate"] = surrogate_leg(eng, stream, a)
     print(json.dumps(out), flush=True)
     eng.close()
+
+
+def surrogate_leg(eng, stream, a):
+    """Accepted pairs/s of the surrogate-screened path against the exhaustive
+    plain path on the same held-out particles (top-k per gene), with recall."""
+    from abc_inference_transcription_amd import screened_abc as S
+    from abc_inference_transcription_amd.surrogate import Surrogate
+    sur = Surrogate(2 * eng.nb * eng.J, seed=0, stream=stream)
+    res = S.run(eng, sur, G=a.genes, n_train=a.sur_train, n_eval=a.sur_eval, k=0,
+                accept_rate=4e-5, z=[4.0, 3.0, 2.5, 2.0], train_steps=a.sur_steps, batch=8192,
+                log=lambda *x: print(*x, file=sys.stderr, flush=True))
+    sur.close()
+    keep = ("plain_accepted_per_s", "screened_accepted_per_s", "speedup_accepted_per_s",
+            "recall", "ssa_fraction_avoided", "ssa_work_fraction_avoided", "plain_pairs",
+            "eval_particles", "train_particles", "heldout_nll", "heldout_nll_of_mean_predictor",
+            "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
+            "train_loss_last", "train_s", "genes", "z", "eps", "accept_rate_target", "by_z")
+    return {k: res[k] for k in keep}
 
 
 def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):

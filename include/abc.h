@@ -137,6 +137,14 @@ int32_t abc_accept_add_dev(abc_ctx* ctx, int64_t n, const double* stats_dev,
 int32_t abc_accept_end(abc_ctx* ctx, int64_t* counts, int64_t* idx, double* err_out,
                        int64_t cap, int64_t* n_candidates);
 
+/* Batches whose rows are an arbitrary subset of particles (e.g. those a
+ * screen let through): row i is global particle pidx_host[i]. */
+int32_t abc_accept_add_idx_dev(abc_ctx* ctx, int64_t n, const double* stats_dev,
+                               const int64_t* pidx_host);
+/* Lower the per-gene thresholds of the open session to thr[g] (G values):
+ * known upper bounds of the k-th error, or the thresholds a screen tests. */
+int32_t abc_accept_set_thresholds(abc_ctx* ctx, const double* thr);
+
 /* The screening GEMM alone: approximate error and its proven bound for every
  * (particle, gene) pair (n x G col-major, float).  For validation. */
 int32_t abc_screen_errors(abc_ctx* ctx, int64_t n, int32_t K, const double* stats, int64_t G,
@@ -151,6 +159,12 @@ int32_t abc_sample_prior_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t
 int32_t abc_simulate_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first, int64_t n,
                          const double* theta_dev, double* stats_dev, double* moments_dev,
                          double* superset_dev, int64_t* steps_dev);
+/* Simulation of an arbitrary subset of particles: row i uses the random
+ * streams of global particle pidx_dev[i] (device int64, < 2^32), so its
+ * results equal those of the same particle in a contiguous run. */
+int32_t abc_simulate_idx_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t n,
+                             const int64_t* pidx_dev, const double* theta_dev, double* stats_dev,
+                             double* moments_dev, double* superset_dev, int64_t* steps_dev);
 int32_t abc_errors_dev(abc_ctx* ctx, int64_t n, int32_t K, const double* stats_dev, int64_t G,
                        const double* data_dev, const double* se_dev, const int32_t* parts,
                        int32_t n_parts, double clamp, double* err_dev);
