@@ -8,6 +8,9 @@ Writes tests/golden/*.npz -- inputs and expected outputs only (data, no code):
                   output; 11 conditions x 5 ages x 5 moments).
   summary_stats.npz : the first G genes of the 14 data/summary_stats matrices.
   capture.npz   : data/capture_efficiencies.txt (all 5,422 cells).
+  julia_floats.txt : float tokens exactly as Julia's writedlm printed them in
+                  the reference's data files (every exponent-form token up to
+                  4,000, 4,000 plain ones, every one >= 1e5), one per line.
 """
 import os
 
@@ -42,5 +45,26 @@ def main(rows=12, genes=20):
                         betas=np.loadtxt(R + "capture_efficiencies.txt"))
 
 
+def julia_floats(n_exp=4000, n_plain=4000):
+    import glob
+    files = sorted(glob.glob(R + "summary_stats/*.txt") +
+                   glob.glob(R + "recovered_statistics/*/*/*.txt") +
+                   glob.glob(R + "posterior_estimates/*.txt"))
+    toks = set()
+    for f in files:
+        for line in open(f):
+            toks.update(line.split())
+    toks = sorted(toks)
+    rng = np.random.default_rng(0)
+    expo = [t for t in toks if "e" in t]
+    plain = [t for t in toks if "e" not in t]
+    big = [t for t in plain if abs(float(t)) >= 1e5]
+    pick = lambda L, k: [L[i] for i in sorted(rng.choice(len(L), min(k, len(L)), replace=False))]
+    out = sorted(set(pick(expo, n_exp) + pick(plain, n_plain) + big))
+    with open(os.path.join(OUT, "julia_floats.txt"), "w") as fh:
+        fh.write("\n".join(out) + "\n")
+
+
 if __name__ == "__main__":
     main()
+    julia_floats()

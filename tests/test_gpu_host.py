@@ -1,0 +1,43 @@
+"""The host mirror driven by libabc.so writes the same bytes as when driven by
+the CPU oracle: statistics files, error matrix and accepted-particle lists."""
+import os
+
+import numpy as np
+import pytest
+
+from abc_inference_transcription_amd import host
+from abc_inference_transcription_amd.design import MODEL_NAMES, reference_design
+
+pytestmark = pytest.mark.gpu
+
+
+def _tree(root):
+    out = {}
+    for d, _, fs in os.walk(root):
+        for f in fs:
+            p = os.path.join(d, f)
+            out[os.path.relpath(p, root)] = open(p, "rb").read()
+    return out
+
+
+def test_host_files_match_oracle_bytes(oracle, engine_factory, golden_summary, tmp_path):
+    from oracle_engine import OracleEngine
+    design = reference_design(cells=275)
+    gpu, cpu = engine_factory(design), OracleEngine(oracle, design)
+    S = {k: golden_summary[k] for k in host.SUMMARY_FILES}
+    stats = {}
+    for tag, eng in (("gpu", gpu), ("cpu", cpu)):
+        root = str(tmp_path / tag)
+        stats[tag] = {}
+        for m in range(1, 6):
+            print(f"[host] {tag} model {m}", flush=True)
+            _, st = host.abc_simulation(eng, m, 7, 2, root=os.path.join(root, "sim"), batch=3)
+            stats[tag][MODEL_NAMES[m - 1]] = st
+        host.compute_errors(eng, "kon", S, path=os.path.join(root, "sim") + "/", ext="_2.txt",
+                            out=os.path.join(root, "errors", "error_kon.txt"))
+        host.accepted_particles(eng, S, stats[tag], eps=4.8, out_dir=os.path.join(root, "post"))
+    a, b = _tree(str(tmp_path / "gpu")), _tree(str(tmp_path / "cpu"))
+    assert sorted(a) == sorted(b)
+    for k in a:
+        assert a[k] == b[k], k
+    assert len(a) == 5 * 7 + 1 + 5
