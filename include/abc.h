@@ -217,8 +217,11 @@ int32_t abc_features_dev(int32_t model, int64_t n, const double* theta_dev, int3
 int32_t abc_targets_dev(int64_t n, const double* stats_dev, int32_t K, const float* mean_dev,
                         const float* sd_dev, float* t_dev, void* hip_stream);
 /* Screen n particles against the genes and thresholds of the open acceptance
- * session: pass[i] = 1 when some gene's interval lower bound (z-sigma
- * surrogate intervals) is within (1 + margin) of its threshold. */
+ * session.  The network's output y = forward(x) gives each statistic a
+ * Gaussian in the standardised asinh space; mapped back (7-point
+ * Gauss-Hermite) it yields the mean and variance of every gene's error.
+ * pass[i] = 1 when for some gene E[err] - z*sd[err] <= thr*(1 + margin), when
+ * a threshold reaches the clamp, or when a prediction is not finite. */
 int32_t abc_screen_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float* x_dev,
                        const float* mean_dev, const float* sd_dev, double z, double margin,
                        uint8_t* pass_dev, int64_t* n_pass);
