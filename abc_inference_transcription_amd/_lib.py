@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(HERE, "libabc.so")
 
 ABC_OK = 0
 ERRORS = {-1: "invalid argument", -2: "HIP runtime error", -3: "device allocation failed",
-          -4: "model index outside 1..5", -5: "output buffer too small"}
+          -4: "model index outside 1..5", -5: "output buffer too small",
+          -6: "not allowed in this state"}
 
 class AbcDesign(C.Structure):
     _fields_ = [
@@ -67,6 +68,11 @@ def _declare(L):
         "abc_ssa_time": (I32, [V, I32, V, V]),
         "abc_set_ssa_blocks": (I32, [V, I32]),
         "abc_get_ssa_blocks": (I32, [V]),
+        "abc_autotune_ssa": (I32, [V, I32, V, U64, I64, I64, V, V]),
+        "abc_graph_begin": (I32, [V]),
+        "abc_graph_end": (I32, [V, C.POINTER(V)]),
+        "abc_graph_launch": (I32, [V, V]),
+        "abc_graph_destroy": (I32, [V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,12 +1,12 @@
 // abc_capi.hip -- host side of libabc.so: the C ABI declared in include/abc.h.
 //
 // Orchestrates, per call, on one HIP stream:
-//   prior_kernel -> rates_kernel -> radix sort of particles by expected cost
-//   (heaviest first) -> ssa_kernel (persistent, lane-level work stealing) ->
-//   stats_kernel -> error / acceptance kernels.
-// Device buffers are owned by the context and grow on demand.
-#include <hipcub/hipcub.hpp>
-
+//   prior_kernel -> rates_kernel -> bucket order of particles by expected cost
+//   (heaviest first) -> item offsets -> ssa_kernel (persistent, lane-level
+//   work stealing) -> stats_kernel -> error / acceptance kernels.
+// Device buffers are owned by the context and grow on demand.  The *_dev
+// sequences use kernels only (no memset, no host wait), so they can be
+// captured into HIP graphs (abc_graph_begin / abc_graph_end).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -52,7 +52,7 @@ struct abc_ctx {
   DesignDev* dd = nullptr;
   double* d_betas = nullptr;
   int64_t* d_goff = nullptr;
-  DevBuf theta, rates, cost, cost_sorted, ident, order, sort_tmp, items, off, scan_tmp, cu, cl,
+  DevBuf theta, rates, cost, hist, order, bsum, off, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
       pflag, bh, bl, cg, gflag, thr, wf, sy;
   AcceptState acc;
@@ -64,14 +64,36 @@ struct abc_ctx {
   int ring_used = 0;
   double ssa_ms_acc = 0.0;
   int64_t ssa_launches = 0;
+  // stream capture (abc_graph_begin .. abc_graph_end)
+  bool capturing = false;
+  bool capture_failed = false;
   std::string msg;
+};
+
+struct abc_graph {
+  hipGraphExec_t exec = nullptr;
 };
 
 namespace {
 
 int fail(abc_ctx* c, int code, const std::string& m) {
-  if (c) c->msg = m;
+  if (c) {
+    c->msg = m;
+    if (c->capturing) c->capture_failed = true;  // abc_graph_end reports and discards
+  }
   return code;
+}
+
+// Calls that wait for the device or copy host memory cannot be captured.
+#define NO_CAPTURE(ctx)                                                             \
+  do {                                                                             \
+    if ((ctx)->capturing)                                                          \
+      return fail(ctx, ABC_E_STATE, "only *_dev entry points can be captured");    \
+  } while (0)
+
+// Timing events are left out of captured graphs.
+hipError_t record(abc_ctx* c, hipEvent_t e) {
+  return c->capturing ? hipSuccess : hipEventRecord(e, c->stream);
 }
 
 #define HIPCHK(ctx, expr)                                                          \
@@ -84,6 +106,10 @@ int fail(abc_ctx* c, int code, const std::string& m) {
 int ensure(abc_ctx* c, DevBuf& b, size_t bytes) {
   if (bytes == 0) bytes = 16;
   if (b.bytes >= bytes) return ABC_OK;
+  if (c->capturing)
+    return fail(c, ABC_E_STATE,
+                "a device buffer would grow during graph capture: run the same call once "
+                "before capturing it");
   if (b.p) {
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return fail(c, ABC_E_HIP, hipGetErrorString(e));
@@ -118,40 +144,34 @@ int check_range(abc_ctx* c, int model, int64_t first, int64_t n) {
 
 int grid1(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// prior (optional) + rates + cost ordering; theta_dev must hold n x n_par
+// rates + heaviest-first queue order + work-item offsets; theta_dev: n x n_par
 int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
   int rc;
+  const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
   if ((rc = ensure(c, c->rates, sizeof(PRates) * n))) return rc;
   if ((rc = ensure(c, c->cost, sizeof(float) * n))) return rc;
-  if ((rc = ensure(c, c->cost_sorted, sizeof(float) * n))) return rc;
-  if ((rc = ensure(c, c->ident, sizeof(int32_t) * n))) return rc;
+  if ((rc = ensure(c, c->hist, sizeof(unsigned) * 2 * ABC_NBUCKET))) return rc;
   if ((rc = ensure(c, c->order, sizeof(int32_t) * n))) return rc;
-  hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
-                     theta_dev, P<PRates>(c->rates), P<float>(c->cost), P<int32_t>(c->ident));
-  HIPCHK(c, hipGetLastError());
-  size_t tmp = 0;
-  HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(
-                nullptr, tmp, P<float>(c->cost), P<float>(c->cost_sorted), P<int32_t>(c->ident),
-                P<int32_t>(c->order), (int)n, 0, 32, c->stream));
-  if ((rc = ensure(c, c->sort_tmp, tmp))) return rc;
-  tmp = c->sort_tmp.bytes;
-  HIPCHK(c, hipcub::DeviceRadixSort::SortPairsDescending(
-                c->sort_tmp.p, tmp, P<float>(c->cost), P<float>(c->cost_sorted),
-                P<int32_t>(c->ident), P<int32_t>(c->order), (int)n, 0, 32, c->stream));
-  // work-item offsets in queue order: off[0] = 0, off[r+1] = off[r] + C * M
-  if ((rc = ensure(c, c->items, sizeof(int64_t) * n))) return rc;
+  if ((rc = ensure(c, c->bsum, sizeof(int64_t) * nblk))) return rc;
   if ((rc = ensure(c, c->off, sizeof(int64_t) * (n + 1)))) return rc;
-  hipLaunchKernelGGL(items_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd,
-                     P<PRates>(c->rates), P<int32_t>(c->order), n, P<int64_t>(c->items),
-                     P<int64_t>(c->off));
+  unsigned* hist = P<unsigned>(c->hist);
+  unsigned* cursor = hist + ABC_NBUCKET;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, c->stream,
+                     P<unsigned long long>(c->hist), (int64_t)(ABC_NBUCKET / 2));
+  hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
+                     theta_dev, P<PRates>(c->rates), P<float>(c->cost));
+  hipLaunchKernelGGL(bucket_hist_kernel, dim3(std::min(grid1(n, 256), 1024)), dim3(256), 0,
+                     c->stream, P<float>(c->cost), n, hist);
+  hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(ABC_NBUCKET), 0, c->stream, hist, cursor);
+  hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
+                     P<float>(c->cost), n, cursor, P<int32_t>(c->order));
+  hipLaunchKernelGGL(items_scan_kernel, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->dd,
+                     P<PRates>(c->rates), P<int32_t>(c->order), n, P<int64_t>(c->off),
+                     P<int64_t>(c->bsum));
+  hipLaunchKernelGGL(bsum_scan_kernel, dim3(1), dim3(256), 0, c->stream, P<int64_t>(c->bsum), nblk);
+  hipLaunchKernelGGL(add_bsum_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
+                     P<int64_t>(c->off), n, P<int64_t>(c->bsum));
   HIPCHK(c, hipGetLastError());
-  size_t stmp = 0;
-  HIPCHK(c, hipcub::DeviceScan::InclusiveSum(nullptr, stmp, P<int64_t>(c->items),
-                                             P<int64_t>(c->off) + 1, (int)n, c->stream));
-  if ((rc = ensure(c, c->scan_tmp, stmp))) return rc;
-  stmp = c->scan_tmp.bytes;
-  HIPCHK(c, hipcub::DeviceScan::InclusiveSum(c->scan_tmp.p, stmp, P<int64_t>(c->items),
-                                             P<int64_t>(c->off) + 1, (int)n, c->stream));
   return ABC_OK;
 }
 
@@ -174,8 +194,9 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   int rc;
   if ((rc = check_range(c, model, first, n))) return rc;
   if (n == 0) return ABC_OK;
+  if (n > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
   const size_t cells = (size_t)n * (size_t)c->hd.C;
-  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(c, record(c, c->ev[0]));
   if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
   const int64_t cell_words = (int64_t)((cells + 1) / 2);  // u32 counts cleared as u64 words
   if ((rc = ensure(c, c->cu, sizeof(uint64_t) * cell_words))) return rc;
@@ -199,12 +220,15 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
                      P<unsigned long long>(c->queue), (int64_t)8);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (c->ring_used == abc_ctx::kRing) {
-    if ((rc = drain_ring(c))) return rc;
+  HIPCHK(c, record(c, c->ev[1]));
+  hipEvent_t* pair = nullptr;
+  if (!c->capturing) {
+    if (c->ring_used == abc_ctx::kRing) {
+      if ((rc = drain_ring(c))) return rc;
+    }
+    pair = c->ring[c->ring_used++];
+    HIPCHK(c, hipEventRecord(pair[0], c->stream));
   }
-  hipEvent_t* pair = c->ring[c->ring_used++];
-  HIPCHK(c, hipEventRecord(pair[0], c->stream));
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
@@ -213,14 +237,14 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps,
                      P<unsigned long long>(c->react), pidx_dev);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(pair[1], c->stream));
-  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  if (pair) HIPCHK(c, hipEventRecord(pair[1], c->stream));
+  HIPCHK(c, record(c, c->ev[2]));
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
                      moments_dev, superset_dev);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-  c->timed[0] = c->timed[1] = c->timed[2] = true;
+  HIPCHK(c, record(c, c->ev[3]));
+  c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
   return ABC_OK;
 }
 
@@ -365,8 +389,7 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
 int32_t abc_destroy(abc_ctx* c) {
   if (!c) return ABC_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
-  DevBuf* bufs[] = {&c->theta, &c->rates, &c->cost, &c->cost_sorted, &c->ident, &c->order,
-                    &c->sort_tmp, &c->items, &c->off, &c->scan_tmp,
+  DevBuf* bufs[] = {&c->theta, &c->rates, &c->cost, &c->hist, &c->order, &c->bsum, &c->off,
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
@@ -397,6 +420,7 @@ int32_t abc_n_stats(const abc_ctx* c) { return c ? 4 * c->hd.nb + 3 * c->hd.J : 
 
 int32_t abc_set_stream(abc_ctx* c, void* s) {
   if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
   hipStreamSynchronize(c->stream);
   if (c->own_stream) hipStreamDestroy(c->stream);
   if (s) {
@@ -411,6 +435,7 @@ int32_t abc_set_stream(abc_ctx* c, void* s) {
 
 int32_t abc_synchronize(abc_ctx* c) {
   if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return ABC_OK;
 }
@@ -441,6 +466,7 @@ int32_t abc_sample_prior_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t f
 int32_t abc_sample_prior(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
                          double* theta) {
   if (!c || (!theta && n > 0)) return ABC_E_ARG;
+  NO_CAPTURE(c);
   int rc = check_range(c, model, first, n);
   if (rc) return rc;
   if (n == 0) return ABC_OK;
@@ -472,6 +498,7 @@ int32_t abc_simulate(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, in
                      const double* theta, double* stats, double* moments, double* superset,
                      int64_t* steps, uint32_t* cell_u, uint32_t* cell_l, uint8_t* cell_bin) {
   if (!c || (n > 0 && (!theta || !stats))) return ABC_E_ARG;
+  NO_CAPTURE(c);
   int rc = check_range(c, model, first, n);
   if (rc) return rc;
   if (n == 0) return ABC_OK;
@@ -511,7 +538,7 @@ int32_t abc_errors_dev(abc_ctx* c, int64_t n, int32_t K, const double* stats_dev
   if (rc) return rc;
   if (n == 0 || G == 0) return ABC_OK;
   if (G > 65535 * 64) return fail(c, ABC_E_ARG, "too many genes for one call");
-  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  HIPCHK(c, record(c, c->ev[3]));
   // grid.y is limited to 65535: split genes into slices
   for (int64_t g0 = 0; g0 < G; g0 += 65535) {
     const int64_t gn = std::min<int64_t>(65535, G - g0);
@@ -519,8 +546,8 @@ int32_t abc_errors_dev(abc_ctx* c, int64_t n, int32_t K, const double* stats_dev
                        stats_dev, n, data_dev, se_dev, G, g0, E, err_dev + n * g0);
   }
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
-  c->timed[3] = true;
+  HIPCHK(c, record(c, c->ev[4]));
+  c->timed[3] = !c->capturing;
   return ABC_OK;
 }
 
@@ -528,6 +555,7 @@ int32_t abc_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_
                    const double* data, const double* se, const int32_t* parts, int32_t n_parts,
                    double clamp, double* err) {
   if (!c || n < 0 || G < 0 || (n > 0 && G > 0 && (!stats || !data || !se || !err))) return ABC_E_ARG;
+  NO_CAPTURE(c);
   ErrSpec E;
   int rc = make_spec(c, K, parts, n_parts, clamp, &E);
   if (rc) return rc;
@@ -649,6 +677,7 @@ int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, c
                          const int32_t* parts, int32_t n_parts, double clamp, int32_t mode,
                          double eps, int64_t k, double margin_rel) {
   if (!c || G < 0 || (G > 0 && (!data || !se))) return ABC_E_ARG;
+  NO_CAPTURE(c);
   if (mode != 0 && mode != 1) return fail(c, ABC_E_ARG, "mode must be 0 (fixed eps) or 1 (top-k)");
   if (mode == 1 && k < 0) return fail(c, ABC_E_ARG, "k must be >= 0");
   if (!(margin_rel >= 0.0)) return fail(c, ABC_E_ARG, "margin_rel must be >= 0");
@@ -694,6 +723,7 @@ int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
                        uint8_t* pass_dev, int64_t* n_pass) {
   if (!c || !mlp || n < 0 || (n > 0 && (!x_dev || !mean_dev || !sd_dev || !pass_dev)))
     return ABC_E_ARG;
+  NO_CAPTURE(c);
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (abc_mlp_n_stats(mlp) != A.E.K) return fail(c, ABC_E_ARG, "surrogate and genes disagree on K");
@@ -722,6 +752,7 @@ int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
 
 int32_t abc_accept_add_dev(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index) {
   if (!c || n < 0 || (n > 0 && !stats_dev) || first_index < 0) return ABC_E_ARG;
+  NO_CAPTURE(c);
   if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (n == 0 || c->acc.G == 0) return ABC_OK;
   return screen_batch(c, n, stats_dev, first_index);
@@ -730,6 +761,7 @@ int32_t abc_accept_add_dev(abc_ctx* c, int64_t n, const double* stats_dev, int64
 int32_t abc_accept_add_idx_dev(abc_ctx* c, int64_t n, const double* stats_dev,
                                const int64_t* pidx_host) {
   if (!c || n < 0 || (n > 0 && (!stats_dev || !pidx_host))) return ABC_E_ARG;
+  NO_CAPTURE(c);
   if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (n == 0 || c->acc.G == 0) return ABC_OK;
   return screen_batch(c, n, stats_dev, 0, pidx_host);
@@ -737,6 +769,7 @@ int32_t abc_accept_add_idx_dev(abc_ctx* c, int64_t n, const double* stats_dev,
 
 int32_t abc_accept_set_thresholds(abc_ctx* c, const double* thr) {
   if (!c || !thr) return ABC_E_ARG;
+  NO_CAPTURE(c);
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   for (int64_t g = 0; g < A.G; ++g) A.thr_host[g] = std::min(A.thr_host[g], thr[g]);
@@ -749,6 +782,7 @@ int32_t abc_accept_set_thresholds(abc_ctx* c, const double* thr) {
 
 int32_t abc_accept_add(abc_ctx* c, int64_t n, const double* stats, int64_t first_index) {
   if (!c || n < 0 || (n > 0 && !stats) || first_index < 0) return ABC_E_ARG;
+  NO_CAPTURE(c);
   if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (n == 0 || c->acc.G == 0) return ABC_OK;
   int rc;
@@ -761,6 +795,7 @@ int32_t abc_accept_add(abc_ctx* c, int64_t n, const double* stats, int64_t first
 int32_t abc_accept_end(abc_ctx* c, int64_t* counts, int64_t* idx, double* err_out, int64_t cap,
                        int64_t* n_candidates) {
   if (!c || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+  NO_CAPTURE(c);
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (A.G > 0 && !counts) return ABC_E_ARG;
@@ -789,6 +824,7 @@ int32_t abc_accept(abc_ctx* c, int64_t n, int32_t K, const double* stats, int64_
                    double clamp, int32_t mode, double eps, int64_t k, int64_t first_index,
                    int64_t* counts, int64_t* idx, double* err_out, int64_t cap) {
   if (!c || n < 0 || G < 0 || (G > 0 && !counts) || cap < 0 || (cap > 0 && !idx)) return ABC_E_ARG;
+  NO_CAPTURE(c);
   int rc = abc_accept_begin(c, K, G, data, se, parts, n_parts, clamp, mode, eps, k, 0.02);
   if (rc) return rc;
   if ((rc = abc_accept_add(c, n, stats, first_index))) { c->acc.active = false; return rc; }
@@ -801,6 +837,7 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
                           const double* data, const double* se, const int32_t* parts,
                           int32_t n_parts, double clamp, float* approx, float* bound) {
   if (!c || n <= 0 || G <= 0 || !stats || !approx || !bound) return ABC_E_ARG;
+  NO_CAPTURE(c);
   int rc = abc_accept_begin(c, K, G, data, se, parts, n_parts, clamp, 0, 0.0, 0, 0.0);
   if (rc) return rc;
   AcceptState& A = c->acc;
@@ -831,6 +868,7 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
 
 int32_t abc_ssa_time(abc_ctx* c, int32_t reset, double* ms_total, int64_t* launches) {
   if (!c || !ms_total || !launches) return ABC_E_ARG;
+  NO_CAPTURE(c);
   int rc = drain_ring(c);
   if (rc) return rc;
   *ms_total = c->ssa_ms_acc;
@@ -844,6 +882,7 @@ int32_t abc_ssa_time(abc_ctx* c, int32_t reset, double* ms_total, int64_t* launc
 
 int32_t abc_last_timing(abc_ctx* c, double* ms) {
   if (!c || !ms) return ABC_E_ARG;
+  NO_CAPTURE(c);
   for (int q = 0; q < 4; ++q) {
     ms[q] = 0.0;
     if (!c->timed[q]) continue;
@@ -852,6 +891,132 @@ int32_t abc_last_timing(abc_ctx* c, double* ms) {
     HIPCHK(c, hipEventElapsedTime(&v, c->ev[q], c->ev[q + 1]));
     ms[q] = v;
   }
+  return ABC_OK;
+}
+
+// ------------------------------------------------------------ HIP graphs
+
+int32_t abc_graph_begin(abc_ctx* c) {
+  if (!c) return ABC_E_ARG;
+  if (c->capturing) return fail(c, ABC_E_STATE, "a capture is already open");
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  c->capturing = true;
+  c->capture_failed = false;
+  return ABC_OK;
+}
+
+int32_t abc_graph_end(abc_ctx* c, abc_graph** g) {
+  if (!c || !g) return ABC_E_ARG;
+  if (!c->capturing) return fail(c, ABC_E_STATE, "no capture is open");
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  c->capturing = false;
+  if (c->capture_failed || e != hipSuccess || !graph) {
+    if (graph) hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    const bool own = c->capture_failed;
+    c->capture_failed = false;
+    return own ? ABC_E_STATE
+               : fail(c, ABC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  if (*g && (*g)->exec) {
+    // same topology with new arguments (other particles, seeds or buffers):
+    // update the executable graph in place
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res = hipGraphExecUpdateError;
+    if (hipGraphExecUpdate((*g)->exec, graph, &bad, &res) == hipSuccess &&
+        res == hipGraphExecUpdateSuccess) {
+      hipGraphDestroy(graph);
+      return ABC_OK;
+    }
+    (void)hipGetLastError();
+    hipGraphExecDestroy((*g)->exec);
+    (*g)->exec = nullptr;
+  }
+  hipGraphExec_t exec = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(c, ABC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  if (!*g) *g = new abc_graph;
+  (*g)->exec = exec;
+  return ABC_OK;
+}
+
+int32_t abc_graph_launch(abc_ctx* c, abc_graph* g) {
+  if (!c || !g || !g->exec) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_graph_destroy(abc_graph* g) {
+  if (!g) return ABC_OK;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  delete g;
+  return ABC_OK;
+}
+
+// -------------------------------------------------------------- autotuner
+
+int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, uint64_t seed,
+                         int64_t first, int64_t n, int32_t* best_blocks, double* best_ms) {
+  if (!c || n_models <= 0 || n_models > 16 || !models || n <= 0) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  int rc;
+  for (int q = 0; q < n_models; ++q)
+    if ((rc = check_range(c, models[q], first, n))) return rc;
+  const int nb = c->hd.nb, K = 4 * nb + 3 * c->hd.J;
+  int np_max = 0;
+  for (int q = 0; q < n_models; ++q) np_max = std::max(np_max, abc_layout_n_params(models[q], nb));
+  if ((rc = ensure(c, c->theta, sizeof(double) * (size_t)n * np_max * n_models))) return rc;
+  if ((rc = ensure(c, c->stats, sizeof(double) * (size_t)n * K))) return rc;
+  double* theta = P<double>(c->theta);
+  for (int q = 0; q < n_models; ++q)
+    if ((rc = abc_sample_prior_dev(c, models[q], seed, first, n, theta + (size_t)q * n * np_max)))
+      return rc;
+  // candidates: 1..16 resident workgroups of SSA_WG lanes per CU, up to occupancy
+  const int per_cu = std::max(1, c->ssa_blocks_default / std::max(1, c->num_cu));
+  std::vector<int> cand;
+  for (int w : {1, 2, 3, 4, 6, 8, 12, 16})
+    if (w <= per_cu) cand.push_back(w * c->num_cu);
+  if (std::find(cand.begin(), cand.end(), c->ssa_blocks_default) == cand.end())
+    cand.push_back(c->ssa_blocks_default);
+  // keep the caller's running SSA time out of the trials
+  if ((rc = drain_ring(c))) return rc;
+  const double acc0 = c->ssa_ms_acc;
+  const int64_t l0 = c->ssa_launches;
+  const int saved = c->ssa_blocks;
+  auto run = [&](int blocks, double* ms) -> int {
+    c->ssa_blocks = blocks;
+    const double a = c->ssa_ms_acc;
+    for (int q = 0; q < n_models; ++q) {
+      const int r = simulate_dev(c, models[q], seed, first, n, theta + (size_t)q * n * np_max,
+                                 P<double>(c->stats), nullptr, nullptr, nullptr);
+      if (r) return r;
+    }
+    const int r = drain_ring(c);
+    *ms = c->ssa_ms_acc - a;
+    return r;
+  };
+  double ms = 0.0, best = 1e300;
+  int bestb = c->ssa_blocks_default;
+  rc = run(c->ssa_blocks_default, &ms);  // warm-up (code load, buffer growth)
+  for (size_t q = 0; rc == ABC_OK && q < cand.size(); ++q) {
+    rc = run(cand[q], &ms);
+    if (rc == ABC_OK && ms < best) {
+      best = ms;
+      bestb = cand[q];
+    }
+  }
+  c->ssa_ms_acc = acc0;
+  c->ssa_launches = l0;
+  if (rc) {
+    c->ssa_blocks = saved;
+    return rc;
+  }
+  c->ssa_blocks = bestb;
+  if (best_blocks) *best_blocks = bestb;
+  if (best_ms) *best_ms = best;
   return ABC_OK;
 }
 

@@ -11,6 +11,8 @@
 #define ABC_MAX_STATS 256
 #define ABC_MAX_PARTS 16
 #define SSA_WG 256
+#define ABC_NBUCKET 256      // cost buckets of the queue order
+#define ABC_SCAN_BLOCK 1024  // ranks per block of the item scan
 
 // Experimental design in device memory (read through uniform scalar loads).
 struct DesignDev {
@@ -61,9 +63,15 @@ int abc_layout_n_params(int model, int nb);
 __global__ void prior_kernel(const DesignDev* D, int model, uint32_t k0, uint32_t k1,
                              uint32_t first, int64_t n, double* theta);
 __global__ void rates_kernel(const DesignDev* D, int model, int64_t n, const double* theta,
-                             PRates* R, float* cost, int32_t* ident);
-__global__ void items_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
-                             int64_t* items, int64_t* off0);
+                             PRates* R, float* cost);
+__global__ void bucket_hist_kernel(const float* cost, int64_t n, unsigned* hist);
+__global__ void bucket_scan_kernel(const unsigned* hist, unsigned* cursor);
+__global__ void bucket_scatter_kernel(const float* cost, int64_t n, unsigned* cursor,
+                                      int32_t* order);
+__global__ void items_scan_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
+                                  int64_t n, int64_t* off, int64_t* bsum);
+__global__ void bsum_scan_kernel(int64_t* bsum, int64_t nblk);
+__global__ void add_bsum_kernel(int64_t* off, int64_t n, const int64_t* bsum);
 __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
                            const int64_t* off, int64_t n,
                            uint32_t first, uint32_t k0, uint32_t k1, const double* betas,

@@ -5,6 +5,7 @@
 //   rates_kernel ...... log10 parameters -> per-particle rates, thinning
 //                       bounds, periodic gene law, burn-in length, cost key
 //                       (scripts/model.jl:1-43)
+//   bucket_* / *scan .. heaviest-first queue order and work-item offsets
 //   ssa_kernel ........ persistent Gillespie simulation, one cell per lane,
 //                       lanes refill from a global cell queue the moment their
 //                       cell ends (lane-level work stealing); replaces the
@@ -109,8 +110,7 @@ __global__ __launch_bounds__(256) void prior_kernel(const DesignDev* __restrict_
 __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict__ D, int model,
                                                     int64_t n, const double* __restrict__ theta,
                                                     PRates* __restrict__ R,
-                                                    float* __restrict__ cost,
-                                                    int32_t* __restrict__ ident) {
+                                                    float* __restrict__ cost) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int nb = D->nb;
@@ -180,19 +180,121 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
   }
   const double per_hour = (sw + (births + loss) / (double)M) / (double)nb;
   cost[i] = (float)(per_hour * T * ((double)B + 0.5) + 4.0);
-  ident[i] = (int32_t)i;
 }
 
-// Work items of particle rank r (heaviest first): cells x sub-lanes.
-__global__ __launch_bounds__(256) void items_kernel(const DesignDev* __restrict__ D,
-                                                    const PRates* __restrict__ R,
-                                                    const int32_t* __restrict__ order, int64_t n,
-                                                    int64_t* __restrict__ items,
-                                                    int64_t* __restrict__ off0) {
+// -------------------------------------------------------------- queue order
+// Particles enter the work queue heaviest first (longest-processing-time
+// order) through a bucket sort of the expected cost on its float exponent and
+// top two mantissa bits (quarter-octave buckets).  Order inside a bucket is
+// whatever the atomics give: every random stream is keyed by particle, cell
+// and sub-lane, never by queue position, so the order changes the schedule
+// and nothing else.  These are plain kernels (no library sort, no memset
+// node), so a simulate call can be captured in a HIP graph.
+__device__ __forceinline__ int cost_bucket(float c) {
+  const int k = (int)((__float_as_uint(c) >> 21) & 0x3FFu) - 127 * 4;  // 4*exponent + 2 bits
+  return k < 0 ? 0 : (k > ABC_NBUCKET - 1 ? ABC_NBUCKET - 1 : k);
+}
+
+__global__ __launch_bounds__(256) void bucket_hist_kernel(const float* __restrict__ cost, int64_t n,
+                                                          unsigned* __restrict__ hist) {
+  __shared__ unsigned h[ABC_NBUCKET];
+  for (int b = threadIdx.x; b < ABC_NBUCKET; b += blockDim.x) h[b] = 0u;
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&h[cost_bucket(cost[i])], 1u);
+  __syncthreads();
+  for (int b = threadIdx.x; b < ABC_NBUCKET; b += blockDim.x)
+    if (h[b]) atomicAdd(&hist[b], h[b]);
+}
+
+// cursor[b] = number of particles in heavier buckets (launch: 1 x ABC_NBUCKET)
+__global__ __launch_bounds__(ABC_NBUCKET) void bucket_scan_kernel(const unsigned* __restrict__ hist,
+                                                                  unsigned* __restrict__ cursor) {
+  __shared__ unsigned s[ABC_NBUCKET];
+  const int t = threadIdx.x;
+  const unsigned x = hist[ABC_NBUCKET - 1 - t];  // heaviest bucket first
+  s[t] = x;
+  __syncthreads();
+  for (int d = 1; d < ABC_NBUCKET; d <<= 1) {
+    const unsigned v = t >= d ? s[t - d] : 0u;
+    __syncthreads();
+    s[t] += v;
+    __syncthreads();
+  }
+  cursor[ABC_NBUCKET - 1 - t] = s[t] - x;
+}
+
+__global__ __launch_bounds__(256) void bucket_scatter_kernel(const float* __restrict__ cost,
+                                                             int64_t n, unsigned* __restrict__ cursor,
+                                                             int32_t* __restrict__ order) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned pos = atomicAdd(&cursor[cost_bucket(cost[i])], 1u);
+  order[pos] = (int32_t)i;
+}
+
+// Work items of particle rank r = cells x sub-lanes.  off[r + 1] = inclusive
+// sum over ranks <= r, computed per block of ABC_SCAN_BLOCK ranks here (block
+// totals to bsum), then offset by bsum_scan_kernel / add_bsum_kernel.
+__global__ __launch_bounds__(256) void items_scan_kernel(const DesignDev* __restrict__ D,
+                                                         const PRates* __restrict__ R,
+                                                         const int32_t* __restrict__ order,
+                                                         int64_t n, int64_t* __restrict__ off,
+                                                         int64_t* __restrict__ bsum) {
+  constexpr int PER = ABC_SCAN_BLOCK / 256;
+  __shared__ int64_t s[256];
+  const int t = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * ABC_SCAN_BLOCK + (int64_t)t * PER;
+  int64_t v[PER];
+  int64_t acc = 0;
+  for (int q = 0; q < PER; ++q) {
+    const int64_t r = base + q;
+    acc += r < n ? (int64_t)D->C * (int64_t)R[order[r]].M : 0;
+    v[q] = acc;
+  }
+  s[t] = acc;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int64_t x = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  const int64_t pre = t ? s[t - 1] : 0;
+  for (int q = 0; q < PER; ++q)
+    if (base + q < n) off[base + q + 1] = pre + v[q];
+  if (t == 255) bsum[blockIdx.x] = s[255];
+  if (blockIdx.x == 0 && t == 0) off[0] = 0;
+}
+
+// Exclusive scan of nblk block totals in place (launch: 1 x 256).
+__global__ __launch_bounds__(256) void bsum_scan_kernel(int64_t* __restrict__ bsum, int64_t nblk) {
+  __shared__ int64_t s[256];
+  const int t = threadIdx.x;
+  int64_t carry = 0;
+  for (int64_t b0 = 0; b0 < nblk; b0 += 256) {
+    const int64_t b = b0 + t;
+    const int64_t x = b < nblk ? bsum[b] : 0;
+    s[t] = x;
+    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int64_t y = t >= d ? s[t - d] : 0;
+      __syncthreads();
+      s[t] += y;
+      __syncthreads();
+    }
+    if (b < nblk) bsum[b] = carry + s[t] - x;
+    carry += s[255];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void add_bsum_kernel(int64_t* __restrict__ off, int64_t n,
+                                                       const int64_t* __restrict__ bsum) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r == 0) off0[0] = 0;
   if (r >= n) return;
-  items[r] = (int64_t)D->C * (int64_t)R[order[r]].M;
+  off[r + 1] += bsum[r / ABC_SCAN_BLOCK];
 }
 
 // ---------------------------------------------------------------- the SSA

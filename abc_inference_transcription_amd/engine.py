@@ -210,3 +210,31 @@ class Engine:
 
     def ssa_blocks(self) -> int:
         return self.L.abc_get_ssa_blocks(self.ctx)
+
+    def autotune(self, models=(1, 2, 3, 4, 5), n: int = 256, seed: int = 0, first: int = 0):
+        """Pick the fastest persistent-SSA grid for this workload (results do not
+        depend on it).  Returns (workgroups, ms of the SSA over the sample)."""
+        ms = np.array(list(models), np.int32)
+        best, t = C.c_int32(), C.c_double()
+        self._chk(self.L.abc_autotune_ssa(self.ctx, len(ms), _p(ms), seed, first, n,
+                                          C.byref(best), C.byref(t)), "autotune_ssa")
+        return best.value, t.value
+
+    def capture(self, fn, graph=None):
+        """Record the *_dev calls ``fn()`` makes on this engine's stream into a
+        HIP graph (a new one, or an in-place update of ``graph``)."""
+        self._chk(self.L.abc_graph_begin(self.ctx), "graph_begin")
+        try:
+            fn()
+        finally:
+            h = C.c_void_p(graph)
+            rc = self.L.abc_graph_end(self.ctx, C.byref(h))
+        self._chk(rc, "graph_end")
+        return h.value
+
+    def replay(self, graph):
+        self._chk(self.L.abc_graph_launch(self.ctx, C.c_void_p(graph)), "graph_launch")
+
+    def drop_graph(self, graph):
+        if graph:
+            self.L.abc_graph_destroy(C.c_void_p(graph))

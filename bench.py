@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-default SSA grid")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
     ap.add_argument("--sur-train", type=int, default=12000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
     ap.add_argument("--sur-steps", type=int, default=3000, help="Adam steps (batch 8192)")
@@ -149,6 +151,12 @@ def main():
         _lib.check(L.abc_ssa_time(eng.ctx, reset, C.byref(ms), C.byref(nl)), eng.ctx, "ssa_time")
         return ms.value, nl.value
 
+    tuned = None
+    if not a.no_autotune:
+        # grid size of the persistent SSA kernel, tuned on one step's worth of other particles
+        blocks, tms = eng.autotune(models=models, n=P, first=50_000_000)
+        tuned = {"ssa_blocks": blocks, "sample_ssa_ms": round(tms, 3)}
+        print(f"[bench] autotune: {blocks} workgroups", file=sys.stderr, flush=True)
     for w in range(a.warmup):
         step(10_000 + w, False)
     torch.cuda.synchronize()
@@ -168,6 +176,25 @@ def main():
     value = n_particles / (gpu_ms / 1e3)
     ssa_ms_total, launches = ssa_time(1)
     total_steps = int(steps_acc.item())
+
+    graph = None
+    if not a.no_graph:
+        # the same K steps, each captured into one HIP graph (updated in place
+        # for the next step's particles) and replayed
+        g = None
+        ev2, ev3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev2.record(stream)
+        for it in range(a.steps):
+            g = eng.capture(lambda: step(it, False), graph=g)
+            eng.replay(g)
+        ev3.record(stream)
+        torch.cuda.synchronize()
+        gms = ev2.elapsed_time(ev3)
+        eng.drop_graph(g)
+        ssa_time(1)
+        graph = {"value": round(n_particles / (gms / 1e3), 3), "ms_per_step": round(gms / a.steps, 3)}
+        print(f"[bench] graph replay: {graph}", file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel (SSA): algorithmic VALU lane-ops
     from abc_inference_transcription_amd.roofline import SSA_OPS_PER_STEP, VALU_PEAK_TOPS
@@ -191,6 +218,8 @@ def main():
                    "mean_steps_per_particle": total_steps / n_particles},
         "roofline": roof,
         "wall_s": round(wall, 3),
+        "ssa_launch": {"workgroups": eng.ssa_blocks(), "wg_lanes": 256, "autotune": tuned},
+        "graph_replay": graph,
     }
 
     if not a.no_cpu:

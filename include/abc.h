@@ -35,6 +35,7 @@ extern "C" {
 #define ABC_E_OOM (-3)      /* device allocation failed               */
 #define ABC_E_MODEL (-4)    /* model index outside 1..5               */
 #define ABC_E_CAPACITY (-5) /* output buffer too small (counts valid) */
+#define ABC_E_STATE (-6)    /* call not allowed now (e.g. inside a graph capture) */
 
 typedef struct abc_ctx abc_ctx;
 
@@ -230,6 +231,32 @@ int32_t abc_screen_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float* x_dev
  * default).  Set by the autotuner. */
 int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
 int32_t abc_get_ssa_blocks(const abc_ctx* ctx);
+
+/* Autotuner of the persistent SSA launch shape (no reference counterpart: the
+ * reference runs one particle at a time, scripts/abc_simulation.jl:88).  Draws
+ * n prior particles (from first) of each listed model, simulates them once per
+ * candidate grid (1, 2, 3, 4, 6, 8, 12, 16 workgroups per CU, up to
+ * occupancy), times ssa_kernel with HIP events and keeps the fastest grid for
+ * later calls.  Outputs never depend on the grid, so tuning changes speed
+ * only.  best_blocks / best_ms (optional) receive the choice and its time. */
+int32_t abc_autotune_ssa(abc_ctx* ctx, int32_t n_models, const int32_t* models, uint64_t seed,
+                         int64_t first, int64_t n, int32_t* best_blocks, double* best_ms);
+
+/* HIP graphs (no reference counterpart).  abc_graph_begin starts capturing the
+ * context stream; the *_dev calls that follow (sample_prior_dev, simulate_dev,
+ * simulate_idx_dev, errors_dev) are recorded instead of run; abc_graph_end
+ * closes the capture into *graph: a new executable graph when *graph is NULL,
+ * else an in-place update of *graph (same call sequence with new particles,
+ * seeds or buffers), re-instantiated if the sequence changed.  Every device
+ * buffer a captured call needs must already have its size: run the same calls
+ * once uncaptured first (ABC_E_STATE otherwise).  Host-pointer and waiting
+ * entry points return ABC_E_STATE during a capture.  Timing events are not
+ * recorded inside graphs.  Graphs replay on the context stream. */
+typedef struct abc_graph abc_graph;
+int32_t abc_graph_begin(abc_ctx* ctx);
+int32_t abc_graph_end(abc_ctx* ctx, abc_graph** graph);
+int32_t abc_graph_launch(abc_ctx* ctx, abc_graph* graph);
+int32_t abc_graph_destroy(abc_graph* graph);
 
 #ifdef __cplusplus
 }
