@@ -41,6 +41,14 @@ def load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is missing: build the HIP library first "
                            "(python -m abc_inference_transcription_amd.build)")
+    # torch ships its own HIP runtime under the same soname (libamdhip64.so.7)
+    # as the system one libabc.so links against; whichever loads first serves
+    # the whole process, and torch only works with its own.  Loading torch
+    # first makes libabc bind to that runtime too.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     _declare(L)
     _lib = L

@@ -7,9 +7,11 @@
 // unseeded global RNG (scripts/abc_simulation.jl:4-9); counter-based streams
 // make every particle reproducible and independent of the batch split.
 //
-// log/exp follow Cody & Waite (1980): power-of-two reduction + 2*atanh series
-// for log, ln2 two-word reduction + degree-13 Taylor polynomial for exp.  Only
-// IEEE-rounded +,-,*,/ are used and the library is compiled with
+// log is table-driven (Tang, ACM TOMS 16(4), 1990): 128 subintervals of
+// [0.707, 1.414), reduction r = z * (1/c) - 1 by one fused multiply-add and a
+// degree-8 log1p polynomial; exp uses Cody & Waite's two-word ln2 reduction
+// and a degree-13 Taylor polynomial.  Only IEEE-rounded +,-,*,/ and fused
+// multiply-adds written as fma() are used, and the library is compiled with
 // -ffp-contract=off, so results are identical on every IEEE machine that
 // performs the same steps; this is what makes GPU runs replayable on a CPU.
 #pragma once
@@ -18,6 +20,9 @@
 
 #define ABC_DEV __device__ __forceinline__
 
+#define ABC_LOGTAB_QUAL __device__ static const
+#include "abc_logtab.h"
+
 namespace abcd {
 
 struct u4 { uint32_t x, y, z, w; };
@@ -25,10 +30,11 @@ struct u4 { uint32_t x, y, z, w; };
 ABC_DEV u4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c0;
-    const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
-    const uint32_t lo1 = 0xCD9E8D57u * c2;
-    const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+    // one 32x32->64 product per multiplier (a single v_mad_u64_u32 each)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+    const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     const uint32_t n0 = hi1 ^ c1 ^ k0;
     const uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
@@ -47,32 +53,41 @@ ABC_DEV uint32_t lemire(uint32_t w, uint32_t n) {
   return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
 }
 
+// Table of the log in LDS: entry j = {1/c_j, -ln(1/c_j) hi, lo, unused}.
+struct LogEntry { double inv, hi, lo, pad; };
+
+// fills a workgroup's LDS copy of the table (call before __syncthreads())
+ABC_DEV void load_log_table(LogEntry* t) {
+  for (int q = threadIdx.x; q < ABC_LOG_N; q += blockDim.x)
+    t[q] = LogEntry{abc_log_inv[q], abc_log_hi[q], abc_log_lo[q], 0.0};
+}
+
 // natural log for finite x > 0 (the callers never pass anything else)
-ABC_DEV double ln(double x) {
-  int64_t bits = __double_as_longlong(x);
-  int e_adj = 0;
+ABC_DEV double ln(double x, const LogEntry* tab) {
+  uint64_t ix = (uint64_t)__double_as_longlong(x);
+  int adj = 0;
   if (x < 0x1p-1022) {
-    bits = __double_as_longlong(x * 0x1p54);
-    e_adj = -54;
+    ix = (uint64_t)__double_as_longlong(x * 0x1p54);
+    adj = -54;
   }
-  int e = (int)((bits >> 52) & 0x7ff) - 1023 + e_adj;
-  double m = __longlong_as_double((bits & 0x000fffffffffffffll) | 0x3ff0000000000000ll);
-  if (m > 0x1.6a09e667f3bcdp+0) { m = m * 0.5; e = e + 1; }
-  const double s = (m - 1.0) / (m + 1.0);
-  const double z = s * s;
-  double p = 0x1.8618618618618p-4;
-  p = p * z + 0x1.af286bca1af28p-4;
-  p = p * z + 0x1.e1e1e1e1e1e1ep-4;
-  p = p * z + 0x1.1111111111111p-3;
-  p = p * z + 0x1.3b13b13b13b14p-3;
-  p = p * z + 0x1.745d1745d1746p-3;
-  p = p * z + 0x1.c71c71c71c71cp-3;
-  p = p * z + 0x1.2492492492492p-2;
-  p = p * z + 0x1.999999999999ap-2;
-  p = p * z + 0x1.5555555555555p-1;
-  const double logm = (s + s) + (s * z) * p;
-  const double de = (double)e;
-  return de * 0x1.62e42fee00000p-1 + (logm + de * 0x1.a39ef35793c76p-33);
+  const uint64_t tmp = ix - ABC_LOG_OFF;
+  const int j = (int)((tmp >> 45) & (uint64_t)(ABC_LOG_N - 1));
+  const int k = (int)((int64_t)tmp >> 52) + adj;
+  const double z = __longlong_as_double((int64_t)(ix - (tmp & (0xfffull << 52))));
+  const LogEntry e = tab[j];
+  const double r = fma(z, e.inv, -1.0);
+  const double r2 = r * r;
+  double p = -0x1p-3;
+  p = fma(p, r, 0x1.2492492492492p-3);
+  p = fma(p, r, -0x1.5555555555555p-3);
+  p = fma(p, r, 0x1.999999999999ap-3);
+  p = fma(p, r, -0x1p-2);
+  p = fma(p, r, 0x1.5555555555555p-2);
+  p = fma(p, r, -0x1p-1);
+  const double lp = fma(p, r2, r);
+  const double kd = (double)k;
+  const double hi = kd * 0x1.62e42fee00000p-1 + e.hi;
+  return hi + ((kd * 0x1.a39ef35793c76p-33 + e.lo) + lp);
 }
 
 ABC_DEV double ex(double x) {
@@ -83,19 +98,19 @@ ABC_DEV double ex(double x) {
   const int k = (int)kd;
   const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
   double p = 0x1.6124613a86d09p-33;
-  p = p * r + 0x1.1eed8eff8d898p-29;
-  p = p * r + 0x1.ae64567f544e4p-26;
-  p = p * r + 0x1.27e4fb7789f5cp-22;
-  p = p * r + 0x1.71de3a556c734p-19;
-  p = p * r + 0x1.a01a01a01a01ap-16;
-  p = p * r + 0x1.a01a01a01a01ap-13;
-  p = p * r + 0x1.6c16c16c16c17p-10;
-  p = p * r + 0x1.1111111111111p-7;
-  p = p * r + 0x1.5555555555555p-5;
-  p = p * r + 0x1.5555555555555p-3;
-  p = p * r + 0.5;
-  p = p * r + 1.0;
-  p = p * r + 1.0;
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);
+  p = fma(p, r, 0x1.ae64567f544e4p-26);
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);
+  p = fma(p, r, 0x1.71de3a556c734p-19);
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);
+  p = fma(p, r, 0x1.1111111111111p-7);
+  p = fma(p, r, 0x1.5555555555555p-5);
+  p = fma(p, r, 0x1.5555555555555p-3);
+  p = fma(p, r, 0.5);
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
   if (k < -1000) return (p * __longlong_as_double((int64_t)(k + 1000 + 1023) << 52)) * 0x1p-1000;
   if (k > 1000) return (p * __longlong_as_double((int64_t)(k - 1000 + 1023) << 52)) * 0x1p1000;
   return p * __longlong_as_double((int64_t)(k + 1023) << 52);

@@ -314,8 +314,8 @@ __device__ __forceinline__ u4 draw(uint64_t ctr, uint32_t stream, uint32_t cell,
   return philox((uint32_t)ctr, ((uint32_t)(ctr >> 32) & 0xFFFFu) | (stream << 16), cell, pid, k0, k1);
 }
 
-__device__ __forceinline__ double wait_time(uint32_t w, double a) {
-  return -ln(u01_32(w)) / a;
+__device__ __forceinline__ double wait_time(uint32_t w, double a, const LogEntry* lt) {
+  return -ln(u01_32(w), lt) / a;
 }
 
 __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict__ D,
@@ -336,6 +336,9 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   const int nb = D->nb, J = D->J, C = D->C;
   const int age_mode = D->age_mode, cap_mode = D->cap_mode;
   const unsigned long long total = (unsigned long long)off[n];
+  __shared__ LogEntry ltab[ABC_LOG_N];
+  load_log_table(ltab);
+  __syncthreads();
 
   // per-lane item state
   bool have = false;
@@ -440,7 +443,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       {
         const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
         st++;
-        tg = t + wait_time(v.x, g ? koff : kon);
+        tg = t + wait_time(v.x, g ? koff : kon, ltab);
       }
       have = true;
     }
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
     if (a0 > 0.0) {
       w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
       st++;
-      tm = t + wait_time(w.x, a0);
+      tm = t + wait_time(w.x, a0, ltab);
     }
     if (tm < tg && tm < tb) {
       t = tm;
@@ -515,7 +518,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       re++;
       const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
       st++;
-      tg = t + wait_time(v.x, g ? koff : kon);
+      tg = t + wait_time(v.x, g ? koff : kon, ltab);
       continue;
     }
     // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
@@ -556,7 +559,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
         // the gene's rate changed: restart its clock (memoryless)
         const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
         st++;
-        tg = t + wait_time(v.x, g ? koff : kon);
+        tg = t + wait_time(v.x, g ? koff : kon, ltab);
       }
       gk = grid_time(knext, nbB, T, nb);
     }

@@ -4,23 +4,24 @@ ssa_kernel is bound by VALU issue (integer Philox + scalar FP64 per lane; it
 reads and writes almost nothing).  Its algorithmic unit is one SSA step (one
 Philox4x32-10 block spent on an event proposal, a boundary crossing or 128
 division coin flips).  Lane-operations per event step, counted from the
-source of the step (each IEEE op, integer op, conversion or compare = 1):
+source of the step (each IEEE op or fused multiply-add, integer op, 32x32->64
+product, conversion, compare or table load = 1):
 
-    Philox4x32-10: 10 rounds x (2 mul_lo + 2 mul_hi + 4 xor + 2 key adds)  100
+    Philox4x32-10: 10 rounds x (2 products + 4 xor + 2 key adds)             80
     counter update and packing                                                4
     uniform for the waiting time (cvt, add, mul)                              3
-    log (exponent/mantissa split 12, s=(m-1)/(m+1) 3, z 1, Horner 18,
-         reconstruction 10)                                                  44
-    waiting time (neg, div), t + tau, boundary compare                        4
-    propensities (3 selects, 3 int adds, 2 cvt, 2 mul, 4 add)                14
+    log (index/exponent/reduction bits 8, table load 1, r and r^2 2,
+         log1p polynomial 7, reconstruction 7)                               25
+    waiting time (neg, div, t + tau), two boundary compares                   5
+    propensities (select, 4 int adds, 2 cvt, 2 mul, 2 add, test)             12
     channel uniform and selection compares                                    6
     state update                                                              4
                                                                       ------
-                                                                        179
+                                                                        139
 
 The peak is one VALU lane-op per lane per clock: 256 CUs x 64 lanes x
 2.4 GHz = 39.3 Top/s (half the published 78.6 TFLOP/s FP64 vector figure,
 which counts an FMA as two flops).
 """
-SSA_OPS_PER_STEP = 179
+SSA_OPS_PER_STEP = 139
 VALU_PEAK_TOPS = 39.3

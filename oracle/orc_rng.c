@@ -20,6 +20,9 @@
 #include <math.h>
 #include <string.h>
 
+#define ABC_LOGTAB_QUAL static const
+#include "abc_logtab.h"
+
 #define PHILOX_M0 0xD2511F53u
 #define PHILOX_M1 0xCD9E8D57u
 #define PHILOX_W0 0x9E3779B9u
@@ -53,36 +56,40 @@ static const double LN2_HI = 0x1.62e42fee00000p-1;
 static const double LN2_LO = 0x1.a39ef35793c76p-33;
 static const double INV_LN2 = 0x1.71547652b82fep+0;
 static const double LN10 = 0x1.26bb1bbb55516p+1;
-static const double SQRT2 = 0x1.6a09e667f3bcdp+0;
 
 static inline uint64_t dbits(double x) { uint64_t u; memcpy(&u, &x, 8); return u; }
 static inline double bitsd(uint64_t u) { double x; memcpy(&x, &u, 8); return x; }
 
+/* Natural log, table-driven (Tang, ACM TOMS 16(4), 1990): x = 2^k z with z in
+ * [0.70703125, 1.4140625); the bits of x minus ABC_LOG_OFF select subinterval
+ * j (bits 45..51), r = z * INV[j] - 1 by one fused multiply-add (|r| <= 2^-7),
+ * ln x = k ln2 + L[j] + log1p(r) with L[j] = -ln(INV[j]) as hi + lo and
+ * log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... - r^6/8).  The table is the
+ * generated abc_logtab.h of the engine (pure data; its entries are pinned by
+ * the accuracy tests, which cover every subinterval). */
 double orc_log(double x) {
   if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
   if (x == INFINITY) return x;
-  int e_adj = 0;
-  if (x < 0x1p-1022) { x = x * 0x1p54; e_adj = -54; }   /* subnormal */
-  uint64_t u = dbits(x);
-  int e = (int)((u >> 52) & 0x7ff) - 1023 + e_adj;
-  double m = bitsd((u & 0x000fffffffffffffull) | 0x3ff0000000000000ull);  /* [1,2) */
-  if (m > SQRT2) { m = m * 0.5; e = e + 1; }
-  /* s in [-0.1716, 0.1716]; log(m) = 2 atanh(s) */
-  double s = (m - 1.0) / (m + 1.0);
-  double z = s * s;
-  double p = 0x1.8618618618618p-4;           /* 2/21 */
-  p = p * z + 0x1.af286bca1af28p-4;          /* 2/19 */
-  p = p * z + 0x1.e1e1e1e1e1e1ep-4;          /* 2/17 */
-  p = p * z + 0x1.1111111111111p-3;          /* 2/15 */
-  p = p * z + 0x1.3b13b13b13b14p-3;          /* 2/13 */
-  p = p * z + 0x1.745d1745d1746p-3;          /* 2/11 */
-  p = p * z + 0x1.c71c71c71c71cp-3;          /* 2/9  */
-  p = p * z + 0x1.2492492492492p-2;          /* 2/7  */
-  p = p * z + 0x1.999999999999ap-2;          /* 2/5  */
-  p = p * z + 0x1.5555555555555p-1;          /* 2/3  */
-  double logm = (s + s) + (s * z) * p;
-  double de = (double)e;
-  return de * LN2_HI + (logm + de * LN2_LO);
+  int adj = 0;
+  if (x < 0x1p-1022) { x = x * 0x1p54; adj = -54; }   /* subnormal */
+  const uint64_t ix = dbits(x);
+  const uint64_t tmp = ix - ABC_LOG_OFF;
+  const int j = (int)((tmp >> 45) & (ABC_LOG_N - 1));
+  const int k = (int)((int64_t)tmp >> 52) + adj;
+  const double z = bitsd(ix - (tmp & (0xfffull << 52)));
+  const double r = fma(z, abc_log_inv[j], -1.0);
+  const double r2 = r * r;
+  double p = -0x1p-3;                         /* -1/8 */
+  p = fma(p, r, 0x1.2492492492492p-3);        /*  1/7 */
+  p = fma(p, r, -0x1.5555555555555p-3);       /* -1/6 */
+  p = fma(p, r, 0x1.999999999999ap-3);        /*  1/5 */
+  p = fma(p, r, -0x1p-2);                     /* -1/4 */
+  p = fma(p, r, 0x1.5555555555555p-2);        /*  1/3 */
+  p = fma(p, r, -0x1p-1);                     /* -1/2 */
+  const double lp = fma(p, r2, r);
+  const double kd = (double)k;
+  const double hi = kd * LN2_HI + abc_log_hi[j];
+  return hi + ((kd * LN2_LO + abc_log_lo[j]) + lp);
 }
 
 double orc_exp(double x) {
@@ -93,19 +100,19 @@ double orc_exp(double x) {
   int k = (int)kd;
   double r = (x - kd * LN2_HI) - kd * LN2_LO;
   double p = 0x1.6124613a86d09p-33;          /* 1/13! */
-  p = p * r + 0x1.1eed8eff8d898p-29;         /* 1/12! */
-  p = p * r + 0x1.ae64567f544e4p-26;         /* 1/11! */
-  p = p * r + 0x1.27e4fb7789f5cp-22;         /* 1/10! */
-  p = p * r + 0x1.71de3a556c734p-19;         /* 1/9!  */
-  p = p * r + 0x1.a01a01a01a01ap-16;         /* 1/8!  */
-  p = p * r + 0x1.a01a01a01a01ap-13;         /* 1/7!  */
-  p = p * r + 0x1.6c16c16c16c17p-10;         /* 1/6!  */
-  p = p * r + 0x1.1111111111111p-7;          /* 1/5!  */
-  p = p * r + 0x1.5555555555555p-5;          /* 1/4!  */
-  p = p * r + 0x1.5555555555555p-3;          /* 1/3!  */
-  p = p * r + 0.5;                           /* 1/2!  */
-  p = p * r + 1.0;
-  p = p * r + 1.0;
+  p = fma(p, r, 0x1.1eed8eff8d898p-29);         /* 1/12! */
+  p = fma(p, r, 0x1.ae64567f544e4p-26);         /* 1/11! */
+  p = fma(p, r, 0x1.27e4fb7789f5cp-22);         /* 1/10! */
+  p = fma(p, r, 0x1.71de3a556c734p-19);         /* 1/9!  */
+  p = fma(p, r, 0x1.a01a01a01a01ap-16);         /* 1/8!  */
+  p = fma(p, r, 0x1.a01a01a01a01ap-13);         /* 1/7!  */
+  p = fma(p, r, 0x1.6c16c16c16c17p-10);         /* 1/6!  */
+  p = fma(p, r, 0x1.1111111111111p-7);          /* 1/5!  */
+  p = fma(p, r, 0x1.5555555555555p-5);          /* 1/4!  */
+  p = fma(p, r, 0x1.5555555555555p-3);          /* 1/3!  */
+  p = fma(p, r, 0.5);                           /* 1/2!  */
+  p = fma(p, r, 1.0);
+  p = fma(p, r, 1.0);
   /* scale by 2^k in two exact steps so subnormal results stay exact-ish */
   if (k < -1000) {
     double s1 = bitsd((uint64_t)(k + 1000 + 1023) << 52);
