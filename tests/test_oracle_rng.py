@@ -5,6 +5,7 @@ kat_vectors: philox4x32_10); log/exp against libm within a few ulp; the
 128-bit conversion against Python's correctly rounded int -> float.
 """
 import math
+import struct
 import random
 
 import numpy as np
@@ -36,13 +37,26 @@ def ulps(a, b):
 
 
 def test_log_accuracy(oracle):
+    # the log is table-driven: 128 subintervals of [0.70703125, 1.4140625)
+    # selected by bits 45..51 of x - 0x3fe6a00000000000; sampling every
+    # subinterval at several binades pins every table entry
     rng = random.Random(1)
     xs = [rng.uniform(1e-12, 1.0) for _ in range(3000)] + [10 ** rng.uniform(-300, 300) for _ in range(3000)]
     xs += [((w + 0.5) * 2.0 ** -32) for w in (0, 1, 2, 2**31, 2**32 - 1)]
-    xs += [5e-324, 2.2250738585072014e-308, 0.5, 2.0, math.sqrt(2), 1 - 2**-53]
+    xs += [5e-324, 2.2250738585072014e-308, 0.5, 2.0, math.sqrt(2), 1 - 2**-53, 1 + 2**-52]
+    off = 0x3fe6a00000000000
+    for j in range(128):
+        for _ in range(12):
+            b = off + (j << 45) + rng.randrange(1 << 45)
+            for k in (-40, -1, 0, 3):
+                xs.append(struct.unpack("<d", struct.pack("<Q", b + (k << 52)))[0])
     worst = max(ulps(oracle.log(x), math.log(x)) for x in xs if x != 1.0)
-    assert worst <= 4.0
+    assert worst <= 2.0
     assert oracle.log(1.0) == 0.0
+    # next to 1 the reduction is exact (1/c = 1): full relative precision
+    for d in (2**-52, -2**-53, 1e-10, -1e-10, 2**-9, -2**-9):
+        x = 1.0 + d
+        assert ulps(oracle.log(x), math.log1p(x - 1.0)) <= 1.0
 
 
 def test_exp_accuracy(oracle):
