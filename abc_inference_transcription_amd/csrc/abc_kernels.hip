@@ -353,11 +353,6 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   uint32_t div_rem = 0, div_keep = 0;
   unsigned long long st = 0, re = 0;
   int sub = 0;
-  // pending molecule candidate: its Philox block and residual Exp(1) hazard
-  // (kept when the gene clock or a boundary comes first; memoryless)
-  bool pend = false;
-  u4 pw{0, 0, 0, 0};
-  double erem = 0.0;
 
   for (;;) {
     if (!have) {
@@ -445,7 +440,6 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       if (ws < tb) tb = ws;
       st = 0;
       re = 0;
-      pend = false;
       {
         const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
         st++;
@@ -489,19 +483,14 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
     const double c1 = a_tx, c2 = c1 + a_spl;
     const double a0 = c2 + a_dec;
     double tm = __longlong_as_double(0x7ff0000000000000ll);
+    u4 w{0, 0, 0, 0};
     if (a0 > 0.0) {
-      if (!pend) {
-        pw = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
-        st++;
-        erem = -ln(u01_32(pw.x), ltab);
-        pend = true;
-      }
-      tm = t + erem / a0;
+      w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
+      st++;
+      tm = t + wait_time(w.x, a0, ltab);
     }
     if (tm < tg && tm < tb) {
       t = tm;
-      pend = false;
-      const u4 w = pw;
       const double rr = u01_32(w.y) * a0;
       if (rr < c1) {
         bool acc = true;
@@ -522,12 +511,6 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
         else n3--;
       }
       continue;
-    }
-    if (pend) {
-      // no molecule event before min(tg, tb): use up a0 * (that interval)
-      const double t1 = tg < tb ? tg : tb;
-      erem = erem - a0 * (t1 - t);
-      if (erem < 0.0) erem = 0.0;
     }
     if (tg < tb) {
       t = tg;

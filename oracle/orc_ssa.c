@@ -306,12 +306,6 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
   draw(&s0, v);
   st++;
   double tg = t + wait(v[0], g ? r->koff[0] : r->kon[0]);
-  /* A molecule block drawn for a candidate event that the gene clock or a
-   * boundary pre-empted stays pending: its channel words are still unused and
-   * its Exp(1) hazard, less what the interval used up, is again Exp(1)
-   * (memoryless), so the next candidate continues from the residual. */
-  int pend = 0;
-  double erem = 0.0;
   for (;;) {
     double tb = age;
     double gk = grid_t(knext, nbB, T, nb);
@@ -326,17 +320,12 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
     double c1 = a_tx, c2 = c1 + a_spl, a0 = c2 + a_dec;
     double tm = INFINITY;
     if (a0 > 0.0) {
-      if (!pend) {
-        draw(&sm, w);
-        st++;
-        erem = -orc_log(orc_u01_32(w[0]));
-        pend = 1;
-      }
-      tm = t + erem / a0;
+      draw(&sm, w);
+      st++;
+      tm = t + wait(w[0], a0);
     }
     if (tm < tg && tm < tb) {
       t = tm;
-      pend = 0;
       double rr = orc_u01_32(w[1]) * a0;
       if (rr < c1) {
         int acc = 1;
@@ -361,12 +350,6 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
         else n3--;
       }
       continue;
-    }
-    if (pend) {
-      /* no molecule event before min(tg, tb): use up a0 * (that interval) */
-      const double t1 = tg < tb ? tg : tb;
-      erem = erem - a0 * (t1 - t);
-      if (erem < 0.0) erem = 0.0;
     }
     if (tg < tb) {
       t = tg;
