@@ -35,7 +35,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=512, help="particles per model per step")
+    ap.add_argument("--particles", type=int, default=4096, help="particles per model per step")
     ap.add_argument("--genes", type=int, default=2000)
     ap.add_argument("--cells", type=int, default=2500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -154,7 +154,7 @@ def main():
     tuned = None
     if not a.no_autotune:
         # grid size of the persistent SSA kernel, tuned on one step's worth of other particles
-        blocks, tms = eng.autotune(models=models, n=P, first=50_000_000)
+        blocks, tms = eng.autotune(models=models, n=max(64, P // 8), first=50_000_000)
         tuned = {"ssa_blocks": blocks, "sample_ssa_ms": round(tms, 3)}
         print(f"[bench] autotune: {blocks} workgroups", file=sys.stderr, flush=True)
     for w in range(a.warmup):
@@ -179,13 +179,14 @@ def main():
 
     graph = None
     if not a.no_graph:
-        # the same K steps, each captured into one HIP graph (updated in place
-        # for the next step's particles) and replayed
+        # the first steps again, each captured into one HIP graph (updated in
+        # place for the next step's particles) and replayed
         g = None
+        gsteps = min(a.steps, 2)
         ev2, ev3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         torch.cuda.synchronize()
         ev2.record(stream)
-        for it in range(a.steps):
+        for it in range(gsteps):
             g = eng.capture(lambda: step(it, False), graph=g)
             eng.replay(g)
         ev3.record(stream)
@@ -193,7 +194,8 @@ def main():
         gms = ev2.elapsed_time(ev3)
         eng.drop_graph(g)
         ssa_time(1)
-        graph = {"value": round(n_particles / (gms / 1e3), 3), "ms_per_step": round(gms / a.steps, 3)}
+        graph = {"value": round(gsteps * P * len(models) / (gms / 1e3), 3),
+                 "ms_per_step": round(gms / gsteps, 3), "steps": gsteps}
         print(f"[bench] graph replay: {graph}", file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel (SSA): algorithmic VALU lane-ops

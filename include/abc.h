@@ -64,7 +64,7 @@ typedef struct {
   int32_t prior_mode;           /* 0 = fix_params ranges, 1 = config priors          */
   double splice_log10;          /* splicing rate for prior_mode 0                   */
   double split_target;          /* expected molecule events per sub-lane (0 = one
-                                   lane per cell), e.g. 16384                       */
+                                   lane per cell), e.g. 65536                       */
   int32_t split_max;            /* sub-lanes per cell cap, 1..4096, e.g. 1024       */
 } abc_design;
 

@@ -153,7 +153,7 @@ class OracleDesign:
             int(d.get("beta_a", 2)), int(d.get("beta_b", 8)), _p(self.betas),
             _p(self.group_off), _p(self.weights), float(d.get("burn_eps", 5e-3)),
             int(d.get("burn_max", 16)), int(d["prior_mode"]), float(d.get("splice_log10", 0.0)),
-            float(d.get("split_target", 16384.0)), int(d.get("split_max", 1024)))
+            float(d.get("split_target", 65536.0)), int(d.get("split_max", 1024)))
 
     @property
     def ref(self):

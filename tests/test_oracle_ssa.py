@@ -159,3 +159,25 @@ def test_cell_stats_edge_cases(oracle):
     assert mo[6 * 2 + 0] == 0 and np.all(mo[6 * 2:6 * 3] == 0)
     assert sr[0] == 0.0 and sr[5] == 0.0          # all-zero counts: mean 0, FF 0/(0+1e-4)
     assert np.all(sr[20:31] == 0.0)               # ratio guard
+
+
+@pytest.mark.parametrize("model,split", [(2, 0.0), (4, 0.0), (4, 2000.0)])
+def test_fast_switching_gene_matches_moments(oracle, model, split):
+    # Gene switching much faster than molecule events (most iterations are
+    # switches, whose candidate molecule draws are discarded and redrawn);
+    # with split cells every sub-lane also replays the fast gene path.
+    kon, koff, al, ga = np.log10(20.0), np.log10(30.0), np.log10(4.0), np.log10(0.7)
+    th_ref = [kon, koff, al, ga, np.log10(0.6)] if model == 2 else \
+        [kon, koff] + [al + 0.1 * q for q in range(5)] + [ga, np.log10(0.6)]
+    theta = np.append(th_ref, 0.3)[None, :]
+    per = 200
+    d = dict(reference_design(cells=55 * per), split_target=split)
+    od = oracle.OracleDesign(d)
+    out = oracle.simulate(od, model, 11, 0, theta, n_threads=0)
+    cfg = oracle.moment_cfg(AGES, REF_PULSE, REF_CHASE, 20.0, transient_mode=1)
+    mom = oracle.recover(model, th_ref, cfg)
+    z = z_scores(out, mom, per)
+    assert len(z) > 150
+    assert abs(z.mean()) < 0.35, z.mean()
+    assert np.abs(z).max() < 5.0
+    assert 0.6 < z.std() < 1.4
