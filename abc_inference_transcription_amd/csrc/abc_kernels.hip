@@ -314,10 +314,6 @@ __device__ __forceinline__ u4 draw(uint64_t ctr, uint32_t stream, uint32_t cell,
   return philox((uint32_t)ctr, ((uint32_t)(ctr >> 32) & 0xFFFFu) | (stream << 16), cell, pid, k0, k1);
 }
 
-__device__ __forceinline__ double wait_time(uint32_t w, double a, const LogEntry* lt) {
-  return -ln(u01_32(w), lt) / a;
-}
-
 __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict__ D,
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
@@ -353,6 +349,11 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   uint32_t div_rem = 0, div_keep = 0;
   unsigned long long st = 0, re = 0;
   int sub = 0;
+  // gene clock to (re)draw; pending molecule candidate (its residual Exp(1)
+  // hazard and unused words)
+  bool gneed = false, pend = false;
+  double erem = 0.0;
+  uint32_t py = 0, pz = 0, pw = 0;
 
   for (;;) {
     if (!have) {
@@ -440,18 +441,33 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       if (ws < tb) tb = ws;
       st = 0;
       re = 0;
-      {
-        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
-        st++;
-        tg = t + wait_time(v.x, g ? koff : kon, ltab);
-      }
+      gneed = true;  // the first iteration draws the gene clock
+      pend = false;
       have = true;
     }
 
-    if (div) {
-      // binomial(count, 1/2) partition of species div_s, 128 coin flips per draw
-      const u4 w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
+    // propensities of this sub-lane's molecule channels (direct method)
+    const double a_tx = g ? atx : 0.0;
+    const uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
+    const double a_spl = bs * (double)nun;
+    const double a_dec = gam * (double)ntot;
+    const double c1 = a_tx, c2 = c1 + a_spl;
+    const double a0 = c2 + a_dec;
+
+    // One Philox block per lane and iteration, in uniform control flow:
+    // division coins, the gene clock (shared stream 0) or a new molecule
+    // candidate (stream 1 + sub).  A candidate pre-empted by the gene clock or
+    // a boundary stays pending: its words are unused and its Exp(1) hazard,
+    // less what the interval used up, is again Exp(1) (memoryless).
+    const int kind = div ? 1 : (gneed ? 2 : ((!pend && a0 > 0.0) ? 3 : 0));
+    const bool mol = kind == 1 || kind == 3;
+    const u4 w = draw(mol ? mctr : gctr, mol ? sstream : 0u, (uint32_t)c, pid, k0, k1);
+    if (kind != 0) {
       st++;
+      if (mol) mctr++; else gctr++;
+    }
+    if (kind == 1) {
+      // binomial(count, 1/2) partition of species div_s, 128 coin flips per block
       const uint32_t m = div_rem < 128u ? div_rem : 128u;
       uint32_t cnt = __popc(m >= 32u ? w.x : (w.x & ((1u << m) - 1u)));
       if (m > 32u) cnt += __popc(m >= 64u ? w.y : (w.y & ((1u << (m - 32u)) - 1u)));
@@ -475,36 +491,38 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       }
       continue;
     }
-
-    const double a_tx = g ? atx : 0.0;
-    const uint32_t nun = n0 + n2, ntot = n0 + n1 + n2 + n3;
-    const double a_spl = bs * (double)nun;
-    const double a_dec = gam * (double)ntot;
-    const double c1 = a_tx, c2 = c1 + a_spl;
-    const double a0 = c2 + a_dec;
-    double tm = __longlong_as_double(0x7ff0000000000000ll);
-    u4 w{0, 0, 0, 0};
-    if (a0 > 0.0) {
-      w = draw(mctr++, sstream, (uint32_t)c, pid, k0, k1);
-      st++;
-      tm = t + wait_time(w.x, a0, ltab);
+    const double E = -ln(u01_32(w.x), ltab);
+    if (kind == 2) {
+      tg = t + E / (g ? koff : kon);
+      gneed = false;
+      if (!pend && a0 > 0.0) continue;  // a molecule candidate comes first
+    } else if (kind == 3) {
+      erem = E;
+      py = w.y;
+      pz = w.z;
+      pw = w.w;
+      pend = true;
     }
+
+    // next event: the molecule candidate, the gene switch or a boundary
+    const double tm = (pend && a0 > 0.0) ? t + erem / a0 : __longlong_as_double(0x7ff0000000000000ll);
     if (tm < tg && tm < tb) {
       t = tm;
-      const double rr = u01_32(w.y) * a0;
+      pend = false;
+      const double rr = u01_32(py) * a0;
       if (rr < c1) {
         bool acc = true;
-        if (scaling) acc = u01_32(w.z) * bound < T + (t - cs);
+        if (scaling) acc = u01_32(pz) * bound < T + (t - cs);
         if (acc) {
           re++;
-          if (in_win && (uint64_t)w.w < thr_lam) n2++; else n0++;
+          if (in_win && (uint64_t)pw < thr_lam) n2++; else n0++;
         }
       } else if (rr < c2) {
         re++;
-        if (lemire(w.z, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+        if (lemire(pz, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
       } else {
         re++;
-        const uint32_t x = lemire(w.z, ntot);
+        const uint32_t x = lemire(pz, ntot);
         if (x < n0) n0--;
         else if (x < n0 + n1) n1--;
         else if (x < n0 + n1 + n2) n2--;
@@ -512,13 +530,17 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       }
       continue;
     }
+    if (pend) {
+      // no molecule event before min(tg, tb): a0 times that interval is used up
+      const double t1 = tg < tb ? tg : tb;
+      erem = erem - a0 * (t1 - t);
+      if (erem < 0.0) erem = 0.0;
+    }
     if (tg < tb) {
       t = tg;
       g = !g;
       re++;
-      const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
-      st++;
-      tg = t + wait_time(v.x, g ? koff : kon, ltab);
+      gneed = true;
       continue;
     }
     // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
@@ -555,12 +577,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
         gam = rp->gam[j];
         bound = D->bound[j];
       }
-      if (gvar) {
-        // the gene's rate changed: restart its clock (memoryless)
-        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
-        st++;
-        tg = t + wait_time(v.x, g ? koff : kon, ltab);
-      }
+      if (gvar) gneed = true;  // the gene's rate changed: restart its clock (memoryless)
       gk = grid_time(knext, nbB, T, nb);
     }
     if (!ws_done && tb == ws) { ws_done = true; in_win = true; }

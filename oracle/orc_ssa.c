@@ -302,10 +302,15 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
   int win_started = 0, in_win = 0;
   uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
   int64_t st = 0, re = 0;
-  /* gene clock (first-reaction method on the shared stream) */
-  draw(&s0, v);
-  st++;
-  double tg = t + wait(v[0], g ? r->koff[0] : r->kon[0]);
+  /* Per iteration one Philox block is drawn: division coins, the gene clock
+   * (first-reaction method on the shared stream 0) or a new molecule
+   * candidate (direct method on stream 1 + sub).  A candidate pre-empted by
+   * the gene clock or a boundary stays pending: its words are still unused and
+   * its Exp(1) hazard, less what the interval used up, is again Exp(1)
+   * (memoryless).  A gene switch only marks the clock for redrawing. */
+  int gneed = 1, pend = 0, div = 0, div_s = 0;
+  uint32_t div_rem = 0, div_keep = 0, pw[3] = {0, 0, 0};
+  double erem = 0.0, tg = 0.0;
   for (;;) {
     double tb = age;
     double gk = grid_t(knext, nbB, T, nb);
@@ -318,32 +323,60 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
     double a_spl = r->bs * (double)nun;
     double a_dec = r->gam[j] * (double)ntot;
     double c1 = a_tx, c2 = c1 + a_spl, a0 = c2 + a_dec;
-    double tm = INFINITY;
-    if (a0 > 0.0) {
-      draw(&sm, w);
-      st++;
-      tm = t + wait(w[0], a0);
+    int kind = div ? 1 : (gneed ? 2 : ((!pend && a0 > 0.0) ? 3 : 0));
+    if (kind == 1 || kind == 3) { draw(&sm, w); st++; }
+    else if (kind == 2) { draw(&s0, w); st++; }
+    if (kind == 1) {
+      /* binomial(count, 1/2) partition of species div_s, 128 coins per block */
+      uint32_t* sp[4] = {&n0, &n1, &n2, &n3};
+      uint32_t m = div_rem < 128u ? div_rem : 128u;
+      for (int b = 0; b < 4; ++b) {
+        int bits = (int)m - 32 * b;
+        if (bits <= 0) break;
+        uint32_t x = bits >= 32 ? w[b] : (w[b] & ((1u << bits) - 1u));
+        div_keep += (uint32_t)__builtin_popcount(x);
+      }
+      div_rem -= m;
+      if (div_rem == 0u) {
+        *sp[div_s] = div_keep;
+        div_s++;
+        while (div_s < 4 && *sp[div_s] == 0u) div_s++;
+        if (div_s < 4) { div_rem = *sp[div_s]; div_keep = 0; }
+        else div = 0;
+      }
+      continue;
     }
+    if (kind == 2) {
+      tg = t + (-orc_log(orc_u01_32(w[0]))) / (g ? r->koff[j] : r->kon[j]);
+      gneed = 0;
+      if (!pend && a0 > 0.0) continue;   /* a molecule candidate comes first */
+    } else if (kind == 3) {
+      erem = -orc_log(orc_u01_32(w[0]));
+      pw[0] = w[1]; pw[1] = w[2]; pw[2] = w[3];
+      pend = 1;
+    }
+    double tm = (pend && a0 > 0.0) ? t + erem / a0 : INFINITY;
     if (tm < tg && tm < tb) {
       t = tm;
-      double rr = orc_u01_32(w[1]) * a0;
+      pend = 0;
+      double rr = orc_u01_32(pw[0]) * a0;
       if (rr < c1) {
         int acc = 1;
         if (r->scaling) {
           /* accept with probability (T + a) / bound_j, a = time since cycle start */
           double cs = grid_t(k - j, nbB, T, nb);
-          acc = orc_u01_32(w[2]) * r->bound[j] < T + (t - cs);
+          acc = orc_u01_32(pw[1]) * r->bound[j] < T + (t - cs);
         }
         if (acc) {
           re++;
-          if (in_win && (uint64_t)w[3] < r->thr_lam) n2++; else n0++;
+          if (in_win && (uint64_t)pw[2] < r->thr_lam) n2++; else n0++;
         }
       } else if (rr < c2) {
         re++;
-        if (lemire(w[2], nun) < n0) { n0--; n1++; } else { n2--; n3++; }
+        if (lemire(pw[1], nun) < n0) { n0--; n1++; } else { n2--; n3++; }
       } else {
         re++;
-        uint32_t x = lemire(w[2], ntot);
+        uint32_t x = lemire(pw[1], ntot);
         if (x < n0) n0--;
         else if (x < n0 + n1) n1--;
         else if (x < n0 + n1 + n2) n2--;
@@ -351,13 +384,17 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       }
       continue;
     }
+    if (pend) {
+      /* no molecule event before min(tg, tb): a0 times that interval is used up */
+      const double t1 = tg < tb ? tg : tb;
+      erem = erem - a0 * (t1 - t);
+      if (erem < 0.0) erem = 0.0;
+    }
     if (tg < tb) {
       t = tg;
       g = !g;
       re++;
-      draw(&s0, v);
-      st++;
-      tg = t + wait(v[0], g ? r->koff[j] : r->kon[j]);
+      gneed = 1;
       continue;
     }
     /* boundary: sampling age, rate-bin / cycle edge, or labelling window edge */
@@ -368,32 +405,14 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       knext += r->step;
       j = (int)(k % nb);
       if (j == 0) {
+        /* division: partition each non-empty species in order, by blocks of
+         * 128 coins drawn by the following iterations */
         uint32_t* sp[4] = {&n0, &n1, &n2, &n3};
-        for (int q = 0; q < 4; ++q) {
-          uint32_t cnt = *sp[q];
-          if (cnt == 0) continue;
-          uint32_t keep = 0, rem = cnt;
-          while (rem > 0) {
-            draw(&sm, v);
-            st++;
-            uint32_t m = rem < 128u ? rem : 128u;
-            for (int b = 0; b < 4; ++b) {
-              int bits = (int)m - 32 * b;
-              if (bits <= 0) break;
-              uint32_t x = bits >= 32 ? v[b] : (v[b] & ((1u << bits) - 1u));
-              keep += (uint32_t)__builtin_popcount(x);
-            }
-            rem -= m;
-          }
-          *sp[q] = keep;
-        }
+        div_s = 0;
+        while (div_s < 4 && *sp[div_s] == 0u) div_s++;
+        if (div_s < 4) { div = 1; div_rem = *sp[div_s]; div_keep = 0; }
       }
-      if (r->gene_varies) {
-        /* the gene's rate changed: restart its clock (memoryless) */
-        draw(&s0, v);
-        st++;
-        tg = t + wait(v[0], g ? r->koff[j] : r->kon[j]);
-      }
+      if (r->gene_varies) gneed = 1;   /* the gene's rate changed: restart its clock */
     }
     if (!win_started && tb == ws) { win_started = 1; in_win = 1; }
     if (in_win && tb == we) in_win = 0;
