@@ -343,7 +343,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   uint64_t gctr = 0, mctr = 0, thr_lam = 0;
   int64_t k = 0, knext = 0, nbB = 0;
   double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, ws = 0.0, we = 0.0, cs = 0.0;
-  double kon = 0.0, koff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0, share = 1.0;
+  double ikon = 0.0, ikoff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0, share = 1.0;
   bool g = false, ws_done = false, in_win = false, div = false;
   int div_s = 0;
   uint32_t div_rem = 0, div_keep = 0;
@@ -426,8 +426,8 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       j = 0;
       t = grid_time(0, nbB, T, nb);
       cs = t;
-      kon = rp->kon[0];
-      koff = rp->koff[0];
+      ikon = 1.0 / rp->kon[0];  // gene clock: waiting time = E * (1 / rate)
+      ikoff = 1.0 / rp->koff[0];
       atx = (rp->amax[0] * bc) / share;
       gam = rp->gam[0];
       bound = D->bound[0];
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
     }
     const double E = -ln(u01_32(w.x), ltab);
     if (kind == 2) {
-      tg = t + E / (g ? koff : kon);
+      tg = t + E * (g ? ikoff : ikon);
       gneed = false;
       if (!pend && a0 > 0.0) continue;  // a molecule candidate comes first
     } else if (kind == 3) {
@@ -571,8 +571,8 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       }
       if (step == 1) {
         const PRates* rp = R + p;
-        kon = rp->kon[j];
-        koff = rp->koff[j];
+        ikon = 1.0 / rp->kon[j];
+        ikoff = 1.0 / rp->koff[j];
         atx = (rp->amax[j] * bc) / share;
         gam = rp->gam[j];
         bound = D->bound[j];

@@ -242,7 +242,6 @@ static inline double grid_t(int64_t k, int64_t nbB, double T, int nb) {
 }
 
 /* Exponential waiting time with rate a from one word. */
-static inline double wait(uint32_t w, double a) { return -orc_log(orc_u01_32(w)) / a; }
 
 /* One sub-lane of cell c: the cell's gene path (stream 0) with a 1/M share
  * of its transcription (stream 1 + sub).  Returns its observed u and l. */
@@ -347,7 +346,8 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       continue;
     }
     if (kind == 2) {
-      tg = t + (-orc_log(orc_u01_32(w[0]))) / (g ? r->koff[j] : r->kon[j]);
+      /* waiting time = Exp(1) * (1 / rate); the reciprocal is a per-bin constant */
+      tg = t + (-orc_log(orc_u01_32(w[0]))) * (1.0 / (g ? r->koff[j] : r->kon[j]));
       gneed = 0;
       if (!pend && a0 > 0.0) continue;   /* a molecule candidate comes first */
     } else if (kind == 3) {
