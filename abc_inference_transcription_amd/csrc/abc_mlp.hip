@@ -33,65 +33,82 @@ constexpr int BM = 128, BK = 32, LDSK = BK + 8;  // +8 bf16 pad against bank con
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // 256 threads, workgroup tile 128 x BN, wave tile 64 x BN/2 (2 x 2 waves).
+// blockIdx.z selects a K chunk [z*K, (z+1)*K) of rows with leading dimension
+// ld (split-K: EPI_F32 then writes its partial tile to Cf + z*M*N).  The next
+// K tile is prefetched into registers while the MFMAs run on the current one.
 template <int BN, int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A,
                                                       const bf16* __restrict__ B, int M, int N,
-                                                      int K, const float* __restrict__ bias,
+                                                      int K, int ld, const float* __restrict__ bias,
                                                       float* __restrict__ Cf,
                                                       bf16* __restrict__ Cb,
                                                       bf16* __restrict__ CbT,
                                                       const float* __restrict__ Zprev,
                                                       float* __restrict__ Zout) {
-  constexpr int WN = BN / 2, NCB = WN / 16;
+  constexpr int WN = BN / 2, NCB = WN / 16, NBQ = BN / 64;
   __shared__ bf16 As[BM][LDSK];
   __shared__ bf16 Bs[BN][LDSK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kbeg = blockIdx.z * K, kend = kbeg + K;
+  if (EPI == EPI_F32) Cf += (size_t)blockIdx.z * M * N;
   f32x4 acc[4][NCB];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int b = 0; b < NCB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < K; k0 += BK) {
+  // this thread's slice of a K tile: rows r = c >> 2, 8 columns at (c & 3) * 8
+  bf16x8 ra[2], rb[NBQ];
+  auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
       const int c = tid + q * 256;
-      const int r = c >> 2, kk = (c & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&As[r][kk]) =
-          *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + r) * K + k0 + kk);
+      ra[q] = *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + (c >> 2)) * ld + k0 + (c & 3) * 8);
     }
 #pragma unroll
-    for (int q = 0; q < BN / 64; ++q) {
+    for (int q = 0; q < NBQ; ++q) {
       const int c = tid + q * 256;
-      const int r = c >> 2, kk = (c & 3) * 8;
-      *reinterpret_cast<bf16x8*>(&Bs[r][kk]) =
-          *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + r) * K + k0 + kk);
+      rb[q] = *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + (c >> 2)) * ld + k0 + (c & 3) * 8);
+    }
+  };
+  load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = tid + q * 256;
+      *reinterpret_cast<bf16x8*>(&As[c >> 2][(c & 3) * 8]) = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < NBQ; ++q) {
+      const int c = tid + q * 256;
+      *reinterpret_cast<bf16x8*>(&Bs[c >> 2][(c & 3) * 8]) = rb[q];
     }
     __syncthreads();
+    if (k0 + BK < kend) load(k0 + BK);  // in flight while the MFMAs below run
     bf16x8 af[4], bfr[NCB];
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
-      af[rb] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + rb * 16 + (lane & 15)][(lane >> 4) * 8]);
+    for (int r4 = 0; r4 < 4; ++r4)
+      af[r4] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + r4 * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
       bfr[cb] = *reinterpret_cast<const bf16x8*>(&Bs[wn * WN + cb * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
-    for (int rb = 0; rb < 4; ++rb)
+    for (int r4 = 0; r4 < 4; ++r4)
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb)
-        acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[rb], bfr[cb], acc[rb][cb], 0, 0, 0);
+        acc[r4][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r4], bfr[cb], acc[r4][cb], 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int rb = 0; rb < 4; ++rb)
+  for (int r4 = 0; r4 < 4; ++r4)
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm * 64 + rb * 16 + (lane >> 4) * 4 + q;
+        const int m = m0 + wm * 64 + r4 * 16 + (lane >> 4) * 4 + q;
         const int n = n0 + wn * WN + cb * 16 + (lane & 15);
-        float v = acc[rb][cb][q];
+        float v = acc[r4][cb][q];
         const size_t o = (size_t)m * N + n;
         if (EPI == EPI_F32) {
           if (bias) v += bias[n];
@@ -110,6 +127,16 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
           CbT[(size_t)n * M + m] = d;
         }
       }
+}
+
+// C[i] = sum over the S split-K partials P[s][i], in the order s = 0, 1, ...
+__global__ void splitk_reduce_kernel(const float* __restrict__ P, int S, int64_t MN,
+                                     float* __restrict__ C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= MN) return;
+  float v = P[i];
+  for (int s = 1; s < S; ++s) v += P[(size_t)s * MN + i];
+  C[i] = v;
 }
 
 // x (B x n_in FP32, row-major) -> X (B x in_pad bf16) and X^T (in_pad x B).
@@ -386,6 +413,8 @@ struct abc_mlp {
   float* O = nullptr;
   bf16 *dA = nullptr, *dAT = nullptr, *dB = nullptr, *dBT = nullptr;
   float* loss = nullptr;
+  float* splitk_ws = nullptr;  // split-K partial tiles of the weight gradients
+  size_t splitk_cap = 0;       // floats
   int64_t step = 0;
   std::string msg;
 };
@@ -403,16 +432,35 @@ int mfail(abc_mlp* p, int code, const std::string& s) {
     if (e_ != hipSuccess) return mfail(p, ABC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// Workgroups of a gemm grid; splitting K fills the machine when M x N is small
+// (the weight gradients: M, N = layer widths, K = batch).
+int splitk_for(int M, int N, int K) {
+  const int tiles = (M / BM) * (N / (N % 128 == 0 ? 128 : 64));
+  int S = 1;
+  while (tiles * S < 256 && K % (2 * S * BK) == 0 && K / (2 * S) >= 4 * BK) S *= 2;
+  return S;
+}
+
 template <int EPI>
 void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const float* bias,
           float* Cf, bf16* Cb, bf16* CbT, const float* Zprev, float* Zout) {
-  if (N % 128 == 0) {
-    hipLaunchKernelGGL((gemm_nt_kernel<128, EPI>), dim3(M / BM, N / 128), dim3(256), 0, p->stream,
-                       A, B, M, N, K, bias, Cf, Cb, CbT, Zprev, Zout);
-  } else {
-    hipLaunchKernelGGL((gemm_nt_kernel<64, EPI>), dim3(M / BM, N / 64), dim3(256), 0, p->stream,
-                       A, B, M, N, K, bias, Cf, Cb, CbT, Zprev, Zout);
+  int S = 1;
+  float* out = Cf;
+  if (EPI == EPI_F32 && !bias && p->splitk_ws) {
+    S = splitk_for(M, N, K);
+    if ((size_t)S * M * N > p->splitk_cap) S = 1;
+    if (S > 1) out = p->splitk_ws;
   }
+  if (N % 128 == 0) {
+    hipLaunchKernelGGL((gemm_nt_kernel<128, EPI>), dim3(M / BM, N / 128, S), dim3(256), 0,
+                       p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
+  } else {
+    hipLaunchKernelGGL((gemm_nt_kernel<64, EPI>), dim3(M / BM, N / 64, S), dim3(256), 0,
+                       p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
+  }
+  if (S > 1)
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(g1((int64_t)M * N, 256)), dim3(256), 0,
+                       p->stream, p->splitk_ws, S, (int64_t)M * N, Cf);
 }
 
 int forward(abc_mlp* p, int64_t Bn, const float* x_dev) {
@@ -530,6 +578,9 @@ int32_t abc_mlp_create(abc_mlp** out, int32_t device, int32_t n_in, int32_t widt
   A(&p->dA, 2 * Bp * dmax); A(&p->dAT, 2 * Bp * dmax);
   A(&p->dB, 2 * Bp * dmax); A(&p->dBT, 2 * Bp * dmax);
   A(&p->loss, 64);
+  // split-K partials: at most 256 tiles of 128 x 128 floats in flight
+  p->splitk_cap = (size_t)256 * BM * 128;
+  A(&p->splitk_ws, 4 * p->splitk_cap);
   if (!ok) { abc_mlp_destroy(p); return ABC_E_OOM; }
   std::vector<float> zeros(off, 0.0f);
   if (hipMemcpy(p->w, w0.data(), 4 * off, hipMemcpyHostToDevice) != hipSuccess ||
@@ -551,7 +602,7 @@ int32_t abc_mlp_destroy(abc_mlp* p) {
   if (!p) return ABC_OK;
   if (p->stream) hipStreamSynchronize(p->stream);
   void* bufs[] = {p->w, p->m, p->v, p->g, p->Wb, p->WbT, p->d_layers, p->d_bf_off, p->X, p->XT,
-                  p->O, p->dA, p->dAT, p->dB, p->dBT, p->loss};
+                  p->O, p->dA, p->dAT, p->dB, p->dBT, p->loss, p->splitk_ws};
   for (void* b : bufs)
     if (b) hipFree(b);
   for (auto* z : p->Z) if (z) hipFree(z);
