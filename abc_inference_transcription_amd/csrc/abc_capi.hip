@@ -2,7 +2,8 @@
 //
 // Orchestrates, per call, on one HIP stream:
 //   prior_kernel -> rates_kernel -> bucket order of particles by expected cost
-//   (heaviest first) -> item offsets -> ssa_kernel (persistent, lane-level
+//   (heaviest first) -> item offsets -> burn_kernel (burn-in of every cell up
+//   to its labelling window) -> ssa_kernel (the window, persistent, lane-level
 //   work stealing) -> stats_kernel -> error / acceptance kernels.
 // Device buffers are owned by the context and grow on demand.  The *_dev
 // sequences use kernels only (no memset, no host wait), so they can be
@@ -48,11 +49,12 @@ struct abc_ctx {
   int num_cu = 0;
   int ssa_blocks = 0;        // 0 = occupancy default
   int ssa_blocks_default = 0;
+  int burn_blocks = 0;       // occupancy default of burn_kernel
   DesignDev hd{};
   DesignDev* dd = nullptr;
   double* d_betas = nullptr;
   int64_t* d_goff = nullptr;
-  DevBuf theta, rates, cost, hist, order, bsum, off, cu, cl,
+  DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
       pflag, bh, bl, cg, gflag, thr, wf, sy;
   AcceptState acc;
@@ -206,6 +208,7 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(cell_words, 256)), dim3(256), 0, c->stream,
                      P<unsigned long long>(c->cl), cell_words);
   if ((rc = ensure(c, c->cb, cells))) return rc;
+  if ((rc = ensure(c, c->cellrec, sizeof(CellStart) * cells))) return rc;
   if ((rc = ensure(c, c->react, sizeof(uint64_t) * n))) return rc;
   if ((rc = ensure(c, c->queue, 64))) return rc;
   unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
@@ -230,12 +233,15 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
     HIPCHK(c, hipEventRecord(pair[0], c->stream));
   }
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
+  hipLaunchKernelGGL(burn_kernel, dim3(c->burn_blocks), dim3(SSA_WG), 0, c->stream, c->dd,
+                     P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
+                     key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
+                     P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev);
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
-                     (uint32_t)first, key0(seed),
-                     key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
-                     P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps,
-                     P<unsigned long long>(c->react), pidx_dev);
+                     (uint32_t)first, key0(seed), key1(seed, model), P<CellStart>(c->cellrec),
+                     P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu), P<uint32_t>(c->cl),
+                     P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react), pidx_dev);
   HIPCHK(c, hipGetLastError());
   if (pair) HIPCHK(c, hipEventRecord(pair[1], c->stream));
   HIPCHK(c, record(c, c->ev[2]));
@@ -318,6 +324,7 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
     h.is_chase[j] = d->is_chase[j];
     wmax = std::max(wmax, d->pulse_h[j] + d->chase_h[j]);
   }
+  h.wmax = wmax;
   h.bmin = std::max(1, (int)std::ceil(wmax / h.T));
   h.bmax = std::max(h.bmin, (int)d->burn_max);
   for (int j = 0; j < h.nb; ++j) h.bound[j] = (h.T * (double)(h.nb + j + 1)) / (double)h.nb;
@@ -364,6 +371,12 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
       per_cu < 1)
     per_cu = 1;
   c->ssa_blocks_default = c->num_cu * per_cu;
+  int per_cu_burn = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_burn, burn_kernel, SSA_WG, 0) !=
+          hipSuccess ||
+      per_cu_burn < 1)
+    per_cu_burn = 1;
+  c->burn_blocks = c->num_cu * per_cu_burn;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) { delete c; return ABC_E_HIP; }
   for (auto& ev : c->ev)
     if (hipEventCreate(&ev) != hipSuccess) { abc_destroy(c); return ABC_E_HIP; }
@@ -390,6 +403,7 @@ int32_t abc_destroy(abc_ctx* c) {
   if (!c) return ABC_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->theta, &c->rates, &c->cost, &c->hist, &c->order, &c->bsum, &c->off,
+                    &c->cellrec,
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,

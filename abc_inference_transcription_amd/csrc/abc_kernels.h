@@ -27,6 +27,7 @@ struct DesignDev {
   double capture_mean;             // mean capture efficiency (cost estimate)
   double split_target;             // molecule events per sub-lane (0 = no split)
   double split_scale;              // capture scale used by the split rule
+  double wmax;                     // longest labelling window (pulse + chase), h
   int split_max, pad1;
   double pulse[ABC_MAX_CONDS], chase[ABC_MAX_CONDS];
   int is_chase[ABC_MAX_CONDS];
@@ -43,6 +44,16 @@ struct __align__(16) PRates {
   int32_t B, step, scaling;        // burn-in cycles, grid stride, size scaling
   int32_t gene_varies;             // kon differs between bins (burst-frequency model)
   int32_t M, pad[3];               // sub-lanes per cell
+};
+
+// A cell at the start of its labelling window (burn_kernel -> ssa_kernel).
+struct __align__(16) CellStart {
+  double age, bc;   // sampling age (h), capture efficiency
+  double tg;        // next gene switch (absolute time)
+  double lu, lt;    // Poisson means of the unspliced / all unlabelled molecules
+  int64_t k;        // last grid edge passed
+  uint64_t gctr;    // gene-stream counter
+  uint8_t bin, g, gneed, pad;
 };
 
 struct ErrSpec {
@@ -72,12 +83,17 @@ __global__ void items_scan_kernel(const DesignDev* D, const PRates* R, const int
                                   int64_t n, int64_t* off, int64_t* bsum);
 __global__ void bsum_scan_kernel(int64_t* bsum, int64_t nblk);
 __global__ void add_bsum_kernel(int64_t* off, int64_t n, const int64_t* bsum);
+__global__ void burn_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
+                            uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
+                            const int64_t* goff, unsigned long long* queue, CellStart* rec,
+                            unsigned long long* steps_out, unsigned long long* react_out,
+                            const int64_t* pidx);
 __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
-                           const int64_t* off, int64_t n,
-                           uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
-                           const int64_t* goff, unsigned long long* queue, uint32_t* out_u,
-                           uint32_t* out_l, uint8_t* out_bin, unsigned long long* steps_out,
-                           unsigned long long* react_out, const int64_t* pidx);
+                           const int64_t* off, int64_t n, uint32_t first, uint32_t k0,
+                           uint32_t k1, const CellStart* rec, unsigned long long* queue,
+                           uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,
+                           unsigned long long* steps_out, unsigned long long* react_out,
+                           const int64_t* pidx);
 __global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
                              const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
                              double* superset);

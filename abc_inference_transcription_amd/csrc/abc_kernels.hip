@@ -6,10 +6,14 @@
 //                       bounds, periodic gene law, burn-in length, cost key
 //                       (scripts/model.jl:1-43)
 //   bucket_* / *scan .. heaviest-first queue order and work-item offsets
-//   ssa_kernel ........ persistent Gillespie simulation, one cell per lane,
-//                       lanes refill from a global cell queue the moment their
-//                       cell ends (lane-level work stealing); replaces the
-//                       moment ODE solves of scripts/model.jl:89-206
+//   burn_kernel ....... burn-in of every cell (gene path + Poisson means of the
+//                       unlabelled molecules) up to its labelling window; the
+//                       stochastic counterpart of scripts/model.jl:114-142
+//   ssa_kernel ........ persistent Gillespie simulation of the labelling
+//                       window, one (cell, sub-lane) item per lane, lanes refill
+//                       from a global item queue the moment their item ends
+//                       (lane-level work stealing); replaces the moment ODE
+//                       solves of scripts/model.jl:146-206
 //   stats_kernel ...... per-particle statistics from observed cell counts with
 //                       exact integer accumulation in LDS
 //                       (scripts/abc_simulation.jl:23-46 semantics on cells,
@@ -163,13 +167,15 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
   if (D->split_target > 0.0) {
     double births = 0.0;
     for (int j = 0; j < nb; ++j) births = births + r.amax[j] * (r.kon[j] / (r.kon[j] + r.koff[j]));
-    const double est = (((births / (double)nb) * D->split_scale) * (T * ((double)B + 1.0))) * 3.0;
+    // molecule events are simulated over the labelling window only
+    const double est = (((births / (double)nb) * D->split_scale) * D->wmax) * 3.0;
     while (M < D->split_max && est > D->split_target * (double)M) M *= 2;
   }
   r.M = M;
   R[i] = r;
   // Expected Philox draws per work item (a scheduling key only, never a
-  // result): gene switches + births (times 1 + splicing + loss) / M.
+  // result): gene switches over the whole path + molecule events (births
+  // times 1 + splicing + loss, a 1/M share) over about half the longest window.
   double sw = 0.0, births = 0.0, loss = 0.0;
   for (int j = 0; j < nb; ++j) {
     const double pon = r.kon[j] / (r.kon[j] + r.koff[j]);
@@ -178,8 +184,8 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
     births += b;
     loss += b * (r.bs / (r.bs + r.gam[j]) + 1.0);
   }
-  const double per_hour = (sw + (births + loss) / (double)M) / (double)nb;
-  cost[i] = (float)(per_hour * T * ((double)B + 0.5) + 4.0);
+  cost[i] = (float)((sw / (double)nb) * T * ((double)B + 0.5) +
+                    ((births + loss) / ((double)nb * (double)M)) * (0.5 * D->wmax) + 4.0);
 }
 
 // -------------------------------------------------------------- queue order
@@ -297,14 +303,10 @@ __global__ __launch_bounds__(256) void add_bsum_kernel(int64_t* __restrict__ off
   off[r + 1] += bsum[r / ABC_SCAN_BLOCK];
 }
 
-// ---------------------------------------------------------------- the SSA
-// Work item = (particle, cell, sub-lane).  A cell is split into M sub-lanes:
-// every sub-lane replays the cell's gene path from the shared stream 0
-// (first-reaction clock) and simulates its own 1/M share of the births and
-// those molecules' fates from stream 1 + sub (direct method).  Molecules never
-// act on the gene, so the sum over sub-lanes has the law of the whole cell.
-// Lanes refill from a global item queue the moment their item ends, so heavy
-// and light particles share the machine without idle lanes.
+// ------------------------------------------------------- shared helpers
+// Random streams of a cell: counter (draw_lo, draw_hi[15:0] | stream << 16,
+// cell, particle); stream 0 holds the set-up draws and the gene clock, stream
+// 1 + sub the molecule events and division coins of sub-lane sub.
 __device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double T, int nb) {
   return ((double)(k - nbB) * T) / (double)nb;
 }
@@ -314,13 +316,293 @@ __device__ __forceinline__ u4 draw(uint64_t ctr, uint32_t stream, uint32_t cell,
   return philox((uint32_t)ctr, ((uint32_t)(ctr >> 32) & 0xFFFFu) | (stream << 16), cell, pid, k0, k1);
 }
 
+// ----- burn-in: before the labelling window molecules are never labelled and
+// never act on the gene or on each other, so given the gene path the observed
+// unspliced and spliced molecules alive at the window start are independent
+// Poisson counts (births a Poisson process of rate alpha(t) beta / M while the
+// gene is on; decay, splicing and halving at division thin every molecule
+// independently; Jahnke & Huisinga, J. Math. Biol. 54, 2007).  The burn-in
+// simulates the gene path only and carries the two Poisson means lu
+// (unspliced: leave at gamma + splice) and lt (all: leave at gamma).
+
+// E1 = int_0^D e^{-k(D-s)} ds, E2 = int_0^D s e^{-k(D-s)} ds, e = e^{-kD}, ik = 1/k
+__device__ __forceinline__ void seg_int(double k, double ik, double D, double e, double& E1,
+                                        double& E2) {
+  const double x = k * D;
+  if (x < 1e-3) {
+    E1 = D * (1.0 - x * (0.5 - x * (1.0 / 6.0 - x * (1.0 / 24.0))));
+    E2 = (D * D) * (0.5 - x * (1.0 / 6.0 - x * (1.0 / 24.0 - x * (1.0 / 120.0))));
+  } else {
+    E1 = (1.0 - e) * ik;
+    E2 = (D - E1) * ik;
+  }
+}
+
+// advance the Poisson means over [t, t + D]: birth rate c0 + c1 s while on;
+// ku, kt are the exit rates (with reciprocals iku, ikt)
+__device__ __forceinline__ void burn_accum(double& lu, double& lt, bool g, double c0, double c1,
+                                           double ku, double iku, double kt, double ikt, double D) {
+  if (!(D > 0.0)) return;
+  const double eu = ex(-(ku * D)), et = ex(-(kt * D));
+  lu = lu * eu;
+  lt = lt * et;
+  if (g) {
+    double E1u, E2u, E1t, E2t;
+    seg_int(ku, iku, D, eu, E1u, E2u);
+    seg_int(kt, ikt, D, et, E1t, E2t);
+    lu = lu + (c0 * E1u + c1 * E2u);
+    lt = lt + (c0 * E1t + c1 * E2t);
+  }
+}
+
+// consecutive words of a sub-lane's molecule stream
+struct WordSrc {
+  u4 w;
+  int i;  // next word; 4 = empty
+  uint64_t& ctr;
+  uint32_t stream, cell, pid, k0, k1;
+  unsigned long long& st;
+  __device__ uint32_t next() {
+    if (i == 4) {
+      w = draw(ctr++, stream, cell, pid, k0, k1);
+      st++;
+      i = 0;
+    }
+    const uint32_t v = i == 0 ? w.x : (i == 1 ? w.y : (i == 2 ? w.z : w.w));
+    i++;
+    return v;
+  }
+};
+
+// ln(k!) for integer-valued k >= 0: table below 10, Stirling series above
+__device__ __forceinline__ double logfact(double kd, const LogEntry* lt) {
+  if (kd < 10.0) return abc_logfact[(int)kd];
+  const double x = kd + 1.0, ix = 1.0 / x, ix2 = ix * ix;
+  return ((x - 0.5) * ln(x, lt) - x) +
+         (0x1.d67f1c864beb5p-1 +
+          ix * (1.0 / 12.0 - ix2 * (1.0 / 360.0 - ix2 * (1.0 / 1260.0 - ix2 * (1.0 / 1680.0)))));
+}
+
+// Poisson(mu): inversion below 10, else the transformed rejection method with
+// squeeze (Hormann, Insurance: Math. Econ. 12, 1993), at most 1000 tries.
+__device__ uint32_t poisson(WordSrc& src, double mu, const LogEntry* lt) {
+  if (!(mu > 0.0)) return 0u;
+  if (mu < 10.0) {
+    const uint32_t w0 = src.next(), w1 = src.next();
+    const double u = u01_53(w0, w1);
+    double p = ex(-mu), F = p;
+    uint32_t k = 0;
+    while (u > F && k < 200u) {
+      k++;
+      p = (p * mu) / (double)k;
+      F = F + p;
+    }
+    return k;
+  }
+  const double lmu = ln(mu, lt), smu = ex(0.5 * lmu);
+  const double bb = 0.931 + 2.53 * smu, aa = -0.059 + 0.02483 * bb;
+  const double inva = 1.1239 + 1.1328 / (bb - 3.4), vr = 0.9277 - 3.6224 / (bb - 2.0);
+  for (int tries = 0; tries < 1000; ++tries) {
+    const double U = u01_32(src.next()) - 0.5;
+    const double V = u01_32(src.next());
+    const double us = 0.5 - fabs(U);
+    const double kd = floor(((2.0 * aa) / us + bb) * U + mu + 0.43);
+    if (kd < 0.0) continue;
+    if (us >= 0.07 && V <= vr) return (uint32_t)kd;
+    if (us < 0.013 && V > us) continue;
+    if ((ln(V, lt) + ln(inva, lt)) - ln(aa / (us * us) + bb, lt) <= (kd * lmu - mu) - logfact(kd, lt))
+      return (uint32_t)kd;
+  }
+  return (uint32_t)floor(mu + 0.5);
+}
+
+// Set-up of cell c of particle p on its stream 0 (counter 0 ...): age and bin,
+// the gene's state from its periodic stationary law, the capture efficiency.
+struct CellSetup {
+  double age, bc;
+  int bin;
+  bool g;
+};
+
+__device__ __forceinline__ CellSetup cell_setup(const DesignDev* __restrict__ D, const PRates* rp,
+                                                int c, uint32_t pid, uint32_t k0, uint32_t k1,
+                                                const double* __restrict__ betas,
+                                                const int64_t* __restrict__ goff, uint64_t& gctr) {
+  const double T = D->T;
+  const int nb = D->nb, J = D->J;
+  CellSetup o;
+  const u4 w = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+  if (D->age_mode == 1) {
+    const double th = u01_53(w.x, w.y);
+    o.age = th * T;
+    o.bin = (int)(th * (double)nb);
+    if (o.bin >= nb) o.bin = nb - 1;
+  } else {
+    o.bin = (c / J) % nb;
+    o.age = ((double)(2 * o.bin + 1) / (double)(2 * nb)) * T;
+  }
+  o.g = (uint64_t)w.z < rp->thr_pi;
+  const int cond = c % J;
+  if (D->cap_mode == 2) {
+    const int grp = o.bin * 2 + D->is_chase[cond];
+    const int64_t off0 = goff[grp];
+    const uint32_t cnt = (uint32_t)(goff[grp + 1] - off0);
+    o.bc = betas[off0 + (int64_t)lemire(w.w, cnt)];
+  } else if (D->cap_mode == 1) {
+    // Beta(a, b) = a-th smallest of a+b-1 uniforms
+    const int a = D->beta_a, nu = D->beta_a + D->beta_b - 1;
+    uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
+    u4 v{0, 0, 0, 0};
+    for (int qq = 0; qq < nu; ++qq) {
+      if ((qq & 3) == 0) v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+      const int sel = qq & 3;
+      const uint32_t x = sel == 0 ? v.x : (sel == 1 ? v.y : (sel == 2 ? v.z : v.w));
+      if (x < s3) {
+        s3 = x;
+        if (s3 < s2) { const uint32_t tt = s2; s2 = s3; s3 = tt; }
+        if (s2 < s1) { const uint32_t tt = s1; s1 = s2; s2 = tt; }
+        if (s1 < s0) { const uint32_t tt = s0; s0 = s1; s1 = tt; }
+      }
+    }
+    o.bc = u01_32(a == 1 ? s0 : (a == 2 ? s1 : (a == 3 ? s2 : s3)));
+  } else {
+    o.bc = 1.0;
+  }
+  return o;
+}
+
+// Burn-in, one lane per cell (persistent, cells of the heaviest particles
+// first): the gene path from the start of the burn-in to the labelling window
+// start ws, carrying the Poisson means of the cell's observed unspliced (lu)
+// and all (lt) molecules; the cell's state at ws goes to rec[p * C + c].
+__global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restrict__ D,
+                                                      const PRates* __restrict__ R,
+                                                      const int32_t* __restrict__ order, int64_t n,
+                                                      uint32_t first, uint32_t k0, uint32_t k1,
+                                                      const double* __restrict__ betas,
+                                                      const int64_t* __restrict__ goff,
+                                                      unsigned long long* __restrict__ queue,
+                                                      CellStart* __restrict__ rec,
+                                                      unsigned long long* __restrict__ steps_out,
+                                                      unsigned long long* __restrict__ react_out,
+                                                      const int64_t* __restrict__ pidx) {
+  const double T = D->T;
+  const int nb = D->nb, J = D->J, C = D->C;
+  const unsigned long long total = (unsigned long long)n * (unsigned long long)C;
+  __shared__ LogEntry ltab[ABC_LOG_N];
+  load_log_table(ltab);
+  __syncthreads();
+  for (;;) {
+    const unsigned long long q = atomicAdd(queue, 1ull);
+    if (q >= total) break;
+    const int p = order[q / (unsigned long long)C];
+    const int c = (int)(q % (unsigned long long)C);
+    const PRates* rp = R + p;
+    const uint32_t pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
+    uint64_t gctr = 0;
+    const CellSetup cs0 = cell_setup(D, rp, c, pid, k0, k1, betas, goff, gctr);
+    const int cond = c % J;
+    const double age = cs0.age, bc = cs0.bc;
+    const double ws = age - D->pulse[cond] - D->chase[cond];
+    const int step = rp->step;
+    const bool scaling = rp->scaling, gvar = rp->gene_varies;
+    const double bs = rp->bs;
+    const int64_t nbB = (int64_t)rp->B * nb;
+    int64_t k = 0, knext = step;
+    int j = 0;
+    double t = grid_time(0, nbB, T, nb), cs = t, gk = grid_time(knext, nbB, T, nb), tg = 0.0;
+    double lu = 0.0, lt = 0.0;
+    bool g = cs0.g, gneed = true;
+    double ikon = 1.0 / rp->kon[0], ikoff = 1.0 / rp->koff[0];
+    double atx = rp->amax[0] * bc, gam = rp->gam[0], bound = D->bound[0];
+    double iku = 1.0 / (gam + bs), ikt = 1.0 / gam, abnd = atx / bound;
+    unsigned long long st = 0, re = 0;
+    for (;;) {
+      double tb = age;
+      if (gk < tb) tb = gk;
+      if (ws < tb) tb = ws;
+      if (gneed) {
+        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+        st++;
+        tg = t + (-ln(u01_32(v.x), ltab)) * (g ? ikoff : ikon);
+        gneed = false;
+      }
+      // births at rate atx (T + a) / bound_j (size scaling) or atx, a = t - cycle start
+      {
+        const double t1 = tg < tb ? tg : tb;
+        double c0 = atx, c1 = 0.0;
+        if (scaling) {
+          c1 = abnd;
+          c0 = c1 * (T + (t - cs));
+        }
+        burn_accum(lu, lt, g, c0, c1, gam + bs, iku, gam, ikt, t1 - t);
+      }
+      if (tg < tb) {
+        t = tg;
+        g = !g;
+        re++;
+        gneed = true;
+        continue;
+      }
+      t = tb;
+      if (tb == gk) {
+        k = knext;
+        knext += step;
+        j = (int)(k % nb);
+        if (j == 0) {
+          cs = tb;
+          lu = lu * 0.5;  // division halves the Poisson means
+          lt = lt * 0.5;
+        }
+        if (step == 1) {
+          ikon = 1.0 / rp->kon[j];
+          ikoff = 1.0 / rp->koff[j];
+          atx = rp->amax[j] * bc;
+          gam = rp->gam[j];
+          bound = D->bound[j];
+          iku = 1.0 / (gam + bs);
+          ikt = 1.0 / gam;
+          abnd = atx / bound;
+        }
+        if (gvar) gneed = true;  // the gene's rate changed: restart its clock
+        gk = grid_time(knext, nbB, T, nb);
+      }
+      if (tb == ws) break;
+    }
+    CellStart r;
+    r.age = age;
+    r.bc = bc;
+    r.tg = tg;
+    r.lu = lu;
+    r.lt = lt;
+    r.k = k;
+    r.gctr = gctr;
+    r.bin = (uint8_t)cs0.bin;
+    r.g = (uint8_t)g;
+    r.gneed = (uint8_t)gneed;
+    r.pad = 0;
+    rec[(size_t)p * (size_t)C + (size_t)c] = r;
+    atomicAdd(steps_out + p, st);
+    atomicAdd(react_out + p, re);
+  }
+}
+
+// ---------------------------------------------------------------- the SSA
+// Labelling window of a work item = (particle, cell, sub-lane).  Sub-lane sub
+// draws its unlabelled molecules at the window start, Poisson(lu / M)
+// unspliced and Poisson((lt - lu) / M) spliced, from its own stream 1 + sub,
+// then simulates the four species to the sampling age: it replays the cell's
+// gene path from the shared stream 0 (continued from the burn-in) and carries
+// its own 1/M share of the births and those molecules' fates (direct method).
+// Molecules never act on the gene, so the sum over sub-lanes has the law of
+// the whole cell.  Lanes refill from a global item queue the moment their item
+// ends, so heavy and light particles share the machine without idle lanes.
 __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict__ D,
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
                                                      const int64_t* __restrict__ off, int64_t n,
                                                      uint32_t first, uint32_t k0, uint32_t k1,
-                                                     const double* __restrict__ betas,
-                                                     const int64_t* __restrict__ goff,
+                                                     const CellStart* __restrict__ rec,
                                                      unsigned long long* __restrict__ queue,
                                                      uint32_t* __restrict__ out_u,
                                                      uint32_t* __restrict__ out_l,
@@ -330,7 +612,6 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
                                                      const int64_t* __restrict__ pidx) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
-  const int age_mode = D->age_mode, cap_mode = D->cap_mode;
   const unsigned long long total = (unsigned long long)off[n];
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
@@ -338,17 +619,16 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
 
   // per-lane item state
   bool have = false;
-  int p = 0, c = 0, bin = 0, j = 0, step = 1, scaling = 0, gvar = 0;
+  int p = 0, c = 0, j = 0, step = 1, scaling = 0, gvar = 0;
   uint32_t pid = 0, sstream = 1, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
   uint64_t gctr = 0, mctr = 0, thr_lam = 0;
   int64_t k = 0, knext = 0, nbB = 0;
-  double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, ws = 0.0, we = 0.0, cs = 0.0;
+  double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, we = 0.0, cs = 0.0;
   double ikon = 0.0, ikoff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0, share = 1.0;
-  bool g = false, ws_done = false, in_win = false, div = false;
+  bool g = false, in_win = false, div = false;
   int div_s = 0;
   uint32_t div_rem = 0, div_keep = 0;
   unsigned long long st = 0, re = 0;
-  int sub = 0;
   // gene clock to (re)draw; pending molecule candidate (its residual Exp(1)
   // hazard and unused words)
   bool gneed = false, pend = false;
@@ -370,79 +650,54 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       const int M = rp->M;
       const int64_t rem = (int64_t)(q - (unsigned long long)off[lo]);
       c = (int)(rem / M);
-      sub = (int)(rem - (int64_t)c * M);
+      const int sub = (int)(rem - (int64_t)c * M);
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       sstream = 1u + (uint32_t)sub;
       share = (double)M;
-      gctr = 0;
+      const CellStart r0 = rec[(size_t)p * (size_t)C + (size_t)c];
+      st = 0;
+      re = 0;
       mctr = 0;
-      u4 w = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
-      if (age_mode == 1) {
-        const double th = u01_53(w.x, w.y);
-        age = th * T;
-        bin = (int)(th * (double)nb);
-        if (bin >= nb) bin = nb - 1;
-      } else {
-        bin = (c / J) % nb;
-        age = ((double)(2 * bin + 1) / (double)(2 * nb)) * T;
+      {
+        // the sub-lane's unlabelled molecules at the window start
+        WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, (uint32_t)c, pid, k0, k1, st};
+        const double ls = r0.lt - r0.lu;
+        n0 = poisson(src, r0.lu / share, ltab);
+        n1 = poisson(src, ls > 0.0 ? ls / share : 0.0, ltab);
       }
-      g = (uint64_t)w.z < rp->thr_pi;
+      n2 = n3 = 0;
       const int cond = c % J;
-      if (cap_mode == 2) {
-        const int grp = bin * 2 + D->is_chase[cond];
-        const int64_t o = goff[grp];
-        const uint32_t cnt = (uint32_t)(goff[grp + 1] - o);
-        bc = betas[o + (int64_t)lemire(w.w, cnt)];
-      } else if (cap_mode == 1) {
-        // Beta(a, b) = a-th smallest of a+b-1 uniforms
-        const int a = D->beta_a, nu = D->beta_a + D->beta_b - 1;
-        uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
-        u4 v{0, 0, 0, 0};
-        for (int qq = 0; qq < nu; ++qq) {
-          if ((qq & 3) == 0) v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
-          const int sel = qq & 3;
-          const uint32_t x = sel == 0 ? v.x : (sel == 1 ? v.y : (sel == 2 ? v.z : v.w));
-          if (x < s3) {
-            s3 = x;
-            if (s3 < s2) { const uint32_t tt = s2; s2 = s3; s3 = tt; }
-            if (s2 < s1) { const uint32_t tt = s1; s1 = s2; s2 = tt; }
-            if (s1 < s0) { const uint32_t tt = s0; s0 = s1; s1 = tt; }
-          }
-        }
-        bc = u01_32(a == 1 ? s0 : (a == 2 ? s1 : (a == 3 ? s2 : s3)));
-      } else {
-        bc = 1.0;
-      }
-      ws = age - D->pulse[cond] - D->chase[cond];
+      age = r0.age;
+      bc = r0.bc;
       we = age - D->chase[cond];
+      if (sub == 0) out_bin[(size_t)p * (size_t)C + (size_t)c] = r0.bin;
       step = rp->step;
       scaling = rp->scaling;
       gvar = rp->gene_varies;
       thr_lam = rp->thr_lam;
       bs = rp->bs;
       nbB = (int64_t)rp->B * nb;
-      k = 0;
-      knext = step;
-      j = 0;
-      t = grid_time(0, nbB, T, nb);
-      cs = t;
-      ikon = 1.0 / rp->kon[0];  // gene clock: waiting time = E * (1 / rate)
-      ikoff = 1.0 / rp->koff[0];
-      atx = (rp->amax[0] * bc) / share;
-      gam = rp->gam[0];
-      bound = D->bound[0];
-      n0 = n1 = n2 = n3 = 0;
-      ws_done = false;
-      in_win = false;
-      div = false;
+      k = r0.k;
+      knext = k + step;
+      j = (int)(k % nb);
+      cs = grid_time(k - j, nbB, T, nb);
       gk = grid_time(knext, nbB, T, nb);
+      t = age - D->pulse[cond] - D->chase[cond];
+      gctr = r0.gctr;
+      g = r0.g != 0;
+      tg = r0.tg;
+      gneed = r0.gneed != 0;
+      ikon = 1.0 / rp->kon[j];  // gene clock: waiting time = E * (1 / rate)
+      ikoff = 1.0 / rp->koff[j];
+      atx = (rp->amax[j] * bc) / share;
+      gam = rp->gam[j];
+      bound = D->bound[j];
+      in_win = true;
+      div = false;
+      pend = false;
       tb = age;
       if (gk < tb) tb = gk;
-      if (ws < tb) tb = ws;
-      st = 0;
-      re = 0;
-      gneed = true;  // the first iteration draws the gene clock
-      pend = false;
+      if (we < tb) tb = we;
       have = true;
     }
 
@@ -543,13 +798,12 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       gneed = true;
       continue;
     }
-    // boundary: sampling age, rate-bin / cycle edge, or labelling window edge
+    // boundary: sampling age, rate-bin / cycle edge, or the window end
     t = tb;
     if (tb == age) {
       const size_t o = (size_t)p * (size_t)C + (size_t)c;
       atomicAdd(out_u + o, n0 + n1);
       atomicAdd(out_l + o, n2 + n3);
-      if (sub == 0) out_bin[o] = (uint8_t)bin;
       atomicAdd(steps_out + p, st);
       atomicAdd(react_out + p, re);
       have = false;
@@ -580,11 +834,9 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       if (gvar) gneed = true;  // the gene's rate changed: restart its clock (memoryless)
       gk = grid_time(knext, nbB, T, nb);
     }
-    if (!ws_done && tb == ws) { ws_done = true; in_win = true; }
     if (in_win && tb == we) in_win = false;
     tb = age;
     if (gk < tb) tb = gk;
-    if (!ws_done && ws < tb) tb = ws;
     if (in_win && we < tb) tb = we;
   }
 }

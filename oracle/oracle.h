@@ -104,6 +104,8 @@ int64_t orc_moment_solve(int model, const double* theta, double cycle, int n_ste
                          double texp, double pulse, double tmin, double tmax, double* y,
                          double rtol, double atol);
 void orc_periodic_boundary(const double* in, double* out);
+/* n draws of the SSA's Poisson sampler (inversion below 10, PTRS above). */
+void orc_poisson_sample(double mu, uint64_t seed, int64_t n, uint32_t* out);
 /* transient_phase (model.jl:114-142); returns iterations. */
 int orc_transient(int model, const double* theta, const orc_moment_cfg* c, double* ss);
 /* run_part_sim without downsampling (recover_statistics.jl:2-11):

@@ -83,6 +83,7 @@ def _declare(L):
     L.orc_error_pair.restype = D
     L.orc_errors.argtypes = [I64, P, C.c_int, I64, P, P, P, C.c_int, D, P, C.c_int]
     L.orc_accept_gene.argtypes = [P, I64, D, P, P]; L.orc_accept_gene.restype = I64
+    L.orc_poisson_sample.argtypes = [D, U64, I64, P]
 
 
 def _p(a):
@@ -193,6 +194,12 @@ def simulate(od: OracleDesign, model, seed, first, theta, n_threads=0, cells=Tru
         _p(out["steps"]), _p(out["reactions"]))
     if rc != 0:
         raise RuntimeError(f"orc_simulate failed: {rc}")
+    return out
+
+
+def poisson_sample(mu, seed, n):
+    out = np.zeros(n, np.uint32)
+    lib().orc_poisson_sample(C.c_double(mu), C.c_uint64(seed), n, _p(out))
     return out
 
 

@@ -28,6 +28,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+
+#define ABC_LOGTAB_QUAL static const
+#include "abc_logtab.h"
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -143,6 +146,15 @@ typedef struct {
  * per cell) <= split_target * M, capped at split_max.  Molecules never act on
  * the gene, so M independent 1/M-thinned birth streams over one shared gene
  * path have the law of the full birth stream. */
+static double window_max(const orc_design* d) {
+  double wmax = 0.0;
+  for (int j = 0; j < d->n_conds; ++j) {
+    double w = d->pulse[j] + d->chase[j];
+    if (w > wmax) wmax = w;
+  }
+  return wmax;
+}
+
 static int split_factor(const orc_design* d, const prates* r) {
   if (!(d->split_target > 0.0)) return 1;
   const int nb = d->n_bins;
@@ -150,19 +162,15 @@ static int split_factor(const orc_design* d, const prates* r) {
   for (int j = 0; j < nb; ++j) births = births + r->amax[j] * (r->kon[j] / (r->kon[j] + r->koff[j]));
   double scale = d->capture_mode == 1 ? (double)d->beta_a / (double)(d->beta_a + d->beta_b)
                                       : (d->capture_mode == 2 ? 0.25 : 1.0);
-  double est = (((births / (double)nb) * scale) * (d->cycle * ((double)r->B + 1.0))) * 3.0;
+  /* molecule events are simulated over the labelling window only */
+  double est = (((births / (double)nb) * scale) * window_max(d)) * 3.0;
   int M = 1;
   while (M < d->split_max && est > d->split_target * (double)M) M *= 2;
   return M;
 }
 
 static int burn_min(const orc_design* d) {
-  double wmax = 0.0;
-  for (int j = 0; j < d->n_conds; ++j) {
-    double w = d->pulse[j] + d->chase[j];
-    if (w > wmax) wmax = w;
-  }
-  int b = (int)ceil(wmax / d->cycle);
+  int b = (int)ceil(window_max(d) / d->cycle);
   return b < 1 ? 1 : b;
 }
 
@@ -243,15 +251,122 @@ static inline double grid_t(int64_t k, int64_t nbB, double T, int nb) {
 
 /* Exponential waiting time with rate a from one word. */
 
-/* One sub-lane of cell c: the cell's gene path (stream 0) with a 1/M share
- * of its transcription (stream 1 + sub).  Returns its observed u and l. */
-static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], uint32_t pidx,
-                   int c, int sub, uint32_t* u_out, uint32_t* l_out, int* bin_out,
-                   int64_t* steps, int64_t* reactions) {
+/* ---------------------------------------------------- burn-in, Poisson
+ * Before the labelling window molecules are never labelled and never act on
+ * the gene or on each other, so given the gene path the observed unspliced
+ * and spliced molecules alive at the window start are independent Poisson
+ * counts (births: a Poisson process of rate alpha(t) beta / M while the gene is
+ * on; decay, splicing and the halving at division thin every molecule
+ * independently; Jahnke & Huisinga, J. Math. Biol. 54, 2007).  The burn-in
+ * therefore simulates the gene path only and carries the two Poisson means:
+ * lu (unspliced, leaving at gamma + splice) and lt (all, leaving at gamma). */
+
+/* E1 = int_0^D e^{-k(D-s)} ds and E2 = int_0^D s e^{-k(D-s)} ds, e = e^{-kD},
+ * ik = 1/k */
+static void seg_int(double k, double ik, double D, double e, double* E1, double* E2) {
+  const double x = k * D;
+  if (x < 1e-3) {
+    *E1 = D * (1.0 - x * (0.5 - x * (1.0 / 6.0 - x * (1.0 / 24.0))));
+    *E2 = (D * D) * (0.5 - x * (1.0 / 6.0 - x * (1.0 / 24.0 - x * (1.0 / 120.0))));
+  } else {
+    *E1 = (1.0 - e) * ik;
+    *E2 = (D - *E1) * ik;
+  }
+}
+
+/* Advance the Poisson means over [t, t + D]: birth rate c0 + c1 s while the
+ * gene is on; ku, kt are the exit rates of the unspliced and of all molecules. */
+static void burn_accum(double* lu, double* lt, int g, double c0, double c1, double ku,
+                       double kt, double D) {
+  if (!(D > 0.0)) return;
+  const double eu = orc_exp(-(ku * D)), et = orc_exp(-(kt * D));
+  *lu = *lu * eu;
+  *lt = *lt * et;
+  if (g) {
+    double E1u, E2u, E1t, E2t;
+    seg_int(ku, 1.0 / ku, D, eu, &E1u, &E2u);
+    seg_int(kt, 1.0 / kt, D, et, &E1t, &E2t);
+    *lu = *lu + (c0 * E1u + c1 * E2u);
+    *lt = *lt + (c0 * E1t + c1 * E2t);
+  }
+}
+
+typedef struct { uint32_t w[4]; int i; } wordbuf;  /* i = next word; 4 = empty */
+
+static uint32_t next_word(cell_stream* s, wordbuf* b, int64_t* st) {
+  if (b->i == 4) { draw(s, b->w); (*st)++; b->i = 0; }
+  return b->w[b->i++];
+}
+
+/* ln(k!) for integer-valued k >= 0: table below 10, Stirling series above */
+static double logfact(double kd) {
+  if (kd < 10.0) return abc_logfact[(int)kd];
+  const double x = kd + 1.0, ix = 1.0 / x, ix2 = ix * ix;
+  return ((x - 0.5) * orc_log(x) - x) +
+         (0x1.d67f1c864beb5p-1 + ix * (1.0 / 12.0 - ix2 * (1.0 / 360.0 - ix2 * (1.0 / 1260.0 - ix2 * (1.0 / 1680.0)))));
+}
+
+/* Poisson(mu): inversion below 10, else the transformed rejection method with
+ * squeeze (Hormann, Insurance: Math. Econ. 12, 1993), at most 1000 tries. */
+static uint32_t poisson(cell_stream* s, wordbuf* b, double mu, int64_t* st) {
+  if (!(mu > 0.0)) return 0u;
+  if (mu < 10.0) {
+    const uint32_t w0 = next_word(s, b, st), w1 = next_word(s, b, st);
+    const double u = orc_u01_53(w0, w1);
+    double p = orc_exp(-mu), F = p;
+    uint32_t k = 0;
+    while (u > F && k < 200u) {
+      k++;
+      p = (p * mu) / (double)k;
+      F = F + p;
+    }
+    return k;
+  }
+  const double lmu = orc_log(mu), smu = orc_exp(0.5 * lmu);
+  const double bb = 0.931 + 2.53 * smu, aa = -0.059 + 0.02483 * bb;
+  const double inva = 1.1239 + 1.1328 / (bb - 3.4), vr = 0.9277 - 3.6224 / (bb - 2.0);
+  for (int tries = 0; tries < 1000; ++tries) {
+    const double U = orc_u01_32(next_word(s, b, st)) - 0.5;
+    const double V = orc_u01_32(next_word(s, b, st));
+    const double us = 0.5 - fabs(U);
+    const double kd = floor(((2.0 * aa) / us + bb) * U + mu + 0.43);
+    if (kd < 0.0) continue;
+    if (us >= 0.07 && V <= vr) return (uint32_t)kd;
+    if (us < 0.013 && V > us) continue;
+    if ((orc_log(V) + orc_log(inva)) - orc_log(aa / (us * us) + bb) <= (kd * lmu - mu) - logfact(kd))
+      return (uint32_t)kd;
+  }
+  return (uint32_t)floor(mu + 0.5);
+}
+
+/* Test hook: n Poisson(mu) draws from the stream (key, cell = 0, pidx = 0,
+ * stream 1), consumed as in a window start. */
+void orc_poisson_sample(double mu, uint64_t seed, int64_t n, uint32_t* out) {
+  cell_stream s = {{(uint32_t)seed, (uint32_t)(seed >> 32)}, 0u, 0u, 1u, 0};
+  wordbuf wb = {{0, 0, 0, 0}, 4};
+  int64_t st = 0;
+  for (int64_t i = 0; i < n; ++i) out[i] = poisson(&s, &wb, mu, &st);
+}
+
+/* A cell at the start of its labelling window, shared by its sub-lanes. */
+typedef struct {
+  double age, bc, tg, lu, lt;
+  int64_t k;         /* last grid edge passed */
+  uint64_t gctr;     /* gene-stream counter */
+  int bin, g, gneed;
+} cell_start;
+
+/* Burn-in of cell c (once per cell): set-up draws, then the gene path from
+ * the start of the burn-in to the window start ws with the Poisson means of
+ * the cell's unspliced (lu) and all (lt) observed molecules.  The gene clock
+ * is a first-reaction clock on stream 0, redrawn after every switch and at
+ * every edge where the gene's rate changes. */
+static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[2],
+                     uint32_t pidx, int c, cell_start* out, int64_t* steps,
+                     int64_t* reactions) {
   const int nb = d->n_bins, J = d->n_conds;
   const double T = d->cycle;
   cell_stream s0 = {{key[0], key[1]}, (uint32_t)c, pidx, 0u, 0};
-  cell_stream sm = {{key[0], key[1]}, (uint32_t)c, pidx, 1u + (uint32_t)sub, 0};
   uint32_t w[4], v[4];
   draw(&s0, w);
   int bin;
@@ -291,30 +406,109 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
     }
     bc = orc_u01_32(lo[a - 1]);
   }
-  const double pulse = d->pulse[cond], chase = d->chase[cond];
-  const double ws = age - pulse - chase, we = age - chase;
+  const double ws = age - d->pulse[cond] - d->chase[cond];
   const int64_t nbB = (int64_t)r->B * nb;
-  const double inv_share = (double)r->M;
   int64_t k = 0, knext = r->step;
   int j = 0;
-  double t = grid_t(0, nbB, T, nb);
-  int win_started = 0, in_win = 0;
-  uint32_t n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+  double t = grid_t(0, nbB, T, nb), tg = 0.0, lu = 0.0, lt = 0.0;
+  int gneed = 1;
   int64_t st = 0, re = 0;
+  for (;;) {
+    double tb = age;
+    double gk = grid_t(knext, nbB, T, nb);
+    if (gk < tb) tb = gk;
+    if (ws < tb) tb = ws;
+    if (gneed) {
+      /* waiting time = Exp(1) * (1 / rate); the reciprocal is a per-bin constant */
+      draw(&s0, w);
+      st++;
+      tg = t + (-orc_log(orc_u01_32(w[0]))) * (1.0 / (g ? r->koff[j] : r->kon[j]));
+      gneed = 0;
+    }
+    /* births at rate amax_j bc (T + a) / bound_j (size scaling) or amax_j bc,
+     * a = time since the cycle start */
+    {
+      const double t1 = tg < tb ? tg : tb;
+      const double atx = r->amax[j] * bc;
+      double c0 = atx, c1 = 0.0;
+      if (r->scaling) {
+        const double cs = grid_t(k - j, nbB, T, nb);
+        c1 = atx / r->bound[j];
+        c0 = c1 * (T + (t - cs));
+      }
+      burn_accum(&lu, &lt, g, c0, c1, r->gam[j] + r->bs, r->gam[j], t1 - t);
+    }
+    if (tg < tb) {
+      t = tg;
+      g = !g;
+      re++;
+      gneed = 1;
+      continue;
+    }
+    t = tb;
+    if (tb == gk) {
+      k = knext;
+      knext += r->step;
+      j = (int)(k % nb);
+      if (j == 0) {
+        /* division halves the Poisson means (each molecule kept w.p. 1/2) */
+        lu = lu * 0.5;
+        lt = lt * 0.5;
+      }
+      if (r->gene_varies) gneed = 1;  /* the gene's rate changed: restart its clock */
+    }
+    if (tb == ws) break;
+  }
+  out->age = age; out->bc = bc; out->tg = tg; out->lu = lu; out->lt = lt;
+  out->k = k; out->gctr = s0.ctr; out->bin = bin; out->g = g; out->gneed = gneed;
+  *steps += st;
+  *reactions += re;
+  return 0;
+}
+
+/* Labelling window of sub-lane sub of cell c: its unlabelled molecules are
+ * Poisson(lu / M) unspliced and Poisson((lt - lu) / M) spliced, drawn from
+ * its own stream 1 + sub; then the four-species simulation from ws to the
+ * sampling age over the cell's gene path (stream 0, continued from the
+ * burn-in) with a 1/M share of the births. */
+static int sim_window(const orc_design* d, const prates* r, const uint32_t key[2],
+                      uint32_t pidx, int c, int sub, const cell_start* cst, uint32_t* u_out,
+                      uint32_t* l_out, int64_t* steps, int64_t* reactions) {
+  const int nb = d->n_bins, J = d->n_conds;
+  const double T = d->cycle;
+  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c, pidx, 0u, cst->gctr};
+  cell_stream sm = {{key[0], key[1]}, (uint32_t)c, pidx, 1u + (uint32_t)sub, 0};
+  uint32_t w[4];
+  const int cond = c % J;
+  const double age = cst->age, bc = cst->bc;
+  const double we = age - d->chase[cond];
+  const int64_t nbB = (int64_t)r->B * nb;
+  const double inv_share = (double)r->M;
+  int64_t st = 0, re = 0;
+  uint32_t n0, n1, n2 = 0, n3 = 0;
+  {
+    wordbuf wb = {{0, 0, 0, 0}, 4};
+    const double ls = cst->lt - cst->lu;
+    n0 = poisson(&sm, &wb, cst->lu / inv_share, &st);
+    n1 = poisson(&sm, &wb, ls > 0.0 ? ls / inv_share : 0.0, &st);
+  }
+  int64_t k = cst->k, knext = cst->k + r->step;
+  int j = (int)(k % nb);
+  double t = age - d->pulse[cond] - d->chase[cond], tg = cst->tg;
+  int g = cst->g, in_win = 1;
   /* Per iteration one Philox block is drawn: division coins, the gene clock
    * (first-reaction method on the shared stream 0) or a new molecule
    * candidate (direct method on stream 1 + sub).  A candidate pre-empted by
    * the gene clock or a boundary stays pending: its words are still unused and
    * its Exp(1) hazard, less what the interval used up, is again Exp(1)
    * (memoryless).  A gene switch only marks the clock for redrawing. */
-  int gneed = 1, pend = 0, div = 0, div_s = 0;
+  int gneed = cst->gneed, pend = 0, div = 0, div_s = 0;
   uint32_t div_rem = 0, div_keep = 0, pw[3] = {0, 0, 0};
-  double erem = 0.0, tg = 0.0;
+  double erem = 0.0;
   for (;;) {
     double tb = age;
     double gk = grid_t(knext, nbB, T, nb);
     if (gk < tb) tb = gk;
-    if (!win_started && ws < tb) tb = ws;
     if (in_win && we < tb) tb = we;
     /* molecule channels of this sub-lane (direct method) */
     double a_tx = g ? (r->amax[j] * bc) / inv_share : 0.0;
@@ -346,7 +540,6 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       continue;
     }
     if (kind == 2) {
-      /* waiting time = Exp(1) * (1 / rate); the reciprocal is a per-bin constant */
       tg = t + (-orc_log(orc_u01_32(w[0]))) * (1.0 / (g ? r->koff[j] : r->kon[j]));
       gneed = 0;
       if (!pend && a0 > 0.0) continue;   /* a molecule candidate comes first */
@@ -397,7 +590,7 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       gneed = 1;
       continue;
     }
-    /* boundary: sampling age, rate-bin / cycle edge, or labelling window edge */
+    /* boundary: sampling age, rate-bin / cycle edge, or the window end */
     t = tb;
     if (tb == age) break;
     if (tb == gk) {
@@ -414,12 +607,10 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
       }
       if (r->gene_varies) gneed = 1;   /* the gene's rate changed: restart its clock */
     }
-    if (!win_started && tb == ws) { win_started = 1; in_win = 1; }
     if (in_win && tb == we) in_win = 0;
   }
   *u_out = n0 + n1;
   *l_out = n2 + n3;
-  *bin_out = bin;
   *steps += st;
   *reactions += re;
   return 0;
@@ -428,18 +619,20 @@ static int sim_sub(const orc_design* d, const prates* r, const uint32_t key[2], 
 static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
                     uint32_t pidx, int c, uint32_t* u_out, uint32_t* l_out,
                     uint8_t* bin_out, int64_t* steps, int64_t* reactions) {
+  cell_start cst;
+  int e = burn_cell(d, r, key, pidx, c, &cst, steps, reactions);
+  if (e) return e;
   uint32_t u = 0, l = 0;
-  int bin = 0;
   for (int sub = 0; sub < r->M; ++sub) {
     uint32_t su, sl;
-    int e = sim_sub(d, r, key, pidx, c, sub, &su, &sl, &bin, steps, reactions);
+    e = sim_window(d, r, key, pidx, c, sub, &cst, &su, &sl, steps, reactions);
     if (e) return e;
     u += su;
     l += sl;
   }
   *u_out = u;
   *l_out = l;
-  *bin_out = (uint8_t)bin;
+  *bin_out = (uint8_t)cst.bin;
   return 0;
 }
 
