@@ -399,9 +399,9 @@ __device__ uint32_t poisson(WordSrc& src, double mu, const LogEntry* lt) {
     }
     return k;
   }
-  const double lmu = ln(mu, lt), smu = ex(0.5 * lmu);
+  const double smu = ex(0.5 * ln(mu, lt));
   const double bb = 0.931 + 2.53 * smu, aa = -0.059 + 0.02483 * bb;
-  const double inva = 1.1239 + 1.1328 / (bb - 3.4), vr = 0.9277 - 3.6224 / (bb - 2.0);
+  const double vr = 0.9277 - 3.6224 / (bb - 2.0);
   for (int tries = 0; tries < 1000; ++tries) {
     const double U = u01_32(src.next()) - 0.5;
     const double V = u01_32(src.next());
@@ -410,7 +410,10 @@ __device__ uint32_t poisson(WordSrc& src, double mu, const LogEntry* lt) {
     if (kd < 0.0) continue;
     if (us >= 0.07 && V <= vr) return (uint32_t)kd;
     if (us < 0.013 && V > us) continue;
-    if ((ln(V, lt) + ln(inva, lt)) - ln(aa / (us * us) + bb, lt) <= (kd * lmu - mu) - logfact(kd, lt))
+    // rare: the envelope constant and ln(mu) are recomputed here (same bits)
+    const double inva = 1.1239 + 1.1328 / (bb - 3.4);
+    if ((ln(V, lt) + ln(inva, lt)) - ln(aa / (us * us) + bb, lt) <=
+        (kd * ln(mu, lt) - mu) - logfact(kd, lt))
       return (uint32_t)kd;
   }
   return (uint32_t)floor(mu + 0.5);
@@ -654,17 +657,25 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       sstream = 1u + (uint32_t)sub;
       share = (double)M;
-      const CellStart r0 = rec[(size_t)p * (size_t)C + (size_t)c];
+      const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
       st = 0;
       re = 0;
       mctr = 0;
+      // every loop-carried value is reset here, so none stays live across the
+      // sampler below
+      div_s = 0;
+      div_rem = 0;
+      div_keep = 0;
+      erem = 0.0;
+      py = pz = pw = 0;
       {
         // the sub-lane's unlabelled molecules at the window start
         WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, (uint32_t)c, pid, k0, k1, st};
-        const double ls = r0.lt - r0.lu;
-        n0 = poisson(src, r0.lu / share, ltab);
+        const double lu0 = rc->lu, ls = rc->lt - lu0;
+        n0 = poisson(src, lu0 / share, ltab);
         n1 = poisson(src, ls > 0.0 ? ls / share : 0.0, ltab);
       }
+      const CellStart r0 = *rc;
       n2 = n3 = 0;
       const int cond = c % J;
       age = r0.age;
