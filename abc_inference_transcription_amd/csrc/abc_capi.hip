@@ -56,15 +56,16 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
-  // ring of (start, end) event pairs around every ssa_kernel launch
+  // ring of (start, after burn_kernel, after ssa_kernel) events around every
+  // simulation; tally[0] counts burn_kernel's draws since the last reset
   static constexpr int kRing = 256;
-  hipEvent_t ring[kRing][2]{};
+  hipEvent_t ring[kRing][3]{};
   int ring_used = 0;
-  double ssa_ms_acc = 0.0;
+  double burn_ms_acc = 0.0, ssa_ms_acc = 0.0;
   int64_t ssa_launches = 0;
   // stream capture (abc_graph_begin .. abc_graph_end)
   bool capturing = false;
@@ -177,13 +178,15 @@ int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
   return ABC_OK;
 }
 
-// fold the pending ssa_kernel event pairs into the running total
+// fold the pending burn_kernel / ssa_kernel event triples into the running totals
 int drain_ring(abc_ctx* c) {
   for (int q = 0; q < c->ring_used; ++q) {
-    float v = 0.f;
-    HIPCHK(c, hipEventSynchronize(c->ring[q][1]));
-    HIPCHK(c, hipEventElapsedTime(&v, c->ring[q][0], c->ring[q][1]));
-    c->ssa_ms_acc += v;
+    float vb = 0.f, vs = 0.f;
+    HIPCHK(c, hipEventSynchronize(c->ring[q][2]));
+    HIPCHK(c, hipEventElapsedTime(&vb, c->ring[q][0], c->ring[q][1]));
+    HIPCHK(c, hipEventElapsedTime(&vs, c->ring[q][1], c->ring[q][2]));
+    c->burn_ms_acc += vb;
+    c->ssa_ms_acc += vs;
     c->ssa_launches++;
   }
   c->ring_used = 0;
@@ -211,6 +214,14 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   if ((rc = ensure(c, c->cellrec, sizeof(CellStart) * cells))) return rc;
   if ((rc = ensure(c, c->react, sizeof(uint64_t) * n))) return rc;
   if ((rc = ensure(c, c->queue, 64))) return rc;
+  if (!c->tally.p) {
+    // created cleared, outside any capture; abc_ssa_time clears it again
+    if (c->capturing)
+      return fail(c, ABC_E_STATE, "simulate once before capturing it");
+    if ((rc = ensure(c, c->tally, 16))) return rc;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                       P<unsigned long long>(c->tally), (int64_t)2);
+  }
   unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
   if (!steps) {
     if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
@@ -224,26 +235,28 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                      P<unsigned long long>(c->queue), (int64_t)8);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, record(c, c->ev[1]));
-  hipEvent_t* pair = nullptr;
+  hipEvent_t* trio = nullptr;
   if (!c->capturing) {
     if (c->ring_used == abc_ctx::kRing) {
       if ((rc = drain_ring(c))) return rc;
     }
-    pair = c->ring[c->ring_used++];
-    HIPCHK(c, hipEventRecord(pair[0], c->stream));
+    trio = c->ring[c->ring_used++];
+    HIPCHK(c, hipEventRecord(trio[0], c->stream));
   }
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(burn_kernel, dim3(c->burn_blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
                      key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
-                     P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev);
+                     P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev,
+                     P<unsigned long long>(c->tally));
+  if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
                      (uint32_t)first, key0(seed), key1(seed, model), P<CellStart>(c->cellrec),
                      P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu), P<uint32_t>(c->cl),
                      P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react), pidx_dev);
   HIPCHK(c, hipGetLastError());
-  if (pair) HIPCHK(c, hipEventRecord(pair[1], c->stream));
+  if (trio) HIPCHK(c, hipEventRecord(trio[2], c->stream));
   HIPCHK(c, record(c, c->ev[2]));
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
@@ -880,16 +893,29 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
   return ABC_OK;
 }
 
-int32_t abc_ssa_time(abc_ctx* c, int32_t reset, double* ms_total, int64_t* launches) {
-  if (!c || !ms_total || !launches) return ABC_E_ARG;
+int32_t abc_ssa_time(abc_ctx* c, int32_t reset, double* ms2, int64_t* burn_steps,
+                     int64_t* launches) {
+  if (!c || !ms2 || !burn_steps || !launches) return ABC_E_ARG;
   NO_CAPTURE(c);
   int rc = drain_ring(c);
   if (rc) return rc;
-  *ms_total = c->ssa_ms_acc;
+  ms2[0] = c->burn_ms_acc;
+  ms2[1] = c->ssa_ms_acc;
   *launches = c->ssa_launches;
+  uint64_t tally = 0;
+  if (c->tally.p) {
+    HIPCHK(c, hipMemcpyAsync(&tally, c->tally.p, sizeof(tally), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  *burn_steps = (int64_t)tally;
   if (reset) {
-    c->ssa_ms_acc = 0.0;
+    c->burn_ms_acc = c->ssa_ms_acc = 0.0;
     c->ssa_launches = 0;
+    if (c->tally.p) {
+      hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                         P<unsigned long long>(c->tally), (int64_t)2);
+      HIPCHK(c, hipGetLastError());
+    }
   }
   return ABC_OK;
 }
@@ -997,8 +1023,13 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
     cand.push_back(c->ssa_blocks_default);
   // keep the caller's running SSA time out of the trials
   if ((rc = drain_ring(c))) return rc;
-  const double acc0 = c->ssa_ms_acc;
+  const double acc0 = c->ssa_ms_acc, bacc0 = c->burn_ms_acc;
   const int64_t l0 = c->ssa_launches;
+  uint64_t tally0 = 0;
+  if (c->tally.p) {
+    HIPCHK(c, hipMemcpyAsync(&tally0, c->tally.p, sizeof(tally0), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   const int saved = c->ssa_blocks;
   auto run = [&](int blocks, double* ms) -> int {
     c->ssa_blocks = blocks;
@@ -1023,7 +1054,12 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
     }
   }
   c->ssa_ms_acc = acc0;
+  c->burn_ms_acc = bacc0;
   c->ssa_launches = l0;
+  if (rc == ABC_OK && c->tally.p) {
+    HIPCHK(c, hipMemcpyAsync(c->tally.p, &tally0, sizeof(tally0), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   if (rc) {
     c->ssa_blocks = saved;
     return rc;

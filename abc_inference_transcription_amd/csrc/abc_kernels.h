@@ -87,7 +87,7 @@ __global__ void burn_kernel(const DesignDev* D, const PRates* R, const int32_t* 
                             uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
                             const int64_t* goff, unsigned long long* queue, CellStart* rec,
                             unsigned long long* steps_out, unsigned long long* react_out,
-                            const int64_t* pidx);
+                            const int64_t* pidx, unsigned long long* burn_tally);
 __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
                            const int64_t* off, int64_t n, uint32_t first, uint32_t k0,
                            uint32_t k1, const CellStart* rec, unsigned long long* queue,

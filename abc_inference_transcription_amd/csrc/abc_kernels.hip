@@ -488,10 +488,12 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
                                                       CellStart* __restrict__ rec,
                                                       unsigned long long* __restrict__ steps_out,
                                                       unsigned long long* __restrict__ react_out,
-                                                      const int64_t* __restrict__ pidx) {
+                                                      const int64_t* __restrict__ pidx,
+                                                      unsigned long long* __restrict__ burn_tally) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
   const unsigned long long total = (unsigned long long)n * (unsigned long long)C;
+  unsigned long long lane_steps = 0;  // this lane's burn-in draws, added to *burn_tally at exit
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
   __syncthreads();
@@ -587,7 +589,9 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     rec[(size_t)p * (size_t)C + (size_t)c] = r;
     atomicAdd(steps_out + p, st);
     atomicAdd(react_out + p, re);
+    lane_steps += st;
   }
+  atomicAdd(burn_tally, lane_steps);
 }
 
 // ---------------------------------------------------------------- the SSA

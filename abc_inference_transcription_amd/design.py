@@ -49,7 +49,7 @@ def reference_design(cells: int = 550, capture_mode: int = CAPTURE_NONE, betas=N
                 betas=betas, beta_bin=beta_bin, beta_is_chase=beta_is_chase,
                 weights=uniform_weights(nb, J) if weights is None else np.asarray(weights),
                 burn_eps=burn_eps, burn_max=16, prior_mode=PRIOR_REFERENCE,
-                splice_log10=0.0, split_target=65536.0, split_max=1024)
+                splice_log10=0.0, split_target=32768.0, split_max=1024)
 
 
 def config_design(cells: int = 2500, prior_mode: int = PRIOR_CONFIG,
@@ -62,7 +62,7 @@ def config_design(cells: int = 2500, prior_mode: int = PRIOR_CONFIG,
                 age_mode=AGE_UNIFORM, capture_mode=capture_mode, beta_a=2, beta_b=8,
                 betas=None, beta_bin=None, beta_is_chase=None,
                 weights=uniform_weights(nb, J), burn_eps=burn_eps, burn_max=16,
-                prior_mode=prior_mode, splice_log10=0.0, split_target=65536.0, split_max=1024)
+                prior_mode=prior_mode, splice_log10=0.0, split_target=32768.0, split_max=1024)
 
 
 def n_ref_params(model: int) -> int:

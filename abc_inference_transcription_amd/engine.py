@@ -48,7 +48,7 @@ class Engine:
             None if bbin is None else bbin.ctypes.data, None if bch is None else bch.ctypes.data,
             nbeta, w.ctypes.data, float(d.get("burn_eps", 5e-3)), int(d.get("burn_max", 16)),
             int(d["prior_mode"]), float(d.get("splice_log10", 0.0)),
-            float(d.get("split_target", 65536.0)), int(d.get("split_max", 1024)))
+            float(d.get("split_target", 32768.0)), int(d.get("split_max", 1024)))
         ctx = C.c_void_p()
         rc = self.L.abc_create(C.byref(ctx), device, C.byref(s))
         if rc != 0:

@@ -64,7 +64,7 @@ typedef struct {
   int32_t prior_mode;           /* 0 = fix_params ranges, 1 = config priors          */
   double splice_log10;          /* splicing rate for prior_mode 0                   */
   double split_target;          /* expected molecule events per sub-lane (0 = one
-                                   lane per cell), e.g. 65536                       */
+                                   lane per cell), e.g. 32768                       */
   int32_t split_max;            /* sub-lanes per cell cap, 1..4096, e.g. 1024       */
 } abc_design;
 
@@ -175,10 +175,16 @@ int32_t abc_errors_dev(abc_ctx* ctx, int64_t n, int32_t K, const double* stats_d
  * kernel, [3] error kernel.  Waits for those events. */
 int32_t abc_last_timing(abc_ctx* ctx, double* ms4);
 
-/* Total milliseconds spent in ssa_kernel over every simulate call since the
- * last reset (HIP events around each launch on the context stream), and the
- * number of launches.  Waits for the pending events; reset != 0 clears. */
-int32_t abc_ssa_time(abc_ctx* ctx, int32_t reset, double* ms_total, int64_t* launches);
+/* SSA work since the last reset, over every simulate call made outside graph
+ * capture: ms2[0] milliseconds in burn_kernel (each cell's gene path and
+ * Poisson means up to its labelling window), ms2[1] milliseconds in
+ * ssa_kernel (the window), both from HIP events on the context stream;
+ * *burn_steps the Philox draws burn_kernel spent (graph replays included;
+ * the per-particle steps output holds both kernels' draws, so the window's
+ * are its sum minus this); *launches the simulate calls timed.  Waits for the
+ * pending work; reset != 0 clears all of them. */
+int32_t abc_ssa_time(abc_ctx* ctx, int32_t reset, double* ms2, int64_t* burn_steps,
+                     int64_t* launches);
 
 /* ---- surrogate (no reference counterpart: the reference simulates every
  * prior particle, scripts/abc_simulation.jl:88-97).  An MLP maps a

@@ -106,6 +106,12 @@ int64_t orc_moment_solve(int model, const double* theta, double cycle, int n_ste
 void orc_periodic_boundary(const double* in, double* out);
 /* n draws of the SSA's Poisson sampler (inversion below 10, PTRS above). */
 void orc_poisson_sample(double mu, uint64_t seed, int64_t n, uint32_t* out);
+/* burn-in records of cells 0..n_cells-1 of one particle: 8 doubles per cell
+ * (age, capture, lu, lt, gene, bin, B, window start) */
+int orc_burn_cells(const orc_design* d, int model, uint64_t seed, uint32_t pidx,
+                   const double* theta, int n_cells, double* out);
+/* one burn-in interval applied to the Poisson means io = {lu, lt} */
+void orc_burn_step(double* io, int g, double c0, double c1, double ku, double kt, double D);
 /* transient_phase (model.jl:114-142); returns iterations. */
 int orc_transient(int model, const double* theta, const orc_moment_cfg* c, double* ss);
 /* run_part_sim without downsampling (recover_statistics.jl:2-11):

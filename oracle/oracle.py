@@ -84,6 +84,9 @@ def _declare(L):
     L.orc_errors.argtypes = [I64, P, C.c_int, I64, P, P, P, C.c_int, D, P, C.c_int]
     L.orc_accept_gene.argtypes = [P, I64, D, P, P]; L.orc_accept_gene.restype = I64
     L.orc_poisson_sample.argtypes = [D, U64, I64, P]
+    L.orc_burn_step.argtypes = [P, C.c_int, D, D, D, D, D]
+    L.orc_burn_cells.argtypes = [P, C.c_int, U64, C.c_uint32, P, C.c_int, P]
+    L.orc_burn_cells.restype = C.c_int
 
 
 def _p(a):
@@ -154,7 +157,7 @@ class OracleDesign:
             int(d.get("beta_a", 2)), int(d.get("beta_b", 8)), _p(self.betas),
             _p(self.group_off), _p(self.weights), float(d.get("burn_eps", 5e-3)),
             int(d.get("burn_max", 16)), int(d["prior_mode"]), float(d.get("splice_log10", 0.0)),
-            float(d.get("split_target", 65536.0)), int(d.get("split_max", 1024)))
+            float(d.get("split_target", 32768.0)), int(d.get("split_max", 1024)))
 
     @property
     def ref(self):
@@ -201,6 +204,24 @@ def poisson_sample(mu, seed, n):
     out = np.zeros(n, np.uint32)
     lib().orc_poisson_sample(C.c_double(mu), C.c_uint64(seed), n, _p(out))
     return out
+
+
+def burn_cells(od: OracleDesign, model, seed, pidx, theta_row, n_cells):
+    """Burn-in records of the first n_cells cells of one particle, as a dict of
+    arrays: age, capture, lu, lt, gene, bin, B (cycles), ws (window start)."""
+    th = np.ascontiguousarray(theta_row, np.float64)
+    out = np.zeros((n_cells, 8), np.float64)
+    rc = lib().orc_burn_cells(od.ref, model, seed, pidx, _p(th), n_cells, _p(out))
+    if rc:
+        raise RuntimeError(f"orc_burn_cells failed: {rc}")
+    return {k: out[:, i] for i, k in enumerate(("age", "capture", "lu", "lt", "gene", "bin", "B", "ws"))}
+
+
+def burn_step(lu, lt, g, c0, c1, ku, kt, D):
+    """The burn-in's Poisson means (lu, lt) after one interval of length D."""
+    io = np.array([lu, lt], np.float64)
+    lib().orc_burn_step(_p(io), int(g), float(c0), float(c1), float(ku), float(kt), float(D))
+    return float(io[0]), float(io[1])
 
 
 def cell_stats(od: OracleDesign, u, l, bins):

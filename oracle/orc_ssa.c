@@ -348,6 +348,12 @@ void orc_poisson_sample(double mu, uint64_t seed, int64_t n, uint32_t* out) {
   for (int64_t i = 0; i < n; ++i) out[i] = poisson(&s, &wb, mu, &st);
 }
 
+/* Test hook: one burn-in interval of length D applied to the Poisson means
+ * (lu, lt) = io[0..1], with the gene on (g != 0) or off. */
+void orc_burn_step(double* io, int g, double c0, double c1, double ku, double kt, double D) {
+  burn_accum(&io[0], &io[1], g, c0, c1, ku, kt, D);
+}
+
 /* A cell at the start of its labelling window, shared by its sub-lanes. */
 typedef struct {
   double age, bc, tg, lu, lt;
@@ -613,6 +619,35 @@ static int sim_window(const orc_design* d, const prates* r, const uint32_t key[2
   *l_out = n2 + n3;
   *steps += st;
   *reactions += re;
+  return 0;
+}
+
+/* Test hook: the burn-in of cells 0 .. n_cells-1 of particle pidx (theta: its
+ * log10 rates).  out[8 c ...] = age, capture, lu, lt, gene state, bin, burn-in
+ * cycles B, window start ws. */
+int orc_burn_cells(const orc_design* d, int model, uint64_t seed, uint32_t pidx,
+                   const double* theta, int n_cells, double* out) {
+  if (model < 1 || model > 5 || n_cells > d->cells) return -1;
+  uint32_t key[2];
+  key_of(seed, model, key);
+  prates r;
+  make_rates(d, model, theta, &r);
+  for (int c = 0; c < n_cells; ++c) {
+    cell_start cst;
+    int64_t st = 0, re = 0;
+    const int e = burn_cell(d, &r, key, pidx, c, &cst, &st, &re);
+    if (e) return e;
+    const int cond = c % d->n_conds;
+    double* o = out + 8 * (size_t)c;
+    o[0] = cst.age;
+    o[1] = cst.bc;
+    o[2] = cst.lu;
+    o[3] = cst.lt;
+    o[4] = cst.g;
+    o[5] = cst.bin;
+    o[6] = r.B;
+    o[7] = cst.age - d->pulse[cond] - d->chase[cond];
+  }
   return 0;
 }
 
