@@ -29,6 +29,16 @@ class AbcDesign(C.Structure):
     ]
 
 
+class AbcMomentCfg(C.Structure):
+    _fields_ = [
+        ("cycle", C.c_double), ("n_steps", C.c_int32), ("n_ages", C.c_int32),
+        ("ages", C.c_void_p), ("n_conds", C.c_int32), ("pulse", C.c_void_p),
+        ("chase", C.c_void_p), ("is_chase", C.c_void_p), ("t0", C.c_double),
+        ("iv", C.c_double * 9), ("transient_mode", C.c_int32), ("rtol", C.c_double),
+        ("atol", C.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -81,6 +91,12 @@ def _declare(L):
         "abc_graph_end": (I32, [V, C.POINTER(V)]),
         "abc_graph_launch": (I32, [V, V]),
         "abc_graph_destroy": (I32, [V]),
+        "abc_recover_dev": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V]),
+        "abc_recover": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V]),
+        "abc_moment_sim_dev": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V, I32, I32,
+                                     V, V, V]),
+        "abc_moment_sim": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V, I32, I32, V,
+                                 V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mom, mth, mst;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -420,7 +420,8 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy};
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss,
+                    &c->mom, &c->mth, &c->mst};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   if (c->dd) hipFree(c->dd);
@@ -890,6 +891,186 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
   HIPCHK(c, hipMemcpyAsync(bound, bp, sizeof(float) * n * G, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   A.active = false;
+  return ABC_OK;
+}
+
+// -------------------------------------------------------------- moment model
+
+namespace {
+
+int moment_cfg(abc_ctx* c, int model, const abc_moment_cfg* u, MomCfgDev* m) {
+  if (model < 1 || model > 5) return fail(c, ABC_E_MODEL, "model must be 1..5");
+  if (!u) return fail(c, ABC_E_ARG, "moment configuration missing");
+  if (!(u->cycle > 0.0) || u->n_steps < 1 || u->n_steps > ABC_MAX_BINS)
+    return fail(c, ABC_E_ARG, "cycle must be > 0 and n_steps 1..8");
+  if (u->n_ages < 1 || u->n_ages > ABC_MOM_MAX_AGES || !u->ages)
+    return fail(c, ABC_E_ARG, "n_ages must be 1..8 with ages given");
+  if (u->n_conds < 1 || u->n_conds > ABC_MOM_MAX_CONDS || !u->pulse || !u->chase)
+    return fail(c, ABC_E_ARG, "n_conds must be 1..16 with pulse and chase given");
+  if (!(u->rtol > 0.0) || !(u->atol > 0.0)) return fail(c, ABC_E_ARG, "rtol and atol must be > 0");
+  if (u->transient_mode != 0 && u->transient_mode != 1)
+    return fail(c, ABC_E_ARG, "transient_mode must be 0 or 1");
+  if (!std::isfinite(u->t0) || u->t0 > 0.0) return fail(c, ABC_E_ARG, "t0 must be finite and <= 0");
+  std::memset(m, 0, sizeof(*m));
+  m->cycle = u->cycle;
+  m->n_steps = u->n_steps;
+  m->n_ages = u->n_ages;
+  m->n_conds = u->n_conds;
+  m->transient_mode = u->transient_mode;
+  m->t0 = u->t0;
+  for (int i = 0; i < 9; ++i) m->iv[i] = u->iv[i];
+  m->rtol = u->rtol;
+  m->atol = u->atol;
+  for (int a = 0; a < u->n_ages; ++a) {
+    if (!std::isfinite(u->ages[a]) || u->ages[a] < 0.0)
+      return fail(c, ABC_E_ARG, "ages must be finite and >= 0");
+    m->ages[a] = u->ages[a];
+  }
+  for (int j = 0; j < u->n_conds; ++j) {
+    if (!(u->pulse[j] >= 0.0) || !(u->chase[j] >= 0.0))
+      return fail(c, ABC_E_ARG, "pulse and chase must be >= 0");
+    m->pulse[j] = u->pulse[j];
+    m->chase[j] = u->chase[j];
+    m->is_chase[j] = u->is_chase ? (u->is_chase[j] != 0) : 0;
+  }
+  m->pulse_cond = m->chase_cond = -1;
+  return ABC_OK;
+}
+
+// transient -> trajectories (-> downsampling and statistics when stats_dev)
+int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_t ld,
+               const MomCfgDev& m, double* out_dev, int64_t* steps_dev, double* stats_dev) {
+  int rc;
+  if ((rc = ensure(c, c->mss, sizeof(double) * 9 * (size_t)n))) return rc;
+  unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
+  hipLaunchKernelGGL(moment_transient_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, model,
+                     n, theta_dev, ld, P<double>(c->mss), steps);
+  HIPCHK(c, hipGetLastError());
+  const int64_t items = n * (int64_t)m.n_conds * m.n_ages;
+  hipLaunchKernelGGL(moment_traj_kernel, dim3(grid1(items, 64)), dim3(64), 0, c->stream, m, model,
+                     n, theta_dev, ld, P<double>(c->mss), out_dev, steps);
+  HIPCHK(c, hipGetLastError());
+  if (stats_dev) {
+    hipLaunchKernelGGL(moment_stats_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,
+                       out_dev, stats_dev);
+    HIPCHK(c, hipGetLastError());
+  }
+  return ABC_OK;
+}
+
+int moment_reference_cols(int model, int nb) {
+  return (model >= 3 ? nb : 1) + 4;  // kon/koff/alpha/gamma/lambda, one of the first four per bin
+}
+
+}  // namespace
+
+int32_t abc_recover_dev(abc_ctx* c, int32_t model, int64_t n, const double* theta_dev,
+                        int64_t ld, const abc_moment_cfg* cfg, double* out_dev,
+                        int64_t* steps_dev) {
+  if (!c) return ABC_E_ARG;
+  MomCfgDev m;
+  int rc = moment_cfg(c, model, cfg, &m);
+  if (rc) return rc;
+  if (n < 0 || ld < n) return fail(c, ABC_E_ARG, "need n >= 0 and ld >= n");
+  if (n == 0) return ABC_OK;
+  if (!theta_dev || !out_dev) return fail(c, ABC_E_ARG, "theta and out are required");
+  if (n > 0x7FFFFFFFll / (m.n_conds * m.n_ages)) return fail(c, ABC_E_ARG, "too many particles per call");
+  if (steps_dev)
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
+                       reinterpret_cast<unsigned long long*>(steps_dev), n);
+  return moment_run(c, model, n, theta_dev, ld, m, out_dev, steps_dev, nullptr);
+}
+
+int32_t abc_recover(abc_ctx* c, int32_t model, int64_t n, const double* theta, int64_t ld,
+                    const abc_moment_cfg* cfg, double* out, int64_t* steps) {
+  if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  MomCfgDev m;
+  int rc = moment_cfg(c, model, cfg, &m);
+  if (rc) return rc;
+  if (n < 0 || ld < n) return fail(c, ABC_E_ARG, "need n >= 0 and ld >= n");
+  if (n == 0) return ABC_OK;
+  if (!theta || !out) return fail(c, ABC_E_ARG, "theta and out are required");
+  const int cols = moment_reference_cols(model, m.n_steps);
+  const size_t outw = (size_t)m.n_conds * m.n_ages * 5;
+  if ((rc = ensure(c, c->mth, sizeof(double) * (size_t)n * cols))) return rc;
+  if ((rc = ensure(c, c->mom, sizeof(double) * (size_t)n * outw))) return rc;
+  if ((rc = ensure(c, c->mst, sizeof(int64_t) * (size_t)n))) return rc;
+  HIPCHK(c, hipMemcpy2DAsync(c->mth.p, sizeof(double) * n, theta, sizeof(double) * ld,
+                             sizeof(double) * n, cols, hipMemcpyHostToDevice, c->stream));
+  rc = abc_recover_dev(c, model, n, P<double>(c->mth), n, cfg, P<double>(c->mom),
+                       steps ? P<int64_t>(c->mst) : nullptr);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->mom.p, sizeof(double) * n * outw, hipMemcpyDeviceToHost, c->stream));
+  if (steps)
+    HIPCHK(c, hipMemcpyAsync(steps, c->mst.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+int32_t abc_moment_sim_dev(abc_ctx* c, int32_t model, int64_t n, const double* theta_dev,
+                           int64_t ld, const abc_moment_cfg* cfg, const double* capture,
+                           const double* weights, int32_t pulse_cond, int32_t chase_cond,
+                           double* stats_dev, double* moments_dev, int64_t* steps_dev) {
+  if (!c) return ABC_E_ARG;
+  MomCfgDev m;
+  int rc = moment_cfg(c, model, cfg, &m);
+  if (rc) return rc;
+  if (n < 0 || ld < n) return fail(c, ABC_E_ARG, "need n >= 0 and ld >= n");
+  if (!weights) return fail(c, ABC_E_ARG, "age weights are required");
+  if (pulse_cond < -1 || pulse_cond >= m.n_conds || chase_cond < -1 || chase_cond >= m.n_conds)
+    return fail(c, ABC_E_ARG, "pulse_cond / chase_cond must be -1 or a condition index");
+  if (n == 0) return ABC_OK;
+  if (!theta_dev || !stats_dev) return fail(c, ABC_E_ARG, "theta and stats are required");
+  if (n > 0x7FFFFFFFll / (m.n_conds * m.n_ages)) return fail(c, ABC_E_ARG, "too many particles per call");
+  for (int i = 0; i < m.n_ages * m.n_conds; ++i) m.weights[i] = weights[i];
+  m.downsample = capture != nullptr;
+  if (capture)
+    for (int i = 0; i < m.n_ages * 2 * 3; ++i) m.cap[i] = capture[i];
+  m.pulse_cond = pulse_cond;
+  m.chase_cond = chase_cond;
+  double* mom = moments_dev;
+  if (!mom) {
+    if ((rc = ensure(c, c->mom, sizeof(double) * (size_t)n * m.n_conds * m.n_ages * 5))) return rc;
+    mom = P<double>(c->mom);
+  }
+  if (steps_dev)
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
+                       reinterpret_cast<unsigned long long*>(steps_dev), n);
+  return moment_run(c, model, n, theta_dev, ld, m, mom, steps_dev, stats_dev);
+}
+
+int32_t abc_moment_sim(abc_ctx* c, int32_t model, int64_t n, const double* theta, int64_t ld,
+                       const abc_moment_cfg* cfg, const double* capture, const double* weights,
+                       int32_t pulse_cond, int32_t chase_cond, double* stats, double* moments,
+                       int64_t* steps) {
+  if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  MomCfgDev m;
+  int rc = moment_cfg(c, model, cfg, &m);
+  if (rc) return rc;
+  if (n < 0 || ld < n) return fail(c, ABC_E_ARG, "need n >= 0 and ld >= n");
+  if (n == 0) return ABC_OK;
+  if (!theta || !stats) return fail(c, ABC_E_ARG, "theta and stats are required");
+  const int cols = moment_reference_cols(model, m.n_steps);
+  const size_t outw = (size_t)m.n_conds * m.n_ages * 5;
+  const int K = 4 * m.n_ages + 3 * m.n_conds;
+  if ((rc = ensure(c, c->mth, sizeof(double) * (size_t)n * cols))) return rc;
+  if ((rc = ensure(c, c->mom, sizeof(double) * (size_t)n * outw))) return rc;
+  if ((rc = ensure(c, c->stats, sizeof(double) * (size_t)n * K))) return rc;
+  if ((rc = ensure(c, c->mst, sizeof(int64_t) * (size_t)n))) return rc;
+  HIPCHK(c, hipMemcpy2DAsync(c->mth.p, sizeof(double) * n, theta, sizeof(double) * ld,
+                             sizeof(double) * n, cols, hipMemcpyHostToDevice, c->stream));
+  rc = abc_moment_sim_dev(c, model, n, P<double>(c->mth), n, cfg, capture, weights, pulse_cond,
+                          chase_cond, P<double>(c->stats), P<double>(c->mom),
+                          steps ? P<int64_t>(c->mst) : nullptr);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(stats, c->stats.p, sizeof(double) * n * K, hipMemcpyDeviceToHost, c->stream));
+  if (moments)
+    HIPCHK(c, hipMemcpyAsync(moments, c->mom.p, sizeof(double) * n * outw, hipMemcpyDeviceToHost, c->stream));
+  if (steps)
+    HIPCHK(c, hipMemcpyAsync(steps, c->mst.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return ABC_OK;
 }
 

@@ -94,6 +94,34 @@ __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* o
                            uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,
                            unsigned long long* steps_out, unsigned long long* react_out,
                            const int64_t* pidx);
+// Moment model (the reference simulator, scripts/model.jl): one call's
+// configuration, passed by value as a kernel argument (about 1.9 kB).
+#define ABC_MOM_MAX_AGES ABC_MAX_BINS
+#define ABC_MOM_MAX_CONDS ABC_MAX_CONDS
+struct MomCfgDev {
+  double cycle;          // cell-cycle length T
+  int n_steps;           // rate bins per cycle
+  int n_ages, n_conds;
+  int transient_mode;    // 0 = 1 % rule (model.jl:114-142), 1 = exact periodic state
+  double t0;             // trajectory start
+  double iv[9];          // transient start state
+  double rtol, atol;     // Dormand-Prince tolerances
+  double ages[ABC_MOM_MAX_AGES];
+  double pulse[ABC_MOM_MAX_CONDS], chase[ABC_MOM_MAX_CONDS];
+  int is_chase[ABC_MOM_MAX_CONDS];
+  int downsample;        // apply the capture moments below
+  int pulse_cond, chase_cond;  // conditions feeding the mean / Fano statistics (-1: none)
+  double cap[ABC_MOM_MAX_AGES * 2 * 3];  // per (age, pulse/chase): E b, E b^2, Var b
+  double weights[ABC_MOM_MAX_AGES * ABC_MOM_MAX_CONDS];  // age weights, n_ages x n_conds
+};
+__global__ void moment_transient_kernel(MomCfgDev cfg, int model, int64_t n,
+                                        const double* theta, int64_t ld, double* ss,
+                                        unsigned long long* steps);
+__global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
+                                   const double* theta, int64_t ld, const double* ss,
+                                   double* out, unsigned long long* steps);
+__global__ void moment_stats_kernel(MomCfgDev cfg, int64_t n, double* mom,
+                                    double* stats);
 __global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
                              const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
                              double* superset);

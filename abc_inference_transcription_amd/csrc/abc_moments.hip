@@ -1,0 +1,489 @@
+// abc_moments.hip -- the reference's own simulator on the GPU: the closed
+// 9-dimensional linear moment model of the telegraph gene and its unlabelled
+// and labelled mRNA (scripts/model.jl), batched over particles.
+//
+//   rates / size scaling ......... scripts/model.jl:1-27
+//   vary map ..................... scripts/model.jl:30-43
+//   labelling window ............. scripts/model.jl:58-64
+//   moment equations ............. scripts/model.jl:67-86
+//   ODE solve .................... scripts/model.jl:89-96 (CVODE_BDF there;
+//                                  Dormand-Prince 5(4) here, split at every
+//                                  rate discontinuity)
+//   periodic boundary ............ scripts/model.jl:98-111
+//   transient phase .............. scripts/model.jl:114-142
+//   trajectories ................. scripts/model.jl:146-174
+//   synthetic data ............... scripts/model.jl:176-187, 191-199
+//   downsample ................... scripts/model.jl:221-239
+//   abc_sim statistics ........... scripts/abc_simulation.jl:23-46
+//   run_part_sim ................. scripts/recover_statistics.jl:2-11
+//
+// Work decomposition.  moment_transient_kernel: one lane per particle
+// iterates the cycle map to the periodic state.  moment_traj_kernel: one lane
+// per (particle, condition, age) integrates that state from t0 to the age;
+// the items of a particle are adjacent, so the lanes of a wavefront share
+// their rates (and their stiffness, which sets the step count).
+// moment_stats_kernel: one lane per particle downsamples the moments and forms
+// the statistics.
+//
+// Every operation is an IEEE-rounded +, -, *, / or sqrt in the order written,
+// the powers use the engine's own exp10 / exp / log, and the library is built
+// with -ffp-contract=off: the step sequence, and so every result bit, is that
+// of the CPU restatement compiled the same way.
+#include "abc_device.h"
+#include "abc_kernels.h"
+
+using namespace abcd;
+
+namespace {
+
+// Dormand-Prince 5(4) tableau (Dormand & Prince 1980), first-same-as-last.
+constexpr double kC2 = 1.0 / 5, kC3 = 3.0 / 10, kC4 = 4.0 / 5, kC5 = 8.0 / 9;
+constexpr double kA21 = 1.0 / 5;
+constexpr double kA31 = 3.0 / 40, kA32 = 9.0 / 40;
+constexpr double kA41 = 44.0 / 45, kA42 = -56.0 / 15, kA43 = 32.0 / 9;
+constexpr double kA51 = 19372.0 / 6561, kA52 = -25360.0 / 2187, kA53 = 64448.0 / 6561,
+                 kA54 = -212.0 / 729;
+constexpr double kA61 = 9017.0 / 3168, kA62 = -355.0 / 33, kA63 = 46732.0 / 5247,
+                 kA64 = 49.0 / 176, kA65 = -5103.0 / 18656;
+constexpr double kB1 = 35.0 / 384, kB3 = 500.0 / 1113, kB4 = 125.0 / 192,
+                 kB5 = -2187.0 / 6784, kB6 = 11.0 / 84;
+constexpr double kE1 = 71.0 / 57600, kE3 = -71.0 / 16695, kE4 = 71.0 / 1920,
+                 kE5 = -17253.0 / 339200, kE6 = 22.0 / 525, kE7 = -1.0 / 40;
+
+// Column offsets of (kon, koff, alpha, gamma) in a reference parameter row
+// and how many bins each spans (model.jl:30-43 with the flags of
+// abc_simulation.jl:78); lambda follows them.
+struct VaryMap {
+  int o[4], cnt[4], o_lam;
+};
+
+ABC_DEV VaryMap vary_map(int model, int nb) {
+  VaryMap v;
+  const int flag[4] = {model == 3, 0, model == 4, model == 5};
+  int k = 0;
+  for (int i = 0; i < 4; ++i) {
+    v.o[i] = k;
+    v.cnt[i] = flag[i] ? nb : 1;
+    k += v.cnt[i];
+  }
+  v.o_lam = k;
+  return v;
+}
+
+// Julia's mod(x, y) for floats: the floored remainder.
+ABC_DEV double jmod(double x, double y) {
+  double r = fmod(x, y);
+  if (r != 0.0 && ((r < 0.0) != (y < 0.0))) r += y;
+  return r;
+}
+
+// A smooth piece of the integration: the rates at its interior point tb.
+struct Piece {
+  double base[4];  // 10^theta of kon, koff, alpha, gamma in tb's bin
+  double lam;      // labelled fraction (0 outside the window)
+  double cs;       // start of the cycle holding tb (size scaling)
+};
+
+struct Particle {
+  const double* th;  // parameter row, column stride ld
+  int64_t ld;
+  VaryMap v;
+  int scaling;
+};
+
+ABC_DEV double theta_at(const Particle& P, int col) { return P.th[(int64_t)col * P.ld]; }
+
+ABC_DEV Piece make_piece(const Particle& P, const MomCfgDev& cfg, double texp, double pulse,
+                         double tb) {
+  Piece pc;
+  const double m = jmod(tb, cfg.cycle);
+  int bin = 0;
+  if (m != 0.0) {
+    const double h = cfg.cycle / cfg.n_steps;
+    bin = (int)ceil(m / h) - 1;
+    if (bin < 0) bin = 0;
+    if (bin >= cfg.n_steps) bin = cfg.n_steps - 1;
+  }
+  for (int i = 0; i < 4; ++i)
+    pc.base[i] = ex10(theta_at(P, P.v.o[i] + (P.v.cnt[i] > 1 ? bin : 0)));
+  pc.lam = (tb >= texp && tb <= texp + pulse) ? ex10(theta_at(P, P.v.o_lam)) : 0.0;
+  pc.cs = floor(tb / cfg.cycle) * cfg.cycle;
+  return pc;
+}
+
+// rates at time t inside the piece: alpha carries the size scaling 1 + a/T
+ABC_DEV void rates(const Piece& pc, const Particle& P, double cycle, double t, double* p) {
+  p[0] = pc.base[0];
+  p[1] = pc.base[1];
+  double al = pc.base[2];
+  if (P.scaling) {
+    const double a = t - pc.cs;
+    al = al * (1.0 + a / cycle);
+  }
+  p[2] = al;
+  p[3] = pc.base[3];
+}
+
+// y = (E g, E u, E l, Var g, Cov gu, Cov gl, Var u, Cov ul, Var l)
+ABC_DEV void rhs(const double* p, double lam, const double* y, double* dy) {
+  dy[0] = p[0] * (1 - y[0]) - p[1] * y[0];
+  dy[1] = (1 - lam) * p[2] * y[0] - p[3] * y[1];
+  dy[2] = lam * p[2] * y[0] - p[3] * y[2];
+  dy[3] = p[1] * y[0] + p[0] * (1 - y[0]) - 2 * (p[0] + p[1]) * y[3];
+  dy[4] = p[2] * (1 - lam) * y[3] - (p[0] + p[1] + p[3]) * y[4];
+  dy[5] = p[2] * lam * y[3] - (p[0] + p[1] + p[3]) * y[5];
+  dy[6] = p[2] * (1 - lam) * y[0] + p[3] * y[1] + 2 * p[2] * (1 - lam) * y[4] - 2 * p[3] * y[6];
+  dy[7] = p[2] * lam * y[4] + p[2] * (1 - lam) * y[5] - 2 * p[3] * y[7];
+  dy[8] = p[2] * lam * y[0] + p[3] * y[2] + 2 * p[2] * lam * y[5] - 2 * p[3] * y[8];
+}
+
+ABC_DEV void eval(const Piece& pc, const Particle& P, double cycle, double t, const double* y,
+                  double* dy) {
+  double p[4];
+  rates(pc, P, cycle, t, p);
+  rhs(p, pc.lam, y, dy);
+}
+
+// Adaptive Dormand-Prince 5(4) over one smooth piece [a, b]; returns the
+// attempted steps.
+ABC_DEV int64_t dopri_piece(const Piece& pc, const Particle& P, const MomCfgDev& cfg, double a,
+                            double b, double* y, const LogEntry* ltab) {
+  const double len = b - a;
+  if (len <= 0) return 0;
+  const double cyc = cfg.cycle, rtol = cfg.rtol, atol = cfg.atol;
+  double t = a;
+  double h = len < 1e-2 ? len : 1e-2;
+  double k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], k7[9], yt[9], yn[9];
+  int64_t steps = 0;
+  eval(pc, P, cyc, t, y, k1);
+  while (t < b) {
+    if (t + h > b) h = b - t;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) yt[i] = y[i] + h * (kA21 * k1[i]);
+    eval(pc, P, cyc, t + kC2 * h, yt, k2);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) yt[i] = y[i] + h * (kA31 * k1[i] + kA32 * k2[i]);
+    eval(pc, P, cyc, t + kC3 * h, yt, k3);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) yt[i] = y[i] + h * (kA41 * k1[i] + kA42 * k2[i] + kA43 * k3[i]);
+    eval(pc, P, cyc, t + kC4 * h, yt, k4);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      yt[i] = y[i] + h * (kA51 * k1[i] + kA52 * k2[i] + kA53 * k3[i] + kA54 * k4[i]);
+    eval(pc, P, cyc, t + kC5 * h, yt, k5);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      yt[i] = y[i] + h * (kA61 * k1[i] + kA62 * k2[i] + kA63 * k3[i] + kA64 * k4[i] + kA65 * k5[i]);
+    eval(pc, P, cyc, t + h, yt, k6);
+#pragma unroll
+    for (int i = 0; i < 9; ++i)
+      yn[i] = y[i] + h * (kB1 * k1[i] + kB3 * k3[i] + kB4 * k4[i] + kB5 * k5[i] + kB6 * k6[i]);
+    eval(pc, P, cyc, t + h, yn, k7);
+    double err = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+      const double ei =
+          h * (kE1 * k1[i] + kE3 * k3[i] + kE4 * k4[i] + kE5 * k5[i] + kE6 * k6[i] + kE7 * k7[i]);
+      const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+      const double r = fabs(ei) / sc;
+      if (r > err) err = r;
+    }
+    steps++;
+    if (err <= 1.0) {
+      t += h;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        y[i] = yn[i];
+        k1[i] = k7[i];
+      }
+      double fac = err == 0.0 ? 5.0 : 0.9 * ex(-0.2 * ln(err, ltab));
+      if (fac > 5.0) fac = 5.0;
+      h *= fac;
+    } else {
+      double fac = 0.9 * ex(-0.2 * ln(err, ltab));
+      if (fac < 0.2) fac = 0.2;
+      h *= fac;
+    }
+    if (h < 1e-14 * (fabs(t) + 1.0)) h = 1e-14 * (fabs(t) + 1.0);
+  }
+  return steps;
+}
+
+// model(): integrate [tmin, tmax], split at every bin edge k * T / n_steps and
+// at the window edges texp, texp + pulse.  The edges are generated in
+// ascending order (a sorted merge of the grid with the two window edges);
+// coincident edges give empty pieces, which are skipped.
+ABC_DEV int64_t moment_solve(const Particle& P, const MomCfgDev& cfg, double texp, double pulse,
+                             double tmin, double tmax, double* y, const LogEntry* ltab) {
+  const double h = cfg.cycle / cfg.n_steps;
+  double k = floor(tmin / h) + 1.0;
+  const double we = texp + pulse;
+  bool w0 = texp > tmin && texp < tmax;
+  bool w1 = we > tmin && we < tmax;
+  int64_t steps = 0;
+  double a = tmin;
+  int grid_left = 4000;  // the same cap on grid edges as the CPU restatement
+  for (;;) {
+    // next edge: the smallest of the pending grid edge and window edges
+    bool have_grid = false;
+    double g = 0.0;
+    while (grid_left > 0 && k * h < tmax) {
+      g = k * h;
+      if (g > tmin) { have_grid = true; break; }
+      k += 1.0;  // an edge at or before tmin is not an edge (and not counted)
+    }
+    double b = tmax;
+    int src = -1;
+    if (have_grid) { b = g; src = 0; }
+    if (w0 && (src < 0 || texp < b)) { b = texp; src = 1; }
+    if (w1 && (src < 0 || we < b)) { b = we; src = 2; }
+    if (src == 0) { k += 1.0; grid_left--; }
+    else if (src == 1) w0 = false;
+    else if (src == 2) w1 = false;
+    if (b > a) {
+      const double tb = 0.5 * (a + b);
+      const Piece pc = make_piece(P, cfg, texp, pulse, tb);
+      steps += dopri_piece(pc, P, cfg, a, b, y, ltab);
+      a = b;
+    }
+    if (src < 0) break;
+  }
+  return steps;
+}
+
+// binomial partitioning at division (model.jl:98-111)
+ABC_DEV void periodic_boundary(const double* e, double* v) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) v[i] = e[i];
+  v[4] = e[4] / 2;
+  v[5] = e[5] / 2;
+  v[6] = e[6] / 4 + e[1] / 4;
+  v[8] = e[8] / 4 + e[2] / 4;
+  v[7] = e[7] / 4;
+  v[1] = e[1] / 2;
+  v[2] = e[2] / 2;
+}
+
+ABC_DEV int64_t cycle_map(const Particle& P, const MomCfgDev& cfg, const double* in, double* out,
+                          const LogEntry* ltab) {
+#pragma unroll
+  for (int i = 0; i < 9; ++i) out[i] = in[i];
+  return moment_solve(P, cfg, -1.0, 0.1, 0.0, cfg.cycle, out, ltab);
+}
+
+ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* theta, int64_t ld,
+                             int64_t p) {
+  Particle P;
+  P.th = theta + p;
+  P.ld = ld;
+  P.v = vary_map(model, cfg.n_steps);
+  P.scaling = model != 2;
+  return P;
+}
+
+}  // namespace
+
+// Periodic state after division, one lane per particle.  Mode 0 is the
+// reference's rule (model.jl:114-142): iterate the cycle map from iv until
+// the gene and unlabelled moments change by at most 1 % (at most 101 maps).
+// Mode 1 solves the affine fixed point exactly: the cycle map is y -> M y + v
+// and the partition P is linear, so the state after division solves
+// (I - P M) x = P v (10 cycle maps and a 9 x 9 elimination with partial
+// pivoting).
+__global__ __launch_bounds__(64) void moment_transient_kernel(const MomCfgDev cfg,
+                                                               int model, int64_t n,
+                                                               const double* __restrict__ theta,
+                                                               int64_t ld, double* __restrict__ ss,
+                                                               unsigned long long* __restrict__ steps) {
+  __shared__ LogEntry ltab[ABC_LOG_N];
+  load_log_table(ltab);
+  __syncthreads();
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const Particle P = particle_of(model, cfg, theta, ld, p);
+  int64_t st = 0;
+  double out[9];
+  if (cfg.transient_mode == 1) {
+    double zero[9], v[9], Pv[9], A[9][9];
+    for (int i = 0; i < 9; ++i) zero[i] = 0.0;
+    st += cycle_map(P, cfg, zero, v, ltab);
+    for (int i = 0; i < 9; ++i) {
+      double e[9], o[9];
+      for (int r = 0; r < 9; ++r) e[r] = 0.0;
+      e[i] = 1.0;
+      st += cycle_map(P, cfg, e, o, ltab);
+      double col[9], pc[9];
+      for (int r = 0; r < 9; ++r) col[r] = o[r] - v[r];
+      periodic_boundary(col, pc);
+      for (int r = 0; r < 9; ++r) A[r][i] = (r == i ? 1.0 : 0.0) - pc[r];
+    }
+    periodic_boundary(v, Pv);
+    // Gaussian elimination with partial pivoting
+    for (int col = 0; col < 9; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < 9; ++r)
+        if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+      if (piv != col) {
+        for (int k = 0; k < 9; ++k) {
+          const double t = A[col][k];
+          A[col][k] = A[piv][k];
+          A[piv][k] = t;
+        }
+        const double t = Pv[col];
+        Pv[col] = Pv[piv];
+        Pv[piv] = t;
+      }
+      for (int r = col + 1; r < 9; ++r) {
+        const double f = A[r][col] / A[col][col];
+        for (int k = col; k < 9; ++k) A[r][k] -= f * A[col][k];
+        Pv[r] -= f * Pv[col];
+      }
+    }
+    for (int r = 8; r >= 0; --r) {
+      double s = Pv[r];
+      for (int k = r + 1; k < 9; ++k) s -= A[r][k] * Pv[k];
+      Pv[r] = s / A[r][r];
+    }
+    for (int i = 0; i < 9; ++i) out[i] = Pv[i];
+  } else {
+    const int check_idx[5] = {0, 1, 3, 4, 6};
+    double e1[9], e2[9], iv[9];
+    st += cycle_map(P, cfg, cfg.iv, e1, ltab);
+    for (int i = 0; i < 9; ++i) e2[i] = e1[i];
+    int k = 0;
+    bool conv = false;
+    while (!conv && k <= 100) {
+      k++;
+      periodic_boundary(e1, iv);
+      st += cycle_map(P, cfg, iv, e2, ltab);
+      int nz = 0, ok = 0;
+      for (int q = 0; q < 5; ++q) {
+        const int i = check_idx[q];
+        if (e1[i] > 0.0) {
+          nz++;
+          if (fabs((e1[i] - e2[i]) / e1[i]) <= 0.01) ok++;
+        }
+      }
+      if (ok == nz) conv = true;
+      for (int i = 0; i < 9; ++i) e1[i] = e2[i];
+    }
+    periodic_boundary(e2, out);
+  }
+  for (int i = 0; i < 9; ++i) ss[p + (int64_t)i * n] = out[i];
+  if (steps) steps[p] = (unsigned long long)st;
+}
+
+// trajectories (model.jl:146-174) for item q = (particle, condition, age):
+// out column (cond * n_ages + age) * 5 + {mean_u, mean_l, var_u, cov_ul, var_l}
+__global__ __launch_bounds__(64) void moment_traj_kernel(const MomCfgDev cfg,
+                                                          int model, int64_t n,
+                                                          const double* __restrict__ theta,
+                                                          int64_t ld,
+                                                          const double* __restrict__ ss,
+                                                          double* __restrict__ out,
+                                                          unsigned long long* __restrict__ steps) {
+  __shared__ LogEntry ltab[ABC_LOG_N];
+  load_log_table(ltab);
+  __syncthreads();
+  const int per = cfg.n_conds * cfg.n_ages;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * per) return;
+  const int64_t p = q / per;
+  const int r = (int)(q - p * per);
+  const int j = r / cfg.n_ages, ai = r - j * cfg.n_ages;
+  const Particle P = particle_of(model, cfg, theta, ld, p);
+  const double T = cfg.cycle, t0 = cfg.t0, age = cfg.ages[ai];
+  const double pulse = cfg.pulse[j], texp = age - cfg.pulse[j] - cfg.chase[j];
+  const int nsols = (age > 0 && age < T) ? (int)floor((age - t0) / T) + 1 : (int)floor((age - t0) / T);
+  double y[9], tmp[9];
+  for (int i = 0; i < 9; ++i) y[i] = ss[p + (int64_t)i * n];
+  double tau = t0;
+  int64_t st = 0;
+  for (int k = 1; k <= nsols; ++k) {
+    const double tf = tau < 0 ? tau + T : age;
+    if (k > 1) {
+      periodic_boundary(y, tmp);
+      for (int i = 0; i < 9; ++i) y[i] = tmp[i];
+    }
+    st += moment_solve(P, cfg, texp, pulse, tau, tf, y, ltab);
+    tau = tf;
+  }
+  double* o = out + p + (int64_t)r * 5 * n;
+  o[0] = y[1];
+  o[n] = y[2];
+  o[2 * n] = y[6];
+  o[3 * n] = y[7];
+  o[4 * n] = y[8];
+  if (steps) atomicAdd(steps + p, (unsigned long long)st);
+}
+
+namespace {
+
+ABC_DEV double wsum(const double* w, const double* s, int col, int nb) {
+  double acc = w[0] * s[col];
+  for (int b = 1; b < nb; ++b) acc = acc + w[b] * s[5 * b + col];
+  return acc;
+}
+
+ABC_DEV double wcov_col(const double* s, int cx, int cy, const double* w, int nb) {
+  const double mx = wsum(w, s, cx, nb), my = wsum(w, s, cy, nb);
+  double acc = w[0] * ((s[cx] - mx) * (s[cy] - my));
+  for (int b = 1; b < nb; ++b) acc = acc + w[b] * ((s[5 * b + cx] - mx) * (s[5 * b + cy] - my));
+  return acc;
+}
+
+}  // namespace
+
+// Per particle: capture downsampling of each condition's moments with the
+// capture moments of its (age, pulse/chase) group (model.jl:221-239), then
+// the statistics of abc_sim (abc_simulation.jl:23-46): labelled ratio,
+// mean correlation and correlation of means per condition, and the mean and
+// Fano factor per age of the pulse and chase conditions, in the order
+// [pulse mean, pulse FF, chase mean, chase FF, ratios, mean_corr, corr_mean].
+// The downsampled moments are written back over the input.
+__global__ __launch_bounds__(64) void moment_stats_kernel(const MomCfgDev cfg,
+                                                           int64_t n,
+                                                           double* __restrict__ mom,
+                                                           double* __restrict__ stats) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int nb = cfg.n_ages, J = cfg.n_conds;
+  double s[ABC_MOM_MAX_AGES * 5];
+  for (int j = 0; j < J; ++j) {
+    for (int a = 0; a < nb; ++a)
+      for (int k = 0; k < 5; ++k) s[5 * a + k] = mom[p + (int64_t)((j * nb + a) * 5 + k) * n];
+    if (cfg.downsample) {
+      const int g = cfg.is_chase[j];
+      for (int i = 0; i < nb; ++i) {
+        double* r = s + 5 * i;
+        const double mb = cfg.cap[(i * 2 + g) * 3 + 0], m2b = cfg.cap[(i * 2 + g) * 3 + 1],
+                     vb = cfg.cap[(i * 2 + g) * 3 + 2];
+        const double mu = r[0], ml = r[1], vu = r[2], cv = r[3], vl = r[4];
+        r[0] = mu * mb;
+        r[1] = ml * mb;
+        r[2] = ((mb - m2b) * mu) + (vb * (mu * mu + vu)) + (mb * mb * vu);
+        r[4] = ((mb - m2b) * ml) + (vb * (ml * ml + vl)) + (mb * mb * vl);
+        r[3] = (vb * (mu * ml + cv)) + (mb * mb * cv);
+      }
+      for (int a = 0; a < nb; ++a)
+        for (int k = 0; k < 5; ++k) mom[p + (int64_t)((j * nb + a) * 5 + k) * n] = s[5 * a + k];
+    }
+    const double* wj = cfg.weights + nb * j;
+    const double su = wsum(wj, s, 0, nb), sl = wsum(wj, s, 1, nb);
+    stats[p + (int64_t)(4 * nb + j) * n] = sl / (su + sl);
+    const double stds = sqrt(fabs((wsum(wj, s, 2, nb) + wcov_col(s, 0, 0, wj, nb)) *
+                                  (wsum(wj, s, 4, nb) + wcov_col(s, 1, 1, wj, nb))));
+    stats[p + (int64_t)(4 * nb + J + j) * n] = wsum(wj, s, 3, nb) / stds;
+    stats[p + (int64_t)(4 * nb + 2 * J + j) * n] = wcov_col(s, 0, 1, wj, nb) / stds;
+    const int slot = j == cfg.pulse_cond ? 0 : (j == cfg.chase_cond ? 1 : -1);
+    if (slot >= 0) {
+      for (int b = 0; b < nb; ++b) {
+        const double* r = s + 5 * b;
+        const double m = r[0] + r[1];
+        const double eps = m == 0.0 ? 0.0001 : 0.0;
+        stats[p + (int64_t)(2 * slot * nb + b) * n] = m;
+        stats[p + (int64_t)((2 * slot + 1) * nb + b) * n] = ((r[2] + 2 * r[3]) + r[4]) / (r[0] + r[1] + eps);
+      }
+    }
+  }
+}

@@ -17,6 +17,50 @@ def _p(a):
     return None if a is None else C.c_void_p(a.ctypes.data)
 
 
+class MomentCfg:
+    """Configuration of the moment model (the reference's run_part_sim /
+    abc_sim arguments, recover_statistics.jl:29-44, abc_simulation.jl:64-76).
+    Defaults are the reference's: 20 h cycle, 5 rate bins, t0 = -3 cycles,
+    the 1 % transient rule; iv is the transient start (recover_statistics.jl
+    starts from E g = 1/2, abc_simulation.jl from E u = 1/2)."""
+
+    def __init__(self, ages, pulse, chase, is_chase=None, cycle=20.0, n_steps=5, t0=None,
+                 iv=None, transient_mode=0, rtol=1e-10, atol=1e-12):
+        self.ages = np.ascontiguousarray(ages, np.float64)
+        self.pulse = np.ascontiguousarray(pulse, np.float64)
+        self.chase = np.ascontiguousarray(chase, np.float64)
+        self.is_chase = None if is_chase is None else np.ascontiguousarray(is_chase, np.int32)
+        iv = [0.5] + [0.0] * 8 if iv is None else list(iv)
+        self.s = _lib.AbcMomentCfg(
+            float(cycle), int(n_steps), len(self.ages), _p(self.ages), len(self.pulse),
+            _p(self.pulse), _p(self.chase), _p(self.is_chase),
+            -3.0 * cycle if t0 is None else float(t0), (C.c_double * 9)(*iv),
+            int(transient_mode), float(rtol), float(atol))
+
+    @property
+    def n_ages(self):
+        return len(self.ages)
+
+    @property
+    def n_conds(self):
+        return len(self.pulse)
+
+
+def capture_moments(betas, bins, is_chase, n_ages):
+    """Per (age, pulse/chase) group: mean, mean of squares and n-1 variance of
+    the cells' capture efficiencies (model.jl:226-234), as the flat array
+    abc_moment_sim expects: [(age * 2 + group) * 3 + k]."""
+    b = np.asarray(betas, np.float64)
+    out = np.zeros((n_ages, 2, 3))
+    for a in range(n_ages):
+        for g in range(2):
+            x = b[(np.asarray(bins) == a) & (np.asarray(is_chase) == g)]
+            if len(x) == 0:
+                continue
+            out[a, g] = [x.mean(), (x * x).mean(), x.var(ddof=1) if len(x) > 1 else np.nan]
+    return out.ravel()
+
+
 class Engine:
     def __init__(self, design: dict, device: int = 0):
         self.L = _lib.load()
@@ -234,6 +278,39 @@ class Engine:
 
     def replay(self, graph):
         self._chk(self.L.abc_graph_launch(self.ctx, C.c_void_p(graph)), "graph_launch")
+
+    def recover(self, model: int, theta, cfg: MomentCfg, steps=False):
+        """The reference's moment model at the given parameter rows (theta:
+        n x reference columns, log10; extra columns are ignored) without
+        downsampling: (n, n_conds, n_ages, 5) of mean_u, mean_l, var_u, cov_ul,
+        var_l (recover_statistics.jl:2-11)."""
+        th = np.asfortranarray(theta, np.float64)
+        n = th.shape[0]
+        out = np.zeros((n, cfg.n_conds * cfg.n_ages * 5), order="F")
+        st = np.zeros(n, np.int64) if steps else None
+        self._chk(self.L.abc_recover(self.ctx, model, n, _p(th), n, C.byref(cfg.s), _p(out),
+                                     _p(st)), "recover")
+        out = out.reshape(n, cfg.n_conds, cfg.n_ages, 5)
+        return (out, st) if steps else out
+
+    def moment_sim(self, model: int, theta, cfg: MomentCfg, weights, capture=None,
+                   pulse_cond: int = 5, chase_cond: int = 6, steps=False):
+        """abc_sim on the moment model (abc_simulation.jl:13-62): returns the
+        statistics (n x (4 n_ages + 3 n_conds), reference order) and the
+        downsampled moments (n, n_conds, n_ages, 5)."""
+        th = np.asfortranarray(theta, np.float64)
+        n = th.shape[0]
+        K = 4 * cfg.n_ages + 3 * cfg.n_conds
+        w = np.ascontiguousarray(np.asarray(weights, np.float64).reshape(-1, order="F"))
+        cap = None if capture is None else np.ascontiguousarray(capture, np.float64)
+        stats = np.zeros((n, K), order="F")
+        mom = np.zeros((n, cfg.n_conds * cfg.n_ages * 5), order="F")
+        st = np.zeros(n, np.int64) if steps else None
+        self._chk(self.L.abc_moment_sim(self.ctx, model, n, _p(th), n, C.byref(cfg.s), _p(cap),
+                                        _p(w), pulse_cond, chase_cond, _p(stats), _p(mom),
+                                        _p(st)), "moment_sim")
+        mom = mom.reshape(n, cfg.n_conds, cfg.n_ages, 5)
+        return (stats, mom, st) if steps else (stats, mom)
 
     def drop_graph(self, graph):
         if graph:

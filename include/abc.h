@@ -186,6 +186,64 @@ int32_t abc_last_timing(abc_ctx* ctx, double* ms4);
 int32_t abc_ssa_time(abc_ctx* ctx, int32_t reset, double* ms2, int64_t* burn_steps,
                      int64_t* launches);
 
+/* ---- moment model: the reference's own simulator (scripts/model.jl), the
+ * exact first and second moments of the telegraph gene and its unlabelled and
+ * labelled mRNA, integrated per particle with adaptive Dormand-Prince 5(4)
+ * (the reference uses CVODE_BDF, model.jl:89-96) split at every rate
+ * discontinuity.  Results are deterministic and do not depend on the batch.
+ * The context supplies the stream and error reporting; the design it was
+ * created with is not used. */
+typedef struct {
+  double cycle;             /* cell-cycle length (h): 20.0 in the reference       */
+  int32_t n_steps;          /* rate bins per cycle (n_clusters): 5, <= 8         */
+  int32_t n_ages;           /* sampling ages (agevec), <= 8                      */
+  const double* ages;       /* n_ages ages (h), host memory                     */
+  int32_t n_conds;          /* labelling conditions, <= 16                       */
+  const double* pulse;      /* n_conds pulse lengths (h)                         */
+  const double* chase;      /* n_conds chase times (h)                           */
+  const int32_t* is_chase;  /* n_conds: capture group 0 (pulse) / 1 (chase); NULL
+                               = all pulse (only used when downsampling)         */
+  double t0;                /* trajectory start: -3 * cycle in the reference     */
+  double iv[9];             /* transient start state (E g, E u, E l, Var g, Cov gu,
+                               Cov gl, Var u, Cov ul, Var l): abc_simulation.jl:71
+                               sets E u = 1/2, recover_statistics.jl:34 E g = 1/2 */
+  int32_t transient_mode;   /* 0 = the reference's 1 % rule (model.jl:114-142),
+                               1 = the exact periodic state                      */
+  double rtol, atol;        /* integrator tolerances, e.g. 1e-10, 1e-12          */
+} abc_moment_cfg;
+
+/* run_part_sim without downsampling (recover_statistics.jl:2-11, the producer
+ * of data/recovered_statistics): theta_dev is n x (reference columns of model:
+ * 5 for models 1-2, 9 for 3-5) column-major with leading dimension ld >= n
+ * (the engine's theta, whose extra splicing column is ignored, fits as is).
+ * out_dev is n x (n_conds * n_ages * 5) column-major; column
+ * (cond * n_ages + age) * 5 + k holds k = mean_u, mean_l, var_u, cov_ul, var_l.
+ * steps_dev (optional, n) receives the Dormand-Prince steps attempted. */
+int32_t abc_recover_dev(abc_ctx* ctx, int32_t model, int64_t n, const double* theta_dev,
+                        int64_t ld, const abc_moment_cfg* cfg, double* out_dev,
+                        int64_t* steps_dev);
+int32_t abc_recover(abc_ctx* ctx, int32_t model, int64_t n, const double* theta, int64_t ld,
+                    const abc_moment_cfg* cfg, double* out, int64_t* steps);
+
+/* abc_sim (abc_simulation.jl:13-62) on the moment model: per particle the
+ * recovered moments, capture downsampling per (age, pulse/chase) group with
+ * capture[(age * 2 + group) * 3 + {0, 1, 2}] = E b, E b^2, Var b (model.jl:
+ * 221-239; NULL = no downsampling), then the 4 * n_ages + 3 * n_conds
+ * statistics in the reference order (s_pulse mean, FF; s_chase mean, FF;
+ * s_ratios; s_mean_corr; s_corr_mean) with the age weights
+ * weights[age + n_ages * cond] (age_id_distribution).  The mean/FF rows come
+ * from conditions pulse_cond and chase_cond (0-based; 5 and 6 in the
+ * reference).  stats_dev is n x K column-major; moments_dev (optional) gets
+ * the downsampled moments in the abc_recover_dev layout. */
+int32_t abc_moment_sim_dev(abc_ctx* ctx, int32_t model, int64_t n, const double* theta_dev,
+                           int64_t ld, const abc_moment_cfg* cfg, const double* capture,
+                           const double* weights, int32_t pulse_cond, int32_t chase_cond,
+                           double* stats_dev, double* moments_dev, int64_t* steps_dev);
+int32_t abc_moment_sim(abc_ctx* ctx, int32_t model, int64_t n, const double* theta, int64_t ld,
+                       const abc_moment_cfg* cfg, const double* capture, const double* weights,
+                       int32_t pulse_cond, int32_t chase_cond, double* stats, double* moments,
+                       int64_t* steps);
+
 /* ---- surrogate (no reference counterpart: the reference simulates every
  * prior particle, scripts/abc_simulation.jl:88-97).  An MLP maps a
  * particle's features to the mean and log-variance of each statistic (in the

@@ -18,6 +18,9 @@
  *   abc_sim statistics .......... scripts/abc_simulation.jl:23-46
  *   weighted_cov ................ scripts/data_summary_statistics.jl:179-181
  *
+ * Powers use the engine's own exp10 / exp / log (as the GPU moment kernels
+ * do), so the two integrate with identical operations and step sequences.
+ *
  * The reference solves with SUNDIALS CVODE at default tolerances
  * (reltol 1e-3, abstol 1e-6, third-party, unpinned version); this restatement
  * integrates to rtol ~1e-10, so fixture comparisons carry CVODE's error.
@@ -69,7 +72,7 @@ static void rates_at(int model, const double* th, double cycle, int n_steps, dou
   int cnt[4] = {v.n_kon, v.n_koff, v.n_al, v.n_ga};
   for (int i = 0; i < 4; ++i) {
     double lp = th[v.o[i] + (cnt[i] > 1 ? bin : 0)];
-    double val = pow(10.0, lp);
+    double val = orc_exp10(lp);
     if (i == 2 && scaling) {
       /* size_scaling: 1 + mod(t,T)/T inside the cycle that holds tb */
       double cs = floor(tb / cycle) * cycle;
@@ -95,7 +98,7 @@ static void rhs(const double* p, double lam, const double* y, double* dy) {
 static double lam_at(const double* th, int model, int n_steps, double texp, double pulse,
                      double tb) {
   vmap_t v = vmap_of(model, n_steps);
-  if (tb >= texp && tb <= texp + pulse) return pow(10.0, th[v.o_lam]);
+  if (tb >= texp && tb <= texp + pulse) return orc_exp10(th[v.o_lam]);
   return 0.0;
 }
 
@@ -160,11 +163,11 @@ static int64_t dopri_piece(int model, const double* th, double cycle, int n_step
       t += h;
       memcpy(y, yn, sizeof yn);
       memcpy(k1, k7, sizeof k7);
-      double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
+      double fac = err == 0.0 ? 5.0 : 0.9 * orc_exp(-0.2 * orc_log(err));
       if (fac > 5.0) fac = 5.0;
       h *= fac;
     } else {
-      double fac = 0.9 * pow(err, -0.2);
+      double fac = 0.9 * orc_exp(-0.2 * orc_log(err));
       if (fac < 0.2) fac = 0.2;
       h *= fac;
     }

@@ -1,0 +1,131 @@
+"""The moment model on the GPU (abc_recover / abc_moment_sim: the reference's
+own simulator, scripts/model.jl, with the statistics of abc_simulation.jl)
+against the oracle's CPU restatement: bit-exact, since both take the same
+Dormand-Prince steps with the same IEEE operations; and against the
+reference's recovered statistics (data/recovered_statistics, golden slice) to
+CVODE's tolerance."""
+import numpy as np
+import pytest
+
+from abc_inference_transcription_amd.design import (MODEL_NAMES, REF_CHASE, REF_PULSE,
+                                                    reference_design)
+
+pytestmark = pytest.mark.gpu
+
+AGES = [(2 * k + 1) / 10 * 20 for k in range(5)]
+IS_CHASE = [0] * 6 + [1] * 5
+
+
+def _cfgs(oracle, **kw):
+    from abc_inference_transcription_amd.engine import MomentCfg
+    g = MomentCfg(AGES, REF_PULSE, REF_CHASE, is_chase=IS_CHASE, cycle=20.0, **kw)
+    o = oracle.moment_cfg(AGES, REF_PULSE, REF_CHASE, cycle=20.0, **kw)
+    return g, o
+
+
+def _prior(oracle, model, n, first):
+    od = oracle.OracleDesign(reference_design(cells=55))
+    return oracle.sample_prior(od, model, 0, first, n)
+
+
+@pytest.fixture(scope="module")
+def eng(engine_factory):
+    return engine_factory(reference_design(cells=55))
+
+
+def _same_bits(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("mode,iv", [(0, None), (0, [0.0, 0.5] + [0.0] * 7), (1, None)])
+def test_recover_bit_exact(oracle, eng, model, mode, iv):
+    """Prior draws of the reference ranges (kon, koff, alpha 1e-3..1e3, gamma
+    1e-3..1e2): every moment of every condition and age, bit for bit."""
+    n = 6 if mode == 0 else 3
+    th = _prior(oracle, model, n, 1000 * model + 17 * mode)
+    g, o = _cfgs(oracle, transient_mode=mode, iv=iv)
+    got, steps = eng.recover(model, th, g, steps=True)
+    assert np.all(steps > 0)
+    for i in range(n):
+        want = oracle.recover(model, th[i], o)
+        assert _same_bits(got[i], want), (model, mode, i, np.max(np.abs(got[i] - want)))
+
+
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
+def test_recover_matches_reference_fixtures(eng, golden_recovered, model):
+    """recover_statistics.jl outputs at the MAP parameter sets (CVODE at
+    reltol 1e-3 there): the same bound the oracle meets."""
+    from abc_inference_transcription_amd.engine import MomentCfg
+    mn = MODEL_NAMES[model - 1]
+    th = golden_recovered[f"{mn}__theta"][:6]
+    want = golden_recovered[f"{mn}__stats"][:6]
+    got = eng.recover(model, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, cycle=20.0))
+    big = np.abs(want) > 1e-3
+    rel = np.abs(got - want)[big] / np.abs(want)[big]
+    assert np.all(np.abs(got - want)[~big] < 5e-5)
+    assert np.median(rel) < 3e-3
+    assert rel.max() < 0.06
+
+
+@pytest.mark.parametrize("model", [2, 4])
+def test_moment_sim_bit_exact(oracle, eng, golden_capture, model):
+    """abc_sim on the moment model: downsampling with per (age, pulse/chase)
+    capture moments, then the 53 statistics, against the oracle's
+    downsample + moment_stats of the same recovered moments."""
+    from abc_inference_transcription_amd.engine import capture_moments
+    n = 5
+    th = _prior(oracle, model, n, 5000 + model)
+    g, o = _cfgs(oracle, iv=[0.0, 0.5] + [0.0] * 7)
+    b = np.asarray(golden_capture, np.float64)[:400]
+    idx = np.arange(len(b))
+    cap = capture_moments(b, idx % 5, (idx // 5) % 2, 5)
+    w = np.random.default_rng(3).uniform(0.2, 1.0, (5, 11))
+    w /= w.sum(axis=0)  # age weights normalised per condition
+    stats, mom = eng.moment_sim(model, th, g, w, capture=cap, pulse_cond=5, chase_cond=6)
+    c3 = cap.reshape(5, 2, 3)
+    for i in range(n):
+        s = oracle.recover(model, th[i], o)
+        for j in range(11):
+            grp = IS_CHASE[j]
+            s[j] = oracle.downsample(s[j], c3[:, grp, 0], c3[:, grp, 1], c3[:, grp, 2])
+        assert _same_bits(mom[i], s), (model, i)
+        want = oracle.moment_stats(s, w, 5, 6)
+        assert _same_bits(stats[i], want), (model, i, np.max(np.abs(stats[i] - want)))
+
+
+def test_moment_sim_without_downsampling_is_recover(oracle, eng):
+    th = _prior(oracle, 1, 4, 7000)
+    g, _ = _cfgs(oracle)
+    w = np.full((5, 11), 0.2)
+    _, mom = eng.moment_sim(1, th, g, w, capture=None)
+    assert _same_bits(mom, eng.recover(1, th, g))
+
+
+def test_recover_batch_invariance_and_extra_columns(oracle, eng):
+    """Rows do not depend on the batch, and columns past the reference's
+    (the engine's splicing column) are ignored."""
+    th = _prior(oracle, 3, 8, 9000)
+    g, _ = _cfgs(oracle)
+    full = eng.recover(3, th, g)
+    assert th.shape[1] == 10  # 9 reference columns + splicing
+    part = eng.recover(3, th[3:6, :9], g)
+    assert _same_bits(full[3:6], part)
+
+
+def test_moment_api_rejects_bad_arguments(eng):
+    from abc_inference_transcription_amd.engine import MomentCfg
+    th = np.zeros((2, 5))
+    with pytest.raises(RuntimeError):
+        eng.recover(6, th, MomentCfg(AGES, REF_PULSE, REF_CHASE))
+    with pytest.raises(RuntimeError):
+        eng.recover(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, rtol=0.0))
+    with pytest.raises(RuntimeError):
+        eng.recover(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, transient_mode=2))
+    with pytest.raises(RuntimeError):
+        eng.recover(2, th, MomentCfg([1.0] * 9, [0.5], [0.0]))  # more than 8 ages
+    with pytest.raises(RuntimeError):
+        eng.moment_sim(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE), np.ones((5, 11)),
+                       pulse_cond=11)
+    assert eng.recover(2, np.zeros((0, 5)), MomentCfg(AGES, REF_PULSE, REF_CHASE)).shape[0] == 0
