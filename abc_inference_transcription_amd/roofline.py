@@ -25,3 +25,22 @@ which counts an FMA as two flops).
 """
 SSA_OPS_PER_STEP = 139
 VALU_PEAK_TOPS = 39.3
+
+# Moment model (moment_transient_kernel, moment_traj_kernel): FP64 operations
+# per Dormand-Prince step, counted from the source of the step (the library
+# is built without contraction, so every add, multiply and divide is one
+# instruction):
+#
+#     stage combinations  9 x (3 + 5 + 7 + 9 + 11) + yn 9 x 11         414
+#     stage times t + c h                                                9
+#     size scaling of alpha at 6 stage times (4 each)                   24
+#     right-hand side, 6 evaluations x 67                              402
+#     error norm 9 x 15 (6 products, 5 sums, h, scale, divide, ...)    135
+#     step-size control (own log and exp)                               60
+#                                                                   ------
+#                                                                     1044
+#
+# The FP64 peak for such unfused operations is one per lane per clock:
+# 39.3 TFLOP/s (the published 78.6 counts a fused multiply-add as two).
+MOMENT_FLOPS_PER_STEP = 1044
+FP64_PEAK_TFLOPS = 39.3

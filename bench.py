@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--sur-train", type=int, default=12000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
     ap.add_argument("--sur-steps", type=int, default=3000, help="Adam steps (batch 8192)")
+    ap.add_argument("--no-moments", action="store_true",
+                    help="skip the leg of the reference's own algorithm (moment model)")
+    ap.add_argument("--mom-particles", type=int, default=8192,
+                    help="particles per model of the moment-model leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (profiler byte-counter scale)")
     return ap.parse_args()
@@ -237,6 +241,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(design, models, P, total_steps / n_particles, a.cpu_seconds)
     if not a.no_surrogate:
         out["surrogate"] = surrogate_leg(eng, stream, a)
+    if not a.no_moments:
+        out["reference_moment_path"] = moment_leg(stream, a)
     print(json.dumps(out), flush=True)
     eng.close()
 
@@ -257,6 +263,98 @@ def surrogate_leg(eng, stream, a):
             "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
             "train_loss_last", "train_s", "genes", "z", "eps", "accept_rate_target", "by_z")
     return {k: res[k] for k in keep}
+
+
+def moment_leg(stream, a):
+    """The reference's own algorithm (scripts/model.jl moment ODE + the abc_sim
+    statistics of abc_simulation.jl, reference design: 20 h cycle, 5 ages at
+    the bin centres, 11 conditions, capture downsampling) on the GPU
+    (abc_moment_sim_dev) and the oracle's C restatement of it on the host
+    cores, both integrating with the same Dormand-Prince steps (rtol 1e-10)."""
+    import torch
+    from abc_inference_transcription_amd import _lib
+    from abc_inference_transcription_amd.design import REF_CHASE, REF_PULSE, reference_design
+    from abc_inference_transcription_amd.engine import Engine, MomentCfg
+    from abc_inference_transcription_amd.roofline import MOMENT_FLOPS_PER_STEP, FP64_PEAK_TFLOPS
+    dev = torch.device("cuda:0")
+    # a context of the reference design, so that the particles are draws of the
+    # reference prior (fix_params ranges), as on the CPU side
+    eng = Engine(reference_design(cells=55), device=0)
+    eng.bind_stream(stream)
+    L = eng.L
+    ages = [(2 * k + 1) / 10 * 20 for k in range(5)]
+    is_chase = [0] * 6 + [1] * 5
+    cfg = MomentCfg(ages, REF_PULSE, REF_CHASE, is_chase=is_chase, cycle=20.0,
+                    iv=[0.0, 0.5] + [0.0] * 7)
+    w = np.ascontiguousarray(np.full(55, 0.2))
+    # Beta(2,8) capture in every (age, pulse/chase) group: E b, E b^2, Var b
+    cap = np.ascontiguousarray(np.tile([0.2, 6.0 / 110.0, 16.0 / 1100.0], 10))
+    n = a.mom_particles
+    models = [1, 2, 3, 4, 5]
+    bufs = []
+    for m in models:
+        th = torch.empty(n * eng.n_params(m), dtype=torch.float64, device=dev)
+        _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, 0, n, C.c_void_p(th.data_ptr())),
+                   eng.ctx, "prior")
+        bufs.append((m, th, torch.empty(n * 53, dtype=torch.float64, device=dev),
+                     torch.zeros(n, dtype=torch.int64, device=dev)))
+    wp, cp = w.ctypes.data_as(C.c_void_p), cap.ctypes.data_as(C.c_void_p)
+
+    def run(m, th, st, steps):
+        _lib.check(L.abc_moment_sim_dev(eng.ctx, m, n, C.c_void_p(th.data_ptr()), n,
+                                        C.byref(cfg.s), cp, wp, 5, 6, C.c_void_p(st.data_ptr()),
+                                        None, C.c_void_p(steps.data_ptr())), eng.ctx,
+                   "moment_sim")
+    run(*bufs[0])  # warm-up (code load, buffer growth)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for b in bufs:
+        run(*b)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    steps = int(sum(int(b[3].sum()) for b in bufs))
+    value = len(models) * n / (ms / 1e3)
+    tflops = steps * MOMENT_FLOPS_PER_STEP / (ms / 1e3) / 1e12
+    eng.close()
+    print(f"[bench] moment path: {value:.1f} particles/s", file=sys.stderr, flush=True)
+    res = {"value": round(value, 3), "unit": "particles/s", "particles": len(models) * n,
+           "ms": round(ms, 3), "rtol": 1e-10,
+           "rk_steps_per_particle": steps / (len(models) * n),
+           "roofline": {"bound": "fp64 valu", "achieved": round(tflops, 3),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+                        "flops_per_step": MOMENT_FLOPS_PER_STEP}}
+    if not a.no_cpu:
+        res["cpu"] = moment_cpu(models, cfg, w, is_chase, cap, a.cpu_seconds)
+        res["gpu_vs_cpu"] = round(value / res["cpu"]["value"], 1)
+    return res
+
+
+def moment_cpu(models, cfg, w, is_chase, cap, budget_s):
+    """The oracle's threaded C restatement of the same moment abc_sim on the
+    host cores, over a bounded sample: 16 particles per model, round-robin,
+    from the start of each model's prior stream."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from abc_inference_transcription_amd.design import reference_design
+    threads = min(os.cpu_count() or 1, 16)
+    od = O.OracleDesign(reference_design(cells=55))
+    ocfg = O.moment_cfg(cfg.ages, cfg.pulse, cfg.chase, cycle=20.0, iv=[0.0, 0.5] + [0.0] * 7)
+    done, t_used, it = 0, 0.0, 0
+    while t_used < budget_s and it < 500:
+        m = models[it % len(models)]
+        th = O.sample_prior(od, m, 0, (it // len(models)) * 16, 16)
+        t = time.perf_counter()
+        O.moment_sim(m, th, ocfg, w.reshape(5, 11, order="F"), is_chase=is_chase, capture=cap,
+                     n_threads=threads)
+        t_used += time.perf_counter() - t
+        done += 16
+        it += 1
+    return {"value": round(done / t_used, 3), "unit": "particles/s", "cores": threads,
+            "kind": "port", "sample": f"{done} particles (16 per model round-robin) in "
+                                      f"{t_used:.1f}s on {threads} threads"}
 
 
 def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):

@@ -125,6 +125,11 @@ void orc_downsample(double* data, int n_ages, const double* mean_b, const double
  * the 0-based conditions feeding s_pulse / s_chase. out has 4*n_ages+3*n_conds. */
 void orc_moment_stats(const double* s, int n_ages, int n_conds, const double* w,
                       int pulse_cond, int chase_cond, double* out);
+/* batch of abc_sim on the moment model (OpenMP over particles) */
+int orc_moment_sim(int model, int64_t n, const double* theta, int64_t ld,
+                   const orc_moment_cfg* c, const int32_t* is_chase, const double* cap,
+                   const double* w, int pulse_cond, int chase_cond, int n_threads,
+                   double* stats, double* moments);
 
 /* ------------------------------------------------- errors and acceptance */
 /* nlsqerror_part + compute_trunc_errors for one (particle, gene) pair in the

@@ -87,6 +87,8 @@ def _declare(L):
     L.orc_burn_step.argtypes = [P, C.c_int, D, D, D, D, D]
     L.orc_burn_cells.argtypes = [P, C.c_int, U64, C.c_uint32, P, C.c_int, P]
     L.orc_burn_cells.restype = C.c_int
+    L.orc_moment_sim.argtypes = [C.c_int, I64, P, I64, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P]
+    L.orc_moment_sim.restype = C.c_int
 
 
 def _p(a):
@@ -267,6 +269,25 @@ def downsample(s, mean_b, m2_b, var_b):
     vb = np.ascontiguousarray(var_b, np.float64)
     lib().orc_downsample(_p(s), s.shape[0], _p(mb), _p(m2), _p(vb))
     return s
+
+
+def moment_sim(model, theta, cfg, w, is_chase=None, capture=None, pulse_cond=5, chase_cond=6,
+               n_threads=0):
+    """abc_sim on the moment model for a batch: (stats n x K, moments
+    (n, n_conds, n_ages, 5))."""
+    th = np.asfortranarray(theta, np.float64)
+    n = th.shape[0]
+    K = 4 * cfg.n_ages + 3 * cfg.n_conds
+    wf = np.ascontiguousarray(np.asarray(w, np.float64).reshape(-1, order="F"))
+    ic = None if is_chase is None else np.ascontiguousarray(is_chase, np.int32)
+    cap = None if capture is None else np.ascontiguousarray(capture, np.float64)
+    stats = np.zeros((n, K), order="F")
+    mom = np.zeros((n, cfg.n_conds * cfg.n_ages * 5), order="F")
+    rc = lib().orc_moment_sim(model, n, _p(th), n, C.byref(cfg), _p(ic), _p(cap), _p(wf),
+                              pulse_cond, chase_cond, n_threads, _p(stats), _p(mom))
+    if rc:
+        raise RuntimeError(f"orc_moment_sim failed: {rc}")
+    return stats, mom.reshape(n, cfg.n_conds, cfg.n_ages, 5)
 
 
 def moment_stats(s, w, pulse_cond, chase_cond):

@@ -27,7 +27,11 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* Julia mod(x, y) for floats: floored remainder */
 static double jmod(double x, double y) {
@@ -373,4 +377,48 @@ void orc_moment_stats(const double* s, int n_ages, int n_conds, const double* w,
       }
     }
   }
+}
+
+/* abc_sim on the moment model for a batch (the CPU baseline of the GPU
+ * moment path): per particle orc_recover, downsampling of each condition with
+ * the capture moments of its (age, pulse/chase) group, then orc_moment_stats.
+ * theta is n x cols column-major with leading dimension ld; cap[(age * 2 +
+ * group) * 3 + {0,1,2}] = E b, E b^2, Var b or NULL; stats is n x K and
+ * moments (optional) n x (n_conds * n_ages * 5), both column-major. */
+int orc_moment_sim(int model, int64_t n, const double* theta, int64_t ld,
+                   const orc_moment_cfg* c, const int32_t* is_chase, const double* cap,
+                   const double* w, int pulse_cond, int chase_cond, int n_threads,
+                   double* stats, double* moments) {
+  if (model < 1 || model > 5 || n < 0 || ld < n) return -1;
+  const int A = c->n_ages, J = c->n_conds, K = 4 * A + 3 * J;
+  const int cols = (model >= 3 ? c->n_steps : 1) + 4;
+  const size_t per = (size_t)J * A * 5;
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int64_t i = 0; i < n; ++i) {
+    double th[32], out[4 * 16 + 3 * 32];
+    double* s = (double*)malloc(sizeof(double) * per);
+    for (int k = 0; k < cols; ++k) th[k] = theta[i + ld * k];
+    orc_recover(model, th, c, s);
+    if (cap) {
+      for (int j = 0; j < J; ++j) {
+        const int g = is_chase ? (is_chase[j] != 0) : 0;
+        double mb[16], m2[16], vb[16];
+        for (int a = 0; a < A; ++a) {
+          mb[a] = cap[(a * 2 + g) * 3 + 0];
+          m2[a] = cap[(a * 2 + g) * 3 + 1];
+          vb[a] = cap[(a * 2 + g) * 3 + 2];
+        }
+        orc_downsample(s + (size_t)j * A * 5, A, mb, m2, vb);
+      }
+    }
+    orc_moment_stats(s, A, J, w, pulse_cond, chase_cond, out);
+    for (int k = 0; k < K; ++k) stats[i + n * k] = out[k];
+    if (moments)
+      for (size_t k = 0; k < per; ++k) moments[i + n * (int64_t)k] = s[k];
+    free(s);
+  }
+  return 0;
 }
