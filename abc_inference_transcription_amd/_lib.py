@@ -97,6 +97,7 @@ def _declare(L):
                                      V, V, V]),
         "abc_moment_sim": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V, I32, I32, V,
                                  V, V]),
+        "abc_model_probs": (I32, [V, I64, I32, V, U64, I32, D, V, V, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mom, mth, mst;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mom, mth, mst, selc, selo;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -421,7 +421,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss,
-                    &c->mom, &c->mth, &c->mst};
+                    &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
   if (c->dd) hipFree(c->dd);
@@ -1070,6 +1070,44 @@ int32_t abc_moment_sim(abc_ctx* c, int32_t model, int64_t n, const double* theta
     HIPCHK(c, hipMemcpyAsync(moments, c->mom.p, sizeof(double) * n * outw, hipMemcpyDeviceToHost, c->stream));
   if (steps)
     HIPCHK(c, hipMemcpyAsync(steps, c->mst.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+// ------------------------------------------------------ model probabilities
+
+int32_t abc_model_probs(abc_ctx* c, int64_t G, int32_t M, const int64_t* counts, uint64_t seed,
+                        int32_t n_boot, double alpha, double* prob, double* lb, double* ub) {
+  if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  if (G < 0 || M < 1 || M > ABC_SEL_MAX_GROUPS)
+    return fail(c, ABC_E_ARG, "need G >= 0 and 1..8 model groups");
+  if (n_boot < 1 || n_boot > ABC_SEL_MAX_BOOT) return fail(c, ABC_E_ARG, "n_boot must be 1..256");
+  if (!(alpha >= 0.0 && alpha <= 1.0)) return fail(c, ABC_E_ARG, "alpha must lie in [0, 1]");
+  if (G == 0) return ABC_OK;
+  if (G > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 genes per call");
+  if (!counts || !prob || !lb || !ub) return fail(c, ABC_E_ARG, "counts and outputs are required");
+  for (int64_t i = 0; i < G * M; ++i)
+    if (counts[i] < 0 || counts[i] > 0xFFFFFFFFll)
+      return fail(c, ABC_E_ARG, "counts must lie in [0, 2^32)");
+  for (int64_t g = 0; g < G; ++g) {
+    int64_t S = 0;
+    for (int m = 0; m < M; ++m) S += counts[g + G * m];
+    if (S > 0xFFFFFFFFll) return fail(c, ABC_E_ARG, "at most 2^32 - 1 accepted particles per gene");
+  }
+  int rc;
+  const size_t gm = (size_t)G * M;
+  if ((rc = ensure(c, c->selc, sizeof(int64_t) * gm))) return rc;
+  if ((rc = ensure(c, c->selo, sizeof(double) * 3 * gm))) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->selc.p, counts, sizeof(int64_t) * gm, hipMemcpyHostToDevice, c->stream));
+  double* o = P<double>(c->selo);
+  hipLaunchKernelGGL(model_prob_kernel, dim3((unsigned)G), dim3(ABC_SEL_WG), 0, c->stream, G, M,
+                     P<int64_t>(c->selc), (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x5851F42Du,
+                     n_boot, alpha, o, o + gm, o + 2 * gm);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(prob, o, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(lb, o + gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ub, o + 2 * gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return ABC_OK;
 }

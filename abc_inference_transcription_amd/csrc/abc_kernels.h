@@ -122,6 +122,13 @@ __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    double* out, unsigned long long* steps);
 __global__ void moment_stats_kernel(MomCfgDev cfg, int64_t n, double* mom,
                                     double* stats);
+// Model probabilities with bootstrap bounds (scripts/model_probs.jl)
+#define ABC_SEL_WG 256
+#define ABC_SEL_MAX_GROUPS 8
+#define ABC_SEL_MAX_BOOT 256
+__global__ void model_prob_kernel(int64_t G, int M, const int64_t* counts, uint32_t k0,
+                                  uint32_t k1, int n_boot, double alpha, double* prob, double* lb,
+                                  double* ub);
 __global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
                              const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
                              double* superset);

@@ -312,6 +312,17 @@ class Engine:
         mom = mom.reshape(n, cfg.n_conds, cfg.n_ages, 5)
         return (stats, mom, st) if steps else (stats, mom)
 
+    def model_probs(self, counts, seed: int = 0, n_boot: int = 100, alpha: float = 0.95):
+        """get_model_probs for every gene (scripts/model_probs.jl:1-63):
+        counts G x M accepted particles per model group -> (prob, l_bound,
+        u_bound), each G x M."""
+        cn = np.asfortranarray(counts, np.int64)
+        G, M = cn.shape
+        prob, lb, ub = (np.zeros((G, M), order="F") for _ in range(3))
+        self._chk(self.L.abc_model_probs(self.ctx, G, M, _p(cn), seed, n_boot, alpha, _p(prob),
+                                         _p(lb), _p(ub)), "model_probs")
+        return prob, lb, ub
+
     def drop_graph(self, graph):
         if graph:
             self.L.abc_graph_destroy(C.c_void_p(graph))

@@ -217,3 +217,105 @@ def accepted_particles(eng, summary: dict, stats_by_model: dict, eps: float = 4.
     for name in res:
         _append(os.path.join(out_dir, f"particles_{name}.txt"), "".join(lines[name]))
     return res
+
+
+# ------------------------------------------------------------ model selection
+# The step after acceptance.  Model probabilities come from libabc
+# (abc_model_probs); the decisions below are the reference's per-gene rules
+# (scripts/model_selection.jl:1-78), on numpy arrays with 1-based gene ids.
+
+def model_probabilities(eng, counts, out_dir: str, prefix: str = "", seed: int = 0):
+    """scripts/model_probs.jl:33-63 (and its constant_ / non_constant_
+    variants with prefix "c_" / "nc_"): per gene, the model-group
+    probabilities and bootstrap bounds from the accepted-particle counts
+    (G x M), appended as one row per gene to model_prob.txt, l_bound.txt and
+    u_bound.txt.  Returns (prob, l_bound, u_bound)."""
+    prob, lb, ub = eng.model_probs(counts, seed=seed)
+    os.makedirs(out_dir, exist_ok=True)
+    for name, a in (("model_prob", prob), ("l_bound", lb), ("u_bound", ub)):
+        _append(os.path.join(out_dir, f"{prefix}{name}.txt"), _rows_text(a))
+    return prob, lb, ub
+
+
+def get_n_particles(posterior):
+    """model_selection.jl:71-78: accepted particles per model and gene from the
+    per-model posterior lists, where a list [0] means none."""
+    out = []
+    for p in posterior:
+        out.append(np.array([0 if list(x) == [0] else len(x) for x in p], np.int64))
+    return out
+
+
+def get_c_nc_genes(prob, l_b, u_b, gene_vec, n_particles, min_particles: int, threshold: float):
+    """model_selection.jl:1-17: constant (1), non-constant (2), undetermined
+    (0) or not fitted (3) per gene; returns the four lists of gene ids."""
+    sel = []
+    for g in gene_vec:
+        r = g - 1
+        if np.any(prob[r] != 0.0) and n_particles[r] >= min_particles:
+            if l_b[r, 0] > threshold:
+                sel.append(1)
+            elif u_b[r, 0] <= threshold:
+                sel.append(2)
+            else:
+                sel.append(0)
+        else:
+            sel.append(3)
+    gv = np.asarray(gene_vec)
+    sel = np.asarray(sel)
+    return [gv[sel == 1], gv[sel == 2], gv[sel == 0], gv[sel == 3]]
+
+
+def cluster_c_genes(prob, l_b, u_b, gene_vec):
+    """model_selection.jl:19-31: scaling (1) vs non-scaling (2) among the
+    constant genes when one model's lower bound reaches the other's upper."""
+    sel = np.zeros(len(gene_vec), np.int64)
+    for i, g in enumerate(gene_vec):
+        r = g - 1
+        if np.any(prob[r] != 0.0):
+            if l_b[r, 0] >= u_b[r, 1]:
+                sel[i] = 1
+            elif l_b[r, 1] >= u_b[r, 0]:
+                sel[i] = 2
+    gv = np.asarray(gene_vec)
+    return [gv[sel == 1], gv[sel == 2], gv[sel == 0]]
+
+
+def cluster_nc_genes(prob, l_b, u_b, gene_vec):
+    """model_selection.jl:34-68: among the non-constant genes, the models
+    whose lower bound reaches another's upper bound.  Returns (sel, [one-model
+    genes per model, two-model genes per pair, 'maybe' genes per model,
+    [undetermined genes]]) with sel[i] a list as in the reference ([m], [m1,
+    m2], [-m] or [0])."""
+    M = prob.shape[1]
+    nc_models = []
+    for g in gene_vec:
+        r = g - 1
+        nc = []
+        for m1 in range(M):
+            for m2 in [k for k in range(M) if k != m1]:
+                if np.any(prob[r, [m1, m2]] != 0.0) and l_b[r, m1] >= u_b[r, m2]:
+                    nc.append(m1 + 1)
+        nc_models.append(nc)
+    sel = []
+    for nc in nc_models:
+        freq = np.zeros(M, np.int64)
+        for m in set(nc):
+            freq[m - 1] = nc.count(m)
+        if len(nc) == M - 2:
+            sel.append([-nc[0]])
+        elif len(nc) == M - 1:
+            sel.append([nc[0]] if len(set(nc)) == 1 else list(nc))
+        elif len(nc) == M:
+            sel.append([int(k) + 1 for k in np.nonzero(freq == M - 1)[0]])
+        else:
+            sel.append([0])
+    gv = np.asarray(gene_vec)
+
+    def pick(want):
+        return gv[np.array([i for i, s in enumerate(sel) if s == want], dtype=np.int64)]
+    one = [pick([m]) for m in range(1, M + 1)]
+    two = [pick([m1, m2]) for m1 in range(1, M) for m2 in range(m1 + 1, M + 1)]
+    maybe = [pick([-m]) for m in range(1, M + 1)]
+    und = [pick([0])]
+    return sel, [one, two, maybe, und]

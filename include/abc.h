@@ -244,6 +244,20 @@ int32_t abc_moment_sim(abc_ctx* ctx, int32_t model, int64_t n, const double* the
                        int32_t pulse_cond, int32_t chase_cond, double* stats, double* moments,
                        int64_t* steps);
 
+/* Replaces get_model_probs and its per-gene branches (scripts/model_probs.jl:
+ * 1-63; constant_model_probs.jl and non_constant_model_probs.jl are the same
+ * with 2 and 3 groups).  counts is G x M column-major: the particles of model
+ * group m accepted for gene g (err <= eps, e.g. the counts of abc_accept).
+ * No group accepted: prob, lb, ub = 0; exactly one: 1 for it; otherwise
+ * prob = counts / sum and lb, ub = Julia's quantile at 1 - alpha and alpha
+ * of the group shares over n_boot resamples of the accepted labels with
+ * replacement (the reference: 100 and 0.95).  Resamples come from Philox
+ * streams keyed by seed (the reference's Julia RNG is unseeded).  M <= 8,
+ * n_boot <= 256, at most 2^32 - 1 accepted particles per gene; outputs are
+ * G x M column-major host arrays. */
+int32_t abc_model_probs(abc_ctx* ctx, int64_t G, int32_t M, const int64_t* counts, uint64_t seed,
+                        int32_t n_boot, double alpha, double* prob, double* lb, double* ub);
+
 /* ---- surrogate (no reference counterpart: the reference simulates every
  * prior particle, scripts/abc_simulation.jl:88-97).  An MLP maps a
  * particle's features to the mean and log-variance of each statistic (in the

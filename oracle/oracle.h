@@ -131,6 +131,15 @@ int orc_moment_sim(int model, int64_t n, const double* theta, int64_t ld,
                    const double* w, int pulse_cond, int chase_cond, int n_threads,
                    double* stats, double* moments);
 
+/* ------------------------------------------------ model probabilities */
+/* get_model_probs (model_probs.jl:1-29) per gene: counts is G x M col-major
+ * (accepted particles per model group); prob, lb, ub are G x M col-major;
+ * boot (optional) receives the resampled counts, [gene][resample][group]. */
+int orc_model_probs(int64_t G, int M, const int64_t* counts, uint64_t seed, int n_boot,
+                    double alpha, double* prob, double* lb, double* ub, int64_t* boot);
+/* Julia's quantile(v, p) of a sorted vector (type 7) */
+double orc_jl_quantile(const double* v, int n, double p);
+
 /* ------------------------------------------------- errors and acceptance */
 /* nlsqerror_part + compute_trunc_errors for one (particle, gene) pair in the
  * reference operation order (compute_errors.jl:30-64). */

@@ -89,6 +89,10 @@ def _declare(L):
     L.orc_burn_cells.restype = C.c_int
     L.orc_moment_sim.argtypes = [C.c_int, I64, P, I64, P, P, P, P, C.c_int, C.c_int, C.c_int, P, P]
     L.orc_moment_sim.restype = C.c_int
+    L.orc_model_probs.argtypes = [I64, C.c_int, P, U64, C.c_int, D, P, P, P, P]
+    L.orc_model_probs.restype = C.c_int
+    L.orc_jl_quantile.argtypes = [P, C.c_int, D]
+    L.orc_jl_quantile.restype = D
 
 
 def _p(a):
@@ -288,6 +292,24 @@ def moment_sim(model, theta, cfg, w, is_chase=None, capture=None, pulse_cond=5, 
     if rc:
         raise RuntimeError(f"orc_moment_sim failed: {rc}")
     return stats, mom.reshape(n, cfg.n_conds, cfg.n_ages, 5)
+
+
+def model_probs(counts, seed=0, n_boot=100, alpha=0.95, boot=False):
+    """get_model_probs for every gene: counts G x M (accepted particles per
+    model group) -> prob, lb, ub (G x M), and the resampled counts if asked."""
+    cn = np.asfortranarray(counts, np.int64)
+    G, M = cn.shape
+    prob, lb, ub = (np.zeros((G, M), order="F") for _ in range(3))
+    bt = np.zeros((G, n_boot, M), np.int64) if boot else None
+    rc = lib().orc_model_probs(G, M, _p(cn), seed, n_boot, alpha, _p(prob), _p(lb), _p(ub), _p(bt))
+    if rc:
+        raise RuntimeError(f"orc_model_probs failed: {rc}")
+    return (prob, lb, ub, bt) if boot else (prob, lb, ub)
+
+
+def jl_quantile(sorted_v, p):
+    v = np.ascontiguousarray(sorted_v, np.float64)
+    return lib().orc_jl_quantile(_p(v), len(v), float(p))
 
 
 def moment_stats(s, w, pulse_cond, chase_cond):
