@@ -243,6 +243,7 @@ def main():
         out["surrogate"] = surrogate_leg(eng, stream, a)
     if not a.no_moments:
         out["reference_moment_path"] = moment_leg(stream, a)
+    out["model_probs"] = model_probs_leg(eng, stream, a)
     print(json.dumps(out), flush=True)
     eng.close()
 
@@ -329,6 +330,37 @@ def moment_leg(stream, a):
     if not a.no_cpu:
         res["cpu"] = moment_cpu(models, cfg, w, is_chase, cap, a.cpu_seconds)
         res["gpu_vs_cpu"] = round(value / res["cpu"]["value"], 1)
+    return res
+
+
+def model_probs_leg(eng, stream, a):
+    """get_model_probs for 2,000 genes (scripts/model_probs.jl): two model
+    groups with 0..20,000 accepted particles each (synthetic counts, seed 0),
+    100 resamples per gene, on the GPU; the oracle's scalar replay of the same
+    resamples timed on a sample of genes.  One unit = one resampled label
+    (a 64-bit uniform, a multiply-high and the group search)."""
+    import torch
+    counts = np.random.default_rng(0).integers(0, 20001, size=(2000, 2))
+    draws = int(counts.sum()) * 100
+    eng.model_probs(counts[:8])  # warm-up
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    eng.model_probs(counts)
+    gpu_s = time.perf_counter() - t
+    res = {"genes": 2000, "groups": 2, "resamples": 100, "draws": draws,
+           "gpu_ms": round(gpu_s * 1e3, 3), "gpu_draws_per_s": draws / gpu_s,
+           "note": "host-array call: includes the copies and the synchronisation"}
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        sample = counts[:20]
+        t = time.perf_counter()
+        O.model_probs(sample)
+        cpu_s = time.perf_counter() - t
+        cd = int(sample.sum()) * 100
+        res["cpu"] = {"draws_per_s": cd / cpu_s, "cores": 1, "kind": "port",
+                      "sample": f"20 genes, {cd} draws in {cpu_s:.2f}s"}
+        res["gpu_vs_cpu"] = round((draws / gpu_s) / (cd / cpu_s), 1)
     return res
 
 
