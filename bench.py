@@ -48,7 +48,7 @@ def parse():
     ap.add_argument("--sur-steps", type=int, default=3000, help="Adam steps (batch 8192)")
     ap.add_argument("--no-moments", action="store_true",
                     help="skip the leg of the reference's own algorithm (moment model)")
-    ap.add_argument("--mom-particles", type=int, default=8192,
+    ap.add_argument("--mom-particles", type=int, default=32768,
                     help="particles per model of the moment-model leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (profiler byte-counter scale)")
