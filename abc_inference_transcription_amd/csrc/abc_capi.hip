@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mom, mth, mst, selc, selo;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -420,7 +420,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss,
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -942,9 +942,13 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
                const MomCfgDev& m, double* out_dev, int64_t* steps_dev, double* stats_dev) {
   int rc;
   if ((rc = ensure(c, c->mss, sizeof(double) * 9 * (size_t)n))) return rc;
+  if ((rc = ensure(c, c->mmap, sizeof(double) * 90 * (size_t)n))) return rc;
   unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
-  hipLaunchKernelGGL(moment_transient_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, model,
-                     n, theta_dev, ld, P<double>(c->mss), steps);
+  hipLaunchKernelGGL(moment_cyclemap_kernel, dim3(grid1(n * 10, 64)), dim3(64), 0, c->stream, m,
+                     model, n, theta_dev, ld, P<double>(c->mmap), steps);
+  HIPCHK(c, hipGetLastError());
+  hipLaunchKernelGGL(moment_transient_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,
+                     P<double>(c->mmap), P<double>(c->mss));
   HIPCHK(c, hipGetLastError());
   const int64_t items = n * (int64_t)m.n_conds * m.n_ages;
   hipLaunchKernelGGL(moment_traj_kernel, dim3(grid1(items, 64)), dim3(64), 0, c->stream, m, model,

@@ -114,9 +114,11 @@ struct MomCfgDev {
   double cap[ABC_MOM_MAX_AGES * 2 * 3];  // per (age, pulse/chase): E b, E b^2, Var b
   double weights[ABC_MOM_MAX_AGES * ABC_MOM_MAX_CONDS];  // age weights, n_ages x n_conds
 };
-__global__ void moment_transient_kernel(MomCfgDev cfg, int model, int64_t n,
-                                        const double* theta, int64_t ld, double* ss,
-                                        unsigned long long* steps);
+__global__ void moment_cyclemap_kernel(MomCfgDev cfg, int model, int64_t n,
+                                       const double* theta, int64_t ld, double* maps,
+                                       unsigned long long* steps);
+__global__ void moment_transient_kernel(MomCfgDev cfg, int64_t n, const double* maps,
+                                        double* ss);
 __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    const double* theta, int64_t ld, const double* ss,
                                    double* out, unsigned long long* steps);

@@ -21,8 +21,10 @@
 // trajectory kernel spills 52 bytes per lane): 12 % faster at 65,536
 // particles than the unconstrained allocation, which runs one wave per SIMD.
 //
-// Work decomposition.  moment_transient_kernel: one lane per particle
-// iterates the cycle map to the periodic state.  moment_traj_kernel: one lane
+// Work decomposition.  moment_cyclemap_kernel: one lane per (particle, start
+// vector) integrates the 10 cycle maps that fix the affine cycle map;
+// moment_transient_kernel: one lane per particle finds the periodic state
+// from it.  moment_traj_kernel: one lane
 // per (particle, condition, age) integrates that state from t0 to the age;
 // the items of a particle are adjacent, so the lanes of a wavefront share
 // their rates (and their stiffness, which sets the step count).
@@ -287,42 +289,56 @@ ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* thet
 
 }  // namespace
 
-// Periodic state after division, one lane per particle.  Mode 0 is the
-// reference's rule (model.jl:114-142): iterate the cycle map from iv until
-// the gene and unlabelled moments change by at most 1 % (at most 101 maps).
-// Mode 1 solves the affine fixed point exactly: the cycle map is y -> M y + v
-// and the partition P is linear, so the state after division solves
-// (I - P M) x = P v (10 cycle maps and a 9 x 9 elimination with partial
-// pivoting).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_transient_kernel(const MomCfgDev cfg,
-                                                               int model, int64_t n,
-                                                               const double* __restrict__ theta,
-                                                               int64_t ld, double* __restrict__ ss,
-                                                               unsigned long long* __restrict__ steps) {
+// The cycle map is affine, y -> M y + v (the moment ODE is linear): v is the
+// map of the zero state and column i of M the map of unit vector i minus v.
+// moment_cyclemap_kernel integrates the 10 maps of every particle, one lane
+// each (item p * 10 + col, col 0 = zero state, col 1 + i = unit vector i),
+// into maps[(p * 10 + col) * 9 + r]; the transient rules below need nothing
+// else, so a particle's longest chain is one cycle whatever its iterations.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_cyclemap_kernel(
+    const MomCfgDev cfg, int model, int64_t n, const double* __restrict__ theta, int64_t ld,
+    double* __restrict__ maps, unsigned long long* __restrict__ steps) {
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
   __syncthreads();
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n * 10) return;
+  const int64_t p = q / 10;
+  const int col = (int)(q - p * 10);
+  const Particle P = particle_of(model, cfg, theta, ld, p);
+  double in[9], out[9];
+  for (int i = 0; i < 9; ++i) in[i] = (col > 0 && i == col - 1) ? 1.0 : 0.0;
+  const int64_t st = cycle_map(P, cfg, in, out, ltab);
+  double* o = maps + q * 9;
+  for (int i = 0; i < 9; ++i) o[i] = out[i];
+  if (steps) atomicAdd(steps + p, (unsigned long long)st);
+}
+
+// Periodic state after division, one lane per particle, from (M, v).  Mode 0
+// is the reference's rule (model.jl:114-142): iterate the cycle map from iv
+// until the gene and unlabelled moments change by at most 1 % (at most 101
+// maps).  Mode 1 solves the fixed point exactly: the partition P is linear,
+// so the state after division solves (I - P M) x = P v (Gaussian elimination
+// with partial pivoting).
+__global__ __launch_bounds__(64) void moment_transient_kernel(const MomCfgDev cfg, int64_t n,
+                                                               const double* __restrict__ maps,
+                                                               double* __restrict__ ss) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const Particle P = particle_of(model, cfg, theta, ld, p);
-  int64_t st = 0;
-  double out[9];
+  const double* mp = maps + p * 90;
+  double M[9][9], v[9], out[9];
+  for (int r = 0; r < 9; ++r) v[r] = mp[r];
+  for (int i = 0; i < 9; ++i)
+    for (int r = 0; r < 9; ++r) M[r][i] = mp[9 * (i + 1) + r] - v[r];
   if (cfg.transient_mode == 1) {
-    double zero[9], v[9], Pv[9], A[9][9];
-    for (int i = 0; i < 9; ++i) zero[i] = 0.0;
-    st += cycle_map(P, cfg, zero, v, ltab);
+    double Pv[9], A[9][9];
     for (int i = 0; i < 9; ++i) {
-      double e[9], o[9];
-      for (int r = 0; r < 9; ++r) e[r] = 0.0;
-      e[i] = 1.0;
-      st += cycle_map(P, cfg, e, o, ltab);
       double col[9], pc[9];
-      for (int r = 0; r < 9; ++r) col[r] = o[r] - v[r];
+      for (int r = 0; r < 9; ++r) col[r] = M[r][i];
       periodic_boundary(col, pc);
       for (int r = 0; r < 9; ++r) A[r][i] = (r == i ? 1.0 : 0.0) - pc[r];
     }
     periodic_boundary(v, Pv);
-    // Gaussian elimination with partial pivoting
     for (int col = 0; col < 9; ++col) {
       int piv = col;
       for (int r = col + 1; r < 9; ++r)
@@ -352,17 +368,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
   } else {
     const int check_idx[5] = {0, 1, 3, 4, 6};
     double e1[9], e2[9], iv[9];
-    st += cycle_map(P, cfg, cfg.iv, e1, ltab);
+    for (int r = 0; r < 9; ++r) {
+      double acc = v[r];
+      for (int i = 0; i < 9; ++i) acc = acc + M[r][i] * cfg.iv[i];
+      e1[r] = acc;
+    }
     for (int i = 0; i < 9; ++i) e2[i] = e1[i];
     int k = 0;
     bool conv = false;
     while (!conv && k <= 100) {
       k++;
       periodic_boundary(e1, iv);
-      st += cycle_map(P, cfg, iv, e2, ltab);
+      for (int r = 0; r < 9; ++r) {
+        double acc = v[r];
+        for (int i = 0; i < 9; ++i) acc = acc + M[r][i] * iv[i];
+        e2[r] = acc;
+      }
       int nz = 0, ok = 0;
-      for (int q = 0; q < 5; ++q) {
-        const int i = check_idx[q];
+      for (int qq = 0; qq < 5; ++qq) {
+        const int i = check_idx[qq];
         if (e1[i] > 0.0) {
           nz++;
           if (fabs((e1[i] - e2[i]) / e1[i]) <= 0.01) ok++;
@@ -374,7 +398,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
     periodic_boundary(e2, out);
   }
   for (int i = 0; i < 9; ++i) ss[p + (int64_t)i * n] = out[i];
-  if (steps) steps[p] = (unsigned long long)st;
 }
 
 // trajectories (model.jl:146-174) for item q = (particle, condition, age):

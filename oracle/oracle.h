@@ -114,6 +114,8 @@ int orc_burn_cells(const orc_design* d, int model, uint64_t seed, uint32_t pidx,
 void orc_burn_step(double* io, int g, double c0, double c1, double ku, double kt, double D);
 /* transient_phase (model.jl:114-142); returns iterations. */
 int orc_transient(int model, const double* theta, const orc_moment_cfg* c, double* ss);
+/* the 1 % rule by integrating every cycle (checks the affine form) */
+int orc_transient_direct(int model, const double* theta, const orc_moment_cfg* c, double* ss);
 /* run_part_sim without downsampling (recover_statistics.jl:2-11):
  * out[cond][age][5] = [mean_u, mean_l, var_u, cov_ul, var_l]. */
 void orc_recover(int model, const double* theta, const orc_moment_cfg* c, double* out);

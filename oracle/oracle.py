@@ -76,6 +76,8 @@ def _declare(L):
     L.orc_moment_solve.restype = I64
     L.orc_periodic_boundary.argtypes = [P, P]
     L.orc_transient.argtypes = [C.c_int, P, P, P]; L.orc_transient.restype = C.c_int
+    L.orc_transient_direct.argtypes = [C.c_int, P, P, P]
+    L.orc_transient_direct.restype = C.c_int
     L.orc_recover.argtypes = [C.c_int, P, P, P]
     L.orc_downsample.argtypes = [P, C.c_int, P, P, P]
     L.orc_moment_stats.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, P]
@@ -260,10 +262,13 @@ def recover(model, theta, cfg):
     return out
 
 
-def transient(model, theta, cfg):
+def transient(model, theta, cfg, direct=False):
+    """Periodic state after division and the iterations the 1 % rule took;
+    direct=True integrates every cycle instead of using the affine map."""
     th = np.ascontiguousarray(theta, np.float64)
     ss = np.zeros(9)
-    k = lib().orc_transient(model, _p(th), C.byref(cfg), _p(ss))
+    f = lib().orc_transient_direct if direct else lib().orc_transient
+    k = f(model, _p(th), C.byref(cfg), _p(ss))
     return ss, k
 
 

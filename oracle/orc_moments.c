@@ -252,18 +252,37 @@ static void solve9(double A[9][9], double* b) {
   }
 }
 
+/* The cycle map is affine, y -> M y + v (the moment ODE is linear): v is the
+ * map of the zero state and column i of M the map of unit vector i minus v.
+ * Both transient rules work on (M, v), so they need 10 integrations that
+ * do not depend on each other, whatever the number of iterations. */
+static void cycle_affine(int model, const double* th, const orc_moment_cfg* c, double M[9][9],
+                         double* v) {
+  double zero[9] = {0};
+  cycle_map(model, th, c, zero, v);
+  for (int i = 0; i < 9; ++i) {
+    double e[9] = {0}, o[9];
+    e[i] = 1.0;
+    cycle_map(model, th, c, e, o);
+    for (int r = 0; r < 9; ++r) M[r][i] = o[r] - v[r];
+  }
+}
+
+static void affine_apply(double M[9][9], const double* v, const double* y, double* out) {
+  for (int r = 0; r < 9; ++r) {
+    double acc = v[r];
+    for (int i = 0; i < 9; ++i) acc = acc + M[r][i] * y[i];
+    out[r] = acc;
+  }
+}
+
 int orc_transient(int model, const double* th, const orc_moment_cfg* c, double* ss) {
+  double M[9][9], v[9];
+  cycle_affine(model, th, c, M, v);
   if (c->transient_mode == 1) {
-    /* exact periodic steady state: the cycle map y -> M y + v is affine, the
-     * partition P is linear, so the state after division solves (I - P M) x = P v */
-    double zero[9] = {0}, v[9], Pv[9], M[9][9], A[9][9];
-    cycle_map(model, th, c, zero, v);
-    for (int i = 0; i < 9; ++i) {
-      double e[9] = {0}, o[9];
-      e[i] = 1.0;
-      cycle_map(model, th, c, e, o);
-      for (int r = 0; r < 9; ++r) M[r][i] = o[r] - v[r];
-    }
+    /* exact periodic steady state: the partition P is linear, so the state
+     * after division solves (I - P M) x = P v */
+    double Pv[9], A[9][9];
     orc_periodic_boundary(v, Pv);
     for (int i = 0; i < 9; ++i) {
       double col[9], pc[9];
@@ -275,14 +294,41 @@ int orc_transient(int model, const double* th, const orc_moment_cfg* c, double* 
     solve9(A, ss);
     return 0;
   }
-  /* reference rule, model.jl:114-142 */
+  /* reference rule, model.jl:114-142, on the affine map */
+  static const int check_idx[5] = {0, 1, 3, 4, 6};
+  double e1[9], e2[9], iv[9];
+  affine_apply(M, v, c->iv, e1);
+  memcpy(e2, e1, sizeof e1);
+  int k = 0, conv = 0;
+  const int max_iter = 100;
+  while (!conv && k <= max_iter) {
+    k++;
+    orc_periodic_boundary(e1, iv);
+    affine_apply(M, v, iv, e2);
+    int nz = 0, ok = 0;
+    for (int q = 0; q < 5; ++q) {
+      int i = check_idx[q];
+      if (e1[i] > 0.0) {
+        nz++;
+        if (fabs((e1[i] - e2[i]) / e1[i]) <= 0.01) ok++;
+      }
+    }
+    if (ok == nz) conv = 1;
+    memcpy(e1, e2, sizeof e2);
+  }
+  orc_periodic_boundary(e2, ss);
+  return k;
+}
+
+/* Test hook: the reference rule by direct integration of every cycle (as
+ * model.jl:128-141 runs it), for checking the affine form above. */
+int orc_transient_direct(int model, const double* th, const orc_moment_cfg* c, double* ss) {
   static const int check_idx[5] = {0, 1, 3, 4, 6};
   double e1[9], e2[9], iv[9];
   cycle_map(model, th, c, c->iv, e1);
   memcpy(e2, e1, sizeof e1);
   int k = 0, conv = 0;
-  const int max_iter = 100;
-  while (!conv && k <= max_iter) {
+  while (!conv && k <= 100) {
     k++;
     orc_periodic_boundary(e1, iv);
     cycle_map(model, th, c, iv, e2);

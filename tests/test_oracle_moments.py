@@ -157,3 +157,20 @@ def test_moment_sim_batch_is_the_composition(oracle):
             s[j] = oracle.downsample(s[j], cap[:, g, 0], cap[:, g, 1], cap[:, g, 2])
         assert np.array_equal(mom[i], s)
         assert np.array_equal(stats[i], oracle.moment_stats(s, w, 5, 6))
+
+
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
+def test_transient_affine_form_matches_direct_iteration(oracle, model):
+    """The 1 % rule run on the cycle map's affine form (10 integrations) takes
+    the same number of iterations as integrating every cycle (model.jl:128-141)
+    and reaches the same state to integration accuracy."""
+    from abc_inference_transcription_amd.design import reference_design
+    od = oracle.OracleDesign(reference_design(cells=55))
+    th = oracle.sample_prior(od, model, 0, 400 + model, 4)
+    for iv in (None, [0.0, 0.5] + [0.0] * 7):
+        cfg = ref_cfg(oracle, iv=iv)
+        for i in range(len(th)):
+            a, ka = oracle.transient(model, th[i], cfg)
+            d, kd = oracle.transient(model, th[i], cfg, direct=True)
+            assert ka == kd
+            assert np.allclose(a, d, rtol=1e-7, atol=1e-12), (model, i, np.max(np.abs(a - d)))
