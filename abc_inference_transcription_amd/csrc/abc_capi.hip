@@ -952,7 +952,7 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   HIPCHK(c, hipGetLastError());
   const int64_t items = n * (int64_t)m.n_conds * m.n_ages;
   hipLaunchKernelGGL(moment_traj_kernel, dim3(grid1(items, 64)), dim3(64), 0, c->stream, m, model,
-                     n, theta_dev, ld, P<double>(c->mss), out_dev, steps);
+                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), out_dev, steps);
   HIPCHK(c, hipGetLastError());
   if (stats_dev) {
     hipLaunchKernelGGL(moment_stats_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,

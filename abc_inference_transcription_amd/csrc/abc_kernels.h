@@ -121,7 +121,7 @@ __global__ void moment_transient_kernel(MomCfgDev cfg, int64_t n, const double* 
                                         double* ss);
 __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    const double* theta, int64_t ld, const double* ss,
-                                   double* out, unsigned long long* steps);
+                                   const double* maps, double* out, unsigned long long* steps);
 __global__ void moment_stats_kernel(MomCfgDev cfg, int64_t n, double* mom,
                                     double* stats);
 // Model probabilities with bootstrap bounds (scripts/model_probs.jl)

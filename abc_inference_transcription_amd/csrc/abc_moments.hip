@@ -401,12 +401,16 @@ __global__ __launch_bounds__(64) void moment_transient_kernel(const MomCfgDev cf
 }
 
 // trajectories (model.jl:146-174) for item q = (particle, condition, age):
-// out column (cond * n_ages + age) * 5 + {mean_u, mean_l, var_u, cov_ul, var_l}
+// out column (cond * n_ages + age) * 5 + {mean_u, mean_l, var_u, cov_ul, var_l}.
+// A whole cycle that starts on a cycle boundary and does not meet the
+// labelling window has the rates of [0, T] shifted by whole cycles and no
+// labelling, so it is the particle's affine cycle map, applied from maps.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_traj_kernel(const MomCfgDev cfg,
                                                           int model, int64_t n,
                                                           const double* __restrict__ theta,
                                                           int64_t ld,
                                                           const double* __restrict__ ss,
+                                                          const double* __restrict__ maps,
                                                           double* __restrict__ out,
                                                           unsigned long long* __restrict__ steps) {
   __shared__ LogEntry ltab[ABC_LOG_N];
@@ -422,6 +426,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
   const double T = cfg.cycle, t0 = cfg.t0, age = cfg.ages[ai];
   const double pulse = cfg.pulse[j], texp = age - cfg.pulse[j] - cfg.chase[j];
   const int nsols = (age > 0 && age < T) ? (int)floor((age - t0) / T) + 1 : (int)floor((age - t0) / T);
+  const double we = texp + pulse;
+  const double* mp = maps + p * 90;
   double y[9], tmp[9];
   for (int i = 0; i < 9; ++i) y[i] = ss[p + (int64_t)i * n];
   double tau = t0;
@@ -432,7 +438,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
       periodic_boundary(y, tmp);
       for (int i = 0; i < 9; ++i) y[i] = tmp[i];
     }
-    st += moment_solve(P, cfg, texp, pulse, tau, tf, y, ltab);
+    if (tau < 0 && jmod(tau, T) == 0.0 && (we <= tau || texp >= tf)) {
+      // y <- v + M y with M[r][i] = map_i[r] - v[r], in the CPU restatement's order
+      for (int r = 0; r < 9; ++r) {
+        const double vr = mp[r];
+        double acc = vr;
+        for (int i = 0; i < 9; ++i) acc = acc + (mp[9 * (i + 1) + r] - vr) * y[i];
+        tmp[r] = acc;
+      }
+      for (int i = 0; i < 9; ++i) y[i] = tmp[i];
+    } else {
+      st += moment_solve(P, cfg, texp, pulse, tau, tf, y, ltab);
+    }
     tau = tf;
   }
   double* o = out + p + (int64_t)r * 5 * n;

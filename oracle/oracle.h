@@ -119,6 +119,9 @@ int orc_transient_direct(int model, const double* theta, const orc_moment_cfg* c
 /* run_part_sim without downsampling (recover_statistics.jl:2-11):
  * out[cond][age][5] = [mean_u, mean_l, var_u, cov_ul, var_l]. */
 void orc_recover(int model, const double* theta, const orc_moment_cfg* c, double* out);
+/* the same, integrating every cycle of every trajectory (checks the affine
+ * shortcut for whole unlabelled cycles) */
+void orc_recover_direct(int model, const double* theta, const orc_moment_cfg* c, double* out);
 /* downsample (model.jl:221-239): data n_ages x 5 row-major in place. */
 void orc_downsample(double* data, int n_ages, const double* mean_b, const double* m2_b,
                     const double* var_b);

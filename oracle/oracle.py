@@ -79,6 +79,7 @@ def _declare(L):
     L.orc_transient_direct.argtypes = [C.c_int, P, P, P]
     L.orc_transient_direct.restype = C.c_int
     L.orc_recover.argtypes = [C.c_int, P, P, P]
+    L.orc_recover_direct.argtypes = [C.c_int, P, P, P]
     L.orc_downsample.argtypes = [P, C.c_int, P, P, P]
     L.orc_moment_stats.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, P]
     L.orc_error_pair.argtypes = [P, P, P, C.c_int, P, C.c_int, D]
@@ -255,10 +256,14 @@ def moment_cfg(ages, pulse, chase, cycle=20.0, n_steps=5, t0=None, iv=None,
     return cfg
 
 
-def recover(model, theta, cfg):
+def recover(model, theta, cfg, direct=False):
+    """run_part_sim without downsampling: (n_conds, n_ages, 5); direct=True
+    integrates every cycle instead of applying the affine cycle map to the
+    whole cycles that do not meet the labelling window."""
     th = np.ascontiguousarray(theta, np.float64)
     out = np.zeros((cfg.n_conds, cfg.n_ages, 5))
-    lib().orc_recover(model, _p(th), C.byref(cfg), _p(out))
+    f = lib().orc_recover_direct if direct else lib().orc_recover
+    f(model, _p(th), C.byref(cfg), _p(out))
     return out
 
 

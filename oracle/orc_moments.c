@@ -276,9 +276,7 @@ static void affine_apply(double M[9][9], const double* v, const double* y, doubl
   }
 }
 
-int orc_transient(int model, const double* th, const orc_moment_cfg* c, double* ss) {
-  double M[9][9], v[9];
-  cycle_affine(model, th, c, M, v);
+static int transient_from(double M[9][9], const double* v, const orc_moment_cfg* c, double* ss) {
   if (c->transient_mode == 1) {
     /* exact periodic steady state: the partition P is linear, so the state
      * after division solves (I - P M) x = P v */
@@ -320,6 +318,12 @@ int orc_transient(int model, const double* th, const orc_moment_cfg* c, double* 
   return k;
 }
 
+int orc_transient(int model, const double* th, const orc_moment_cfg* c, double* ss) {
+  double M[9][9], v[9];
+  cycle_affine(model, th, c, M, v);
+  return transient_from(M, v, c, ss);
+}
+
 /* Test hook: the reference rule by direct integration of every cycle (as
  * model.jl:128-141 runs it), for checking the affine form above. */
 int orc_transient_direct(int model, const double* th, const orc_moment_cfg* c, double* ss) {
@@ -347,31 +351,52 @@ int orc_transient_direct(int model, const double* th, const orc_moment_cfg* c, d
   return k;
 }
 
-/* trajectories, model.jl:146-174 */
+/* trajectories, model.jl:146-174.  A whole cycle that starts on a cycle
+ * boundary and does not meet the labelling window [texp, texp + pulse] has
+ * the rates of [0, T] shifted by whole cycles and no labelling, so it is the
+ * affine cycle map (M, v) itself; with M = NULL every cycle is integrated. */
 static void trajectory(int model, const double* th, const orc_moment_cfg* c, const double* iv,
-                       double age, double pulse, double texp, double* out5) {
-  double T = c->cycle, t0 = c->t0;
+                       double M[9][9], const double* v, double age, double pulse, double texp,
+                       double* out5) {
+  double T = c->cycle, t0 = c->t0, we = texp + pulse;
   int nsols = (age > 0 && age < T) ? (int)floor((age - t0) / T) + 1 : (int)floor((age - t0) / T);
   double tau = t0, y[9], tmp[9];
   memcpy(y, iv, sizeof y);
   for (int k = 1; k <= nsols; ++k) {
     double tf = tau < 0 ? tau + T : age;
     if (k > 1) { orc_periodic_boundary(y, tmp); memcpy(y, tmp, sizeof y); }
-    orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol);
+    if (M && tau < 0 && jmod(tau, T) == 0.0 && (we <= tau || texp >= tf)) {
+      affine_apply(M, v, y, tmp);
+      memcpy(y, tmp, sizeof y);
+    } else {
+      orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol);
+    }
     tau = tf;
   }
   out5[0] = y[1]; out5[1] = y[2]; out5[2] = y[6]; out5[3] = y[7]; out5[4] = y[8];
 }
 
-void orc_recover(int model, const double* th, const orc_moment_cfg* c, double* out) {
-  double ss[9];
-  orc_transient(model, th, c, ss);
+static void recover(int model, const double* th, const orc_moment_cfg* c, double* out,
+                    int direct) {
+  double M[9][9], v[9], ss[9];
+  cycle_affine(model, th, c, M, v);
+  transient_from(M, v, c, ss);
   for (int j = 0; j < c->n_conds; ++j)
     for (int a = 0; a < c->n_ages; ++a) {
       double age = c->ages[a];
       double texp = age - c->pulse[j] - c->chase[j];
-      trajectory(model, th, c, ss, age, c->pulse[j], texp, out + (j * c->n_ages + a) * 5);
+      trajectory(model, th, c, ss, direct ? NULL : M, v, age, c->pulse[j], texp,
+                 out + (j * c->n_ages + a) * 5);
     }
+}
+
+void orc_recover(int model, const double* th, const orc_moment_cfg* c, double* out) {
+  recover(model, th, c, out, 0);
+}
+
+/* Test hook: orc_recover integrating every cycle of every trajectory. */
+void orc_recover_direct(int model, const double* th, const orc_moment_cfg* c, double* out) {
+  recover(model, th, c, out, 1);
 }
 
 void orc_downsample(double* s, int n_ages, const double* mb, const double* m2b,

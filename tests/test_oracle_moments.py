@@ -174,3 +174,18 @@ def test_transient_affine_form_matches_direct_iteration(oracle, model):
             d, kd = oracle.transient(model, th[i], cfg, direct=True)
             assert ka == kd
             assert np.allclose(a, d, rtol=1e-7, atol=1e-12), (model, i, np.max(np.abs(a - d)))
+
+
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
+def test_whole_unlabelled_cycles_are_the_affine_map(oracle, model):
+    """Trajectories that apply the affine cycle map to whole cycles away from
+    the labelling window equal trajectories that integrate every cycle, to
+    integration accuracy, at every condition and age."""
+    from abc_inference_transcription_amd.design import reference_design
+    od = oracle.OracleDesign(reference_design(cells=55))
+    th = oracle.sample_prior(od, model, 0, 600 + model, 3)
+    cfg = ref_cfg(oracle)
+    for i in range(len(th)):
+        a = oracle.recover(model, th[i], cfg)
+        d = oracle.recover(model, th[i], cfg, direct=True)
+        assert np.allclose(a, d, rtol=1e-7, atol=1e-12), (model, i, np.max(np.abs(a - d)))
