@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -168,8 +168,22 @@ int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
   hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(ABC_NBUCKET), 0, c->stream, hist, cursor);
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
                      P<float>(c->cost), n, cursor, P<int32_t>(c->order));
-  hipLaunchKernelGGL(items_scan_kernel, dim3((unsigned)nblk), dim3(256), 0, c->stream, c->dd,
-                     P<PRates>(c->rates), P<int32_t>(c->order), n, P<int64_t>(c->off),
+  HIPCHK(c, hipGetLastError());
+  return ABC_OK;
+}
+
+// after the burn-in: each cell's sub-lanes -> per-particle cell offsets and
+// item counts -> the item offsets of the particles in queue order
+int stage_items(abc_ctx* c, int64_t n) {
+  int rc;
+  const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
+  const int C = c->hd.C;
+  if ((rc = ensure(c, c->coff, sizeof(int32_t) * (size_t)n * (size_t)(C + 1)))) return rc;
+  if ((rc = ensure(c, c->pitems, sizeof(int64_t) * (size_t)n))) return rc;
+  hipLaunchKernelGGL(cell_items_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, n, C,
+                     P<CellStart>(c->cellrec), P<int32_t>(c->coff), P<int64_t>(c->pitems));
+  hipLaunchKernelGGL(items_scan_kernel, dim3((unsigned)nblk), dim3(256), 0, c->stream,
+                     P<int64_t>(c->pitems), P<int32_t>(c->order), n, P<int64_t>(c->off),
                      P<int64_t>(c->bsum));
   hipLaunchKernelGGL(bsum_scan_kernel, dim3(1), dim3(256), 0, c->stream, P<int64_t>(c->bsum), nblk);
   hipLaunchKernelGGL(add_bsum_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
@@ -249,11 +263,13 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
                      key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
                      P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev,
                      P<unsigned long long>(c->tally));
+  HIPCHK(c, hipGetLastError());
+  if ((rc = stage_items(c, n))) return rc;
   if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
                      (uint32_t)first, key0(seed), key1(seed, model), P<CellStart>(c->cellrec),
-                     P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu), P<uint32_t>(c->cl),
+                     P<int32_t>(c->coff), P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu), P<uint32_t>(c->cl),
                      P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react), pidx_dev);
   HIPCHK(c, hipGetLastError());
   if (trio) HIPCHK(c, hipEventRecord(trio[2], c->stream));
@@ -420,7 +436,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap,
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);

@@ -39,11 +39,12 @@ struct DesignDev {
 struct __align__(16) PRates {
   double kon[ABC_MAX_BINS], koff[ABC_MAX_BINS], amax[ABC_MAX_BINS], gam[ABC_MAX_BINS];
   double bs;                       // splicing rate
+  double brate;                    // mean over the bins of alpha_max * P(gene on)
   uint64_t thr_lam;                // floor(lambda * 2^32): labelling threshold
   uint64_t thr_pi;                 // floor(P(gene on at cycle start) * 2^32)
   int32_t B, step, scaling;        // burn-in cycles, grid stride, size scaling
   int32_t gene_varies;             // kon differs between bins (burst-frequency model)
-  int32_t M, pad[3];               // sub-lanes per cell
+  int32_t M, pad[3];               // typical sub-lanes per cell (the queue-order key only)
 };
 
 // A cell at the start of its labelling window (burn_kernel -> ssa_kernel).
@@ -53,7 +54,8 @@ struct __align__(16) CellStart {
   double lu, lt;    // Poisson means of the unspliced / all unlabelled molecules
   int64_t k;        // last grid edge passed
   uint64_t gctr;    // gene-stream counter
-  uint8_t bin, g, gneed, pad;
+  uint8_t bin, g, gneed;
+  uint8_t msh;      // log2 of the cell's sub-lanes M
 };
 
 struct ErrSpec {
@@ -79,10 +81,12 @@ __global__ void bucket_hist_kernel(const float* cost, int64_t n, unsigned* hist)
 __global__ void bucket_scan_kernel(const unsigned* hist, unsigned* cursor);
 __global__ void bucket_scatter_kernel(const float* cost, int64_t n, unsigned* cursor,
                                       int32_t* order);
-__global__ void items_scan_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
-                                  int64_t n, int64_t* off, int64_t* bsum);
+__global__ void items_scan_kernel(const int64_t* pitems, const int32_t* order, int64_t n,
+                                  int64_t* off, int64_t* bsum);
 __global__ void bsum_scan_kernel(int64_t* bsum, int64_t nblk);
 __global__ void add_bsum_kernel(int64_t* off, int64_t n, const int64_t* bsum);
+__global__ void cell_items_kernel(int64_t n, int C, const CellStart* rec, int32_t* coff,
+                                  int64_t* pitems);
 __global__ void burn_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
                             uint32_t first, uint32_t k0, uint32_t k1, const double* betas,
                             const int64_t* goff, unsigned long long* queue, CellStart* rec,
@@ -90,7 +94,8 @@ __global__ void burn_kernel(const DesignDev* D, const PRates* R, const int32_t* 
                             const int64_t* pidx, unsigned long long* burn_tally);
 __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
                            const int64_t* off, int64_t n, uint32_t first, uint32_t k0,
-                           uint32_t k1, const CellStart* rec, unsigned long long* queue,
+                           uint32_t k1, const CellStart* rec, const int32_t* coff,
+                           unsigned long long* queue,
                            uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,
                            unsigned long long* steps_out, unsigned long long* react_out,
                            const int64_t* pidx);

@@ -160,15 +160,16 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
   while (S > D->burn_eps && B < D->bmax) { S = S * s; B++; }
   r.B = B;
   r.gene_varies = L.n_kon > 1;
-  // Sub-lanes per cell: smallest power of two M with (expected molecule
-  // events per cell) <= split_target * M.  This value selects random streams,
-  // so it is computed with the same operations as the CPU replay.
+  // Mean birth rate with the gene's activity (before capture): the per-cell
+  // split below uses it with the operations of the CPU replay, since it
+  // selects random streams.  The typical sub-lanes M of an average cell
+  // (longest window, mean capture) only orders the work queue.
+  double births = 0.0;
+  for (int j = 0; j < nb; ++j) births = births + r.amax[j] * (r.kon[j] / (r.kon[j] + r.koff[j]));
+  r.brate = births / (double)nb;
   int M = 1;
   if (D->split_target > 0.0) {
-    double births = 0.0;
-    for (int j = 0; j < nb; ++j) births = births + r.amax[j] * (r.kon[j] / (r.kon[j] + r.koff[j]));
-    // molecule events are simulated over the labelling window only
-    const double est = (((births / (double)nb) * D->split_scale) * D->wmax) * 3.0;
+    const double est = ((r.brate * D->split_scale) * D->wmax) * 3.0;
     while (M < D->split_max && est > D->split_target * (double)M) M *= 2;
   }
   r.M = M;
@@ -176,16 +177,16 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
   // Expected Philox draws per work item (a scheduling key only, never a
   // result): gene switches over the whole path + molecule events (births
   // times 1 + splicing + loss, a 1/M share) over about half the longest window.
-  double sw = 0.0, births = 0.0, loss = 0.0;
+  double sw = 0.0, obs = 0.0, loss = 0.0;
   for (int j = 0; j < nb; ++j) {
     const double pon = r.kon[j] / (r.kon[j] + r.koff[j]);
     sw += 2.0 * r.kon[j] * (1.0 - pon);
     const double b = r.amax[j] * pon * D->capture_mean;
-    births += b;
+    obs += b;
     loss += b * (r.bs / (r.bs + r.gam[j]) + 1.0);
   }
   cost[i] = (float)((sw / (double)nb) * T * ((double)B + 0.5) +
-                    ((births + loss) / ((double)nb * (double)M)) * (0.5 * D->wmax) + 4.0);
+                    ((obs + loss) / ((double)nb * (double)M)) * (0.5 * D->wmax) + 4.0);
 }
 
 // -------------------------------------------------------------- queue order
@@ -240,11 +241,47 @@ __global__ __launch_bounds__(256) void bucket_scatter_kernel(const float* __rest
   order[pos] = (int32_t)i;
 }
 
-// Work items of particle rank r = cells x sub-lanes.  off[r + 1] = inclusive
-// sum over ranks <= r, computed per block of ABC_SCAN_BLOCK ranks here (block
-// totals to bsum), then offset by bsum_scan_kernel / add_bsum_kernel.
-__global__ __launch_bounds__(256) void items_scan_kernel(const DesignDev* __restrict__ D,
-                                                         const PRates* __restrict__ R,
+// Sub-lane offsets of every particle's cells after the burn-in: one
+// workgroup per particle scans its cells' M = 2^msh into coff[p * (C + 1) + c]
+// (exclusive; coff[.. + C] is the total) and pitems[p].
+__global__ __launch_bounds__(256) void cell_items_kernel(int64_t n, int C,
+                                                         const CellStart* __restrict__ rec,
+                                                         int32_t* __restrict__ coff,
+                                                         int64_t* __restrict__ pitems) {
+  __shared__ int32_t s[256];
+  const int64_t p = blockIdx.x;
+  if (p >= n) return;
+  const int t = threadIdx.x;
+  const int per = (C + 255) / 256;
+  const int c0 = t * per, c1 = min(C, c0 + per);
+  const CellStart* rp = rec + (size_t)p * (size_t)C;
+  int32_t acc = 0;
+  for (int c = c0; c < c1; ++c) acc += 1 << rp[c].msh;
+  s[t] = acc;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const int32_t x = t >= d ? s[t - d] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  int32_t run = t ? s[t - 1] : 0;
+  int32_t* co = coff + (size_t)p * (size_t)(C + 1);
+  for (int c = c0; c < c1; ++c) {
+    co[c] = run;
+    run += 1 << rp[c].msh;
+  }
+  if (t == 255) {
+    co[C] = s[255];
+    pitems[p] = s[255];
+  }
+}
+
+// Work items of particle rank r = the sub-lanes of all its cells (pitems).
+// off[r + 1] = inclusive sum over ranks <= r, computed per block of
+// ABC_SCAN_BLOCK ranks here (block totals to bsum), then offset by
+// bsum_scan_kernel / add_bsum_kernel.
+__global__ __launch_bounds__(256) void items_scan_kernel(const int64_t* __restrict__ pitems,
                                                          const int32_t* __restrict__ order,
                                                          int64_t n, int64_t* __restrict__ off,
                                                          int64_t* __restrict__ bsum) {
@@ -256,7 +293,7 @@ __global__ __launch_bounds__(256) void items_scan_kernel(const DesignDev* __rest
   int64_t acc = 0;
   for (int q = 0; q < PER; ++q) {
     const int64_t r = base + q;
-    acc += r < n ? (int64_t)D->C * (int64_t)R[order[r]].M : 0;
+    acc += r < n ? pitems[order[r]] : 0;
     v[q] = acc;
   }
   s[t] = acc;
@@ -585,7 +622,23 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     r.bin = (uint8_t)cs0.bin;
     r.g = (uint8_t)g;
     r.gneed = (uint8_t)gneed;
-    r.pad = 0;
+    // sub-lanes of this cell: smallest power of two M with the cell's expected
+    // molecule events in its window <= split_target * M (M selects random
+    // streams: the CPU replay uses the same operations).  Events: the
+    // unlabelled molecules at the window start (at most 2 each) and births
+    // at the cell's own capture over its own window (3 events each).
+    {
+      int M = 1, msh = 0;
+      if (D->split_target > 0.0) {
+        const double w = D->pulse[cond] + D->chase[cond];
+        const double est = (lu + lt) + ((rp->brate * bc) * w) * 3.0;
+        while (M < D->split_max && est > D->split_target * (double)M) {
+          M *= 2;
+          msh++;
+        }
+      }
+      r.msh = (uint8_t)msh;
+    }
     rec[(size_t)p * (size_t)C + (size_t)c] = r;
     atomicAdd(steps_out + p, st);
     atomicAdd(react_out + p, re);
@@ -610,6 +663,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
                                                      const int64_t* __restrict__ off, int64_t n,
                                                      uint32_t first, uint32_t k0, uint32_t k1,
                                                      const CellStart* __restrict__ rec,
+                                                     const int32_t* __restrict__ coff,
                                                      unsigned long long* __restrict__ queue,
                                                      uint32_t* __restrict__ out_u,
                                                      uint32_t* __restrict__ out_l,
@@ -654,14 +708,20 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       }
       p = order[lo];
       const PRates* rp = R + p;
-      const int M = rp->M;
-      const int64_t rem = (int64_t)(q - (unsigned long long)off[lo]);
-      c = (int)(rem / M);
-      const int sub = (int)(rem - (int64_t)c * M);
+      // cell: largest c with coff[c] <= rem among the particle's C cells
+      const int32_t rem = (int32_t)(q - (unsigned long long)off[lo]);
+      const int32_t* co = coff + (size_t)p * (size_t)(C + 1);
+      int clo = 0, chi = C;
+      while (chi - clo > 1) {
+        const int mid = (clo + chi) >> 1;
+        if (co[mid] <= rem) clo = mid; else chi = mid;
+      }
+      c = clo;
+      const int sub = rem - co[c];
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       sstream = 1u + (uint32_t)sub;
-      share = (double)M;
       const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
+      share = (double)(1 << rc->msh);
       st = 0;
       re = 0;
       mctr = 0;

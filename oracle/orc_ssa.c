@@ -139,13 +139,9 @@ typedef struct {
   double bs;
   uint64_t thr_lam, thr_pi;
   int B, step, scaling, gene_varies;
-  int M; /* sub-lanes per cell: each carries a 1/M share of the births */
+  double brate; /* mean over the bins of alpha_max * P(gene on): the split rule */
 } prates;
 
-/* Split factor: the smallest power of two M with (expected molecule events
- * per cell) <= split_target * M, capped at split_max.  Molecules never act on
- * the gene, so M independent 1/M-thinned birth streams over one shared gene
- * path have the law of the full birth stream. */
 static double window_max(const orc_design* d) {
   double wmax = 0.0;
   for (int j = 0; j < d->n_conds; ++j) {
@@ -153,20 +149,6 @@ static double window_max(const orc_design* d) {
     if (w > wmax) wmax = w;
   }
   return wmax;
-}
-
-static int split_factor(const orc_design* d, const prates* r) {
-  if (!(d->split_target > 0.0)) return 1;
-  const int nb = d->n_bins;
-  double births = 0.0;
-  for (int j = 0; j < nb; ++j) births = births + r->amax[j] * (r->kon[j] / (r->kon[j] + r->koff[j]));
-  double scale = d->capture_mode == 1 ? (double)d->beta_a / (double)(d->beta_a + d->beta_b)
-                                      : (d->capture_mode == 2 ? 0.25 : 1.0);
-  /* molecule events are simulated over the labelling window only */
-  double est = (((births / (double)nb) * scale) * window_max(d)) * 3.0;
-  int M = 1;
-  while (M < d->split_max && est > d->split_target * (double)M) M *= 2;
-  return M;
 }
 
 static int burn_min(const orc_design* d) {
@@ -222,7 +204,11 @@ static void make_rates(const orc_design* d, int model, const double* th, prates*
   int B = bmin;
   while (S > d->burn_eps && B < d->burn_max) { S = S * s; B++; }
   r->B = B;
-  r->M = split_factor(d, r);
+  {
+    double births = 0.0;
+    for (int j = 0; j < nb; ++j) births = births + r->amax[j] * (r->kon[j] / (r->kon[j] + r->koff[j]));
+    r->brate = births / (double)nb;
+  }
 }
 
 /* --------------------------------------------------------------- cells */
@@ -360,6 +346,7 @@ typedef struct {
   int64_t k;         /* last grid edge passed */
   uint64_t gctr;     /* gene-stream counter */
   int bin, g, gneed;
+  int M;             /* sub-lanes of this cell */
 } cell_start;
 
 /* Burn-in of cell c (once per cell): set-up draws, then the gene path from
@@ -467,6 +454,22 @@ static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[2]
   }
   out->age = age; out->bc = bc; out->tg = tg; out->lu = lu; out->lt = lt;
   out->k = k; out->gctr = s0.ctr; out->bin = bin; out->g = g; out->gneed = gneed;
+  /* Sub-lanes of this cell: the smallest power of two M with the cell's
+   * expected molecule events in its window <= split_target * M (up to
+   * split_max): the unlabelled molecules at the window start (at most 2
+   * events each) and births at the cell's capture over its window (3 events
+   * each).  Molecules never act on the gene, so M independent 1/M-thinned
+   * birth streams and 1/M Poisson shares of the unlabelled molecules over
+   * one shared gene path have the law of the whole cell. */
+  {
+    int M = 1;
+    if (d->split_target > 0.0) {
+      const double w = d->pulse[cond] + d->chase[cond];
+      const double est = (lu + lt) + ((r->brate * bc) * w) * 3.0;
+      while (M < d->split_max && est > d->split_target * (double)M) M *= 2;
+    }
+    out->M = M;
+  }
   *steps += st;
   *reactions += re;
   return 0;
@@ -489,7 +492,7 @@ static int sim_window(const orc_design* d, const prates* r, const uint32_t key[2
   const double age = cst->age, bc = cst->bc;
   const double we = age - d->chase[cond];
   const int64_t nbB = (int64_t)r->B * nb;
-  const double inv_share = (double)r->M;
+  const double inv_share = (double)cst->M;
   int64_t st = 0, re = 0;
   uint32_t n0, n1, n2 = 0, n3 = 0;
   {
@@ -658,7 +661,7 @@ static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
   int e = burn_cell(d, r, key, pidx, c, &cst, steps, reactions);
   if (e) return e;
   uint32_t u = 0, l = 0;
-  for (int sub = 0; sub < r->M; ++sub) {
+  for (int sub = 0; sub < cst.M; ++sub) {
     uint32_t su, sl;
     e = sim_window(d, r, key, pidx, c, sub, &cst, &su, &sl, steps, reactions);
     if (e) return e;
