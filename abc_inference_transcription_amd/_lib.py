@@ -82,6 +82,7 @@ def _declare(L):
         "abc_sample_prior_dev": (I32, [V, I32, U64, I64, I64, V]),
         "abc_simulate_dev": (I32, [V, I32, U64, I64, I64, V, V, V, V, V]),
         "abc_errors_dev": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V]),
+        "abc_simulate_multi_dev": (I32, [V, I32, V, U64, V, V, V, V, V, V, V]),
         "abc_last_timing": (I32, [V, V]),
         "abc_ssa_time": (I32, [V, I32, V, V, V]),
         "abc_set_ssa_blocks": (I32, [V, I32]),

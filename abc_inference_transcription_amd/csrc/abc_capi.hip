@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -147,8 +147,8 @@ int check_range(abc_ctx* c, int model, int64_t first, int64_t n) {
 
 int grid1(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// rates + heaviest-first queue order + work-item offsets; theta_dev: n x n_par
-int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
+// buffers of the rates, the queue order and the item offsets of n particles
+int ensure_order(abc_ctx* c, int64_t n) {
   int rc;
   const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
   if ((rc = ensure(c, c->rates, sizeof(PRates) * n))) return rc;
@@ -157,12 +157,21 @@ int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
   if ((rc = ensure(c, c->order, sizeof(int32_t) * n))) return rc;
   if ((rc = ensure(c, c->bsum, sizeof(int64_t) * nblk))) return rc;
   if ((rc = ensure(c, c->off, sizeof(int64_t) * (n + 1)))) return rc;
+  return ABC_OK;
+}
+
+// rates of n particles of one model into rates/cost from index a on
+void launch_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_t a) {
+  hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
+                     theta_dev, P<PRates>(c->rates) + a, P<float>(c->cost) + a);
+}
+
+// heaviest-first queue order of all n particles (bucket sort of the cost)
+int launch_order(abc_ctx* c, int64_t n) {
   unsigned* hist = P<unsigned>(c->hist);
   unsigned* cursor = hist + ABC_NBUCKET;
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, c->stream,
                      P<unsigned long long>(c->hist), (int64_t)(ABC_NBUCKET / 2));
-  hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
-                     theta_dev, P<PRates>(c->rates), P<float>(c->cost));
   hipLaunchKernelGGL(bucket_hist_kernel, dim3(std::min(grid1(n, 256), 1024)), dim3(256), 0,
                      c->stream, P<float>(c->cost), n, hist);
   hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(ABC_NBUCKET), 0, c->stream, hist, cursor);
@@ -170,6 +179,14 @@ int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
                      P<float>(c->cost), n, cursor, P<int32_t>(c->order));
   HIPCHK(c, hipGetLastError());
   return ABC_OK;
+}
+
+// rates + heaviest-first queue order; theta_dev: n x n_par
+int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
+  int rc;
+  if ((rc = ensure_order(c, n))) return rc;
+  launch_rates(c, model, n, theta_dev, 0);
+  return launch_order(c, n);
 }
 
 // after the burn-in: each cell's sub-lanes -> per-particle cell offsets and
@@ -207,16 +224,14 @@ int drain_ring(abc_ctx* c) {
   return ABC_OK;
 }
 
-int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
-                 const double* theta_dev, double* stats_dev, double* moments_dev,
-                 double* superset_dev, int64_t* steps_dev, const int64_t* pidx_dev = nullptr) {
+// cells of n particles (rates and queue order staged): burn-in, sub-lane
+// offsets, the window SSA.  Particle p is global particle pidx[p] (first + p
+// without pidx) with key word k1s[p] (k1 without k1s).  Cell counts go to
+// cu / cl / cb, Philox draws per particle to steps.
+int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t k1,
+                  const uint32_t* k1s, const int64_t* pidx_dev, unsigned long long* steps) {
   int rc;
-  if ((rc = check_range(c, model, first, n))) return rc;
-  if (n == 0) return ABC_OK;
-  if (n > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
   const size_t cells = (size_t)n * (size_t)c->hd.C;
-  HIPCHK(c, record(c, c->ev[0]));
-  if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
   const int64_t cell_words = (int64_t)((cells + 1) / 2);  // u32 counts cleared as u64 words
   if ((rc = ensure(c, c->cu, sizeof(uint64_t) * cell_words))) return rc;
   if ((rc = ensure(c, c->cl, sizeof(uint64_t) * cell_words))) return rc;
@@ -235,11 +250,6 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
     if ((rc = ensure(c, c->tally, 16))) return rc;
     hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
                        P<unsigned long long>(c->tally), (int64_t)2);
-  }
-  unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
-  if (!steps) {
-    if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
-    steps = P<unsigned long long>(c->steps);
   }
   // counters cleared by a kernel (no memset nodes, so the sequence can be captured)
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, steps, n);
@@ -260,7 +270,7 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(burn_kernel, dim3(c->burn_blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
-                     key1(seed, model), c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
+                     k1, k1s, c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
                      P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev,
                      P<unsigned long long>(c->tally));
   HIPCHK(c, hipGetLastError());
@@ -268,12 +278,32 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
-                     (uint32_t)first, key0(seed), key1(seed, model), P<CellStart>(c->cellrec),
-                     P<int32_t>(c->coff), P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu), P<uint32_t>(c->cl),
-                     P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react), pidx_dev);
+                     (uint32_t)first, key0(seed), k1, k1s, P<CellStart>(c->cellrec),
+                     P<int32_t>(c->coff), P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu),
+                     P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react),
+                     pidx_dev);
   HIPCHK(c, hipGetLastError());
   if (trio) HIPCHK(c, hipEventRecord(trio[2], c->stream));
   HIPCHK(c, record(c, c->ev[2]));
+  return ABC_OK;
+}
+
+int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
+                 const double* theta_dev, double* stats_dev, double* moments_dev,
+                 double* superset_dev, int64_t* steps_dev, const int64_t* pidx_dev = nullptr) {
+  int rc;
+  if ((rc = check_range(c, model, first, n))) return rc;
+  if (n == 0) return ABC_OK;
+  if (n > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
+  HIPCHK(c, record(c, c->ev[0]));
+  if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
+  unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
+  if (!steps) {
+    if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
+    steps = P<unsigned long long>(c->steps);
+  }
+  if ((rc = simulate_core(c, seed, n, first, key1(seed, model), nullptr, pidx_dev, steps)))
+    return rc;
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
                      moments_dev, superset_dev);
@@ -436,7 +466,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems,
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -528,6 +558,62 @@ int32_t abc_simulate_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first
   if (!c || (n > 0 && (!theta_dev || !stats_dev))) return ABC_E_ARG;
   return simulate_dev(c, model, seed, first, n, theta_dev, stats_dev, moments_dev, superset_dev,
                       steps_dev);
+}
+
+int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* models, uint64_t seed,
+                               const int64_t* first, const int64_t* n,
+                               const double* const* theta_dev, double* const* stats_dev,
+                               double* const* moments_dev, double* const* superset_dev,
+                               int64_t* const* steps_dev) {
+  if (!c) return ABC_E_ARG;
+  if (n_models < 1 || n_models > 16 || !models || !first || !n || !theta_dev || !stats_dev)
+    return fail(c, ABC_E_ARG, "need 1..16 models with first, n, theta and stats");
+  int rc;
+  int64_t total = 0, a[17];
+  for (int m = 0; m < n_models; ++m) {
+    if ((rc = check_range(c, models[m], first[m], n[m]))) return rc;
+    if (n[m] > 0 && (!theta_dev[m] || !stats_dev[m]))
+      return fail(c, ABC_E_ARG, "theta and stats are required for every model with particles");
+    a[m] = total;
+    total += n[m];
+  }
+  a[n_models] = total;
+  if (total == 0) return ABC_OK;
+  if (total > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
+  HIPCHK(c, record(c, c->ev[0]));
+  if ((rc = ensure_order(c, total))) return rc;
+  if ((rc = ensure(c, c->mpidx, sizeof(int64_t) * total))) return rc;
+  if ((rc = ensure(c, c->mk1, sizeof(uint32_t) * total))) return rc;
+  if ((rc = ensure(c, c->steps, sizeof(uint64_t) * total))) return rc;
+  for (int m = 0; m < n_models; ++m) {
+    if (n[m] == 0) continue;
+    launch_rates(c, models[m], n[m], theta_dev[m], a[m]);
+    hipLaunchKernelGGL(fill_ids_kernel, dim3(grid1(n[m], 256)), dim3(256), 0, c->stream, n[m],
+                       first[m], key1(seed, models[m]), P<int64_t>(c->mpidx) + a[m],
+                       P<uint32_t>(c->mk1) + a[m]);
+  }
+  HIPCHK(c, hipGetLastError());
+  if ((rc = launch_order(c, total))) return rc;
+  unsigned long long* steps = P<unsigned long long>(c->steps);
+  if ((rc = simulate_core(c, seed, total, 0, 0u, P<uint32_t>(c->mk1), P<int64_t>(c->mpidx),
+                          steps)))
+    return rc;
+  const size_t C = (size_t)c->hd.C;
+  for (int m = 0; m < n_models; ++m) {
+    if (n[m] == 0) continue;
+    const size_t o = (size_t)a[m] * C;
+    hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n[m]), dim3(256), 0, c->stream, c->dd,
+                       P<uint32_t>(c->cu) + o, P<uint32_t>(c->cl) + o, P<uint8_t>(c->cb) + o, n[m],
+                       stats_dev[m], moments_dev ? moments_dev[m] : nullptr,
+                       superset_dev ? superset_dev[m] : nullptr);
+    if (steps_dev && steps_dev[m])
+      hipLaunchKernelGGL(copy_u64_kernel, dim3(grid1(n[m], 256)), dim3(256), 0, c->stream,
+                         steps + a[m], reinterpret_cast<unsigned long long*>(steps_dev[m]), n[m]);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, record(c, c->ev[3]));
+  c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
+  return ABC_OK;
 }
 
 int32_t abc_simulate_idx_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t n,

@@ -241,6 +241,25 @@ __global__ __launch_bounds__(256) void bucket_scatter_kernel(const float* __rest
   order[pos] = (int32_t)i;
 }
 
+// Identity of the particles of one model in a multi-model batch: global
+// particle index first + i and the model's key word.
+__global__ __launch_bounds__(256) void fill_ids_kernel(int64_t n, int64_t first, uint32_t k1,
+                                                       int64_t* __restrict__ pidx,
+                                                       uint32_t* __restrict__ k1s) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    pidx[i] = first + i;
+    k1s[i] = k1;
+  }
+}
+
+__global__ __launch_bounds__(256) void copy_u64_kernel(const unsigned long long* __restrict__ src,
+                                                       unsigned long long* __restrict__ dst,
+                                                       int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 // Sub-lane offsets of every particle's cells after the burn-in: one
 // workgroup per particle scans its cells' M = 2^msh into coff[p * (C + 1) + c]
 // (exclusive; coff[.. + C] is the total) and pitems[p].
@@ -518,7 +537,8 @@ __device__ __forceinline__ CellSetup cell_setup(const DesignDev* __restrict__ D,
 __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restrict__ D,
                                                       const PRates* __restrict__ R,
                                                       const int32_t* __restrict__ order, int64_t n,
-                                                      uint32_t first, uint32_t k0, uint32_t k1,
+                                                      uint32_t first, uint32_t k0, uint32_t k1_all,
+                                                      const uint32_t* __restrict__ k1s,
                                                       const double* __restrict__ betas,
                                                       const int64_t* __restrict__ goff,
                                                       unsigned long long* __restrict__ queue,
@@ -541,6 +561,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     const int c = (int)(q % (unsigned long long)C);
     const PRates* rp = R + p;
     const uint32_t pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
+    const uint32_t k1 = k1s ? k1s[p] : k1_all;  // the particle's model enters the key
     uint64_t gctr = 0;
     const CellSetup cs0 = cell_setup(D, rp, c, pid, k0, k1, betas, goff, gctr);
     const int cond = c % J;
@@ -661,7 +682,8 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
                                                      const int64_t* __restrict__ off, int64_t n,
-                                                     uint32_t first, uint32_t k0, uint32_t k1,
+                                                     uint32_t first, uint32_t k0, uint32_t k1_all,
+                                                     const uint32_t* __restrict__ k1s,
                                                      const CellStart* __restrict__ rec,
                                                      const int32_t* __restrict__ coff,
                                                      unsigned long long* __restrict__ queue,
@@ -678,6 +700,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
   load_log_table(ltab);
   __syncthreads();
 
+  uint32_t k1 = k1_all;  // key of the current item's particle (its model)
   // per-lane item state
   bool have = false;
   int p = 0, c = 0, j = 0, step = 1, scaling = 0, gvar = 0;
@@ -719,6 +742,7 @@ __global__ __launch_bounds__(SSA_WG) void ssa_kernel(const DesignDev* __restrict
       c = clo;
       const int sub = rem - co[c];
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
+      k1 = k1s ? k1s[p] : k1_all;
       sstream = 1u + (uint32_t)sub;
       const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
       share = (double)(1 << rc->msh);

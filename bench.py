@@ -136,14 +136,26 @@ def main():
             err=torch.empty(P * G, dtype=torch.float64, device=dev),
         )
     steps_acc = torch.zeros((), dtype=torch.int64, device=dev)
+    # all five models' particles go through one burn-in and one SSA launch per step
+    nm = len(models)
+    VA = C.c_void_p * nm
+    m_models = (C.c_int32 * nm)(*models)
+    m_n = (C.c_int64 * nm)(*([P] * nm))
+    m_theta = VA(*[bufs[m]["theta"].data_ptr() for m in models])
+    m_stats = VA(*[bufs[m]["stats"].data_ptr() for m in models])
+    m_sup = VA(*[bufs[m]["sup"].data_ptr() for m in models])
+    m_steps = VA(*[bufs[m]["steps"].data_ptr() for m in models])
 
     def step(it, timed):
         first = it * P
         for m in models:
+            _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, first, P, ptr(bufs[m]["theta"])),
+                       eng.ctx, "prior")
+        _lib.check(L.abc_simulate_multi_dev(eng.ctx, nm, m_models, 0, (C.c_int64 * nm)(*([first] * nm)),
+                                            m_n, m_theta, m_stats, None, m_sup, m_steps),
+                   eng.ctx, "simulate")
+        for m in models:
             b = bufs[m]
-            _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, first, P, ptr(b["theta"])), eng.ctx, "prior")
-            _lib.check(L.abc_simulate_dev(eng.ctx, m, 0, first, P, ptr(b["theta"]), ptr(b["stats"]),
-                                          None, ptr(b["sup"]), ptr(b["steps"])), eng.ctx, "simulate")
             _lib.check(L.abc_errors_dev(eng.ctx, P, KS, ptr(b["sup"]), G, ptr(data_d), ptr(se_d),
                                         parts_c.ctypes.data_as(C.c_void_p), 2, 10.0, ptr(b["err"])),
                        eng.ctx, "errors")

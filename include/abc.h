@@ -160,6 +160,20 @@ int32_t abc_sample_prior_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t
 int32_t abc_simulate_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first, int64_t n,
                          const double* theta_dev, double* stats_dev, double* moments_dev,
                          double* superset_dev, int64_t* steps_dev);
+
+/* Several models in one call (no reference counterpart; the bodies of several
+ * abc_simulation.jl jobs, one per model, at once): the particles of all
+ * models share one burn-in launch and one SSA launch, so the machine drains
+ * once instead of once per model.  Arrays of n_models (<= 16) entries: model
+ * index, first particle, particle count and the device pointers of each
+ * model's theta / stats / moments / superset / steps (moments, superset and
+ * steps may be NULL, as may the whole moments, superset or steps array).
+ * Results equal those of one abc_simulate_dev call per model, bit for bit. */
+int32_t abc_simulate_multi_dev(abc_ctx* ctx, int32_t n_models, const int32_t* models,
+                               uint64_t seed, const int64_t* first, const int64_t* n,
+                               const double* const* theta_dev, double* const* stats_dev,
+                               double* const* moments_dev, double* const* superset_dev,
+                               int64_t* const* steps_dev);
 /* Simulation of an arbitrary subset of particles: row i uses the random
  * streams of global particle pidx_dev[i] (device int64, < 2^32), so its
  * results equal those of the same particle in a contiguous run. */
