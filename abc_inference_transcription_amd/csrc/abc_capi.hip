@@ -369,6 +369,12 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
   h.burn_eps = d->burn_eps;
   h.splice_log10 = d->splice_log10;
   if (d->split_target < 0.0 || d->split_max < 1 || d->split_max > 4096) { delete c; return ABC_E_ARG; }
+  // a cell's sub-lanes reach at most 2 * split_max - 1 (powers of two up to
+  // the cap); a particle's item offsets are 32-bit
+  if ((int64_t)d->cells_per_particle * (2 * (int64_t)d->split_max) > 0x7FFFFFFFll) {
+    delete c;
+    return ABC_E_ARG;
+  }
   h.split_target = d->split_target;
   h.split_max = d->split_max;
   h.split_scale = d->capture_mode == 1 ? (double)d->beta_a / (double)(d->beta_a + d->beta_b)

@@ -65,7 +65,8 @@ typedef struct {
   double splice_log10;          /* splicing rate for prior_mode 0                   */
   double split_target;          /* expected molecule events per sub-lane (0 = one
                                    lane per cell), e.g. 32768                       */
-  int32_t split_max;            /* sub-lanes per cell cap, 1..4096, e.g. 1024       */
+  int32_t split_max;            /* sub-lanes per cell cap, 1..4096, e.g. 1024, with
+                                   cells_per_particle * 2 * split_max < 2^31       */
 } abc_design;
 
 /* Context: owns the HIP stream, device buffers and launch shapes.
