@@ -75,7 +75,8 @@ def main():
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py measures one GPU: multi-GPU runs are not part of this build "
-              "(particles shard by contiguous ranges with no communication; see DESIGN.md)",
+              "(particles would shard by contiguous ranges of the global particle index, with no "
+              "communication: each range's streams are keyed by that index)",
               file=sys.stderr)
         sys.exit(2)
     import torch
