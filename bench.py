@@ -74,7 +74,7 @@ def main():
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: multi-GPU runs are not part of this build "
+        print("bench.py measures one GPU; runs on several GPUs are not supported "
               "(particles would shard by contiguous ranges of the global particle index, with no "
               "communication: each range's streams are keyed by that index)",
               file=sys.stderr)
