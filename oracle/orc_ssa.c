@@ -801,7 +801,7 @@ void orc_cell_stats(const orc_design* d, const uint32_t* u, const uint32_t* l,
   }
 }
 
-/* ------------------------------------------------------------ driver */
+/* ----------------------------------------------------- batch entry point */
 int orc_simulate(const orc_design* d, int model, uint64_t seed, int64_t first, int64_t n,
                  const double* theta, int n_threads, uint32_t* cell_u, uint32_t* cell_l,
                  uint8_t* cell_bin, double* stats_ref, double* moments, double* superset,
