@@ -43,9 +43,9 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-default SSA grid")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
-    ap.add_argument("--sur-train", type=int, default=12000, help="training particles per model")
+    ap.add_argument("--sur-train", type=int, default=48000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
-    ap.add_argument("--sur-steps", type=int, default=3000, help="Adam steps (batch 8192)")
+    ap.add_argument("--sur-steps", type=int, default=6000, help="Adam steps (batch 8192)")
     ap.add_argument("--no-moments", action="store_true",
                     help="skip the leg of the reference's own algorithm (moment model)")
     ap.add_argument("--mom-particles", type=int, default=32768,
