@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -472,7 +472,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1,
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -1051,16 +1051,27 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   int rc;
   if ((rc = ensure(c, c->mss, sizeof(double) * 9 * (size_t)n))) return rc;
   if ((rc = ensure(c, c->mmap, sizeof(double) * 90 * (size_t)n))) return rc;
+  if ((rc = ensure(c, c->mcost, sizeof(unsigned long long) * (size_t)n))) return rc;
+  if ((rc = ensure(c, c->mord, sizeof(int32_t) * (size_t)n))) return rc;
   unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
+  unsigned long long* cost = P<unsigned long long>(c->mcost);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, cost, n);
   hipLaunchKernelGGL(moment_cyclemap_kernel, dim3(grid1(n * 10, 64)), dim3(64), 0, c->stream, m,
-                     model, n, theta_dev, ld, P<double>(c->mmap), steps);
+                     model, n, theta_dev, ld, P<double>(c->mmap), cost, steps);
+  HIPCHK(c, hipGetLastError());
+  // trajectories run in descending order of the particle's cycle-map steps,
+  // so that the 64 lanes of a wave integrate alike (results do not depend on it)
+  hipLaunchKernelGGL(moment_order_kernel, dim3(1), dim3(ABC_MOM_ORDER_WG), 0, c->stream, n, cost,
+                     P<int32_t>(c->mord));
   HIPCHK(c, hipGetLastError());
   hipLaunchKernelGGL(moment_transient_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,
                      P<double>(c->mmap), P<double>(c->mss));
   HIPCHK(c, hipGetLastError());
-  const int64_t items = n * (int64_t)m.n_conds * m.n_ages;
-  hipLaunchKernelGGL(moment_traj_kernel, dim3(grid1(items, 64)), dim3(64), 0, c->stream, m, model,
-                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), out_dev, steps);
+  // one wave per (group of 64 ordered particles, condition x age item)
+  const int64_t waves = (n + 63) / 64 * (int64_t)m.n_conds * m.n_ages;
+  hipLaunchKernelGGL(moment_traj_kernel, dim3((unsigned)waves), dim3(64), 0, c->stream, m, model,
+                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), P<int32_t>(c->mord),
+                     out_dev, steps);
   HIPCHK(c, hipGetLastError());
   if (stats_dev) {
     hipLaunchKernelGGL(moment_stats_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,

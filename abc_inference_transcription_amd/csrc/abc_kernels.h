@@ -127,12 +127,17 @@ struct MomCfgDev {
 };
 __global__ void moment_cyclemap_kernel(MomCfgDev cfg, int model, int64_t n,
                                        const double* theta, int64_t ld, double* maps,
-                                       unsigned long long* steps);
+                                       unsigned long long* cost, unsigned long long* steps);
+// particle order for the trajectory kernel: descending cycle-map cost
+#define ABC_MOM_ORDER_WG 1024
+#define ABC_MOM_COST_BUCKETS 1024
+__global__ void moment_order_kernel(int64_t n, const unsigned long long* cost, int32_t* order);
 __global__ void moment_transient_kernel(MomCfgDev cfg, int64_t n, const double* maps,
                                         double* ss);
 __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    const double* theta, int64_t ld, const double* ss,
-                                   const double* maps, double* out, unsigned long long* steps);
+                                   const double* maps, const int32_t* order, double* out,
+                                   unsigned long long* steps);
 __global__ void moment_stats_kernel(MomCfgDev cfg, int64_t n, double* mom,
                                     double* stats);
 // Model probabilities with bootstrap bounds (scripts/model_probs.jl)

@@ -297,7 +297,8 @@ ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* thet
 // else, so a particle's longest chain is one cycle whatever its iterations.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_cyclemap_kernel(
     const MomCfgDev cfg, int model, int64_t n, const double* __restrict__ theta, int64_t ld,
-    double* __restrict__ maps, unsigned long long* __restrict__ steps) {
+    double* __restrict__ maps, unsigned long long* __restrict__ cost,
+    unsigned long long* __restrict__ steps) {
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
   __syncthreads();
@@ -311,7 +312,48 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
   const int64_t st = cycle_map(P, cfg, in, out, ltab);
   double* o = maps + q * 9;
   for (int i = 0; i < 9; ++i) o[i] = out[i];
+  atomicAdd(cost + p, (unsigned long long)st);
   if (steps) atomicAdd(steps + p, (unsigned long long)st);
+}
+
+// Counting sort of the particles by descending cycle-map steps, on a log
+// scale of 16 buckets per octave (bucket = 16 * floor(log2 c) + the next 4
+// bits of c), in one workgroup.  A particle's trajectories integrate the
+// same cycles as its neighbours' in this order, at a step size set by the
+// same rates, so a wave of 64 of them finishes nearly together.  Order
+// within a bucket follows LDS atomics and is not fixed; each trajectory's
+// result is independent of it.
+__global__ __launch_bounds__(ABC_MOM_ORDER_WG) void moment_order_kernel(
+    int64_t n, const unsigned long long* __restrict__ cost, int32_t* __restrict__ order) {
+  __shared__ unsigned int hist[ABC_MOM_COST_BUCKETS];
+  __shared__ unsigned int scan[ABC_MOM_COST_BUCKETS];
+  static_assert(ABC_MOM_COST_BUCKETS == ABC_MOM_ORDER_WG, "one bucket per lane");
+  const int tid = threadIdx.x;
+  auto bucket = [](unsigned long long c) -> int {
+    if (c < 16ull) return (int)c;
+    const int e = 63 - __clzll(c);  // >= 4
+    return 16 * (e - 3) + (int)((c >> (e - 4)) & 15ull);
+  };
+  hist[tid] = 0u;
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += ABC_MOM_ORDER_WG) atomicAdd(&hist[bucket(cost[p])], 1u);
+  __syncthreads();
+  // descending: lane t holds bucket B - 1 - t; inclusive Hillis-Steele scan
+  const unsigned int own = hist[ABC_MOM_COST_BUCKETS - 1 - tid];
+  scan[tid] = own;
+  __syncthreads();
+  for (int d = 1; d < ABC_MOM_ORDER_WG; d <<= 1) {
+    const unsigned int add = tid >= d ? scan[tid - d] : 0u;
+    __syncthreads();
+    scan[tid] += add;
+    __syncthreads();
+  }
+  hist[ABC_MOM_COST_BUCKETS - 1 - tid] = scan[tid] - own;  // first slot of the bucket
+  __syncthreads();
+  for (int64_t p = tid; p < n; p += ABC_MOM_ORDER_WG) {
+    const unsigned int at = atomicAdd(&hist[bucket(cost[p])], 1u);
+    order[at] = (int32_t)p;
+  }
 }
 
 // Periodic state after division, one lane per particle, from (M, v).  Mode 0
@@ -400,8 +442,11 @@ __global__ __launch_bounds__(64) void moment_transient_kernel(const MomCfgDev cf
   for (int i = 0; i < 9; ++i) ss[p + (int64_t)i * n] = out[i];
 }
 
-// trajectories (model.jl:146-174) for item q = (particle, condition, age):
-// out column (cond * n_ages + age) * 5 + {mean_u, mean_l, var_u, cov_ul, var_l}.
+// trajectories (model.jl:146-174) for item (particle, condition, age): out
+// column (cond * n_ages + age) * 5 + {mean_u, mean_l, var_u, cov_ul, var_l}.
+// Wave w takes one (condition, age) item for the 64 particles
+// order[64 g .. 64 g + 63], g = w / items, so its lanes share the item's
+// cycle structure and have similar step counts.
 // A whole cycle that starts on a cycle boundary and does not meet the
 // labelling window has the rates of [0, T] shifted by whole cycles and no
 // labelling, so it is the particle's affine cycle map, applied from maps.
@@ -411,16 +456,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
                                                           int64_t ld,
                                                           const double* __restrict__ ss,
                                                           const double* __restrict__ maps,
+                                                          const int32_t* __restrict__ order,
                                                           double* __restrict__ out,
                                                           unsigned long long* __restrict__ steps) {
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
   __syncthreads();
   const int per = cfg.n_conds * cfg.n_ages;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * per) return;
-  const int64_t p = q / per;
-  const int r = (int)(q - p * per);
+  const int64_t w = blockIdx.x;  // 64 lanes per workgroup
+  const int64_t g = w / per;
+  const int r = (int)(w - g * per);
+  const int64_t i = g * 64 + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = order[i];
   const int j = r / cfg.n_ages, ai = r - j * cfg.n_ages;
   const Particle P = particle_of(model, cfg, theta, ld, p);
   const double T = cfg.cycle, t0 = cfg.t0, age = cfg.ages[ai];

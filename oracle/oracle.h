@@ -122,6 +122,9 @@ void orc_recover(int model, const double* theta, const orc_moment_cfg* c, double
 /* the same, integrating every cycle of every trajectory (checks the affine
  * shortcut for whole unlabelled cycles) */
 void orc_recover_direct(int model, const double* theta, const orc_moment_cfg* c, double* out);
+/* orc_recover plus the integration steps of each trajectory [cond * n_ages + age] */
+void orc_recover_steps(int model, const double* theta, const orc_moment_cfg* c, double* out,
+                       int64_t* traj_steps);
 /* downsample (model.jl:221-239): data n_ages x 5 row-major in place. */
 void orc_downsample(double* data, int n_ages, const double* mean_b, const double* m2_b,
                     const double* var_b);

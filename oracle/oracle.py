@@ -80,6 +80,7 @@ def _declare(L):
     L.orc_transient_direct.restype = C.c_int
     L.orc_recover.argtypes = [C.c_int, P, P, P]
     L.orc_recover_direct.argtypes = [C.c_int, P, P, P]
+    L.orc_recover_steps.argtypes = [C.c_int, P, P, P, P]
     L.orc_downsample.argtypes = [P, C.c_int, P, P, P]
     L.orc_moment_stats.argtypes = [P, C.c_int, C.c_int, P, C.c_int, C.c_int, P]
     L.orc_error_pair.argtypes = [P, P, P, C.c_int, P, C.c_int, D]

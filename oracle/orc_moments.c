@@ -355,12 +355,13 @@ int orc_transient_direct(int model, const double* th, const orc_moment_cfg* c, d
  * boundary and does not meet the labelling window [texp, texp + pulse] has
  * the rates of [0, T] shifted by whole cycles and no labelling, so it is the
  * affine cycle map (M, v) itself; with M = NULL every cycle is integrated. */
-static void trajectory(int model, const double* th, const orc_moment_cfg* c, const double* iv,
-                       double M[9][9], const double* v, double age, double pulse, double texp,
-                       double* out5) {
+static int64_t trajectory(int model, const double* th, const orc_moment_cfg* c, const double* iv,
+                          double M[9][9], const double* v, double age, double pulse, double texp,
+                          double* out5) {
   double T = c->cycle, t0 = c->t0, we = texp + pulse;
   int nsols = (age > 0 && age < T) ? (int)floor((age - t0) / T) + 1 : (int)floor((age - t0) / T);
   double tau = t0, y[9], tmp[9];
+  int64_t st = 0;
   memcpy(y, iv, sizeof y);
   for (int k = 1; k <= nsols; ++k) {
     double tf = tau < 0 ? tau + T : age;
@@ -369,15 +370,16 @@ static void trajectory(int model, const double* th, const orc_moment_cfg* c, con
       affine_apply(M, v, y, tmp);
       memcpy(y, tmp, sizeof y);
     } else {
-      orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol);
+      st += orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol);
     }
     tau = tf;
   }
   out5[0] = y[1]; out5[1] = y[2]; out5[2] = y[6]; out5[3] = y[7]; out5[4] = y[8];
+  return st;
 }
 
 static void recover(int model, const double* th, const orc_moment_cfg* c, double* out,
-                    int direct) {
+                    int direct, int64_t* traj_steps) {
   double M[9][9], v[9], ss[9];
   cycle_affine(model, th, c, M, v);
   transient_from(M, v, c, ss);
@@ -385,18 +387,26 @@ static void recover(int model, const double* th, const orc_moment_cfg* c, double
     for (int a = 0; a < c->n_ages; ++a) {
       double age = c->ages[a];
       double texp = age - c->pulse[j] - c->chase[j];
-      trajectory(model, th, c, ss, direct ? NULL : M, v, age, c->pulse[j], texp,
-                 out + (j * c->n_ages + a) * 5);
+      const int64_t st = trajectory(model, th, c, ss, direct ? NULL : M, v, age, c->pulse[j],
+                                    texp, out + (j * c->n_ages + a) * 5);
+      if (traj_steps) traj_steps[j * c->n_ages + a] = st;
     }
 }
 
 void orc_recover(int model, const double* th, const orc_moment_cfg* c, double* out) {
-  recover(model, th, c, out, 0);
+  recover(model, th, c, out, 0, NULL);
+}
+
+/* Test hook: orc_recover, also returning the integration steps of each
+ * trajectory, [cond * n_ages + age]. */
+void orc_recover_steps(int model, const double* th, const orc_moment_cfg* c, double* out,
+                       int64_t* traj_steps) {
+  recover(model, th, c, out, 0, traj_steps);
 }
 
 /* Test hook: orc_recover integrating every cycle of every trajectory. */
 void orc_recover_direct(int model, const double* th, const orc_moment_cfg* c, double* out) {
-  recover(model, th, c, out, 1);
+  recover(model, th, c, out, 1, NULL);
 }
 
 void orc_downsample(double* s, int n_ages, const double* mb, const double* m2b,
