@@ -1051,18 +1051,28 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   int rc;
   if ((rc = ensure(c, c->mss, sizeof(double) * 9 * (size_t)n))) return rc;
   if ((rc = ensure(c, c->mmap, sizeof(double) * 90 * (size_t)n))) return rc;
-  if ((rc = ensure(c, c->mcost, sizeof(unsigned long long) * (size_t)n))) return rc;
-  if ((rc = ensure(c, c->mord, sizeof(int32_t) * (size_t)n))) return rc;
+  if ((rc = ensure(c, c->mcost, sizeof(unsigned long long) * 2 * (size_t)n))) return rc;
+  if ((rc = ensure(c, c->mord, sizeof(int32_t) * 2 * (size_t)n))) return rc;
   unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
-  unsigned long long* cost = P<unsigned long long>(c->mcost);
+  unsigned long long* key = P<unsigned long long>(c->mcost);
+  unsigned long long* cost = key + n;
+  int32_t* ord_map = P<int32_t>(c->mord);
+  int32_t* ord = ord_map + n;
+  // Lanes of a wave run in lockstep, so each kernel takes the particles in an
+  // order of descending cost (the results do not depend on it): the cycle
+  // maps by the cost predicted from the rates, the trajectories by the steps
+  // the cycle maps took.
+  hipLaunchKernelGGL(moment_key_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, m, model, n,
+                     theta_dev, ld, key);
+  hipLaunchKernelGGL(moment_order_kernel, dim3(1), dim3(ABC_MOM_ORDER_WG), 0, c->stream, n, key,
+                     ord_map);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, cost, n);
-  hipLaunchKernelGGL(moment_cyclemap_kernel, dim3(grid1(n * 10, 64)), dim3(64), 0, c->stream, m,
-                     model, n, theta_dev, ld, P<double>(c->mmap), cost, steps);
+  hipLaunchKernelGGL(moment_cyclemap_kernel, dim3((unsigned)((n + 63) / 64 * 10)), dim3(64), 0,
+                     c->stream, m, model, n, theta_dev, ld, ord_map, P<double>(c->mmap), cost,
+                     steps);
   HIPCHK(c, hipGetLastError());
-  // trajectories run in descending order of the particle's cycle-map steps,
-  // so that the 64 lanes of a wave integrate alike (results do not depend on it)
   hipLaunchKernelGGL(moment_order_kernel, dim3(1), dim3(ABC_MOM_ORDER_WG), 0, c->stream, n, cost,
-                     P<int32_t>(c->mord));
+                     ord);
   HIPCHK(c, hipGetLastError());
   hipLaunchKernelGGL(moment_transient_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,
                      P<double>(c->mmap), P<double>(c->mss));
@@ -1070,8 +1080,7 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   // one wave per (group of 64 ordered particles, condition x age item)
   const int64_t waves = (n + 63) / 64 * (int64_t)m.n_conds * m.n_ages;
   hipLaunchKernelGGL(moment_traj_kernel, dim3((unsigned)waves), dim3(64), 0, c->stream, m, model,
-                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), P<int32_t>(c->mord),
-                     out_dev, steps);
+                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), ord, out_dev, steps);
   HIPCHK(c, hipGetLastError());
   if (stats_dev) {
     hipLaunchKernelGGL(moment_stats_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,

@@ -126,9 +126,12 @@ struct MomCfgDev {
   double weights[ABC_MOM_MAX_AGES * ABC_MOM_MAX_CONDS];  // age weights, n_ages x n_conds
 };
 __global__ void moment_cyclemap_kernel(MomCfgDev cfg, int model, int64_t n,
-                                       const double* theta, int64_t ld, double* maps,
-                                       unsigned long long* cost, unsigned long long* steps);
-// particle order for the trajectory kernel: descending cycle-map cost
+                                       const double* theta, int64_t ld, const int32_t* order,
+                                       double* maps, unsigned long long* cost,
+                                       unsigned long long* steps);
+__global__ void moment_key_kernel(MomCfgDev cfg, int model, int64_t n, const double* theta,
+                                  int64_t ld, unsigned long long* cost);
+// particle order for the cycle-map and trajectory kernels: descending cost
 #define ABC_MOM_ORDER_WG 1024
 #define ABC_MOM_COST_BUCKETS 1024
 __global__ void moment_order_kernel(int64_t n, const unsigned long long* cost, int32_t* order);

@@ -291,32 +291,56 @@ ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* thet
 
 // The cycle map is affine, y -> M y + v (the moment ODE is linear): v is the
 // map of the zero state and column i of M the map of unit vector i minus v.
-// moment_cyclemap_kernel integrates the 10 maps of every particle, one lane
-// each (item p * 10 + col, col 0 = zero state, col 1 + i = unit vector i),
-// into maps[(p * 10 + col) * 9 + r]; the transient rules below need nothing
-// else, so a particle's longest chain is one cycle whatever its iterations.
+// Column c of particle p (c = 0 the zero state, c = 1 + i unit vector i) goes
+// to maps[(p * 10 + c) * 9 + r]; the transient rules below need nothing else,
+// so a particle's longest chain is one cycle whatever its iterations.
+// Wave w integrates column w % 10 for the 64 particles order[64 (w / 10) ..]
+// (an order by predicted cost) and adds each integration's steps to cost[p].
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_cyclemap_kernel(
     const MomCfgDev cfg, int model, int64_t n, const double* __restrict__ theta, int64_t ld,
-    double* __restrict__ maps, unsigned long long* __restrict__ cost,
-    unsigned long long* __restrict__ steps) {
+    const int32_t* __restrict__ order, double* __restrict__ maps,
+    unsigned long long* __restrict__ cost, unsigned long long* __restrict__ steps) {
   __shared__ LogEntry ltab[ABC_LOG_N];
   load_log_table(ltab);
   __syncthreads();
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n * 10) return;
-  const int64_t p = q / 10;
-  const int col = (int)(q - p * 10);
+  const int64_t w = blockIdx.x;  // 64 lanes per workgroup
+  const int64_t g = w / 10;
+  const int col = (int)(w - g * 10);
+  const int64_t i = g * 64 + threadIdx.x;
+  if (i >= n) return;
+  const int64_t p = order[i];
   const Particle P = particle_of(model, cfg, theta, ld, p);
   double in[9], out[9];
-  for (int i = 0; i < 9; ++i) in[i] = (col > 0 && i == col - 1) ? 1.0 : 0.0;
+  for (int k = 0; k < 9; ++k) in[k] = (col > 0 && k == col - 1) ? 1.0 : 0.0;
   const int64_t st = cycle_map(P, cfg, in, out, ltab);
-  double* o = maps + q * 9;
-  for (int i = 0; i < 9; ++i) o[i] = out[i];
+  double* o = maps + (p * 10 + col) * 9;
+  for (int k = 0; k < 9; ++k) o[k] = out[k];
   atomicAdd(cost + p, (unsigned long long)st);
   if (steps) atomicAdd(steps + p, (unsigned long long)st);
 }
 
-// Counting sort of the particles by descending cycle-map steps, on a log
+// Predicted cycle-map cost of particle p, written to cost[p]: the moment
+// ODE's fastest rate, 2 max(kon + koff, gamma), summed over the rate bins of
+// the cycle (the explicit integrator's step is held near a fixed multiple of
+// its inverse), in units of 1/1024.
+__global__ __launch_bounds__(256) void moment_key_kernel(const MomCfgDev cfg, int model, int64_t n,
+                                                         const double* __restrict__ theta,
+                                                         int64_t ld,
+                                                         unsigned long long* __restrict__ cost) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const Particle P = particle_of(model, cfg, theta, ld, p);
+  double key = 0.0;
+  for (int b = 0; b < cfg.n_steps; ++b) {
+    double r[4];
+    for (int k = 0; k < 4; ++k) r[k] = ex10(theta_at(P, P.v.o[k] + (P.v.cnt[k] > 1 ? b : 0)));
+    key += 2.0 * fmax(r[0] + r[1], r[3]);
+  }
+  key *= 1024.0;
+  cost[p] = key < 1.8e19 ? (unsigned long long)key : 18000000000000000000ull;
+}
+
+// Counting sort of the particles by descending cost, on a log
 // scale of 16 buckets per octave (bucket = 16 * floor(log2 c) + the next 4
 // bits of c), in one workgroup.  A particle's trajectories integrate the
 // same cycles as its neighbours' in this order, at a step size set by the
