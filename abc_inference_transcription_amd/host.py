@@ -178,20 +178,42 @@ def load_s_data(path: str, model_name: str, ext: str):
 
 def compute_errors(eng, model_name: str, summary: dict, path: str = "data/simulations/",
                    ext: str = ".txt", out: str | None = None, batch: int = 65536,
-                   clamp: float = 10.0):
+                   clamp: float = 10.0, columns: str | None = None, text: bool = True):
     """scripts/compute_errors.jl:45-78: errors of every simulated particle
     against every gene, clamped at ``clamp``, one row per particle appended to
     ``out`` (default ``data/errors/error_<model>.txt``; the reference writes to
-    a fixed cluster path).  Returns the P x G matrix."""
+    a fixed cluster path).
+
+    ``columns`` also writes the matrix as a column-major FP64 ``.npy`` file
+    (P x G, Fortran order), filled batch by batch: gene g's errors are one
+    contiguous column, as in the per-gene columns ``x<g>`` of the JDF file
+    that scripts/process_error_files.jl makes from the text file and
+    accepted_particles.jl:1-17 reads.  ``text=False`` skips the text file,
+    whose size (about 20 bytes per pair) rules it out at a million particles.
+    Returns the P x G matrix, or the memory map of ``columns`` when the text
+    file is skipped."""
     stats = join_stats(load_s_data(path, model_name, ext))
     data, se = data_matrices(summary)
     out = out or os.path.join("data", "errors", f"error_{model_name}.txt")
+    P, G = stats.shape[0], data.shape[0]
+    cols = None
+    if columns:
+        os.makedirs(os.path.dirname(os.path.abspath(columns)), exist_ok=True)
+        cols = np.lib.format.open_memmap(columns, mode="w+", dtype=np.float64, shape=(P, G),
+                                         fortran_order=True)
     mats = []
-    for i0 in range(0, stats.shape[0], batch):
+    for i0 in range(0, P, batch):
         err = eng.errors(stats[i0:i0 + batch], data, se, clamp=clamp)
-        _append(out, _rows_text(err))
-        mats.append(err)
-    return np.vstack(mats) if mats else np.zeros((0, data.shape[0]))
+        if text:
+            _append(out, _rows_text(err))
+            mats.append(err)
+        if cols is not None:
+            cols[i0:i0 + err.shape[0]] = err
+    if cols is not None:
+        cols.flush()
+        if not text:
+            return cols
+    return np.vstack(mats) if mats else np.zeros((0, G))
 
 
 # ------------------------------------------------------ accepted_particles
@@ -217,6 +239,39 @@ def accepted_particles(eng, summary: dict, stats_by_model: dict, eps: float = 4.
     for name in res:
         _append(os.path.join(out_dir, f"particles_{name}.txt"), "".join(lines[name]))
     return res
+
+
+def accepted_from_columns(columns_by_model: dict, eps: float = 4.8,
+                          out_dir: str = "data/posteriors"):
+    """scripts/accepted_particles.jl:1-32 on error matrices stored by
+    ``compute_errors(columns=...)``: for each gene g and each model, the
+    1-based indices of the particles whose error in column g is <= eps,
+    ascending by error with ties by index (Julia's sortperm is stable),
+    appended as one line to ``particles_<model>.txt``; ``0`` when none
+    passes.  Each column is read from a memory map, so one gene's errors of
+    every model are in memory at a time.  Returns {model: number of accepted
+    particles per gene}."""
+    mm = {name: np.load(p, mmap_mode="r", allow_pickle=False)
+          for name, p in columns_by_model.items()}
+    G = {m.shape[1] for m in mm.values()}
+    if len(G) != 1:
+        raise ValueError("error matrices disagree on the number of genes")
+    G = G.pop()
+    lines = {name: [] for name in mm}
+    counts = {name: np.zeros(G, np.int64) for name in mm}
+    for g in range(G):
+        for name, m in mm.items():
+            col = np.asarray(m[:, g])
+            v = np.nonzero(col <= eps)[0]
+            counts[name][g] = len(v)
+            if len(v):
+                idx = v[np.argsort(col[v], kind="stable")] + 1
+                lines[name].append("\t".join(map(str, idx.tolist())) + "\n")
+            else:
+                lines[name].append("0\n")
+    for name in mm:
+        _append(os.path.join(out_dir, f"particles_{name}.txt"), "".join(lines[name]))
+    return counts
 
 
 # ------------------------------------------------------------ model selection

@@ -1,5 +1,6 @@
 """The host mirror driven by libabc.so writes the same bytes as when driven by
-the CPU oracle: statistics files, error matrix and accepted-particle lists."""
+the CPU oracle: statistics files, error matrix (text and gene-major columns)
+and accepted-particle lists."""
 import os
 
 import numpy as np
@@ -34,10 +35,11 @@ def test_host_files_match_oracle_bytes(oracle, engine_factory, golden_summary, t
             _, st = host.abc_simulation(eng, m, 7, 2, root=os.path.join(root, "sim"), batch=3)
             stats[tag][MODEL_NAMES[m - 1]] = st
         host.compute_errors(eng, "kon", S, path=os.path.join(root, "sim") + "/", ext="_2.txt",
-                            out=os.path.join(root, "errors", "error_kon.txt"))
+                            out=os.path.join(root, "errors", "error_kon.txt"),
+                            columns=os.path.join(root, "errors", "error_kon.npy"))
         host.accepted_particles(eng, S, stats[tag], eps=4.8, out_dir=os.path.join(root, "post"))
     a, b = _tree(str(tmp_path / "gpu")), _tree(str(tmp_path / "cpu"))
     assert sorted(a) == sorted(b)
     for k in a:
         assert a[k] == b[k], k
-    assert len(a) == 5 * 7 + 1 + 5
+    assert len(a) == 5 * 7 + 2 + 5

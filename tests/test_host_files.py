@@ -101,3 +101,51 @@ def test_errors_and_acceptance_files(oracle, small_design, golden_summary, tmp_p
         for g, line in enumerate(lines):
             want = [i + 1 for i in sorted(np.nonzero(e[:, g] <= eps)[0], key=lambda i: (e[i, g], i))]
             assert line == ("\t".join(map(str, want)) if want else "0")
+
+
+def test_columnar_error_files_and_acceptance(oracle, small_design, golden_summary, tmp_path):
+    """compute_errors(columns=...) stores the matrix gene-major (one contiguous
+    column per gene, the JDF layout accepted_particles.jl reads), filled over
+    ragged particle batches; acceptance from those columns writes the same
+    particles_<model>.txt bytes as acceptance in memory."""
+    from oracle_engine import OracleEngine
+    eng = OracleEngine(oracle, small_design)
+    root = str(tmp_path / "sim")
+    stats = {}
+    for m, name in [(1, "const"), (3, "kon")]:
+        _, st = host.abc_simulation(eng, m, 7, 1, root=root)
+        stats[name] = st
+    S = {k: golden_summary[k] for k in host.SUMMARY_FILES}
+    data, se = host.data_matrices(S)
+    cols = {}
+    for name in stats:
+        cols[name] = str(tmp_path / "errors" / f"error_{name}.npy")
+        err = host.compute_errors(eng, name, S, path=root + "/", ext="_1.txt", batch=3,
+                                  out=str(tmp_path / "errors" / f"error_{name}.txt"),
+                                  columns=cols[name])
+        np.testing.assert_array_equal(err, oracle.errors(stats[name], data, se, eng.parts))
+        back = np.load(cols[name], allow_pickle=False)
+        assert back.flags["F_CONTIGUOUS"] and back.shape == err.shape
+        np.testing.assert_array_equal(back, err)
+        # columns only: no text file, the memory map is returned
+        only = str(tmp_path / "errors" / f"only_{name}.npy")
+        mm = host.compute_errors(eng, name, S, path=root + "/", ext="_1.txt", batch=5,
+                                 out=str(tmp_path / "errors" / "unused.txt"), columns=only,
+                                 text=False)
+        np.testing.assert_array_equal(np.asarray(mm), err)
+    assert not os.path.exists(tmp_path / "errors" / "unused.txt")
+    all_err = np.concatenate([np.load(p) for p in cols.values()])
+    eps = float(np.quantile(all_err, 0.3))
+    counts = host.accepted_from_columns(cols, eps=eps, out_dir=str(tmp_path / "from_cols"))
+    host.accepted_particles(eng, S, stats, eps=eps, out_dir=str(tmp_path / "in_mem"))
+    for name in stats:
+        a = open(tmp_path / "from_cols" / f"particles_{name}.txt", "rb").read()
+        b = open(tmp_path / "in_mem" / f"particles_{name}.txt", "rb").read()
+        assert a == b
+        e = np.load(cols[name])
+        np.testing.assert_array_equal(counts[name], (e <= eps).sum(axis=0))
+    bad = str(tmp_path / "errors" / "bad.npy")
+    np.save(bad, np.zeros((3, data.shape[0] + 1)))
+    with pytest.raises(ValueError):
+        host.accepted_from_columns({"const": cols["const"], "bad": bad}, eps=eps,
+                                   out_dir=str(tmp_path / "x"))
