@@ -35,7 +35,7 @@ class AbcMomentCfg(C.Structure):
         ("ages", C.c_void_p), ("n_conds", C.c_int32), ("pulse", C.c_void_p),
         ("chase", C.c_void_p), ("is_chase", C.c_void_p), ("t0", C.c_double),
         ("iv", C.c_double * 9), ("transient_mode", C.c_int32), ("rtol", C.c_double),
-        ("atol", C.c_double),
+        ("atol", C.c_double), ("method", C.c_int32),
     ]
 
 

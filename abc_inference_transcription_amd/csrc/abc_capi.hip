@@ -1018,6 +1018,7 @@ int moment_cfg(abc_ctx* c, int model, const abc_moment_cfg* u, MomCfgDev* m) {
   if (!(u->rtol > 0.0) || !(u->atol > 0.0)) return fail(c, ABC_E_ARG, "rtol and atol must be > 0");
   if (u->transient_mode != 0 && u->transient_mode != 1)
     return fail(c, ABC_E_ARG, "transient_mode must be 0 or 1");
+  if (u->method != 0 && u->method != 1) return fail(c, ABC_E_ARG, "method must be 0 or 1");
   if (!std::isfinite(u->t0) || u->t0 > 0.0) return fail(c, ABC_E_ARG, "t0 must be finite and <= 0");
   std::memset(m, 0, sizeof(*m));
   m->cycle = u->cycle;
@@ -1029,6 +1030,7 @@ int moment_cfg(abc_ctx* c, int model, const abc_moment_cfg* u, MomCfgDev* m) {
   for (int i = 0; i < 9; ++i) m->iv[i] = u->iv[i];
   m->rtol = u->rtol;
   m->atol = u->atol;
+  m->method = u->method;
   for (int a = 0; a < u->n_ages; ++a) {
     if (!std::isfinite(u->ages[a]) || u->ages[a] < 0.0)
       return fail(c, ABC_E_ARG, "ages must be finite and >= 0");
@@ -1067,9 +1069,14 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   hipLaunchKernelGGL(moment_order_kernel, dim3(1), dim3(ABC_MOM_ORDER_WG), 0, c->stream, n, key,
                      ord_map);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, cost, n);
-  hipLaunchKernelGGL(moment_cyclemap_kernel, dim3((unsigned)((n + 63) / 64 * 10)), dim3(64), 0,
-                     c->stream, m, model, n, theta_dev, ld, ord_map, P<double>(c->mmap), cost,
-                     steps);
+  if (m.method == 1)
+    hipLaunchKernelGGL(moment_cyclemap_kernel<1>, dim3((unsigned)((n + 63) / 64 * 10)), dim3(64),
+                       0, c->stream, m, model, n, theta_dev, ld, ord_map, P<double>(c->mmap), cost,
+                       steps);
+  else
+    hipLaunchKernelGGL(moment_cyclemap_kernel<0>, dim3((unsigned)((n + 63) / 64 * 10)), dim3(64),
+                       0, c->stream, m, model, n, theta_dev, ld, ord_map, P<double>(c->mmap), cost,
+                       steps);
   HIPCHK(c, hipGetLastError());
   hipLaunchKernelGGL(moment_order_kernel, dim3(1), dim3(ABC_MOM_ORDER_WG), 0, c->stream, n, cost,
                      ord);
@@ -1079,8 +1086,14 @@ int moment_run(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_
   HIPCHK(c, hipGetLastError());
   // one wave per (group of 64 ordered particles, condition x age item)
   const int64_t waves = (n + 63) / 64 * (int64_t)m.n_conds * m.n_ages;
-  hipLaunchKernelGGL(moment_traj_kernel, dim3((unsigned)waves), dim3(64), 0, c->stream, m, model,
-                     n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), ord, out_dev, steps);
+  if (m.method == 1)
+    hipLaunchKernelGGL(moment_traj_kernel<1>, dim3((unsigned)waves), dim3(64), 0, c->stream, m,
+                       model, n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), ord,
+                       out_dev, steps);
+  else
+    hipLaunchKernelGGL(moment_traj_kernel<0>, dim3((unsigned)waves), dim3(64), 0, c->stream, m,
+                       model, n, theta_dev, ld, P<double>(c->mss), P<double>(c->mmap), ord,
+                       out_dev, steps);
   HIPCHK(c, hipGetLastError());
   if (stats_dev) {
     hipLaunchKernelGGL(moment_stats_kernel, dim3(grid1(n, 64)), dim3(64), 0, c->stream, m, n,

@@ -117,6 +117,7 @@ struct MomCfgDev {
   double t0;             // trajectory start
   double iv[9];          // transient start state
   double rtol, atol;     // Dormand-Prince tolerances
+  int method;            // 0 = Dormand-Prince on all nine moments, 1 = gene block exact
   double ages[ABC_MOM_MAX_AGES];
   double pulse[ABC_MOM_MAX_CONDS], chase[ABC_MOM_MAX_CONDS];
   int is_chase[ABC_MOM_MAX_CONDS];
@@ -125,6 +126,7 @@ struct MomCfgDev {
   double cap[ABC_MOM_MAX_AGES * 2 * 3];  // per (age, pulse/chase): E b, E b^2, Var b
   double weights[ABC_MOM_MAX_AGES * ABC_MOM_MAX_CONDS];  // age weights, n_ages x n_conds
 };
+template <int METHOD>
 __global__ void moment_cyclemap_kernel(MomCfgDev cfg, int model, int64_t n,
                                        const double* theta, int64_t ld, const int32_t* order,
                                        double* maps, unsigned long long* cost,
@@ -137,6 +139,7 @@ __global__ void moment_key_kernel(MomCfgDev cfg, int model, int64_t n, const dou
 __global__ void moment_order_kernel(int64_t n, const unsigned long long* cost, int32_t* order);
 __global__ void moment_transient_kernel(MomCfgDev cfg, int64_t n, const double* maps,
                                         double* ss);
+template <int METHOD>
 __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    const double* theta, int64_t ld, const double* ss,
                                    const double* maps, const int32_t* order, double* out,

@@ -215,10 +215,202 @@ ABC_DEV int64_t dopri_piece(const Piece& pc, const Particle& P, const MomCfgDev&
   return steps;
 }
 
+// Method 1: the gene block -- E g (0), Var g (3), Cov gu (4), Cov gl (5) --
+// in closed form on a smooth piece [a, b], where kon, koff, gamma and the
+// labelled fraction lam are constant and alpha(a + s) = al0 + al1 s.  It is a
+// lower-triangular linear system with decay rates K, 2K and nu = K + gamma
+// (K = kon + koff), the stiff part of the moment ODE.  With g_inf = kon / K
+// and D = E g(a) - g_inf:
+//   E g(s)    = g_inf + D e^{-Ks}
+//   Var g(s)  = Pv + A e^{-Ks} + B e^{-2Ks},  Pv = g_inf koff / K,
+//               A = (koff - kon) D / K,  B = Var g(a) - Pv - A
+//   Cov gu(s) = Cov gu(a) e^{-nu s} + (1 - lam) J(s),  Cov gl likewise with lam,
+//   J(s) = int_0^s e^{-nu (s - r)} (al0 + al1 r) Var g(r) dr
+//        = sum over (C, mu) in {(Pv, 0), (A, K), (B, 2K)} of C (al0 E1 + al1 E2),
+//   E1 = (e^{-mu s} - e^{-nu s}) / d,  E2 = (e^{-mu s} (d s - 1) + e^{-nu s}) / d^2
+//   (as products with 1 / d, formed once per piece),
+// d = nu - mu (nu, gamma, gamma - K); for |d s| < 1/2, E1 = e^{-mu s} s phi1(d s)
+// and E2 = e^{-mu s} s^2 phi2(d s) by the series phi1(x) = sum (-x)^k / (k+1)!,
+// phi2(x) = sum (-x)^k / (k+2)!, k < 16.  E u, E l, Var u, Cov ul and Var l
+// (decay gamma and 2 gamma) take the Dormand-Prince steps of method 0 with the
+// gene block evaluated exactly at each stage time, and the error norm runs
+// over those five, so the step is no longer held below 3.3 / (2K).  The CPU
+// restatement performs the same operations in the same order.
+struct GBlock {
+  double K, gam, nu, ginf, D, Pv, A, B, al0, al1, cu, cl, c4, c5;
+  double r0, rK, r2;  // 1 / d for the three sources: 1 / nu, 1 / gamma, 1 / (gamma - K)
+};
+
+ABC_DEV void gb_e12(double d, double rd, double s, double emu, double enu, double& E1,
+                    double& E2) {
+  constexpr double kInvFact[17] = {
+      0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+      0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+      0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29,
+      0x1.6124613a86d09p-33, 0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45,
+      0x1.952c77030ad4ap-49};  // 1/1! .. 1/17!
+  const double x = d * s;
+  if (fabs(x) < 0.5) {
+    const double y = -x;
+    double p1 = kInvFact[15], p2 = kInvFact[16];
+#pragma unroll
+    for (int k = 14; k >= 0; --k) {
+      p1 = p1 * y + kInvFact[k];
+      p2 = p2 * y + kInvFact[k + 1];
+    }
+    E1 = emu * (s * p1);
+    E2 = emu * ((s * s) * p2);
+  } else {
+    E1 = (emu - enu) * rd;
+    E2 = (emu * (x - 1.0) + enu) * (rd * rd);
+  }
+}
+
+// p: kon, koff, alpha(a), gamma; al1: d alpha / dt; y: the state at a
+ABC_DEV void gb_setup(GBlock& q, const double* p, double al1, double lam, const double* y) {
+  q.K = p[0] + p[1];
+  q.gam = p[3];
+  q.nu = q.K + q.gam;
+  q.ginf = p[0] / q.K;
+  q.D = y[0] - q.ginf;
+  q.Pv = q.ginf * (p[1] / q.K);
+  q.A = ((p[1] - p[0]) * q.D) / q.K;
+  q.B = (y[3] - q.Pv) - q.A;
+  q.al0 = p[2];
+  q.al1 = al1;
+  q.cu = 1 - lam;
+  q.cl = lam;
+  q.c4 = y[4];
+  q.c5 = y[5];
+  q.r0 = 1.0 / q.nu;
+  q.rK = 1.0 / q.gam;
+  q.r2 = 1.0 / (q.gam - q.K);  // unused (the series) when gamma = K
+}
+
+// the gene block at a + s into y[0], y[3], y[4], y[5]
+ABC_DEV void gb_eval(const GBlock& q, double s, double* y) {
+  const double e1 = ex(-(q.K * s));
+  const double eg = ex(-(q.gam * s));
+  const double e2 = e1 * e1;
+  const double en = e1 * eg;
+  double E10, E20, E1K, E2K, E12, E22;
+  gb_e12(q.nu, q.r0, s, 1.0, en, E10, E20);
+  gb_e12(q.gam, q.rK, s, e1, en, E1K, E2K);
+  gb_e12(q.gam - q.K, q.r2, s, e2, en, E12, E22);
+  const double J = q.al0 * ((q.Pv * E10 + q.A * E1K) + q.B * E12) +
+                   q.al1 * ((q.Pv * E20 + q.A * E2K) + q.B * E22);
+  y[0] = q.ginf + q.D * e1;
+  y[3] = (q.Pv + q.A * e1) + q.B * e2;
+  y[4] = q.c4 * en + q.cu * J;
+  y[5] = q.c5 * en + q.cl * J;
+}
+
+// Method 1 on one smooth piece [a, b]; returns the attempted steps.
+ABC_DEV int64_t egb_piece(const Piece& pc, const Particle& P, const MomCfgDev& cfg, double a,
+                          double b, double* y, const LogEntry* ltab) {
+  constexpr int Z[5] = {1, 2, 6, 7, 8};
+  const double len = b - a;
+  if (len <= 0) return 0;
+  const double cyc = cfg.cycle, rtol = cfg.rtol, atol = cfg.atol;
+  double p[4];
+  rates(pc, P, cyc, a, p);
+  const double al1 = P.scaling ? pc.base[2] / cyc : 0.0;
+  GBlock q;
+  gb_setup(q, p, al1, pc.lam, y);
+  double t = a;
+  double h = len < 1e-2 ? len : 1e-2;
+  double k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], k7[9], yt[9], yn[9];
+  int64_t steps = 0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) yt[i] = y[i];
+  gb_eval(q, t - a, yt);
+  rhs(p, pc.lam, yt, k1);
+  while (t < b) {
+    if (t + h > b) h = b - t;
+    double ts;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { const int i = Z[j]; yt[i] = y[i] + h * (kA21 * k1[i]); }
+    ts = t + kC2 * h;
+    gb_eval(q, ts - a, yt);
+    rates(pc, P, cyc, ts, p);
+    rhs(p, pc.lam, yt, k2);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) { const int i = Z[j]; yt[i] = y[i] + h * (kA31 * k1[i] + kA32 * k2[i]); }
+    ts = t + kC3 * h;
+    gb_eval(q, ts - a, yt);
+    rates(pc, P, cyc, ts, p);
+    rhs(p, pc.lam, yt, k3);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int i = Z[j];
+      yt[i] = y[i] + h * (kA41 * k1[i] + kA42 * k2[i] + kA43 * k3[i]);
+    }
+    ts = t + kC4 * h;
+    gb_eval(q, ts - a, yt);
+    rates(pc, P, cyc, ts, p);
+    rhs(p, pc.lam, yt, k4);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int i = Z[j];
+      yt[i] = y[i] + h * (kA51 * k1[i] + kA52 * k2[i] + kA53 * k3[i] + kA54 * k4[i]);
+    }
+    ts = t + kC5 * h;
+    gb_eval(q, ts - a, yt);
+    rates(pc, P, cyc, ts, p);
+    rhs(p, pc.lam, yt, k5);
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int i = Z[j];
+      yt[i] = y[i] + h * (kA61 * k1[i] + kA62 * k2[i] + kA63 * k3[i] + kA64 * k4[i] + kA65 * k5[i]);
+    }
+    ts = t + h;
+    gb_eval(q, ts - a, yt);
+    rates(pc, P, cyc, ts, p);
+    rhs(p, pc.lam, yt, k6);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) yn[i] = yt[i];  // the gene block at t + h
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int i = Z[j];
+      yn[i] = y[i] + h * (kB1 * k1[i] + kB3 * k3[i] + kB4 * k4[i] + kB5 * k5[i] + kB6 * k6[i]);
+    }
+    rhs(p, pc.lam, yn, k7);
+    double err = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+      const int i = Z[j];
+      const double ei =
+          h * (kE1 * k1[i] + kE3 * k3[i] + kE4 * k4[i] + kE5 * k5[i] + kE6 * k6[i] + kE7 * k7[i]);
+      const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+      const double r = fabs(ei) / sc;
+      if (r > err) err = r;
+    }
+    steps++;
+    if (err <= 1.0) {
+      t += h;
+#pragma unroll
+      for (int i = 0; i < 9; ++i) {
+        y[i] = yn[i];
+        k1[i] = k7[i];
+      }
+      double fac = err == 0.0 ? 5.0 : 0.9 * ex(-0.2 * ln(err, ltab));
+      if (fac > 5.0) fac = 5.0;
+      h *= fac;
+    } else {
+      double fac = 0.9 * ex(-0.2 * ln(err, ltab));
+      if (fac < 0.2) fac = 0.2;
+      h *= fac;
+    }
+    if (h < 1e-14 * (fabs(t) + 1.0)) h = 1e-14 * (fabs(t) + 1.0);
+  }
+  return steps;
+}
+
 // model(): integrate [tmin, tmax], split at every bin edge k * T / n_steps and
 // at the window edges texp, texp + pulse.  The edges are generated in
 // ascending order (a sorted merge of the grid with the two window edges);
 // coincident edges give empty pieces, which are skipped.
+template <int METHOD>
 ABC_DEV int64_t moment_solve(const Particle& P, const MomCfgDev& cfg, double texp, double pulse,
                              double tmin, double tmax, double* y, const LogEntry* ltab) {
   const double h = cfg.cycle / cfg.n_steps;
@@ -249,7 +441,8 @@ ABC_DEV int64_t moment_solve(const Particle& P, const MomCfgDev& cfg, double tex
     if (b > a) {
       const double tb = 0.5 * (a + b);
       const Piece pc = make_piece(P, cfg, texp, pulse, tb);
-      steps += dopri_piece(pc, P, cfg, a, b, y, ltab);
+      steps += METHOD == 1 ? egb_piece(pc, P, cfg, a, b, y, ltab)
+                           : dopri_piece(pc, P, cfg, a, b, y, ltab);
       a = b;
     }
     if (src < 0) break;
@@ -270,11 +463,12 @@ ABC_DEV void periodic_boundary(const double* e, double* v) {
   v[2] = e[2] / 2;
 }
 
+template <int METHOD>
 ABC_DEV int64_t cycle_map(const Particle& P, const MomCfgDev& cfg, const double* in, double* out,
                           const LogEntry* ltab) {
 #pragma unroll
   for (int i = 0; i < 9; ++i) out[i] = in[i];
-  return moment_solve(P, cfg, -1.0, 0.1, 0.0, cfg.cycle, out, ltab);
+  return moment_solve<METHOD>(P, cfg, -1.0, 0.1, 0.0, cfg.cycle, out, ltab);
 }
 
 ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* theta, int64_t ld,
@@ -296,6 +490,7 @@ ABC_DEV Particle particle_of(int model, const MomCfgDev& cfg, const double* thet
 // so a particle's longest chain is one cycle whatever its iterations.
 // Wave w integrates column w % 10 for the 64 particles order[64 (w / 10) ..]
 // (an order by predicted cost) and adds each integration's steps to cost[p].
+template <int METHOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_cyclemap_kernel(
     const MomCfgDev cfg, int model, int64_t n, const double* __restrict__ theta, int64_t ld,
     const int32_t* __restrict__ order, double* __restrict__ maps,
@@ -312,17 +507,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
   const Particle P = particle_of(model, cfg, theta, ld, p);
   double in[9], out[9];
   for (int k = 0; k < 9; ++k) in[k] = (col > 0 && k == col - 1) ? 1.0 : 0.0;
-  const int64_t st = cycle_map(P, cfg, in, out, ltab);
+  const int64_t st = cycle_map<METHOD>(P, cfg, in, out, ltab);
   double* o = maps + (p * 10 + col) * 9;
   for (int k = 0; k < 9; ++k) o[k] = out[k];
   atomicAdd(cost + p, (unsigned long long)st);
   if (steps) atomicAdd(steps + p, (unsigned long long)st);
 }
 
-// Predicted cycle-map cost of particle p, written to cost[p]: the moment
-// ODE's fastest rate, 2 max(kon + koff, gamma), summed over the rate bins of
-// the cycle (the explicit integrator's step is held near a fixed multiple of
-// its inverse), in units of 1/1024.
+// Predicted cycle-map cost of particle p, written to cost[p] in units of
+// 2^-20, summed over the rate bins of the cycle.  Method 0: the moment ODE's
+// fastest rate, 2 max(kon + koff, gamma) (the explicit step is held near a
+// fixed multiple of its inverse).  Method 1 (gene block exact): gamma
+// alpha^0.3 (a fit of its step counts over the reference prior gave
+// steps ~ gamma^0.3 alpha^0.1 and no dependence on kon + koff; the ranking
+// is what counts here).
 __global__ __launch_bounds__(256) void moment_key_kernel(const MomCfgDev cfg, int model, int64_t n,
                                                          const double* __restrict__ theta,
                                                          int64_t ld,
@@ -332,11 +530,14 @@ __global__ __launch_bounds__(256) void moment_key_kernel(const MomCfgDev cfg, in
   const Particle P = particle_of(model, cfg, theta, ld, p);
   double key = 0.0;
   for (int b = 0; b < cfg.n_steps; ++b) {
-    double r[4];
-    for (int k = 0; k < 4; ++k) r[k] = ex10(theta_at(P, P.v.o[k] + (P.v.cnt[k] > 1 ? b : 0)));
-    key += 2.0 * fmax(r[0] + r[1], r[3]);
+    double lg[4];
+    for (int k = 0; k < 4; ++k) lg[k] = theta_at(P, P.v.o[k] + (P.v.cnt[k] > 1 ? b : 0));
+    if (cfg.method == 1)
+      key += ex10(lg[3] + 0.3 * lg[2]);
+    else
+      key += 2.0 * fmax(ex10(lg[0]) + ex10(lg[1]), ex10(lg[3]));
   }
-  key *= 1024.0;
+  key *= 1048576.0;
   cost[p] = key < 1.8e19 ? (unsigned long long)key : 18000000000000000000ull;
 }
 
@@ -474,6 +675,7 @@ __global__ __launch_bounds__(64) void moment_transient_kernel(const MomCfgDev cf
 // A whole cycle that starts on a cycle boundary and does not meet the
 // labelling window has the rates of [0, T] shifted by whole cycles and no
 // labelling, so it is the particle's affine cycle map, applied from maps.
+template <int METHOD>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void moment_traj_kernel(const MomCfgDev cfg,
                                                           int model, int64_t n,
                                                           const double* __restrict__ theta,
@@ -520,7 +722,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void mo
       }
       for (int i = 0; i < 9; ++i) y[i] = tmp[i];
     } else {
-      st += moment_solve(P, cfg, texp, pulse, tau, tf, y, ltab);
+      st += moment_solve<METHOD>(P, cfg, texp, pulse, tau, tf, y, ltab);
     }
     tau = tf;
   }
@@ -603,3 +805,18 @@ __global__ __launch_bounds__(64) void moment_stats_kernel(const MomCfgDev cfg,
     }
   }
 }
+
+// one instantiation per integration method (cfg.method), so that each has its
+// own register allocation
+template __global__ void moment_cyclemap_kernel<0>(MomCfgDev, int, int64_t, const double*, int64_t,
+                                                   const int32_t*, double*, unsigned long long*,
+                                                   unsigned long long*);
+template __global__ void moment_cyclemap_kernel<1>(MomCfgDev, int, int64_t, const double*, int64_t,
+                                                   const int32_t*, double*, unsigned long long*,
+                                                   unsigned long long*);
+template __global__ void moment_traj_kernel<0>(MomCfgDev, int, int64_t, const double*, int64_t,
+                                               const double*, const double*, const int32_t*,
+                                               double*, unsigned long long*);
+template __global__ void moment_traj_kernel<1>(MomCfgDev, int, int64_t, const double*, int64_t,
+                                               const double*, const double*, const int32_t*,
+                                               double*, unsigned long long*);

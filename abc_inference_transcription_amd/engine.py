@@ -22,10 +22,12 @@ class MomentCfg:
     abc_sim arguments, recover_statistics.jl:29-44, abc_simulation.jl:64-76).
     Defaults are the reference's: 20 h cycle, 5 rate bins, t0 = -3 cycles,
     the 1 % transient rule; iv is the transient start (recover_statistics.jl
-    starts from E g = 1/2, abc_simulation.jl from E u = 1/2)."""
+    starts from E g = 1/2, abc_simulation.jl from E u = 1/2).  method 1 (the
+    default) solves the stiff gene block in closed form on each smooth piece
+    and integrates the other five moments; method 0 integrates all nine."""
 
     def __init__(self, ages, pulse, chase, is_chase=None, cycle=20.0, n_steps=5, t0=None,
-                 iv=None, transient_mode=0, rtol=1e-10, atol=1e-12):
+                 iv=None, transient_mode=0, rtol=1e-10, atol=1e-12, method=1):
         self.ages = np.ascontiguousarray(ages, np.float64)
         self.pulse = np.ascontiguousarray(pulse, np.float64)
         self.chase = np.ascontiguousarray(chase, np.float64)
@@ -35,7 +37,7 @@ class MomentCfg:
             float(cycle), int(n_steps), len(self.ages), _p(self.ages), len(self.pulse),
             _p(self.pulse), _p(self.chase), _p(self.is_chase),
             -3.0 * cycle if t0 is None else float(t0), (C.c_double * 9)(*iv),
-            int(transient_mode), float(rtol), float(atol))
+            int(transient_mode), float(rtol), float(atol), int(method))
 
     @property
     def n_ages(self):

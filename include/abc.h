@@ -225,6 +225,10 @@ typedef struct {
   int32_t transient_mode;   /* 0 = the reference's 1 % rule (model.jl:114-142),
                                1 = the exact periodic state                      */
   double rtol, atol;        /* integrator tolerances, e.g. 1e-10, 1e-12          */
+  int32_t method;           /* 0 = Dormand-Prince 5(4) on all nine moments; 1 = the
+                               stiff gene block (E g, Var g, Cov gu, Cov gl) in
+                               closed form on each smooth piece, Dormand-Prince on
+                               the other five (several times fewer steps)          */
 } abc_moment_cfg;
 
 /* run_part_sim without downsampling (recover_statistics.jl:2-11, the producer

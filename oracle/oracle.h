@@ -94,6 +94,8 @@ typedef struct {
   int32_t transient_mode;                   /* 0 = reference 1% rule,
                                                1 = iterate to fixed point   */
   double rtol, atol;                        /* integrator tolerances        */
+  int32_t method;                           /* 0 = Dormand-Prince on all 9,
+                                               1 = gene block in closed form */
 } orc_moment_cfg;
 
 /* y' = f(y) at time t (model.jl:67-86). */
@@ -102,7 +104,7 @@ void orc_moment_rhs(int model, const double* theta, double cycle, int n_steps,
 /* model(): integrate [tmin,tmax] (model.jl:89-96). Returns steps taken. */
 int64_t orc_moment_solve(int model, const double* theta, double cycle, int n_steps,
                          double texp, double pulse, double tmin, double tmax, double* y,
-                         double rtol, double atol);
+                         double rtol, double atol, int method);
 void orc_periodic_boundary(const double* in, double* out);
 /* n draws of the SSA's Poisson sampler (inversion below 10, PTRS above). */
 void orc_poisson_sample(double mu, uint64_t seed, int64_t n, uint32_t* out);

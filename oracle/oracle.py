@@ -52,6 +52,7 @@ class MomentCfg(C.Structure):
         ("ages", C.c_void_p), ("n_conds", C.c_int32), ("pulse", C.c_void_p),
         ("chase", C.c_void_p), ("t0", C.c_double), ("iv", C.c_double * 9),
         ("transient_mode", C.c_int32), ("rtol", C.c_double), ("atol", C.c_double),
+        ("method", C.c_int32),
     ]
 
 
@@ -72,7 +73,7 @@ def _declare(L):
     L.orc_simulate.restype = C.c_int
     L.orc_cell_stats.argtypes = [P, P, P, P, P, P, P]
     L.orc_moment_rhs.argtypes = [C.c_int, P, D, C.c_int, D, D, D, P, P]
-    L.orc_moment_solve.argtypes = [C.c_int, P, D, C.c_int, D, D, D, D, P, D, D]
+    L.orc_moment_solve.argtypes = [C.c_int, P, D, C.c_int, D, D, D, D, P, D, D, C.c_int]
     L.orc_moment_solve.restype = I64
     L.orc_periodic_boundary.argtypes = [P, P]
     L.orc_transient.argtypes = [C.c_int, P, P, P]; L.orc_transient.restype = C.c_int
@@ -246,13 +247,13 @@ def cell_stats(od: OracleDesign, u, l, bins):
 
 # -------------------------------------------------------------- moment model
 def moment_cfg(ages, pulse, chase, cycle=20.0, n_steps=5, t0=None, iv=None,
-               transient_mode=0, rtol=1e-10, atol=1e-12):
+               transient_mode=0, rtol=1e-10, atol=1e-12, method=1):
     ages = np.ascontiguousarray(ages, np.float64)
     pulse = np.ascontiguousarray(pulse, np.float64)
     chase = np.ascontiguousarray(chase, np.float64)
     ivv = (C.c_double * 9)(*(iv if iv is not None else [0.5] + [0.0] * 8))
     cfg = MomentCfg(cycle, n_steps, len(ages), _p(ages), len(pulse), _p(pulse), _p(chase),
-                    -3 * cycle if t0 is None else t0, ivv, transient_mode, rtol, atol)
+                    -3 * cycle if t0 is None else t0, ivv, transient_mode, rtol, atol, method)
     cfg._keep = (ages, pulse, chase)
     return cfg
 

@@ -8,7 +8,8 @@
  *   momentodes / f .............. scripts/model.jl:67-86
  *   model (CVODE_BDF solve) ..... scripts/model.jl:89-96 -> adaptive
  *                                 Dormand-Prince 5(4) at tight tolerance,
- *                                 split at every rate discontinuity
+ *                                 split at every rate discontinuity; method 1
+ *                                 solves the stiff gene block in closed form
  *   periodic_boundary ........... scripts/model.jl:98-111
  *   transient_phase ............. scripts/model.jl:114-142
  *   trajectories ................ scripts/model.jl:146-174
@@ -180,9 +181,192 @@ static int64_t dopri_piece(int model, const double* th, double cycle, int n_step
   return steps;
 }
 
+/* ---- method 1: the gene block in closed form ----------------------------
+ * On a smooth piece [a, b] kon, koff, gamma and the labelled fraction lam are
+ * constant and alpha(a + s) = al0 + al1 s (al1 = 0 without size scaling).
+ * The gene block -- E g (0), Var g (3), Cov gu (4), Cov gl (5) -- is a
+ * lower-triangular linear system with decay rates K, 2K and nu = K + gamma
+ * (K = kon + koff), the stiff part of the moment ODE.  With g_inf = kon / K
+ * and D = E g(a) - g_inf:
+ *   E g(s)    = g_inf + D e^{-Ks}
+ *   Var g(s)  = Pv + A e^{-Ks} + B e^{-2Ks},  Pv = g_inf koff / K,
+ *               A = (koff - kon) D / K,  B = Var g(a) - Pv - A
+ *   Cov gu(s) = Cov gu(a) e^{-nu s} + (1 - lam) J(s),  Cov gl likewise with lam,
+ *   J(s)      = int_0^s e^{-nu (s - r)} (al0 + al1 r) Var g(r) dr
+ *             = sum over (C, mu) in {(Pv, 0), (A, K), (B, 2K)} of C (al0 E1 + al1 E2),
+ *   E1 = int_0^s e^{-nu (s-r)} e^{-mu r} dr        = (e^{-mu s} - e^{-nu s}) / d,
+ *   E2 = int_0^s e^{-nu (s-r)} r e^{-mu r} dr      = (e^{-mu s} (d s - 1) + e^{-nu s}) / d^2,
+ * d = nu - mu (nu, gamma, gamma - K), as products with 1 / d formed once per
+ * piece.  For |d s| < 1/2 the same integrals are
+ * e^{-mu s} s phi1(d s) and e^{-mu s} s^2 phi2(d s), with the series
+ * phi1(x) = sum_k (-x)^k / (k+1)!, phi2(x) = sum_k (-x)^k / (k+2)!, k < 16.
+ * The other five moments (E u, E l, Var u, Cov ul, Var l: decay gamma and
+ * 2 gamma) are integrated by the same Dormand-Prince 5(4) as method 0, with
+ * the gene block evaluated exactly at every stage time; the error norm runs
+ * over those five.  The explicit method's step is then no longer held below
+ * 3.3 / (2K). */
+static const double kInvFact[17] = {
+    0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5,
+    0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13, 0x1.a01a01a01a01ap-16,
+    0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26, 0x1.1eed8eff8d898p-29,
+    0x1.6124613a86d09p-33, 0x1.93974a8c07c9dp-37, 0x1.ae7f3e733b81fp-41, 0x1.ae7f3e733b81fp-45,
+    0x1.952c77030ad4ap-49};  /* 1/1! .. 1/17! */
+
+typedef struct {
+  double K, gam, nu, ginf, D, Pv, A, B, al0, al1, cu, cl, c4, c5;
+  double r0, rK, r2;  /* 1 / d for the three sources: 1 / nu, 1 / gamma, 1 / (gamma - K) */
+} gblock_t;
+
+/* E1, E2 for decay nu and source decay mu, d = nu - mu, emu = e^{-mu s}, enu = e^{-nu s} */
+static void gb_e12(double d, double rd, double s, double emu, double enu, double* E1,
+                   double* E2) {
+  const double x = d * s;
+  if (fabs(x) < 0.5) {
+    const double y = -x;
+    double p1 = kInvFact[15], p2 = kInvFact[16];
+    for (int k = 14; k >= 0; --k) {
+      p1 = p1 * y + kInvFact[k];
+      p2 = p2 * y + kInvFact[k + 1];
+    }
+    *E1 = emu * (s * p1);
+    *E2 = emu * ((s * s) * p2);
+  } else {
+    *E1 = (emu - enu) * rd;
+    *E2 = (emu * (x - 1.0) + enu) * (rd * rd);
+  }
+}
+
+/* p: kon, koff, alpha(a), gamma; al1: d alpha / dt; y: the state at a */
+static void gb_setup(gblock_t* q, const double* p, double al1, double lam, const double* y) {
+  q->K = p[0] + p[1];
+  q->gam = p[3];
+  q->nu = q->K + q->gam;
+  q->ginf = p[0] / q->K;
+  q->D = y[0] - q->ginf;
+  q->Pv = q->ginf * (p[1] / q->K);
+  q->A = ((p[1] - p[0]) * q->D) / q->K;
+  q->B = (y[3] - q->Pv) - q->A;
+  q->al0 = p[2];
+  q->al1 = al1;
+  q->cu = 1 - lam;
+  q->cl = lam;
+  q->c4 = y[4];
+  q->c5 = y[5];
+  q->r0 = 1.0 / q->nu;
+  q->rK = 1.0 / q->gam;
+  q->r2 = 1.0 / (q->gam - q->K);  /* unused (the series) when gamma = K */
+}
+
+/* the gene block at a + s into y[0], y[3], y[4], y[5] */
+static void gb_eval(const gblock_t* q, double s, double* y) {
+  const double e1 = orc_exp(-(q->K * s));
+  const double eg = orc_exp(-(q->gam * s));
+  const double e2 = e1 * e1;
+  const double en = e1 * eg;
+  double E10, E20, E1K, E2K, E12, E22;
+  gb_e12(q->nu, q->r0, s, 1.0, en, &E10, &E20);
+  gb_e12(q->gam, q->rK, s, e1, en, &E1K, &E2K);
+  gb_e12(q->gam - q->K, q->r2, s, e2, en, &E12, &E22);
+  const double J = q->al0 * ((q->Pv * E10 + q->A * E1K) + q->B * E12) +
+                   q->al1 * ((q->Pv * E20 + q->A * E2K) + q->B * E22);
+  y[0] = q->ginf + q->D * e1;
+  y[3] = (q->Pv + q->A * e1) + q->B * e2;
+  y[4] = q->c4 * en + q->cu * J;
+  y[5] = q->c5 * en + q->cl * J;
+}
+
+/* Method 1 on one smooth piece [a, b]: Dormand-Prince 5(4) over the five
+ * moments of Z, the gene block exact. */
+static int64_t egb_piece(int model, const double* th, double cycle, int n_steps,
+                         double lam, double tb, double a, double b, double* y,
+                         double rtol, double atol) {
+  static const int Z[5] = {1, 2, 6, 7, 8};
+  static const double c2 = 1.0 / 5, c3 = 3.0 / 10, c4 = 4.0 / 5, c5 = 8.0 / 9;
+  static const double a21 = 1.0 / 5;
+  static const double a31 = 3.0 / 40, a32 = 9.0 / 40;
+  static const double a41 = 44.0 / 45, a42 = -56.0 / 15, a43 = 32.0 / 9;
+  static const double a51 = 19372.0 / 6561, a52 = -25360.0 / 2187, a53 = 64448.0 / 6561,
+                      a54 = -212.0 / 729;
+  static const double a61 = 9017.0 / 3168, a62 = -355.0 / 33, a63 = 46732.0 / 5247,
+                      a64 = 49.0 / 176, a65 = -5103.0 / 18656;
+  static const double b1 = 35.0 / 384, b3 = 500.0 / 1113, b4 = 125.0 / 192,
+                      b5 = -2187.0 / 6784, b6 = 11.0 / 84;
+  static const double e1 = 71.0 / 57600, e3 = -71.0 / 16695, e4 = 71.0 / 1920,
+                      e5 = -17253.0 / 339200, e6 = 22.0 / 525, e7 = -1.0 / 40;
+  double t = a, len = b - a;
+  if (len <= 0) return 0;
+  double p[4], pb[4];
+  rates_at(model, th, cycle, n_steps, a, tb, p);
+  rates_at(model, th, cycle, n_steps, floor(tb / cycle) * cycle, tb, pb);
+  const double al1 = model != 2 ? pb[2] / cycle : 0.0;
+  gblock_t q;
+  gb_setup(&q, p, al1, lam, y);
+  double h = len < 1e-2 ? len : 1e-2;
+  double k1[9], k2[9], k3[9], k4[9], k5[9], k6[9], k7[9], yt[9], yn[9];
+  int64_t steps = 0;
+  memcpy(yt, y, sizeof yt);
+  gb_eval(&q, t - a, yt);
+  rhs(p, lam, yt, k1);
+  while (t < b) {
+    if (t + h > b) h = b - t;
+    int j, i;
+    double ts;
+    for (j = 0; j < 5; ++j) { i = Z[j]; yt[i] = y[i] + h * (a21 * k1[i]); }
+    ts = t + c2 * h; gb_eval(&q, ts - a, yt);
+    rates_at(model, th, cycle, n_steps, ts, tb, p); rhs(p, lam, yt, k2);
+    for (j = 0; j < 5; ++j) { i = Z[j]; yt[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]); }
+    ts = t + c3 * h; gb_eval(&q, ts - a, yt);
+    rates_at(model, th, cycle, n_steps, ts, tb, p); rhs(p, lam, yt, k3);
+    for (j = 0; j < 5; ++j) { i = Z[j]; yt[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]); }
+    ts = t + c4 * h; gb_eval(&q, ts - a, yt);
+    rates_at(model, th, cycle, n_steps, ts, tb, p); rhs(p, lam, yt, k4);
+    for (j = 0; j < 5; ++j) {
+      i = Z[j];
+      yt[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
+    }
+    ts = t + c5 * h; gb_eval(&q, ts - a, yt);
+    rates_at(model, th, cycle, n_steps, ts, tb, p); rhs(p, lam, yt, k5);
+    for (j = 0; j < 5; ++j) {
+      i = Z[j];
+      yt[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
+    }
+    ts = t + h; gb_eval(&q, ts - a, yt);
+    rates_at(model, th, cycle, n_steps, ts, tb, p); rhs(p, lam, yt, k6);
+    memcpy(yn, yt, sizeof yn);  /* the gene block at t + h */
+    for (j = 0; j < 5; ++j) {
+      i = Z[j];
+      yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b5 * k5[i] + b6 * k6[i]);
+    }
+    rhs(p, lam, yn, k7);
+    double err = 0.0;
+    for (j = 0; j < 5; ++j) {
+      i = Z[j];
+      double ei = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i] + e7 * k7[i]);
+      double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+      double r = fabs(ei) / sc;
+      if (r > err) err = r;
+    }
+    steps++;
+    if (err <= 1.0) {
+      t += h;
+      memcpy(y, yn, sizeof yn);
+      memcpy(k1, k7, sizeof k7);
+      double fac = err == 0.0 ? 5.0 : 0.9 * orc_exp(-0.2 * orc_log(err));
+      if (fac > 5.0) fac = 5.0;
+      h *= fac;
+    } else {
+      double fac = 0.9 * orc_exp(-0.2 * orc_log(err));
+      if (fac < 0.2) fac = 0.2;
+      h *= fac;
+    }
+    if (h < 1e-14 * (fabs(t) + 1.0)) h = 1e-14 * (fabs(t) + 1.0);
+  }
+  return steps;
+}
+
 int64_t orc_moment_solve(int model, const double* th, double cycle, int n_steps, double texp,
                          double pulse, double tmin, double tmax, double* y, double rtol,
-                         double atol) {
+                         double atol, int method) {
   /* breakpoints: every bin boundary k*cycle/n_steps and the window edges */
   double bp[4096];
   int nbp = 0;
@@ -208,7 +392,8 @@ int64_t orc_moment_solve(int model, const double* th, double cycle, int n_steps,
     if (b <= a) continue;
     double tb = 0.5 * (a + b);
     double lam = lam_at(th, model, n_steps, texp, pulse, tb);
-    steps += dopri_piece(model, th, cycle, n_steps, lam, tb, a, b, y, rtol, atol);
+    steps += method == 1 ? egb_piece(model, th, cycle, n_steps, lam, tb, a, b, y, rtol, atol)
+                         : dopri_piece(model, th, cycle, n_steps, lam, tb, a, b, y, rtol, atol);
     a = b;
   }
   return steps;
@@ -227,7 +412,7 @@ static void cycle_map(int model, const double* th, const orc_moment_cfg* c, cons
                       double* out) {
   memcpy(out, in, 9 * sizeof(double));
   orc_moment_solve(model, th, c->cycle, c->n_steps, -1.0, 0.1, 0.0, c->cycle, out, c->rtol,
-                   c->atol);
+                   c->atol, c->method);
 }
 
 /* Solve A x = b (9x9) by Gaussian elimination with partial pivoting. */
@@ -370,7 +555,8 @@ static int64_t trajectory(int model, const double* th, const orc_moment_cfg* c, 
       affine_apply(M, v, y, tmp);
       memcpy(y, tmp, sizeof y);
     } else {
-      st += orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol);
+      st += orc_moment_solve(model, th, T, c->n_steps, texp, pulse, tau, tf, y, c->rtol, c->atol,
+                             c->method);
     }
     tau = tf;
   }

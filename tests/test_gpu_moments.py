@@ -38,14 +38,17 @@ def _same_bits(a, b):
     return np.array_equal(a.view(np.int64), b.view(np.int64))
 
 
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("mode,iv", [(0, None), (0, [0.0, 0.5] + [0.0] * 7), (1, None)])
-def test_recover_bit_exact(oracle, eng, model, mode, iv):
+def test_recover_bit_exact(oracle, eng, model, mode, iv, method):
     """Prior draws of the reference ranges (kon, koff, alpha 1e-3..1e3, gamma
-    1e-3..1e2): every moment of every condition and age, bit for bit."""
+    1e-3..1e2): every moment of every condition and age, bit for bit, with
+    either integration method (0: Dormand-Prince on all nine moments; 1: the
+    gene block in closed form)."""
     n = 6 if mode == 0 else 3
     th = _prior(oracle, model, n, 1000 * model + 17 * mode)
-    g, o = _cfgs(oracle, transient_mode=mode, iv=iv)
+    g, o = _cfgs(oracle, transient_mode=mode, iv=iv, method=method)
     got, steps = eng.recover(model, th, g, steps=True)
     assert np.all(steps > 0)
     for i in range(n):
@@ -123,6 +126,8 @@ def test_moment_api_rejects_bad_arguments(eng):
         eng.recover(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, rtol=0.0))
     with pytest.raises(RuntimeError):
         eng.recover(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, transient_mode=2))
+    with pytest.raises(RuntimeError):
+        eng.recover(2, th, MomentCfg(AGES, REF_PULSE, REF_CHASE, method=2))
     with pytest.raises(RuntimeError):
         eng.recover(2, th, MomentCfg([1.0] * 9, [0.5], [0.0]))  # more than 8 ages
     with pytest.raises(RuntimeError):

@@ -189,3 +189,66 @@ def test_whole_unlabelled_cycles_are_the_affine_map(oracle, model):
         a = oracle.recover(model, th[i], cfg)
         d = oracle.recover(model, th[i], cfg, direct=True)
         assert np.allclose(a, d, rtol=1e-7, atol=1e-12), (model, i, np.max(np.abs(a - d)))
+
+
+def _solve(O, model, th, y0, method, rtol=1e-12, atol=1e-15, t1=20.0):
+    import ctypes as C
+    y = np.array(y0, np.float64)
+    t = np.ascontiguousarray(th, np.float64)
+    st = O.lib().orc_moment_solve(model, t.ctypes.data_as(C.c_void_p), 20.0, 5, 7.0, 2.0, 0.0,
+                                  t1, y.ctypes.data_as(C.c_void_p), rtol, atol, method)
+    return y, st
+
+
+# log10 (kon, koff, alpha, gamma, lam) of the edge cases for the closed form:
+# gamma = K (the 2K source resonates with nu = K + gamma), gamma a little
+# above K (|d s| crosses 1/2 inside a piece), a stiff gene with a slow mRNA,
+# gamma far above K, and every rate slow (nu s < 1/2: the series for E1(0)).
+EGB_CASES = {
+    "gamma=K": [0.0, 0.0, 1.0, math.log10(2.0), -0.3],
+    "gamma~K": [0.0, 0.0, 1.0, math.log10(2.3), -0.3],
+    "stiff gene": [3.0, 3.0, 1.0, -3.0, -0.3],
+    "gamma>>K": [-3.0, -2.0, 2.0, math.log10(50.0), -0.1],
+    "all slow": [-3.0, -3.0, 0.5, -3.0, -0.5],
+}
+
+
+@pytest.mark.parametrize("model", [1, 2])
+@pytest.mark.parametrize("case", sorted(EGB_CASES))
+def test_gene_block_closed_form_matches_integration(oracle, model, case):
+    """Method 1 (the gene block E g, Var g, Cov gu, Cov gl in closed form, the
+    rest by Dormand-Prince) against method 0 (Dormand-Prince on all nine
+    moments) over one cycle through the labelling window, from a generic
+    state, at rtol 1e-12: two different algorithms for the same linear ODE.
+    Model 1 carries the size scaling alpha(t) = alpha (1 + t/T) (the al1
+    terms), model 2 a constant alpha.  A dropped source term, a wrong decay
+    rate or a sign error in any of the closed forms moves some moment by far
+    more than the bound."""
+    th = np.array(EGB_CASES[case] + [0.0])
+    y0 = [0.3, 2.0, 0.5, 0.1, 0.05, 0.02, 3.0, 0.1, 0.6]
+    a, _ = _solve(oracle, model, th, y0, 0)
+    b, _ = _solve(oracle, model, th, y0, 1)
+    assert np.all(np.isfinite(b))
+    err = np.abs(b - a) / (np.abs(a) + 1e-9 * np.abs(a).max())
+    assert err.max() < 1e-7, (case, err)
+
+
+@pytest.mark.parametrize("model", [1, 2, 3, 4, 5])
+def test_gene_block_closed_form_on_prior_draws(oracle, model):
+    """The same comparison over reference-prior particles (kon, koff 1e-3..1e3,
+    gamma 1e-3..1e2; per-bin rates for models 3-5), starting from the
+    reference's transient start state; and method 1 takes fewer steps on
+    the stiff ones.  Moments that have decayed to about zero carry method
+    0's absolute error (atol 1e-15 per step over thousands of steps), hence
+    the absolute floor of 1e-12."""
+    from abc_inference_transcription_amd.design import reference_design
+    od = oracle.OracleDesign(reference_design(cells=55))
+    th = oracle.sample_prior(od, model, 0, 500 + model, 24)
+    iv = [0.0, 0.5] + [0.0] * 7
+    ratio = []
+    for i in range(len(th)):
+        a, s0 = _solve(oracle, model, th[i], iv, 0)
+        b, s1 = _solve(oracle, model, th[i], iv, 1)
+        assert np.all(np.abs(b - a) <= 1e-6 * np.abs(a) + 1e-12), (i, a, b)
+        ratio.append(s0 / s1)
+    assert np.median(ratio) > 1.0 and max(ratio) > 3.0
