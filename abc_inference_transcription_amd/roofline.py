@@ -43,4 +43,20 @@ VALU_PEAK_TOPS = 39.3
 # The FP64 peak for such unfused operations is one per lane per clock:
 # 39.3 TFLOP/s (the published 78.6 counts a fused multiply-add as two).
 MOMENT_FLOPS_PER_STEP = 1044
+
+# Method 1 (the gene block in closed form, Dormand-Prince over the other five
+# moments), counted the same way per step:
+#
+#     stage combinations  5 x (3 + 5 + 7 + 9 + 11) + yn 5 x 11         230
+#     stage times t + c h and offsets from the piece start              14
+#     size scaling of alpha at 5 stage times (4 each)                   20
+#     right-hand side of the five moments, 6 evaluations x 41          246
+#     gene block at 5 stage times x 102 (two own exp of 21 and two
+#       products, three (E1, E2) pairs of 10 and gamma - K, the
+#       source sum J of 13, the four moments of 12)                    510
+#     error norm 5 x 15                                                 75
+#     step-size control (own log and exp)                               60
+#                                                                   ------
+#                                                                     1155
+MOMENT_FLOPS_PER_STEP_BY_METHOD = {0: MOMENT_FLOPS_PER_STEP, 1: 1155}
 FP64_PEAK_TFLOPS = 39.3

@@ -50,6 +50,9 @@ def parse():
                     help="skip the leg of the reference's own algorithm (moment model)")
     ap.add_argument("--mom-particles", type=int, default=32768,
                     help="particles per model of the moment-model leg")
+    ap.add_argument("--mom-method", type=int, default=1, choices=[0, 1],
+                    help="moment-model integration: 1 = gene block in closed form, "
+                         "0 = Dormand-Prince on all nine moments")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (profiler byte-counter scale)")
     return ap.parse_args()
@@ -284,12 +287,16 @@ def moment_leg(stream, a):
     statistics of abc_simulation.jl, reference design: 20 h cycle, 5 ages at
     the bin centres, 11 conditions, capture downsampling) on the GPU
     (abc_moment_sim_dev) and the oracle's C restatement of it on the host
-    cores, both integrating with the same Dormand-Prince steps (rtol 1e-10)."""
+    cores, both integrating with the same steps (rtol 1e-10; method 1 solves the
+    stiff gene block in closed form and takes Dormand-Prince steps over the
+    other five moments, method 0 takes them over all nine)."""
     import torch
     from abc_inference_transcription_amd import _lib
     from abc_inference_transcription_amd.design import REF_CHASE, REF_PULSE, reference_design
     from abc_inference_transcription_amd.engine import Engine, MomentCfg
-    from abc_inference_transcription_amd.roofline import MOMENT_FLOPS_PER_STEP, FP64_PEAK_TFLOPS
+    from abc_inference_transcription_amd.roofline import (FP64_PEAK_TFLOPS,
+                                                          MOMENT_FLOPS_PER_STEP_BY_METHOD)
+    flops_per_step = MOMENT_FLOPS_PER_STEP_BY_METHOD[a.mom_method]
     dev = torch.device("cuda:0")
     # a context of the reference design, so that the particles are draws of the
     # reference prior (fix_params ranges), as on the CPU side
@@ -299,7 +306,7 @@ def moment_leg(stream, a):
     ages = [(2 * k + 1) / 10 * 20 for k in range(5)]
     is_chase = [0] * 6 + [1] * 5
     cfg = MomentCfg(ages, REF_PULSE, REF_CHASE, is_chase=is_chase, cycle=20.0,
-                    iv=[0.0, 0.5] + [0.0] * 7)
+                    iv=[0.0, 0.5] + [0.0] * 7, method=a.mom_method)
     w = np.ascontiguousarray(np.full(55, 0.2))
     # Beta(2,8) capture in every (age, pulse/chase) group: E b, E b^2, Var b
     cap = np.ascontiguousarray(np.tile([0.2, 6.0 / 110.0, 16.0 / 1100.0], 10))
@@ -330,18 +337,18 @@ def moment_leg(stream, a):
     ms = e0.elapsed_time(e1)
     steps = int(sum(int(b[3].sum()) for b in bufs))
     value = len(models) * n / (ms / 1e3)
-    tflops = steps * MOMENT_FLOPS_PER_STEP / (ms / 1e3) / 1e12
+    tflops = steps * flops_per_step / (ms / 1e3) / 1e12
     eng.close()
     print(f"[bench] moment path: {value:.1f} particles/s", file=sys.stderr, flush=True)
     res = {"value": round(value, 3), "unit": "particles/s", "particles": len(models) * n,
-           "ms": round(ms, 3), "rtol": 1e-10,
+           "ms": round(ms, 3), "rtol": 1e-10, "method": a.mom_method,
            "rk_steps_per_particle": steps / (len(models) * n),
            "roofline": {"bound": "fp64 valu", "achieved": round(tflops, 3),
                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
-                        "flops_per_step": MOMENT_FLOPS_PER_STEP}}
+                        "flops_per_step": flops_per_step}}
     if not a.no_cpu:
-        res["cpu"] = moment_cpu(models, cfg, w, is_chase, cap, a.cpu_seconds)
+        res["cpu"] = moment_cpu(models, cfg, w, is_chase, cap, a.cpu_seconds, a.mom_method)
         res["gpu_vs_cpu"] = round(value / res["cpu"]["value"], 1)
     return res
 
@@ -377,7 +384,7 @@ def model_probs_leg(eng, stream, a):
     return res
 
 
-def moment_cpu(models, cfg, w, is_chase, cap, budget_s):
+def moment_cpu(models, cfg, w, is_chase, cap, budget_s, method):
     """The oracle's threaded C restatement of the same moment abc_sim on the
     host cores, over a bounded sample: 16 particles per model, round-robin,
     from the start of each model's prior stream."""
@@ -386,7 +393,8 @@ def moment_cpu(models, cfg, w, is_chase, cap, budget_s):
     from abc_inference_transcription_amd.design import reference_design
     threads = min(os.cpu_count() or 1, 16)
     od = O.OracleDesign(reference_design(cells=55))
-    ocfg = O.moment_cfg(cfg.ages, cfg.pulse, cfg.chase, cycle=20.0, iv=[0.0, 0.5] + [0.0] * 7)
+    ocfg = O.moment_cfg(cfg.ages, cfg.pulse, cfg.chase, cycle=20.0, iv=[0.0, 0.5] + [0.0] * 7,
+                        method=method)
     done, t_used, it = 0, 0.0, 0
     while t_used < budget_s and it < 500:
         m = models[it % len(models)]
