@@ -867,25 +867,26 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (tm < tg && tm < tb) {
       t = tm;
       pend = false;
+      // The channel, then its molecule, as predicated arithmetic: lanes of a
+      // wave that fire different channels run one instruction stream.
+      //   transcription (rr < c1): kept by the size-scaling thinning test on
+      //     pz, labelled when pw < thr_lam inside the window;
+      //   splicing (c1 <= rr < c2): molecule lemire(pz, n0 + n2) of the
+      //     unspliced, unlabelled first;
+      //   decay (rr >= c2): molecule lemire(pz, ntot) in species order.
       const double rr = u01_32(py) * a0;
-      if (rr < c1) {
-        bool acc = true;
-        if (scaling) acc = u01_32(pz) * bound < T + (t - cs);
-        if (acc) {
-          re++;
-          if (in_win && (uint64_t)pw < thr_lam) n2++; else n0++;
-        }
-      } else if (rr < c2) {
-        re++;
-        if (lemire(pz, nun) < n0) { n0--; n1++; } else { n2--; n3++; }
-      } else {
-        re++;
-        const uint32_t x = lemire(pz, ntot);
-        if (x < n0) n0--;
-        else if (x < n0 + n1) n1--;
-        else if (x < n0 + n1 + n2) n2--;
-        else n3--;
-      }
+      const bool tx = rr < c1;
+      const bool spl = !tx && rr < c2;
+      const bool dec = !tx && !spl;
+      const bool acc = tx && (!scaling || u01_32(pz) * bound < T + (t - cs));
+      const bool lab = in_win && (uint64_t)pw < thr_lam;
+      const uint32_t x = lemire(pz, spl ? nun : ntot);
+      const bool ge0 = x >= n0, ge1 = x >= n0 + n1, ge2 = x >= n0 + n1 + n2;
+      re += (!tx || acc) ? 1ull : 0ull;
+      n0 = n0 + (uint32_t)(acc && !lab) - (uint32_t)(!tx && !ge0);
+      n1 = n1 + (uint32_t)(spl && !ge0) - (uint32_t)(dec && ge0 && !ge1);
+      n2 = n2 + (uint32_t)(acc && lab) - (uint32_t)(spl && ge0) - (uint32_t)(dec && ge1 && !ge2);
+      n3 = n3 + (uint32_t)(spl && ge0) - (uint32_t)(dec && ge2);
       continue;
     }
     if (pend) {
