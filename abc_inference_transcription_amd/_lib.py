@@ -99,6 +99,10 @@ def _declare(L):
         "abc_moment_sim": (I32, [V, I32, I64, V, I64, C.POINTER(AbcMomentCfg), V, V, I32, I32, V,
                                  V, V]),
         "abc_model_probs": (I32, [V, I64, I32, V, U64, I32, D, V, V, V]),
+        "abc_data_stats": (I32, [V, I64, I64, V, V, I32, V, I32, V, V, I64, V, I64, V, I32, U64,
+                                 V]),
+        "abc_data_stats_dev": (I32, [V, I64, I64, V, V, I32, V, I32, V, V, I64, V, I64, V, I32,
+                                     U64, V]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

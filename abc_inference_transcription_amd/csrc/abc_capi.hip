@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -472,7 +472,8 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cu, &c->cl, &c->cb, &c->steps, &c->react, &c->queue,
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
-                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord,
+                    &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
+                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -1253,6 +1254,136 @@ int32_t abc_model_probs(abc_ctx* c, int64_t G, int32_t M, const int64_t* counts,
   HIPCHK(c, hipMemcpyAsync(prob, o, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(lb, o + gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(ub, o + 2 * gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return ABC_OK;
+}
+
+// --------------------------------------------- data-side summary statistics
+
+namespace {
+
+constexpr int64_t kDataMaxCells = 1ll << 21;  // with counts < 2^20 every sum fits 64 bits
+
+// validate the cells' metadata and copy it to the device
+int data_meta(abc_ctx* c, int64_t G, int64_t n, int32_t nc, const int32_t* age, int32_t J,
+              const int32_t* cond, const int32_t* pulse, int64_t np, const int32_t* chase,
+              int64_t nch, const double* w, int32_t n_boot) {
+  if (G < 0 || G > 0x7FFFFFFFll) return fail(c, ABC_E_ARG, "need 0 <= genes < 2^31");
+  if (n < 0 || n > kDataMaxCells) return fail(c, ABC_E_ARG, "need 0 <= cells <= 2^21");
+  if (nc < 1 || nc > ABC_DATA_MAX_CLUSTERS) return fail(c, ABC_E_ARG, "n_clusters must be 1..8");
+  if (J < 1 || J > ABC_DATA_MAX_CONDS) return fail(c, ABC_E_ARG, "n_conds must be 1..16");
+  if (n_boot < 2 || n_boot > (1 << 20)) return fail(c, ABC_E_ARG, "n_boot must be 2..2^20");
+  if (np < 0 || nch < 0 || np > kDataMaxCells || nch > kDataMaxCells)
+    return fail(c, ABC_E_ARG, "pulse and chase lists: 0..2^21 cells");
+  if (n > 0 && (!age || !cond)) return fail(c, ABC_E_ARG, "age and cond are required");
+  if ((np > 0 && !pulse) || (nch > 0 && !chase) || !w)
+    return fail(c, ABC_E_ARG, "pulse, chase and age_id_dist are required");
+  for (int64_t i = 0; i < n; ++i) {
+    if (age[i] < 0 || age[i] >= nc) return fail(c, ABC_E_ARG, "age clusters must lie in 0..n_clusters-1");
+    if (cond[i] < -1 || cond[i] >= J) return fail(c, ABC_E_ARG, "conditions must lie in -1..n_conds-1");
+  }
+  for (int64_t i = 0; i < np; ++i)
+    if (pulse[i] < 0 || pulse[i] >= n) return fail(c, ABC_E_ARG, "pulse cells must lie in 0..n-1");
+  for (int64_t i = 0; i < nch; ++i)
+    if (chase[i] < 0 || chase[i] >= n) return fail(c, ABC_E_ARG, "chase cells must lie in 0..n-1");
+  for (int64_t i = 0; i < (int64_t)nc * J; ++i)
+    if (!std::isfinite(w[i])) return fail(c, ABC_E_ARG, "age_id_dist must be finite");
+  int rc;
+  const size_t ni = (size_t)(2 * n + np + nch);
+  if ((rc = ensure(c, c->dmeta, sizeof(int32_t) * (ni > 0 ? ni : 1)))) return rc;
+  if ((rc = ensure(c, c->dw, sizeof(double) * (size_t)nc * J))) return rc;
+  int32_t* m = P<int32_t>(c->dmeta);
+  if (n > 0) {
+    HIPCHK(c, hipMemcpyAsync(m, age, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(m + n, cond, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  }
+  if (np > 0)
+    HIPCHK(c, hipMemcpyAsync(m + 2 * n, pulse, sizeof(int32_t) * np, hipMemcpyHostToDevice, c->stream));
+  if (nch > 0)
+    HIPCHK(c, hipMemcpyAsync(m + 2 * n + np, chase, sizeof(int32_t) * nch, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->dw.p, w, sizeof(double) * nc * J, hipMemcpyHostToDevice, c->stream));
+  return ABC_OK;
+}
+
+// the kernel over every gene, in launches whose resample scratch stays near 1 GiB
+int data_run(abc_ctx* c, int64_t G, int64_t n, const uint32_t* u, const uint32_t* l, int32_t nc,
+             int32_t J, int64_t np, int64_t nch, int32_t n_boot, uint64_t seed, double* out) {
+  int rc;
+  const int64_t K1 = 4 * nc + 3 * J;
+  const int64_t per_gene = (int64_t)n_boot * K1 * (int64_t)sizeof(double);
+  int64_t chunk = (1ll << 30) / per_gene;
+  if (chunk < 1) chunk = 1;
+  if (chunk > G) chunk = G;
+  if ((rc = ensure(c, c->dboot, (size_t)(chunk * per_gene)))) return rc;
+  if ((rc = ensure(c, c->dflag, sizeof(unsigned long long)))) return rc;
+  unsigned long long* flag = P<unsigned long long>(c->dflag);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, flag, (int64_t)1);
+  const int32_t* m = P<int32_t>(c->dmeta);
+  for (int64_t g0 = 0; g0 < G; g0 += chunk) {
+    const int64_t cnt = G - g0 < chunk ? G - g0 : chunk;
+    hipLaunchKernelGGL(data_stats_kernel, dim3((unsigned)cnt), dim3(ABC_DATA_WG), 0, c->stream, G,
+                       g0, n, u, l, m, m + n, nc, J, m + 2 * n, np, m + 2 * n + np, nch,
+                       P<double>(c->dw), n_boot, (uint32_t)seed,
+                       (uint32_t)(seed >> 32) ^ 0x2545F491u, P<double>(c->dboot), out,
+                       reinterpret_cast<unsigned int*>(flag));
+    HIPCHK(c, hipGetLastError());
+  }
+  unsigned long long bad = 0;
+  HIPCHK(c, hipMemcpyAsync(&bad, flag, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (bad) return fail(c, ABC_E_ARG, "counts must be below 2^20");
+  return ABC_OK;
+}
+
+}  // namespace
+
+int32_t abc_data_stats_dev(abc_ctx* c, int64_t n_genes, int64_t n_cells, const uint32_t* u_dev,
+                           const uint32_t* l_dev, int32_t n_clusters, const int32_t* age,
+                           int32_t n_conds, const int32_t* cond, const int32_t* pulse,
+                           int64_t n_pulse, const int32_t* chase, int64_t n_chase,
+                           const double* age_id_dist, int32_t n_boot, uint64_t seed,
+                           double* out_dev) {
+  if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  int rc = data_meta(c, n_genes, n_cells, n_clusters, age, n_conds, cond, pulse, n_pulse, chase,
+                     n_chase, age_id_dist, n_boot);
+  if (rc) return rc;
+  if (n_genes == 0) return ABC_OK;
+  if ((n_cells > 0 && (!u_dev || !l_dev)) || !out_dev)
+    return fail(c, ABC_E_ARG, "u, l and out are required");
+  return data_run(c, n_genes, n_cells, u_dev, l_dev, n_clusters, n_conds, n_pulse, n_chase, n_boot,
+                  seed, out_dev);
+}
+
+int32_t abc_data_stats(abc_ctx* c, int64_t n_genes, int64_t n_cells, const uint32_t* u,
+                       const uint32_t* l, int32_t n_clusters, const int32_t* age, int32_t n_conds,
+                       const int32_t* cond, const int32_t* pulse, int64_t n_pulse,
+                       const int32_t* chase, int64_t n_chase, const double* age_id_dist,
+                       int32_t n_boot, uint64_t seed, double* out) {
+  if (!c) return ABC_E_ARG;
+  NO_CAPTURE(c);
+  int rc = data_meta(c, n_genes, n_cells, n_clusters, age, n_conds, cond, pulse, n_pulse, chase,
+                     n_chase, age_id_dist, n_boot);
+  if (rc) return rc;
+  if (n_genes == 0) return ABC_OK;
+  if ((n_cells > 0 && (!u || !l)) || !out) return fail(c, ABC_E_ARG, "u, l and out are required");
+  const size_t nu = (size_t)n_genes * (size_t)n_cells;
+  for (size_t i = 0; i < nu; ++i)
+    if ((u[i] | l[i]) >= (1u << 20)) return fail(c, ABC_E_ARG, "counts must be below 2^20");
+  const size_t K = (size_t)(8 * n_clusters + 6 * n_conds);
+  if ((rc = ensure(c, c->du, sizeof(uint32_t) * (nu > 0 ? nu : 1)))) return rc;
+  if ((rc = ensure(c, c->dl, sizeof(uint32_t) * (nu > 0 ? nu : 1)))) return rc;
+  if ((rc = ensure(c, c->dout, sizeof(double) * (size_t)n_genes * K))) return rc;
+  if (nu > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->du.p, u, sizeof(uint32_t) * nu, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->dl.p, l, sizeof(uint32_t) * nu, hipMemcpyHostToDevice, c->stream));
+  }
+  rc = data_run(c, n_genes, n_cells, P<uint32_t>(c->du), P<uint32_t>(c->dl), n_clusters, n_conds,
+                n_pulse, n_chase, n_boot, seed, P<double>(c->dout));
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->dout.p, sizeof(double) * (size_t)n_genes * K,
+                           hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return ABC_OK;
 }

@@ -1,0 +1,245 @@
+// abc_data.hip -- the data-side summary statistics and their bootstrap
+// standard errors (scripts/data_summary_statistics.jl:1-196: get_ccp_data,
+// get_ccp_se, get_ratios, get_ratio_se, get_correlations,
+// get_correlation_se, weighted_cov, get_summary_stats).
+//
+// One workgroup per gene.  From the integer counts u, l of the n cells (age
+// cluster 0..nc-1, condition 0..J-1 or -1) it forms the statistics of the
+// data themselves, then of n_boot resamples with replacement: four
+// independent ones per resample as in the reference -- the pulse cells
+// (mean and Fano factor of u + l per cluster), the chase cells (the same),
+// all cells (labelled ratio per condition) and all cells again (mean_corr,
+// corr_mean per condition, weighted by that resample's cluster shares within
+// the condition; the data's own statistics use the caller's shares).  Draw r
+// of resample b of family f for gene g is Philox4x32-10 with key (seed_lo,
+// seed_hi ^ 0x2545F491) and counter (r / 2, b, g, f): words (0,1) or (2,3)
+// as a 64-bit u, position floor(u * m / 2^64) among the family's m cells.
+//
+// Per set, the cells' sums go to LDS with 64-bit integer atomics (exact and
+// independent of their order); then up to 2 nc + 2 J lanes finish the
+// statistics, one cluster or condition each, in the CPU restatement's
+// floating-point order: variances and covariances (n S_xy - S_x S_y) /
+// (n (n - 1)) from 128-bit integers, NaN for one cell; missing clusters 0;
+// the Fano factor adds 1e-4 to a zero mean; an empty condition has ratio 0.
+// The standard error of each statistic is sqrt((1 / (B - 1)) sum_b (x_b -
+// mean)^2), summed in resample order.  The result therefore equals the CPU
+// replay bit for bit whatever the launch shape.
+#include "abc_device.h"
+#include "abc_kernels.h"
+
+using namespace abcd;
+
+namespace {
+
+// position among m cells from word pair h of a Philox block
+ABC_DEV int64_t pos_of(const u4& w, int h, int64_t m) {
+  const uint64_t u = h == 0 ? ((uint64_t)w.y << 32 | w.x) : ((uint64_t)w.w << 32 | w.z);
+  return (int64_t)__umul64hi(u, (uint64_t)m);
+}
+
+// (n Sxy - Sx Sy) / (n (n - 1)) from exact 128-bit products
+ABC_DEV double cov_int(int64_t n, uint64_t Sx, uint64_t Sy, uint64_t Sxy) {
+  const uint64_t ahi = __umul64hi((uint64_t)n, Sxy), alo = (uint64_t)n * Sxy;
+  const uint64_t bhi = __umul64hi(Sx, Sy), blo = Sx * Sy;
+  const double den = (double)n * (double)(n - 1);
+  const bool ge = ahi > bhi || (ahi == bhi && alo >= blo);
+  if (ge) {
+    const uint64_t lo = alo - blo, hi = ahi - bhi - (alo < blo ? 1ull : 0ull);
+    return u128_to_f64(hi, lo) / den;
+  }
+  const uint64_t lo = blo - alo, hi = bhi - ahi - (blo < alo ? 1ull : 0ull);
+  return -(u128_to_f64(hi, lo) / den);
+}
+
+ABC_DEV double wcov(const double* x, const double* y, const double* w, int n) {
+  double mx = w[0] * x[0], my = w[0] * y[0];
+  for (int i = 1; i < n; ++i) mx = mx + w[i] * x[i];
+  for (int i = 1; i < n; ++i) my = my + w[i] * y[i];
+  double acc = w[0] * ((x[0] - mx) * (y[0] - my));
+  for (int i = 1; i < n; ++i) acc = acc + w[i] * ((x[i] - mx) * (y[i] - my));
+  return acc;
+}
+
+constexpr int MC = ABC_DATA_MAX_CLUSTERS, MJ = ABC_DATA_MAX_CONDS;
+
+struct DataLds {
+  unsigned long long ccp[2][3][MC];      // pulse / chase: cells, S, SS of u + l per cluster
+  unsigned long long rat[3][MJ];         // cells, S_u, S_l per condition
+  unsigned long long cor[6][MJ][MC];     // cells, S_u, S_l, S_uu, S_ll, S_ul per (condition, cluster)
+  double pt[4 * MC + 3 * MJ];            // the data's own statistics
+  unsigned int bad;                      // a count of 2^20 or more was met
+};
+
+}  // namespace
+
+// genes g0 .. g0 + gridDim.x - 1 of G; boot holds this launch's resampled
+// statistics, gridDim.x x n_boot x (4 nc + 3 J)
+__global__ __launch_bounds__(ABC_DATA_WG) void data_stats_kernel(
+    int64_t G, int64_t g0, int64_t n, const uint32_t* __restrict__ u, const uint32_t* __restrict__ l,
+    const int32_t* __restrict__ age, const int32_t* __restrict__ cond, int nc, int J,
+    const int32_t* __restrict__ pulse, int64_t np, const int32_t* __restrict__ chase, int64_t nch,
+    const double* __restrict__ w_point, int n_boot, uint32_t k0, uint32_t k1,
+    double* __restrict__ boot, double* __restrict__ out, unsigned int* __restrict__ bad_out) {
+  __shared__ DataLds S;
+  const int64_t g = g0 + blockIdx.x;
+  const int tid = threadIdx.x;
+  const uint32_t* ug = u + g * n;
+  const uint32_t* lg = l + g * n;
+  const int K1 = 4 * nc + 3 * J;
+  double* bg = boot + (size_t)blockIdx.x * (size_t)n_boot * (size_t)K1;
+  if (tid == 0) S.bad = 0u;
+  for (int b = -1; b < n_boot; ++b) {
+    unsigned long long* z = &S.ccp[0][0][0];
+    for (int i = tid; i < 2 * 3 * MC + 3 * MJ + 6 * MJ * MC; i += ABC_DATA_WG) {
+      if (i < 2 * 3 * MC) z[i] = 0ull;
+      else if (i < 2 * 3 * MC + 3 * MJ) (&S.rat[0][0])[i - 2 * 3 * MC] = 0ull;
+      else (&S.cor[0][0][0])[i - 2 * 3 * MC - 3 * MJ] = 0ull;
+    }
+    __syncthreads();
+    // pulse and chase cells: u + l per age cluster
+    for (int f = 0; f < 2; ++f) {
+      const int32_t* list = f == 0 ? pulse : chase;
+      const int64_t m = f == 0 ? np : nch;
+      for (int64_t q = tid; 2 * q < m; q += ABC_DATA_WG) {
+        u4 w{0, 0, 0, 0};
+        if (b >= 0) w = philox((uint32_t)q, (uint32_t)b, (uint32_t)g, (uint32_t)f, k0, k1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t r = 2 * q + h;
+          if (r >= m) break;
+          const int64_t pos = b < 0 ? r : pos_of(w, h, m);
+          const int32_t i = list[pos];
+          if (i < 0 || i >= n) continue;
+          const int c = age[i];
+          if (c < 0 || c >= nc) continue;
+          const uint32_t x0 = ug[i], x1 = lg[i];
+          if ((x0 | x1) >= (1u << 20)) { atomicOr(&S.bad, 1u); continue; }
+          const unsigned long long x = (unsigned long long)x0 + x1;
+          atomicAdd(&S.ccp[f][0][c], 1ull);
+          atomicAdd(&S.ccp[f][1][c], x);
+          atomicAdd(&S.ccp[f][2][c], x * x);
+        }
+      }
+    }
+    // all cells, twice: ratios (f = 2) and correlations (f = 3)
+    for (int f = 2; f < 4; ++f) {
+      for (int64_t q = tid; 2 * q < n; q += ABC_DATA_WG) {
+        u4 w{0, 0, 0, 0};
+        if (b >= 0) w = philox((uint32_t)q, (uint32_t)b, (uint32_t)g, (uint32_t)f, k0, k1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t r = 2 * q + h;
+          if (r >= n) break;
+          const int64_t i = b < 0 ? r : pos_of(w, h, n);
+          const int j = cond[i];
+          if (j < 0 || j >= J) continue;
+          const uint32_t x0 = ug[i], x1 = lg[i];
+          if ((x0 | x1) >= (1u << 20)) { atomicOr(&S.bad, 1u); continue; }
+          if (f == 2) {
+            atomicAdd(&S.rat[0][j], 1ull);
+            atomicAdd(&S.rat[1][j], (unsigned long long)x0);
+            atomicAdd(&S.rat[2][j], (unsigned long long)x1);
+          } else {
+            const int c = age[i];
+            if (c < 0 || c >= nc) continue;
+            const unsigned long long x = x0, y = x1;
+            atomicAdd(&S.cor[0][j][c], 1ull);
+            atomicAdd(&S.cor[1][j][c], x);
+            atomicAdd(&S.cor[2][j][c], y);
+            atomicAdd(&S.cor[3][j][c], x * x);
+            atomicAdd(&S.cor[4][j][c], y * y);
+            atomicAdd(&S.cor[5][j][c], x * y);
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // one lane per cluster (pulse, chase) or condition (ratio, correlations)
+    double* st = b < 0 ? S.pt : bg + (size_t)b * K1;
+    if (tid < 2 * nc) {
+      const int f = tid / nc, c = tid - f * nc;
+      const int64_t cnt = (int64_t)S.ccp[f][0][c];
+      double mean = 0.0, ff = 0.0;
+      if (cnt > 0) {
+        const uint64_t s1 = S.ccp[f][1][c], s2 = S.ccp[f][2][c];
+        mean = (double)s1 / (double)cnt;
+        const double v = cov_int(cnt, s1, s1, s2);
+        const double eps = mean == 0.0 ? 0.0001 : 0.0;
+        ff = v / (mean + eps);
+      }
+      st[2 * f * nc + c] = mean;
+      st[(2 * f + 1) * nc + c] = ff;
+    } else if (tid < 2 * nc + J) {
+      const int j = tid - 2 * nc;
+      const int64_t cnt = (int64_t)S.rat[0][j];
+      double r = 0.0;
+      if (cnt > 0) {
+        const double mu = (double)S.rat[1][j] / (double)cnt, ml = (double)S.rat[2][j] / (double)cnt;
+        r = mu + ml > 0.0 ? ml / (mu + ml) : 0.0;
+      }
+      st[4 * nc + j] = r;
+    } else if (tid < 2 * nc + 2 * J) {
+      const int j = tid - 2 * nc - J;
+      double m1[MC], m2[MC], v1[MC], v2[MC], c12[MC], w[MC];
+      bool any_cov = false;
+      int64_t tot = 0;
+      for (int c = 0; c < nc; ++c) tot += (int64_t)S.cor[0][j][c];
+      for (int c = 0; c < nc; ++c) {
+        const int64_t cnt = (int64_t)S.cor[0][j][c];
+        if (cnt == 0) {
+          m1[c] = m2[c] = v1[c] = v2[c] = c12[c] = 0.0;
+        } else {
+          const uint64_t su = S.cor[1][j][c], sl = S.cor[2][j][c];
+          m1[c] = (double)su / (double)cnt;
+          m2[c] = (double)sl / (double)cnt;
+          v1[c] = cov_int(cnt, su, su, S.cor[3][j][c]);
+          v2[c] = cov_int(cnt, sl, sl, S.cor[4][j][c]);
+          c12[c] = cov_int(cnt, su, sl, S.cor[5][j][c]);
+        }
+        if (!(c12[c] == 0.0)) any_cov = true;
+        if (b < 0) w[c] = w_point[c + (int64_t)nc * j];
+        else w[c] = tot > 0 ? (double)cnt / (double)tot : 0.0;
+      }
+      double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
+      for (int c = 1; c < nc; ++c) {
+        s1 = s1 + w[c] * v1[c];
+        s2 = s2 + w[c] * v2[c];
+        s12 = s12 + w[c] * c12[c];
+      }
+      const double tv1 = s1 + wcov(m1, m1, w, nc), tv2 = s2 + wcov(m2, m2, w, nc);
+      const bool nz = !(tv1 == 0.0) && !(tv2 == 0.0);
+      st[4 * nc + J + j] = (any_cov && nz) ? s12 / sqrt(fabs(tv1 * tv2)) : 0.0;
+      st[4 * nc + 2 * J + j] = nz ? wcov(m1, m2, w, nc) / sqrt(fabs(tv1 * tv2)) : 0.0;
+    }
+    __syncthreads();
+  }
+  // standard errors over the resamples; output columns in the order of the
+  // 14 summary files: pulse_mean, pulse_ff, pulse_mean_se, pulse_ff_se,
+  // chase_* (nc each), ratio, ratio_se, mean_corr, mean_corr_se, corr_mean,
+  // corr_mean_se (J each)
+  for (int k = tid; k < K1; k += ABC_DATA_WG) {
+    double s = bg[k];
+    for (int b = 1; b < n_boot; ++b) s = s + bg[(size_t)b * K1 + k];
+    const double m = s / (double)n_boot;
+    double q = (bg[k] - m) * (bg[k] - m);
+    for (int b = 1; b < n_boot; ++b) {
+      const double d = bg[(size_t)b * K1 + k] - m;
+      q = q + d * d;
+    }
+    const double se = sqrt((1.0 / (double)(n_boot - 1)) * q);
+    int colv, cols;
+    if (k < 4 * nc) {  // pulse / chase block f, mean or FF
+      const int blk = k / nc, c = k - blk * nc;  // blk 0 p-mean, 1 p-ff, 2 c-mean, 3 c-ff
+      const int f = blk / 2, kind = blk & 1;
+      colv = 4 * nc * f + kind * nc + c;
+      cols = colv + 2 * nc;
+    } else {  // ratio, mean_corr, corr_mean
+      const int kk = k - 4 * nc, qd = kk / J, j = kk - qd * J;
+      colv = 8 * nc + 2 * J * qd + j;
+      cols = colv + J;
+    }
+    out[g + G * (int64_t)colv] = S.pt[k];
+    out[g + G * (int64_t)cols] = se;
+  }
+  if (tid == 0 && S.bad) atomicOr(bad_out, 1u);
+}

@@ -146,6 +146,16 @@ __global__ void moment_traj_kernel(MomCfgDev cfg, int model, int64_t n,
                                    unsigned long long* steps);
 __global__ void moment_stats_kernel(MomCfgDev cfg, int64_t n, double* mom,
                                     double* stats);
+// Data-side summary statistics with bootstrap SEs (scripts/data_summary_statistics.jl)
+#define ABC_DATA_WG 256
+#define ABC_DATA_MAX_CLUSTERS 8
+#define ABC_DATA_MAX_CONDS 16
+__global__ void data_stats_kernel(int64_t G, int64_t g0, int64_t n, const uint32_t* u,
+                                  const uint32_t* l,
+                                  const int32_t* age, const int32_t* cond, int nc, int J,
+                                  const int32_t* pulse, int64_t np, const int32_t* chase,
+                                  int64_t nch, const double* w_point, int n_boot, uint32_t k0,
+                                  uint32_t k1, double* boot, double* out, unsigned int* bad_out);
 // Model probabilities with bootstrap bounds (scripts/model_probs.jl)
 #define ABC_SEL_WG 256
 #define ABC_SEL_MAX_GROUPS 8

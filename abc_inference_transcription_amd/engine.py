@@ -325,6 +325,31 @@ class Engine:
                                          _p(lb), _p(ub)), "model_probs")
         return prob, lb, ub
 
+    def data_stats(self, u, l, age, cond, n_clusters, n_conds, pulse, chase, age_id_dist,
+                   n_boot: int = 100, seed: int = 0):
+        """get_summary_stats for every gene (scripts/data_summary_statistics.jl:
+        1-196): u, l (cells x genes) integer counts, age 0..n_clusters-1,
+        cond 0..n_conds-1 or -1, pulse / chase 0-based cell lists,
+        age_id_dist (n_clusters x n_conds).  Returns the genes x (8 n_clusters
+        + 6 n_conds) matrix whose column blocks are the 14 summary files."""
+        u = np.asfortranarray(u, np.uint32)
+        l = np.asfortranarray(l, np.uint32)
+        n, G = u.shape
+        if l.shape != u.shape:
+            raise ValueError("u and l must have the same shape")
+        age = np.ascontiguousarray(age, np.int32)
+        cond = np.ascontiguousarray(cond, np.int32)
+        pulse = np.ascontiguousarray(pulse, np.int32)
+        chase = np.ascontiguousarray(chase, np.int32)
+        w = np.asfortranarray(age_id_dist, np.float64)
+        if len(age) != n or len(cond) != n or w.shape != (n_clusters, n_conds):
+            raise ValueError("age, cond and age_id_dist do not match the cells and groups")
+        out = np.zeros((G, 8 * n_clusters + 6 * n_conds), order="F")
+        self._chk(self.L.abc_data_stats(self.ctx, G, n, _p(u), _p(l), n_clusters, _p(age), n_conds,
+                                        _p(cond), _p(pulse), len(pulse), _p(chase), len(chase),
+                                        _p(w), n_boot, seed, _p(out)), "data_stats")
+        return out
+
     def drop_graph(self, graph):
         if graph:
             self.L.abc_graph_destroy(C.c_void_p(graph))

@@ -152,6 +152,40 @@ SUMMARY_FILES = ["pulse_mean", "pulse_ff", "pulse_mean_se", "pulse_ff_se", "chas
                  "mean_corr_data", "mean_corr_se", "corr_mean_data", "corr_mean_se"]
 
 
+def summary_statistics(eng, u_data, l_data, n_clusters: int, age, experiment, cond_vec,
+                       pulse_idx, chase_idx, age_id_distribution, n_boot: int = 100,
+                       seed: int = 0, out_dir: str | None = "data/summary_stats"):
+    """scripts/data_summary_statistics.jl:183-241: get_summary_stats for every
+    gene of the data, in the reference's conventions: u_data, l_data cells x
+    genes counts; age the 1-based age cluster of each cell; experiment its
+    labelling experiment id and cond_vec the ids of the conditions, in order;
+    pulse_idx, chase_idx 1-based cells; age_id_distribution one vector of
+    cluster weights per condition.  The 14 statistics (genes x clusters or
+    genes x conditions) are appended one row per gene to the files of
+    SUMMARY_FILES under out_dir (none when out_dir is None) and returned as a
+    dict.  The bootstrap resamples come from Philox streams keyed by seed."""
+    u = np.asarray(u_data)
+    l = np.asarray(l_data)
+    if np.any(u != np.round(u)) or np.any(l != np.round(l)) or u.min(initial=0) < 0 or l.min(initial=0) < 0:
+        raise ValueError("u_data and l_data must hold non-negative integer counts")
+    cond_vec = list(cond_vec)
+    pos = {e: j for j, e in enumerate(cond_vec)}
+    cond = np.array([pos.get(e, -1) for e in np.asarray(experiment).tolist()], np.int32)
+    w = np.column_stack([np.asarray(v, np.float64) for v in age_id_distribution])
+    out = eng.data_stats(u.astype(np.uint32), l.astype(np.uint32), np.asarray(age, np.int64) - 1,
+                         cond, n_clusters, len(cond_vec), np.asarray(pulse_idx, np.int64) - 1,
+                         np.asarray(chase_idx, np.int64) - 1, w, n_boot=n_boot, seed=seed)
+    widths = [n_clusters] * 8 + [len(cond_vec)] * 6
+    res, c0 = {}, 0
+    for name, wd in zip(SUMMARY_FILES, widths):
+        res[name] = np.ascontiguousarray(out[:, c0:c0 + wd])
+        c0 += wd
+    if out_dir is not None:
+        for name in SUMMARY_FILES:
+            _append(os.path.join(out_dir, name + ".txt"), _rows_text(res[name]))
+    return res
+
+
 def load_summary_stats(path: str = "data/summary_stats/", ext: str = ".txt") -> dict:
     """The 14 data matrices wrapper.jl:42 unpacks (genes x ages or genes x
     conditions)."""

@@ -277,6 +277,46 @@ int32_t abc_moment_sim(abc_ctx* ctx, int32_t model, int64_t n, const double* the
 int32_t abc_model_probs(abc_ctx* ctx, int64_t G, int32_t M, const int64_t* counts, uint64_t seed,
                         int32_t n_boot, double alpha, double* prob, double* lb, double* ub);
 
+/* Replaces get_summary_stats and the functions it calls
+ * (scripts/data_summary_statistics.jl:1-196: get_ccp_data, get_ccp_se,
+ * get_ratios, get_ratio_se, get_correlations, get_correlation_se): the data's
+ * summary statistics and their bootstrap standard errors, for every gene.
+ *   u, l         n_cells x n_genes, gene-major (column g = gene g), integer
+ *                counts < 2^20 (the reference's u_data, l_data);
+ *   age          n_cells age clusters, 0-based (age .- 1), < n_clusters <= 8;
+ *   cond         n_cells condition of each cell, 0..n_conds-1 (the position
+ *                of experiment[i] in cond_vec), or -1 for none; n_conds <= 16;
+ *   pulse, chase 0-based cells of pulse_idx and chase_idx;
+ *   age_id_dist  n_clusters x n_conds column-major: the weights of the data's
+ *                own correlations (the reference's age_id_distribution);
+ *   n_boot       resamples per gene (the reference: 100), seed their Philox
+ *                streams (the reference's Julia RNG is unseeded);
+ *   out          n_genes x (8 n_clusters + 6 n_conds) column-major, column
+ *                blocks in the order of the 14 summary files: pulse_mean,
+ *                pulse_ff, pulse_mean_se, pulse_ff_se, chase_mean, chase_ff,
+ *                chase_mean_se, chase_ff_se (n_clusters columns each),
+ *                ratio_data, ratio_se, mean_corr_data, mean_corr_se,
+ *                corr_mean_data, corr_mean_se (n_conds columns each).
+ * As in the reference, each resample draws four independent samples of
+ * cells: the pulse cells, the chase cells, all cells for the ratios and all
+ * cells for the correlations, whose weights are that sample's age-cluster
+ * shares within each condition.  Variances use n - 1 and are NaN for one
+ * cell; a missing cluster gives 0.  n_cells <= 2^21.  abc_data_stats takes
+ * host arrays; abc_data_stats_dev takes u, l and out in device memory (the
+ * metadata stays on the host) and returns ABC_E_ARG if a count is 2^20 or
+ * more.  Both wait for the result. */
+int32_t abc_data_stats(abc_ctx* ctx, int64_t n_genes, int64_t n_cells, const uint32_t* u,
+                       const uint32_t* l, int32_t n_clusters, const int32_t* age, int32_t n_conds,
+                       const int32_t* cond, const int32_t* pulse, int64_t n_pulse,
+                       const int32_t* chase, int64_t n_chase, const double* age_id_dist,
+                       int32_t n_boot, uint64_t seed, double* out);
+int32_t abc_data_stats_dev(abc_ctx* ctx, int64_t n_genes, int64_t n_cells, const uint32_t* u_dev,
+                           const uint32_t* l_dev, int32_t n_clusters, const int32_t* age,
+                           int32_t n_conds, const int32_t* cond, const int32_t* pulse,
+                           int64_t n_pulse, const int32_t* chase, int64_t n_chase,
+                           const double* age_id_dist, int32_t n_boot, uint64_t seed,
+                           double* out_dev);
+
 /* ---- surrogate (no reference counterpart: the reference simulates every
  * prior particle, scripts/abc_simulation.jl:88-97).  An MLP maps a
  * particle's features to the mean and log-variance of each statistic (in the

@@ -145,6 +145,20 @@ int orc_moment_sim(int model, int64_t n, const double* theta, int64_t ld,
 /* get_model_probs (model_probs.jl:1-29) per gene: counts is G x M col-major
  * (accepted particles per model group); prob, lb, ub are G x M col-major;
  * boot (optional) receives the resampled counts, [gene][resample][group]. */
+/* Data-side summary statistics with bootstrap SEs (data_summary_statistics.jl):
+ * u, l n x G (gene-major) integer counts <= 2^20, n <= 2^21; age 0..nc-1; cond
+ * 0..J-1 or -1; pulse / chase: 0-based cell lists; w_point nc x J.  out:
+ * G x (8 nc + 6 J) column-major in the order of the 14 summary files; boot
+ * (optional): G x n_boot x (4 nc + 3 J) resampled statistics. */
+#define ORC_DATA_MAX_CLUSTERS 8
+#define ORC_DATA_MAX_CONDS 16
+int orc_data_stats(int64_t G, int64_t n, const uint32_t* u, const uint32_t* l,
+                   const int32_t* age, const int32_t* cond, int nc, int J,
+                   const int32_t* pulse, int64_t np, const int32_t* chase, int64_t nch,
+                   const double* w_point, int n_boot, uint64_t seed, double* out, double* boot,
+                   int n_threads);
+/* the positions (0..m-1) drawn by resample b of family f for gene g */
+void orc_data_resample(int64_t g, int b, int f, int64_t m, uint64_t seed, int64_t* pos);
 int orc_model_probs(int64_t G, int M, const int64_t* counts, uint64_t seed, int n_boot,
                     double alpha, double* prob, double* lb, double* ub, int64_t* boot);
 /* Julia's quantile(v, p) of a sorted vector (type 7) */

@@ -98,6 +98,11 @@ def _declare(L):
     L.orc_model_probs.restype = C.c_int
     L.orc_jl_quantile.argtypes = [P, C.c_int, D]
     L.orc_jl_quantile.restype = D
+    L.orc_data_stats.argtypes = [I64, I64, P, P, P, P, C.c_int, C.c_int, P, I64, P, I64, P,
+                                 C.c_int, U64, P, P, C.c_int]
+    L.orc_data_stats.restype = C.c_int
+    L.orc_data_resample.argtypes = [I64, C.c_int, C.c_int, I64, U64, P]
+    L.orc_data_resample.restype = None
 
 
 def _p(a):
@@ -317,6 +322,34 @@ def model_probs(counts, seed=0, n_boot=100, alpha=0.95, boot=False):
     if rc:
         raise RuntimeError(f"orc_model_probs failed: {rc}")
     return (prob, lb, ub, bt) if boot else (prob, lb, ub)
+
+
+def data_stats(u, l, age, cond, nc, J, pulse, chase, w_point, n_boot=100, seed=0,
+               boot=False, n_threads=0):
+    """get_summary_stats for every gene: u, l (n cells x G genes) integer
+    counts, age 0..nc-1, cond 0..J-1 or -1, pulse / chase 0-based cell
+    lists, w_point (nc x J).  Returns the G x (8 nc + 6 J) matrix whose column
+    blocks are the 14 summary files in order (and the resampled statistics,
+    G x n_boot x (4 nc + 3 J), if asked)."""
+    u = np.asfortranarray(u, np.uint32); l = np.asfortranarray(l, np.uint32)
+    n, G = u.shape
+    age = np.ascontiguousarray(age, np.int32); cond = np.ascontiguousarray(cond, np.int32)
+    pulse = np.ascontiguousarray(pulse, np.int32); chase = np.ascontiguousarray(chase, np.int32)
+    w = np.asfortranarray(w_point, np.float64)
+    out = np.zeros((G, 8 * nc + 6 * J), order="F")
+    bt = np.zeros((G, n_boot, 4 * nc + 3 * J)) if boot else None
+    rc = lib().orc_data_stats(G, n, _p(u), _p(l), _p(age), _p(cond), nc, J, _p(pulse), len(pulse),
+                              _p(chase), len(chase), _p(w), n_boot, seed, _p(out), _p(bt),
+                              n_threads or (os.cpu_count() or 1))
+    if rc:
+        raise RuntimeError(f"orc_data_stats failed: {rc}")
+    return (out, bt) if boot else out
+
+
+def data_resample(g, b, f, m, seed=0):
+    pos = np.zeros(m, np.int64)
+    lib().orc_data_resample(g, b, f, m, seed, _p(pos))
+    return pos
 
 
 def jl_quantile(sorted_v, p):

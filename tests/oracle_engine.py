@@ -28,3 +28,8 @@ class OracleEngine:
         assert mode == 0
         err = self.errors(stats, data, se, parts, clamp)
         return [self.O.accept_gene(err[:, g], eps) for g in range(err.shape[1])]
+
+    def data_stats(self, u, l, age, cond, n_clusters, n_conds, pulse, chase, age_id_dist,
+                   n_boot=100, seed=0):
+        return self.O.data_stats(u, l, age, cond, n_clusters, n_conds, pulse, chase, age_id_dist,
+                                 n_boot=n_boot, seed=seed)

@@ -149,3 +149,39 @@ def test_columnar_error_files_and_acceptance(oracle, small_design, golden_summar
     with pytest.raises(ValueError):
         host.accepted_from_columns({"const": cols["const"], "bad": bad}, eps=eps,
                                    out_dir=str(tmp_path / "x"))
+
+
+def test_summary_statistics_files(oracle, small_design, tmp_path):
+    """host.summary_statistics in the reference's conventions (1-based ages and
+    cells, experiment ids with cond_vec, one weight vector per condition):
+    one row per gene in each of the 14 files of data_summary_statistics.jl,
+    with the widths the reference's summary_stats files have (clusters or
+    conditions), holding the engine's values."""
+    from oracle_engine import OracleEngine
+    eng = OracleEngine(oracle, small_design)
+    rng = np.random.default_rng(4)
+    n, G, nc = 240, 3, 5
+    ids = list(range(0, 11))                 # experiment ids 0..10; 11 marks a cell in no condition
+    experiment = rng.integers(0, 12, n)
+    age = rng.integers(1, nc + 1, n)
+    u = rng.poisson(3.0, (n, G)).astype(float)
+    l = rng.poisson(1.0, (n, G)).astype(float)
+    pulse_idx = [i + 1 for i in range(n) if 0 <= experiment[i] <= 5]
+    chase_idx = [i + 1 for i in range(n) if 6 <= experiment[i] <= 10]
+    aid = [np.full(nc, 1.0 / nc) for _ in ids]
+    res = host.summary_statistics(eng, u, l, nc, age, experiment, ids, pulse_idx, chase_idx, aid,
+                                  n_boot=8, out_dir=str(tmp_path / "ss"))
+    cond = np.where(experiment < 11, experiment, -1)
+    want = oracle.data_stats(u.astype(np.uint32), l.astype(np.uint32), age - 1, cond, nc, 11,
+                             np.array(pulse_idx) - 1, np.array(chase_idx) - 1,
+                             np.full((nc, 11), 1.0 / nc), n_boot=8)
+    back = host.load_summary_stats(str(tmp_path / "ss") + "/")
+    c0 = 0
+    for name, wd in zip(host.SUMMARY_FILES, [nc] * 8 + [11] * 6):
+        assert res[name].shape == (G, wd) and back[name].shape == (G, wd), name
+        np.testing.assert_array_equal(res[name], want[:, c0:c0 + wd])
+        np.testing.assert_array_equal(back[name], res[name])
+        c0 += wd
+    with pytest.raises(ValueError):
+        host.summary_statistics(eng, u + 0.5, l, nc, age, experiment, ids, pulse_idx, chase_idx,
+                                aid, n_boot=8, out_dir=None)
