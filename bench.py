@@ -260,6 +260,7 @@ def main():
     if not a.no_moments:
         out["reference_moment_path"] = moment_leg(stream, a)
     out["model_probs"] = model_probs_leg(eng, stream, a)
+    out["data_summary_stats"] = data_stats_leg(eng, stream, a)
     print(json.dumps(out), flush=True)
     eng.close()
 
@@ -381,6 +382,73 @@ def model_probs_leg(eng, stream, a):
         res["cpu"] = {"draws_per_s": cd / cpu_s, "cores": 1, "kind": "port",
                       "sample": f"20 genes, {cd} draws in {cpu_s:.2f}s"}
         res["gpu_vs_cpu"] = round((draws / gpu_s) / (cd / cpu_s), 1)
+    return res
+
+
+DATA_OPS_PER_DRAW = 60  # half a Philox4x32-10 block, a 64 x 64 multiply-high, checks, LDS atomics
+
+
+def data_stats_leg(eng, stream, a):
+    """get_summary_stats with bootstrap SEs (scripts/data_summary_statistics.jl)
+    at the reference data's size: 3,419 genes, 5,422 cells, 5 age clusters,
+    11 conditions, 100 resamples; synthetic negative-binomial counts (the
+    raw data are not shipped), counts already in device memory.  One unit =
+    one resampled cell (each resample draws the pulse cells, the chase cells
+    and all cells twice).  The oracle's threaded C restatement runs on the
+    first 256 genes."""
+    import torch
+    from abc_inference_transcription_amd.roofline import VALU_PEAK_TOPS
+    rng = np.random.default_rng(0)
+    G, n, nc, J, B = 3419, 5422, 5, 11, 100
+    age = rng.integers(0, nc, n).astype(np.int32)
+    cond = rng.integers(-1, J, n).astype(np.int32)
+    u = rng.negative_binomial(2, rng.uniform(0.05, 0.8, G), (n, G)).astype(np.uint32)
+    l = rng.negative_binomial(1, rng.uniform(0.2, 0.95, G), (n, G)).astype(np.uint32)
+    pulse = np.nonzero((cond >= 0) & (cond <= 5))[0].astype(np.int32)
+    chase = np.nonzero(cond >= 6)[0].astype(np.int32)
+    w = np.asfortranarray([[np.mean(age[cond == j] == c) for j in range(J)] for c in range(nc)])
+    dev = torch.device("cuda:0")
+    ut = torch.from_numpy(np.ascontiguousarray(u.T).view(np.int32)).to(dev)
+    lt = torch.from_numpy(np.ascontiguousarray(l.T).view(np.int32)).to(dev)
+    out = torch.empty(G * (8 * nc + 6 * J), dtype=torch.float64, device=dev)
+    vp = lambda x: C.c_void_p(x.ctypes.data)
+
+    def run(ng):
+        from abc_inference_transcription_amd import _lib
+        _lib.check(eng.L.abc_data_stats_dev(eng.ctx, ng, n, C.c_void_p(ut.data_ptr()),
+                                            C.c_void_p(lt.data_ptr()), nc, vp(age), J, vp(cond),
+                                            vp(pulse), len(pulse), vp(chase), len(chase), vp(w),
+                                            B, 0, C.c_void_p(out.data_ptr())), eng.ctx,
+                   "data_stats")
+    run(8)  # warm-up
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    run(G)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    draws = G * B * (len(pulse) + len(chase) + 2 * n)
+    tops = draws * DATA_OPS_PER_DRAW / (ms / 1e3) / 1e12
+    res = {"genes": G, "cells": n, "resamples": B, "value": round(G / (ms / 1e3), 1),
+           "unit": "genes/s", "ms": round(ms, 3), "resampled_cells": draws,
+           "roofline": {"bound": "valu", "achieved": round(tops, 3), "peak": VALU_PEAK_TOPS,
+                        "unit": "Top/s", "frac": round(tops / VALU_PEAK_TOPS, 4),
+                        "ops_per_resampled_cell": DATA_OPS_PER_DRAW}}
+    print(f"[bench] data summary statistics: {res['value']:.1f} genes/s", file=sys.stderr,
+          flush=True)
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        threads = min(os.cpu_count() or 1, 16)
+        k = 256
+        t = time.perf_counter()
+        O.data_stats(u[:, :k], l[:, :k], age, cond, nc, J, pulse, chase, w, n_boot=B,
+                     n_threads=threads)
+        cpu_s = time.perf_counter() - t
+        res["cpu"] = {"value": round(k / cpu_s, 3), "unit": "genes/s", "cores": threads,
+                      "kind": "port", "sample": f"{k} genes in {cpu_s:.2f}s on {threads} threads"}
+        res["gpu_vs_cpu"] = round(res["value"] / res["cpu"]["value"], 1)
     return res
 
 
