@@ -5,9 +5,10 @@ reads and writes almost nothing).  Its algorithmic unit is one SSA step (one
 Philox4x32-10 block spent on an event proposal, a boundary crossing or 128
 division coin flips).  Lane-operations per event step, counted from the
 source of the step (each IEEE op or fused multiply-add, integer op, 32x32->64
-product, conversion, compare or table load = 1):
+product, three-input xor, conversion, compare or table load = 1; gfx950 does a
+three-input xor in one v_bitop3_b32):
 
-    Philox4x32-10: 10 rounds x (2 products + 4 xor + 2 key adds)             80
+    Philox4x32-10: 10 rounds x (2 products + 2 three-input xor + 2 key adds) 60
     counter update and packing                                                4
     uniform for the waiting time (cvt, add, mul)                              3
     log (index/exponent/reduction bits 8, table load 1, r and r^2 2,
@@ -17,13 +18,13 @@ product, conversion, compare or table load = 1):
     channel uniform and selection compares                                    6
     state update                                                              4
                                                                       ------
-                                                                        139
+                                                                        119
 
 The peak is one VALU lane-op per lane per clock: 256 CUs x 64 lanes x
 2.4 GHz = 39.3 Top/s (half the published 78.6 TFLOP/s FP64 vector figure,
 which counts an FMA as two flops).
 """
-SSA_OPS_PER_STEP = 139
+SSA_OPS_PER_STEP = 119
 VALU_PEAK_TOPS = 39.3
 
 # Moment model (moment_transient_kernel, moment_traj_kernel): FP64 operations
