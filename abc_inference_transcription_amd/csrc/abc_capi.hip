@@ -133,10 +133,11 @@ int ensure(abc_ctx* c, DevBuf& b, size_t bytes) {
 template <typename T>
 T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
 
+// Philox key (seed_lo, seed_hi), the same for every model and particle; the
+// model goes to the counter's cell word (cell | model << 24; cells < 2^24)
 uint32_t key0(uint64_t seed) { return (uint32_t)seed; }
-uint32_t key1(uint64_t seed, int model) {
-  return (uint32_t)(seed >> 32) ^ (0x9E3779B9u * (uint32_t)model);
-}
+uint32_t key1(uint64_t seed) { return (uint32_t)(seed >> 32); }
+uint32_t mtag(int model) { return (uint32_t)model << 24; }
 
 int check_range(abc_ctx* c, int model, int64_t first, int64_t n) {
   if (model < 1 || model > 5) return fail(c, ABC_E_MODEL, "model must be 1..5");
@@ -226,10 +227,10 @@ int drain_ring(abc_ctx* c) {
 
 // cells of n particles (rates and queue order staged): burn-in, sub-lane
 // offsets, the window SSA.  Particle p is global particle pidx[p] (first + p
-// without pidx) with key word k1s[p] (k1 without k1s).  Cell counts go to
+// without pidx) with model tag mtags[p] (mtag_all without mtags).  Cell counts go to
 // cu / cl / cb, Philox draws per particle to steps.
-int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t k1,
-                  const uint32_t* k1s, const int64_t* pidx_dev, unsigned long long* steps) {
+int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t mtag_all,
+                  const uint32_t* mtags, const int64_t* pidx_dev, unsigned long long* steps) {
   int rc;
   const size_t cells = (size_t)n * (size_t)c->hd.C;
   const int64_t cell_words = (int64_t)((cells + 1) / 2);  // u32 counts cleared as u64 words
@@ -270,7 +271,8 @@ int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t 
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(burn_kernel, dim3(c->burn_blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
-                     k1, k1s, c->d_betas, c->d_goff, P<unsigned long long>(c->queue),
+                     key1(seed), mtag_all, mtags, c->d_betas, c->d_goff,
+                     P<unsigned long long>(c->queue),
                      P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev,
                      P<unsigned long long>(c->tally));
   HIPCHK(c, hipGetLastError());
@@ -278,7 +280,8 @@ int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t 
   if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
                      P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
-                     (uint32_t)first, key0(seed), k1, k1s, P<CellStart>(c->cellrec),
+                     (uint32_t)first, key0(seed), key1(seed), mtag_all, mtags,
+                     P<CellStart>(c->cellrec),
                      P<int32_t>(c->coff), P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu),
                      P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react),
                      pidx_dev);
@@ -302,7 +305,7 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
     if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
     steps = P<unsigned long long>(c->steps);
   }
-  if ((rc = simulate_core(c, seed, n, first, key1(seed, model), nullptr, pidx_dev, steps)))
+  if ((rc = simulate_core(c, seed, n, first, mtag(model), nullptr, pidx_dev, steps)))
     return rc;
   hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
                      P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
@@ -539,7 +542,7 @@ int32_t abc_sample_prior_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t f
   if (rc) return rc;
   if (n == 0) return ABC_OK;
   hipLaunchKernelGGL(prior_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model,
-                     key0(seed), key1(seed, model), (uint32_t)first, n, theta_dev);
+                     key0(seed), key1(seed), (uint32_t)first, n, theta_dev);
   HIPCHK(c, hipGetLastError());
   return ABC_OK;
 }
@@ -596,7 +599,7 @@ int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* mode
     if (n[m] == 0) continue;
     launch_rates(c, models[m], n[m], theta_dev[m], a[m]);
     hipLaunchKernelGGL(fill_ids_kernel, dim3(grid1(n[m], 256)), dim3(256), 0, c->stream, n[m],
-                       first[m], key1(seed, models[m]), P<int64_t>(c->mpidx) + a[m],
+                       first[m], mtag(models[m]), P<int64_t>(c->mpidx) + a[m],
                        P<uint32_t>(c->mk1) + a[m]);
   }
   HIPCHK(c, hipGetLastError());

@@ -2,8 +2,9 @@
 // the gfx950 kernels.
 //
 // Philox4x32-10 (Salmon et al., SC'11): one independent stream per
-// (particle, cell), counter = (draw_lo, draw_hi, cell, particle), key =
-// (seed_lo, seed_hi ^ model * 0x9E3779B9).  The reference draws from Julia's
+// (particle, cell), counter = (draw_lo, draw_hi, cell | model << 24,
+// particle), key = (seed_lo, seed_hi), the same in every lane of a launch so
+// that the round keys stay scalar.  The reference draws from Julia's
 // unseeded global RNG (scripts/abc_simulation.jl:4-9); counter-based streams
 // make every particle reproducible and independent of the batch split.
 //

@@ -85,21 +85,22 @@ __global__ void items_scan_kernel(const int64_t* pitems, const int32_t* order, i
                                   int64_t* off, int64_t* bsum);
 __global__ void bsum_scan_kernel(int64_t* bsum, int64_t nblk);
 __global__ void add_bsum_kernel(int64_t* off, int64_t n, const int64_t* bsum);
-__global__ void fill_ids_kernel(int64_t n, int64_t first, uint32_t k1, int64_t* pidx,
-                                uint32_t* k1s);
+__global__ void fill_ids_kernel(int64_t n, int64_t first, uint32_t mtag, int64_t* pidx,
+                                uint32_t* mtags);
 __global__ void copy_u64_kernel(const unsigned long long* src, unsigned long long* dst,
                                 int64_t n);
 __global__ void cell_items_kernel(int64_t n, int C, const CellStart* rec, int32_t* coff,
                                   int64_t* pitems);
 __global__ void burn_kernel(const DesignDev* D, const PRates* R, const int32_t* order, int64_t n,
-                            uint32_t first, uint32_t k0, uint32_t k1, const uint32_t* k1s,
-                            const double* betas,
+                            uint32_t first, uint32_t k0, uint32_t k1, uint32_t mtag_all,
+                            const uint32_t* mtags, const double* betas,
                             const int64_t* goff, unsigned long long* queue, CellStart* rec,
                             unsigned long long* steps_out, unsigned long long* react_out,
                             const int64_t* pidx, unsigned long long* burn_tally);
 __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* order,
                            const int64_t* off, int64_t n, uint32_t first, uint32_t k0,
-                           uint32_t k1, const uint32_t* k1s, const CellStart* rec,
+                           uint32_t k1, uint32_t mtag_all, const uint32_t* mtags,
+                           const CellStart* rec,
                            const int32_t* coff,
                            unsigned long long* queue,
                            uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,

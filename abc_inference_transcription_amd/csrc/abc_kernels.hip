@@ -50,12 +50,12 @@ int abc_layout_n_params(int model, int nb) { return layout_of(model, nb).n_par; 
 
 // ------------------------------------------------------------------- prior
 struct PriorStream {
-  uint32_t k0, k1, pid;
+  uint32_t k0, k1, tag, pid;  // tag: 0xFF000000 | model, disjoint from every cell word
   int q;
   u4 w;
   ABC_DEV double next() {
     const int call = q >> 1, half = q & 1;
-    if (half == 0) w = philox((uint32_t)call, 0u, 0xFFFFFFFFu, pid, k0, k1);
+    if (half == 0) w = philox((uint32_t)call, 0u, tag, pid, k0, k1);
     q++;
     return half == 0 ? u01_53(w.x, w.y) : u01_53(w.z, w.w);
   }
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void prior_kernel(const DesignDev* __restrict_
   if (i >= n) return;
   const int nb = D->nb;
   const Layout L = layout_of(model, nb);
-  PriorStream s{k0, k1, first + (uint32_t)i, 0, u4{0, 0, 0, 0}};
+  PriorStream s{k0, k1, 0xFF000000u | (uint32_t)model, first + (uint32_t)i, 0, u4{0, 0, 0, 0}};
   double* th = theta + i;  // column c at th[n * c]
   if (D->prior_mode == 0) {
     for (int c = 0; c < L.n_kon; ++c) th[n * (L.o_kon + c)] = s.unif(-3.0, 3.0);
@@ -242,14 +242,15 @@ __global__ __launch_bounds__(256) void bucket_scatter_kernel(const float* __rest
 }
 
 // Identity of the particles of one model in a multi-model batch: global
-// particle index first + i and the model's key word.
-__global__ __launch_bounds__(256) void fill_ids_kernel(int64_t n, int64_t first, uint32_t k1,
+// particle index first + i and the model's tag (model << 24) for the cell
+// word of its counters.
+__global__ __launch_bounds__(256) void fill_ids_kernel(int64_t n, int64_t first, uint32_t mtag,
                                                        int64_t* __restrict__ pidx,
-                                                       uint32_t* __restrict__ k1s) {
+                                                       uint32_t* __restrict__ mtags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
     pidx[i] = first + i;
-    k1s[i] = k1;
+    mtags[i] = mtag;
   }
 }
 
@@ -361,8 +362,9 @@ __global__ __launch_bounds__(256) void add_bsum_kernel(int64_t* __restrict__ off
 
 // ------------------------------------------------------- shared helpers
 // Random streams of a cell: counter (draw_lo, draw_hi[15:0] | stream << 16,
-// cell, particle); stream 0 holds the set-up draws and the gene clock, stream
-// 1 + sub the molecule events and division coins of sub-lane sub.
+// cell | model << 24, particle), key (seed_lo, seed_hi); stream 0 holds the
+// set-up draws and the gene clock, stream 1 + sub the molecule events and
+// division coins of sub-lane sub.
 __device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double T, int nb) {
   return ((double)(k - nbB) * T) / (double)nb;
 }
@@ -484,13 +486,14 @@ struct CellSetup {
 };
 
 __device__ __forceinline__ CellSetup cell_setup(const DesignDev* __restrict__ D, const PRates* rp,
-                                                int c, uint32_t pid, uint32_t k0, uint32_t k1,
+                                                int c, uint32_t cw, uint32_t pid, uint32_t k0,
+                                                uint32_t k1,
                                                 const double* __restrict__ betas,
                                                 const int64_t* __restrict__ goff, uint64_t& gctr) {
   const double T = D->T;
   const int nb = D->nb, J = D->J;
   CellSetup o;
-  const u4 w = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+  const u4 w = draw(gctr++, 0u, cw, pid, k0, k1);
   if (D->age_mode == 1) {
     const double th = u01_53(w.x, w.y);
     o.age = th * T;
@@ -513,7 +516,7 @@ __device__ __forceinline__ CellSetup cell_setup(const DesignDev* __restrict__ D,
     uint32_t s0 = 0xFFFFFFFFu, s1 = 0xFFFFFFFFu, s2 = 0xFFFFFFFFu, s3 = 0xFFFFFFFFu;
     u4 v{0, 0, 0, 0};
     for (int qq = 0; qq < nu; ++qq) {
-      if ((qq & 3) == 0) v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+      if ((qq & 3) == 0) v = draw(gctr++, 0u, cw, pid, k0, k1);
       const int sel = qq & 3;
       const uint32_t x = sel == 0 ? v.x : (sel == 1 ? v.y : (sel == 2 ? v.z : v.w));
       if (x < s3) {
@@ -537,8 +540,9 @@ __device__ __forceinline__ CellSetup cell_setup(const DesignDev* __restrict__ D,
 __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restrict__ D,
                                                       const PRates* __restrict__ R,
                                                       const int32_t* __restrict__ order, int64_t n,
-                                                      uint32_t first, uint32_t k0, uint32_t k1_all,
-                                                      const uint32_t* __restrict__ k1s,
+                                                      uint32_t first, uint32_t k0, uint32_t k1,
+                                                      uint32_t mtag_all,
+                                                      const uint32_t* __restrict__ mtags,
                                                       const double* __restrict__ betas,
                                                       const int64_t* __restrict__ goff,
                                                       unsigned long long* __restrict__ queue,
@@ -561,9 +565,10 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     const int c = (int)(q % (unsigned long long)C);
     const PRates* rp = R + p;
     const uint32_t pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
-    const uint32_t k1 = k1s ? k1s[p] : k1_all;  // the particle's model enters the key
+    // counter word: the cell and the particle's model (the key is the seed alone)
+    const uint32_t cw = (uint32_t)c | (mtags ? mtags[p] : mtag_all);
     uint64_t gctr = 0;
-    const CellSetup cs0 = cell_setup(D, rp, c, pid, k0, k1, betas, goff, gctr);
+    const CellSetup cs0 = cell_setup(D, rp, c, cw, pid, k0, k1, betas, goff, gctr);
     const int cond = c % J;
     const double age = cs0.age, bc = cs0.bc;
     const double ws = age - D->pulse[cond] - D->chase[cond];
@@ -585,7 +590,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
       if (gk < tb) tb = gk;
       if (ws < tb) tb = ws;
       if (gneed) {
-        const u4 v = draw(gctr++, 0u, (uint32_t)c, pid, k0, k1);
+        const u4 v = draw(gctr++, 0u, cw, pid, k0, k1);
         st++;
         tg = t + (-ln(u01_32(v.x), ltab)) * (g ? ikoff : ikon);
         gneed = false;
@@ -686,8 +691,9 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
                                                      const int64_t* __restrict__ off, int64_t n,
-                                                     uint32_t first, uint32_t k0, uint32_t k1_all,
-                                                     const uint32_t* __restrict__ k1s,
+                                                     uint32_t first, uint32_t k0, uint32_t k1,
+                                                     uint32_t mtag_all,
+                                                     const uint32_t* __restrict__ mtags,
                                                      const CellStart* __restrict__ rec,
                                                      const int32_t* __restrict__ coff,
                                                      unsigned long long* __restrict__ queue,
@@ -704,7 +710,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
   load_log_table(ltab);
   __syncthreads();
 
-  uint32_t k1 = k1_all;  // key of the current item's particle (its model)
+  uint32_t cw = 0;  // counter word of the current item: cell | model << 24
   // per-lane item state
   bool have = false;
   int p = 0, c = 0, j = 0, step = 1, scaling = 0, gvar = 0;
@@ -746,7 +752,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       c = clo;
       const int sub = rem - co[c];
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
-      k1 = k1s ? k1s[p] : k1_all;
+      cw = (uint32_t)c | (mtags ? mtags[p] : mtag_all);
       sstream = 1u + (uint32_t)sub;
       const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
       share = (double)(1 << rc->msh);
@@ -762,7 +768,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       py = pz = pw = 0;
       {
         // the sub-lane's unlabelled molecules at the window start
-        WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, (uint32_t)c, pid, k0, k1, st};
+        WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, cw, pid, k0, k1, st};
         const double lu0 = rc->lu, ls = rc->lt - lu0;
         n0 = poisson(src, lu0 / share, ltab);
         n1 = poisson(src, ls > 0.0 ? ls / share : 0.0, ltab);
@@ -819,7 +825,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     // less what the interval used up, is again Exp(1) (memoryless).
     const int kind = div ? 1 : (gneed ? 2 : ((!pend && a0 > 0.0) ? 3 : 0));
     const bool mol = kind == 1 || kind == 3;
-    const u4 w = draw(mol ? mctr : gctr, mol ? sstream : 0u, (uint32_t)c, pid, k0, k1);
+    const u4 w = draw(mol ? mctr : gctr, mol ? sstream : 0u, cw, pid, k0, k1);
     if (kind != 0) {
       st++;
       if (mol) mctr++; else gctr++;

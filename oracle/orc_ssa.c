@@ -59,20 +59,23 @@ static layout_t layout_of(int model, int nb) {
 
 int orc_n_params(int model) { return layout_of(model, 5).n_par; }
 
-static void key_of(uint64_t seed, int model, uint32_t key[2]) {
+/* key (seed_lo, seed_hi) for every model; key[2] = model << 24 goes into the
+ * counter's cell word (cells < 2^24), 0xFF000000 | model into the prior's */
+static void key_of(uint64_t seed, int model, uint32_t key[3]) {
   key[0] = (uint32_t)seed;
-  key[1] = (uint32_t)(seed >> 32) ^ (0x9E3779B9u * (uint32_t)model);
+  key[1] = (uint32_t)(seed >> 32);
+  key[2] = (uint32_t)model << 24;
 }
 
 /* ---------------------------------------------------------------- prior */
 typedef struct {
-  uint32_t key[2]; uint32_t pidx; int q; uint32_t w[4];
+  uint32_t key[3]; uint32_t pidx; int q; uint32_t w[4];
 } prior_stream;
 
 static double prior_u(prior_stream* s) {
   int call = s->q >> 1, half = s->q & 1;
   if (half == 0) {
-    uint32_t ctr[4] = {(uint32_t)call, 0u, 0xFFFFFFFFu, s->pidx};
+    uint32_t ctr[4] = {(uint32_t)call, 0u, 0xFF000000u | (s->key[2] >> 24), s->pidx};
     orc_philox4x32_10(ctr, s->key, s->w);
   }
   s->q++;
@@ -216,8 +219,8 @@ static inline uint32_t lemire(uint32_t w, uint32_t n) {
   return (uint32_t)(((uint64_t)w * (uint64_t)n) >> 32);
 }
 
-/* Stream s of a cell: counter (draw_lo, draw_hi[15:0] | s << 16, cell,
- * particle).  Stream 0 holds the cell set-up draws and the gene clock and is
+/* Stream s of a cell: counter (draw_lo, draw_hi[15:0] | s << 16,
+ * cell | model << 24, particle).  Stream 0 holds the cell set-up draws and the gene clock and is
  * replayed identically by every sub-lane; stream 1 + sub holds sub-lane sub's
  * molecule events and division coin flips. */
 typedef struct {
@@ -354,12 +357,12 @@ typedef struct {
  * the cell's unspliced (lu) and all (lt) observed molecules.  The gene clock
  * is a first-reaction clock on stream 0, redrawn after every switch and at
  * every edge where the gene's rate changes. */
-static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[2],
+static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[3],
                      uint32_t pidx, int c, cell_start* out, int64_t* steps,
                      int64_t* reactions) {
   const int nb = d->n_bins, J = d->n_conds;
   const double T = d->cycle;
-  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c, pidx, 0u, 0};
+  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c | key[2], pidx, 0u, 0};
   uint32_t w[4], v[4];
   draw(&s0, w);
   int bin;
@@ -480,13 +483,13 @@ static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[2]
  * its own stream 1 + sub; then the four-species simulation from ws to the
  * sampling age over the cell's gene path (stream 0, continued from the
  * burn-in) with a 1/M share of the births. */
-static int sim_window(const orc_design* d, const prates* r, const uint32_t key[2],
+static int sim_window(const orc_design* d, const prates* r, const uint32_t key[3],
                       uint32_t pidx, int c, int sub, const cell_start* cst, uint32_t* u_out,
                       uint32_t* l_out, int64_t* steps, int64_t* reactions) {
   const int nb = d->n_bins, J = d->n_conds;
   const double T = d->cycle;
-  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c, pidx, 0u, cst->gctr};
-  cell_stream sm = {{key[0], key[1]}, (uint32_t)c, pidx, 1u + (uint32_t)sub, 0};
+  cell_stream s0 = {{key[0], key[1]}, (uint32_t)c | key[2], pidx, 0u, cst->gctr};
+  cell_stream sm = {{key[0], key[1]}, (uint32_t)c | key[2], pidx, 1u + (uint32_t)sub, 0};
   uint32_t w[4];
   const int cond = c % J;
   const double age = cst->age, bc = cst->bc;
@@ -631,7 +634,7 @@ static int sim_window(const orc_design* d, const prates* r, const uint32_t key[2
 int orc_burn_cells(const orc_design* d, int model, uint64_t seed, uint32_t pidx,
                    const double* theta, int n_cells, double* out) {
   if (model < 1 || model > 5 || n_cells > d->cells) return -1;
-  uint32_t key[2];
+  uint32_t key[3];
   key_of(seed, model, key);
   prates r;
   make_rates(d, model, theta, &r);
@@ -654,7 +657,7 @@ int orc_burn_cells(const orc_design* d, int model, uint64_t seed, uint32_t pidx,
   return 0;
 }
 
-static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[2],
+static int sim_cell(const orc_design* d, const prates* r, const uint32_t key[3],
                     uint32_t pidx, int c, uint32_t* u_out, uint32_t* l_out,
                     uint8_t* bin_out, int64_t* steps, int64_t* reactions) {
   cell_start cst;
@@ -812,7 +815,7 @@ int orc_simulate(const orc_design* d, int model, uint64_t seed, int64_t first, i
   const int C = d->cells, nb = d->n_bins, J = d->n_conds;
   const int K = 4 * nb + 3 * J, M = nb * J * 6, KS = 2 * nb * J;
   const int np = layout_of(model, nb).n_par;
-  uint32_t key[2];
+  uint32_t key[3];
   key_of(seed, model, key);
   int err = 0;
   prates* R = (prates*)malloc(sizeof(prates) * (n > 0 ? n : 1));
