@@ -684,9 +684,9 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
 // the whole cell.  Lanes refill from a global item queue the moment their item
 // ends, so heavy and light particles share the machine without idle lanes.
 // Held to 128 VGPRs, four waves per SIMD: the compiler's own allocation (140
-// VGPRs, three waves) leaves more of the FP64 and Philox latency exposed;
-// the four-wave build spills 10 VGPRs (44 bytes of scratch per lane) and
-// measured 7 % less SSA time per step.
+// VGPRs, three waves) leaves more of the FP64 and Philox latency exposed.
+// The four-wave build measured 7 % less SSA time per step; with the Philox
+// key the same in every lane it fits 128 VGPRs without spilling.
 __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) void ssa_kernel(const DesignDev* __restrict__ D,
                                                      const PRates* __restrict__ R,
                                                      const int32_t* __restrict__ order,
