@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -476,7 +476,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
-                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout,
+                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -1306,6 +1306,45 @@ int data_meta(abc_ctx* c, int64_t G, int64_t n, int32_t nc, const int32_t* age, 
     HIPCHK(c, hipMemcpyAsync(m + 2 * n + np, chase, sizeof(int32_t) * nch, hipMemcpyHostToDevice,
                              c->stream));
   HIPCHK(c, hipMemcpyAsync(c->dw.p, w, sizeof(double) * nc * J, hipMemcpyHostToDevice, c->stream));
+  // each family's positions in group order (a stable counting sort) and the
+  // group of each sorted position, for data_stats_sorted_kernel: pulse and
+  // chase cells by cluster, all cells by condition, all cells by
+  // (condition, cluster); -1 for a cell in no condition
+  const int64_t total = np + nch + 2 * n;
+  std::vector<int32_t> perm((size_t)(total > 0 ? total : 1)), grp((size_t)(total > 0 ? total : 1));
+  auto order = [&](int64_t base, int64_t m, int ngrp, auto key) {
+    // counts per key (key -1 counted in slot ngrp, placed last), then offsets
+    std::vector<int64_t> at((size_t)ngrp + 1, 0);
+    for (int64_t i = 0; i < m; ++i) {
+      const int k = key(i);
+      at[(size_t)(k < 0 ? ngrp : k)]++;
+    }
+    int64_t acc = 0;
+    for (int k = 0; k <= ngrp; ++k) {
+      const int64_t t = at[(size_t)k];
+      at[(size_t)k] = acc;
+      acc += t;
+    }
+    for (int64_t i = 0; i < m; ++i) {
+      const int k = key(i);
+      const int64_t to = at[(size_t)(k < 0 ? ngrp : k)]++;
+      perm[(size_t)(base + to)] = (int32_t)i;
+      grp[(size_t)(base + to)] = k;
+    }
+  };
+  order(0, np, nc, [&](int64_t i) { return (int)age[pulse[i]]; });
+  order(np, nch, nc, [&](int64_t i) { return (int)age[chase[i]]; });
+  order(np + nch, n, J, [&](int64_t i) { return (int)cond[i]; });
+  order(np + nch + n, n, J * nc, [&](int64_t i) { return cond[i] < 0 ? -1 : (int)(cond[i] * nc + age[i]); });
+  if ((rc = ensure(c, c->dsort, sizeof(int32_t) * 2 * (size_t)(total > 0 ? total : 1)))) return rc;
+  if (total > 0) {
+    HIPCHK(c, hipMemcpyAsync(c->dsort.p, perm.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(P<int32_t>(c->dsort) + total, grp.data(), sizeof(int32_t) * total,
+                             hipMemcpyHostToDevice, c->stream));
+    // the host vectors must outlive the copies
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
   return ABC_OK;
 }
 
@@ -1319,17 +1358,39 @@ int data_run(abc_ctx* c, int64_t G, int64_t n, const uint32_t* u, const uint32_t
   if (chunk < 1) chunk = 1;
   if (chunk > G) chunk = G;
   if ((rc = ensure(c, c->dboot, (size_t)(chunk * per_gene)))) return rc;
+  if ((rc = ensure(c, c->dpt, sizeof(double) * (size_t)(chunk * K1)))) return rc;
   if ((rc = ensure(c, c->dflag, sizeof(unsigned long long)))) return rc;
   unsigned long long* flag = P<unsigned long long>(c->dflag);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, flag, (int64_t)1);
   const int32_t* m = P<int32_t>(c->dmeta);
+  // families of at most ABC_DATA_SORTED_MAX cells: per-cell draw counts in LDS,
+  // then sums over the cells in group order; larger ones: sums per draw
+  int64_t mmax = n;
+  if (np > mmax) mmax = np;
+  if (nch > mmax) mmax = nch;
+  const bool sorted = mmax <= ABC_DATA_SORTED_MAX;
+  const int64_t total = np + nch + 2 * n;
+  const int32_t* perm = P<int32_t>(c->dsort);
   for (int64_t g0 = 0; g0 < G; g0 += chunk) {
     const int64_t cnt = G - g0 < chunk ? G - g0 : chunk;
-    hipLaunchKernelGGL(data_stats_kernel, dim3((unsigned)cnt), dim3(ABC_DATA_WG), 0, c->stream, G,
-                       g0, n, u, l, m, m + n, nc, J, m + 2 * n, np, m + 2 * n + np, nch,
-                       P<double>(c->dw), n_boot, (uint32_t)seed,
-                       (uint32_t)(seed >> 32) ^ 0x2545F491u, P<double>(c->dboot), out,
-                       reinterpret_cast<unsigned int*>(flag));
+    if (sorted) {
+      const int spb = ABC_DATA_SETS_PER_WG;
+      const unsigned nby = (unsigned)((n_boot + 1 + spb - 1) / spb);
+      hipLaunchKernelGGL(data_stats_sorted_kernel, dim3((unsigned)cnt, nby), dim3(ABC_DATA_WG),
+                         (size_t)((mmax + 1) / 2 > 0 ? (mmax + 1) / 2 : 1) * sizeof(uint32_t),
+                         c->stream, n, u, l, g0, m + 2 * n, np, m + 2 * n + np, nch, perm,
+                         perm + total, nc, J, P<double>(c->dw), n_boot, spb, (uint32_t)seed,
+                         (uint32_t)(seed >> 32) ^ 0x2545F491u, P<double>(c->dpt),
+                         P<double>(c->dboot), reinterpret_cast<unsigned int*>(flag));
+      HIPCHK(c, hipGetLastError());
+      hipLaunchKernelGGL(data_se_kernel, dim3(grid1(cnt * K1, 256)), dim3(256), 0, c->stream, G,
+                         g0, cnt, nc, J, n_boot, P<double>(c->dpt), P<double>(c->dboot), out);
+    } else
+      hipLaunchKernelGGL(data_stats_kernel, dim3((unsigned)cnt), dim3(ABC_DATA_WG), 0, c->stream,
+                         G, g0, n, u, l, m, m + n, nc, J, m + 2 * n, np, m + 2 * n + np, nch,
+                         P<double>(c->dw), n_boot, (uint32_t)seed,
+                         (uint32_t)(seed >> 32) ^ 0x2545F491u, P<double>(c->dboot), out,
+                         reinterpret_cast<unsigned int*>(flag));
     HIPCHK(c, hipGetLastError());
   }
   unsigned long long bad = 0;

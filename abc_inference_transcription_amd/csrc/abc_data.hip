@@ -24,6 +24,13 @@
 // The standard error of each statistic is sqrt((1 / (B - 1)) sum_b (x_b -
 // mean)^2), summed in resample order.  The result therefore equals the CPU
 // replay bit for bit whatever the launch shape.
+//
+// data_stats_kernel below does this with one workgroup per gene and the
+// per-draw atomics just described; it serves families of more than
+// ABC_DATA_SORTED_MAX cells.  Smaller families (the reference data's 5,422
+// cells) take data_stats_sorted_kernel, which counts the draws per cell
+// first and spreads a gene's sets over several workgroups, and
+// data_se_kernel; both give the same integers and so the same bits.
 #include "abc_device.h"
 #include "abc_kernels.h"
 
@@ -69,6 +76,105 @@ struct DataLds {
   double pt[4 * MC + 3 * MJ];            // the data's own statistics
   unsigned int bad;                      // a count of 2^20 or more was met
 };
+
+// The statistics of one set from its sums, one lane per cluster (pulse,
+// chase) or condition (ratio, correlations); b < 0: the data's own set, whose
+// correlations use the caller's weights w_point.
+ABC_DEV void finish_set(const DataLds& S, int tid, int b, int nc, int J,
+                        const double* __restrict__ w_point, double* st) {
+  if (tid < 2 * nc) {
+    const int f = tid / nc, c = tid - f * nc;
+    const int64_t cnt = (int64_t)S.ccp[f][0][c];
+    double mean = 0.0, ff = 0.0;
+    if (cnt > 0) {
+      const uint64_t s1 = S.ccp[f][1][c], s2 = S.ccp[f][2][c];
+      mean = (double)s1 / (double)cnt;
+      const double v = cov_int(cnt, s1, s1, s2);
+      const double eps = mean == 0.0 ? 0.0001 : 0.0;
+      ff = v / (mean + eps);
+    }
+    st[2 * f * nc + c] = mean;
+    st[(2 * f + 1) * nc + c] = ff;
+  } else if (tid < 2 * nc + J) {
+    const int j = tid - 2 * nc;
+    const int64_t cnt = (int64_t)S.rat[0][j];
+    double r = 0.0;
+    if (cnt > 0) {
+      const double mu = (double)S.rat[1][j] / (double)cnt, ml = (double)S.rat[2][j] / (double)cnt;
+      r = mu + ml > 0.0 ? ml / (mu + ml) : 0.0;
+    }
+    st[4 * nc + j] = r;
+  } else if (tid < 2 * nc + 2 * J) {
+    const int j = tid - 2 * nc - J;
+    double m1[MC], m2[MC], v1[MC], v2[MC], c12[MC], w[MC];
+    bool any_cov = false;
+    int64_t tot = 0;
+    for (int c = 0; c < nc; ++c) tot += (int64_t)S.cor[0][j][c];
+    for (int c = 0; c < nc; ++c) {
+      const int64_t cnt = (int64_t)S.cor[0][j][c];
+      if (cnt == 0) {
+        m1[c] = m2[c] = v1[c] = v2[c] = c12[c] = 0.0;
+      } else {
+        const uint64_t su = S.cor[1][j][c], sl = S.cor[2][j][c];
+        m1[c] = (double)su / (double)cnt;
+        m2[c] = (double)sl / (double)cnt;
+        v1[c] = cov_int(cnt, su, su, S.cor[3][j][c]);
+        v2[c] = cov_int(cnt, sl, sl, S.cor[4][j][c]);
+        c12[c] = cov_int(cnt, su, sl, S.cor[5][j][c]);
+      }
+      if (!(c12[c] == 0.0)) any_cov = true;
+      if (b < 0) w[c] = w_point[c + (int64_t)nc * j];
+      else w[c] = tot > 0 ? (double)cnt / (double)tot : 0.0;
+    }
+    double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
+    for (int c = 1; c < nc; ++c) {
+      s1 = s1 + w[c] * v1[c];
+      s2 = s2 + w[c] * v2[c];
+      s12 = s12 + w[c] * c12[c];
+    }
+    const double tv1 = s1 + wcov(m1, m1, w, nc), tv2 = s2 + wcov(m2, m2, w, nc);
+    const bool nz = !(tv1 == 0.0) && !(tv2 == 0.0);
+    st[4 * nc + J + j] = (any_cov && nz) ? s12 / sqrt(fabs(tv1 * tv2)) : 0.0;
+    st[4 * nc + 2 * J + j] = nz ? wcov(m1, m2, w, nc) / sqrt(fabs(tv1 * tv2)) : 0.0;
+  }
+}
+
+// Statistic k of gene g: its standard error over the n_boot sets in bg and
+// the data's own value, to the output columns in the order of the 14 summary
+// files: pulse_mean, pulse_ff, pulse_mean_se, pulse_ff_se, chase_* (nc each),
+// ratio, ratio_se, mean_corr, mean_corr_se, corr_mean, corr_mean_se (J each).
+ABC_DEV void se_column(int64_t g, int64_t G, int k, int n_boot, int nc, int J, double point,
+                       const double* bg, double* __restrict__ out) {
+  const int K1 = 4 * nc + 3 * J;
+  double s = bg[k];
+  for (int b = 1; b < n_boot; ++b) s = s + bg[(size_t)b * K1 + k];
+  const double m = s / (double)n_boot;
+  double q = (bg[k] - m) * (bg[k] - m);
+  for (int b = 1; b < n_boot; ++b) {
+    const double d = bg[(size_t)b * K1 + k] - m;
+    q = q + d * d;
+  }
+  const double se = sqrt((1.0 / (double)(n_boot - 1)) * q);
+  int colv, cols;
+  if (k < 4 * nc) {  // pulse / chase block, mean or FF
+    const int blk = k / nc, c = k - blk * nc;  // blk 0 p-mean, 1 p-ff, 2 c-mean, 3 c-ff
+    const int f = blk / 2, kind = blk & 1;
+    colv = 4 * nc * f + kind * nc + c;
+    cols = colv + 2 * nc;
+  } else {  // ratio, mean_corr, corr_mean
+    const int kk = k - 4 * nc, qd = kk / J, j = kk - qd * J;
+    colv = 8 * nc + 2 * J * qd + j;
+    cols = colv + J;
+  }
+  out[g + G * (int64_t)colv] = point;
+  out[g + G * (int64_t)cols] = se;
+}
+
+ABC_DEV void write_se(const DataLds& S, int tid, int64_t g, int64_t G, int n_boot, int nc, int J,
+                      const double* bg, double* __restrict__ out) {
+  const int K1 = 4 * nc + 3 * J;
+  for (int k = tid; k < K1; k += ABC_DATA_WG) se_column(g, G, k, n_boot, nc, J, S.pt[k], bg, out);
+}
 
 }  // namespace
 
@@ -154,92 +260,151 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_kernel(
       }
     }
     __syncthreads();
-    // one lane per cluster (pulse, chase) or condition (ratio, correlations)
-    double* st = b < 0 ? S.pt : bg + (size_t)b * K1;
-    if (tid < 2 * nc) {
-      const int f = tid / nc, c = tid - f * nc;
-      const int64_t cnt = (int64_t)S.ccp[f][0][c];
-      double mean = 0.0, ff = 0.0;
-      if (cnt > 0) {
-        const uint64_t s1 = S.ccp[f][1][c], s2 = S.ccp[f][2][c];
-        mean = (double)s1 / (double)cnt;
-        const double v = cov_int(cnt, s1, s1, s2);
-        const double eps = mean == 0.0 ? 0.0001 : 0.0;
-        ff = v / (mean + eps);
-      }
-      st[2 * f * nc + c] = mean;
-      st[(2 * f + 1) * nc + c] = ff;
-    } else if (tid < 2 * nc + J) {
-      const int j = tid - 2 * nc;
-      const int64_t cnt = (int64_t)S.rat[0][j];
-      double r = 0.0;
-      if (cnt > 0) {
-        const double mu = (double)S.rat[1][j] / (double)cnt, ml = (double)S.rat[2][j] / (double)cnt;
-        r = mu + ml > 0.0 ? ml / (mu + ml) : 0.0;
-      }
-      st[4 * nc + j] = r;
-    } else if (tid < 2 * nc + 2 * J) {
-      const int j = tid - 2 * nc - J;
-      double m1[MC], m2[MC], v1[MC], v2[MC], c12[MC], w[MC];
-      bool any_cov = false;
-      int64_t tot = 0;
-      for (int c = 0; c < nc; ++c) tot += (int64_t)S.cor[0][j][c];
-      for (int c = 0; c < nc; ++c) {
-        const int64_t cnt = (int64_t)S.cor[0][j][c];
-        if (cnt == 0) {
-          m1[c] = m2[c] = v1[c] = v2[c] = c12[c] = 0.0;
-        } else {
-          const uint64_t su = S.cor[1][j][c], sl = S.cor[2][j][c];
-          m1[c] = (double)su / (double)cnt;
-          m2[c] = (double)sl / (double)cnt;
-          v1[c] = cov_int(cnt, su, su, S.cor[3][j][c]);
-          v2[c] = cov_int(cnt, sl, sl, S.cor[4][j][c]);
-          c12[c] = cov_int(cnt, su, sl, S.cor[5][j][c]);
-        }
-        if (!(c12[c] == 0.0)) any_cov = true;
-        if (b < 0) w[c] = w_point[c + (int64_t)nc * j];
-        else w[c] = tot > 0 ? (double)cnt / (double)tot : 0.0;
-      }
-      double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
-      for (int c = 1; c < nc; ++c) {
-        s1 = s1 + w[c] * v1[c];
-        s2 = s2 + w[c] * v2[c];
-        s12 = s12 + w[c] * c12[c];
-      }
-      const double tv1 = s1 + wcov(m1, m1, w, nc), tv2 = s2 + wcov(m2, m2, w, nc);
-      const bool nz = !(tv1 == 0.0) && !(tv2 == 0.0);
-      st[4 * nc + J + j] = (any_cov && nz) ? s12 / sqrt(fabs(tv1 * tv2)) : 0.0;
-      st[4 * nc + 2 * J + j] = nz ? wcov(m1, m2, w, nc) / sqrt(fabs(tv1 * tv2)) : 0.0;
-    }
+    finish_set(S, tid, b, nc, J, w_point, b < 0 ? S.pt : bg + (size_t)b * K1);
     __syncthreads();
   }
-  // standard errors over the resamples; output columns in the order of the
-  // 14 summary files: pulse_mean, pulse_ff, pulse_mean_se, pulse_ff_se,
-  // chase_* (nc each), ratio, ratio_se, mean_corr, mean_corr_se, corr_mean,
-  // corr_mean_se (J each)
-  for (int k = tid; k < K1; k += ABC_DATA_WG) {
-    double s = bg[k];
-    for (int b = 1; b < n_boot; ++b) s = s + bg[(size_t)b * K1 + k];
-    const double m = s / (double)n_boot;
-    double q = (bg[k] - m) * (bg[k] - m);
-    for (int b = 1; b < n_boot; ++b) {
-      const double d = bg[(size_t)b * K1 + k] - m;
-      q = q + d * d;
+  write_se(S, tid, g, G, n_boot, nc, J, bg, out);
+  if (tid == 0 && S.bad) atomicOr(bad_out, 1u);
+}
+
+// The same statistics for genes whose families hold at most
+// ABC_DATA_SORTED_MAX cells, with the sets spread over workgroups: workgroup
+// (x, y) takes gene g0 + x and sets y * spb .. y * spb + spb - 1 of 1 + n_boot
+// (set 0: the data themselves, into pt; set 1 + b: resample b, into boot).
+// Each resample first counts how often it draws each cell of a family
+// (16-bit halves of 32-bit LDS words, atomics at random addresses; a count
+// never exceeds the family's size), then every lane walks a contiguous slice
+// of the family's cells in group order (perm, with the group of each
+// position in grp, -1 for a cell in no condition), sums count-weighted values
+// in registers and adds them to the group's sums where its slice changes
+// group.  The integer sums are those of data_stats_kernel, regrouped, so the
+// statistics are identical.  perm and grp hold the four families one after
+// another: pulse (np), chase (nch), all cells (n) for the ratios and all
+// cells (n) for the correlations, whose groups are cond * nc + cluster.
+// data_se_kernel then forms the standard errors.
+__global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
+    int64_t n, const uint32_t* __restrict__ u, const uint32_t* __restrict__ l, int64_t g0,
+    const int32_t* __restrict__ pulse, int64_t np, const int32_t* __restrict__ chase, int64_t nch,
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ grp, int nc, int J,
+    const double* __restrict__ w_point, int n_boot, int spb, uint32_t k0, uint32_t k1,
+    double* __restrict__ pt, double* __restrict__ boot, unsigned int* __restrict__ bad_out) {
+  extern __shared__ uint32_t mult_sh[];  // two 16-bit counters per word
+  __shared__ DataLds S;
+  const int64_t gl = blockIdx.x;  // gene within this launch
+  const int64_t g = g0 + gl;
+  const int tid = threadIdx.x;
+  const uint32_t* ug = u + g * n;
+  const uint32_t* lg = l + g * n;
+  const int K1 = 4 * nc + 3 * J;
+  double* bg = boot + (size_t)gl * (size_t)n_boot * (size_t)K1;
+  if (tid == 0) S.bad = 0u;
+  const int s_lo = (int)blockIdx.y * spb;
+  const int s_hi = s_lo + spb < n_boot + 1 ? s_lo + spb : n_boot + 1;
+  for (int set = s_lo; set < s_hi; ++set) {
+    const int b = set - 1;
+    for (int i = tid; i < 2 * 3 * MC + 3 * MJ + 6 * MJ * MC; i += ABC_DATA_WG) {
+      if (i < 2 * 3 * MC) (&S.ccp[0][0][0])[i] = 0ull;
+      else if (i < 2 * 3 * MC + 3 * MJ) (&S.rat[0][0])[i - 2 * 3 * MC] = 0ull;
+      else (&S.cor[0][0][0])[i - 2 * 3 * MC - 3 * MJ] = 0ull;
     }
-    const double se = sqrt((1.0 / (double)(n_boot - 1)) * q);
-    int colv, cols;
-    if (k < 4 * nc) {  // pulse / chase block f, mean or FF
-      const int blk = k / nc, c = k - blk * nc;  // blk 0 p-mean, 1 p-ff, 2 c-mean, 3 c-ff
-      const int f = blk / 2, kind = blk & 1;
-      colv = 4 * nc * f + kind * nc + c;
-      cols = colv + 2 * nc;
-    } else {  // ratio, mean_corr, corr_mean
-      const int kk = k - 4 * nc, qd = kk / J, j = kk - qd * J;
-      colv = 8 * nc + 2 * J * qd + j;
-      cols = colv + J;
+    for (int f = 0; f < 4; ++f) {
+      const int64_t m = f == 0 ? np : (f == 1 ? nch : n);
+      const int64_t base = f == 0 ? 0 : (f == 1 ? np : (f == 2 ? np + nch : np + nch + n));
+      const int32_t* list = f == 0 ? pulse : (f == 1 ? chase : nullptr);
+      if (b >= 0) {
+        for (int64_t i = tid; 2 * i < m; i += ABC_DATA_WG) mult_sh[i] = 0u;
+        __syncthreads();
+        for (int64_t q = tid; 2 * q < m; q += ABC_DATA_WG) {
+          const u4 w = philox((uint32_t)q, (uint32_t)b, (uint32_t)g, (uint32_t)f, k0, k1);
+          const int64_t p0 = pos_of(w, 0, m);
+          atomicAdd(&mult_sh[p0 >> 1], 1u << (16 * (int)(p0 & 1)));
+          if (2 * q + 1 < m) {
+            const int64_t p1 = pos_of(w, 1, m);
+            atomicAdd(&mult_sh[p1 >> 1], 1u << (16 * (int)(p1 & 1)));
+          }
+        }
+      }
+      __syncthreads();
+      const int64_t chunk = (m + ABC_DATA_WG - 1) / ABC_DATA_WG;
+      const int64_t k_lo = (int64_t)tid * chunk;
+      const int64_t k_hi = k_lo + chunk < m ? k_lo + chunk : m;
+      int cur = -1;
+      unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+      auto flush = [&]() {
+        if (cur < 0 || a0 == 0ull) return;
+        if (f < 2) {
+          atomicAdd(&S.ccp[f][0][cur], a0);
+          atomicAdd(&S.ccp[f][1][cur], a1);
+          atomicAdd(&S.ccp[f][2][cur], a2);
+        } else if (f == 2) {
+          atomicAdd(&S.rat[0][cur], a0);
+          atomicAdd(&S.rat[1][cur], a1);
+          atomicAdd(&S.rat[2][cur], a2);
+        } else {
+          const int j = cur / nc, c = cur - (cur / nc) * nc;
+          atomicAdd(&S.cor[0][j][c], a0);
+          atomicAdd(&S.cor[1][j][c], a1);
+          atomicAdd(&S.cor[2][j][c], a2);
+          atomicAdd(&S.cor[3][j][c], a3);
+          atomicAdd(&S.cor[4][j][c], a4);
+          atomicAdd(&S.cor[5][j][c], a5);
+        }
+      };
+      for (int64_t k = k_lo; k < k_hi; ++k) {
+        const int gk = grp[base + k];
+        if (gk != cur) {
+          flush();
+          cur = gk;
+          a0 = a1 = a2 = a3 = a4 = a5 = 0ull;
+        }
+        if (gk < 0) continue;
+        const int32_t pos = perm[base + k];
+        const uint32_t mult = b < 0 ? 1u : (mult_sh[pos >> 1] >> (16 * (pos & 1))) & 0xFFFFu;
+        if (mult == 0u) continue;
+        const int64_t i = list ? list[pos] : pos;
+        const uint32_t x0 = ug[i], x1 = lg[i];
+        if ((x0 | x1) >= (1u << 20)) {
+          atomicOr(&S.bad, 1u);
+          continue;
+        }
+        const unsigned long long mm = mult;
+        a0 += mm;
+        if (f < 2) {
+          const unsigned long long x = (unsigned long long)x0 + x1;
+          a1 += mm * x;
+          a2 += mm * (x * x);
+        } else {
+          const unsigned long long x = x0, y = x1;
+          a1 += mm * x;
+          a2 += mm * y;
+          if (f == 3) {
+            a3 += mm * (x * x);
+            a4 += mm * (y * y);
+            a5 += mm * (x * y);
+          }
+        }
+      }
+      flush();
+      __syncthreads();
     }
-    out[g + G * (int64_t)colv] = S.pt[k];
-    out[g + G * (int64_t)cols] = se;
+    finish_set(S, tid, b, nc, J, w_point, b < 0 ? pt + (size_t)gl * K1 : bg + (size_t)b * K1);
+    __syncthreads();
   }
   if (tid == 0 && S.bad) atomicOr(bad_out, 1u);
+}
+
+// Standard errors of data_stats_sorted_kernel's sets: one lane per (gene,
+// statistic) of the launch, the same sums in resample order as write_se.
+__global__ __launch_bounds__(256) void data_se_kernel(int64_t G, int64_t g0, int64_t genes, int nc,
+                                                      int J, int n_boot,
+                                                      const double* __restrict__ pt,
+                                                      const double* __restrict__ boot,
+                                                      double* __restrict__ out) {
+  const int K1 = 4 * nc + 3 * J;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= genes * K1) return;
+  const int64_t gl = q / K1;
+  const int k = (int)(q - gl * K1);
+  const double* bg = boot + (size_t)gl * (size_t)n_boot * (size_t)K1;
+  se_column(g0 + gl, G, k, n_boot, nc, J, pt[gl * K1 + k], bg, out);
 }

@@ -126,3 +126,14 @@ def test_data_stats_rejects_bad_arguments(eng):
                           **ok).shape == (0, 8 * NC + 6 * J)
     empty = eng.data_stats(np.zeros((0, 2)), np.zeros((0, 2)), [], [], NC, J, [], [], w, **ok)
     assert empty.shape == (2, 8 * NC + 6 * J) and np.all(empty == 0.0)
+
+
+def test_data_stats_large_family_path(oracle, eng):
+    """Families of more than 8,192 cells take the per-draw path
+    (data_stats_kernel); smaller ones count draws per cell first
+    (data_stats_sorted_kernel).  Both equal the CPU restatement."""
+    u, l, age, cond, pulse, chase, w = synthetic(8, n=9000, G=3)
+    print("\n[data_stats] 9,000 cells (per-draw path)", flush=True)
+    got = eng.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
+    want = oracle.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
+    assert _same_bits(got, want)
