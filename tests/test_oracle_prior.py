@@ -85,3 +85,19 @@ def test_prior_depends_only_on_particle_index(oracle):
     assert not np.any(a == c)
     d = oracle.sample_prior(od, 4, 0, 0, 100)           # another model: other stream
     assert not np.any(a[:, 6] == d[:, 6])
+
+
+def test_cell_streams_differ_between_models(oracle):
+    """The model is part of every cell's Philox counter (cell | model << 24), so
+    the same particle index and cell draw different set-ups under different
+    models: models 1 and 2 (the same parameter columns, so the same rates
+    here) draw each cell's capture efficiency from their own stream, and the
+    same model twice draws identical ones."""
+    from abc_inference_transcription_amd.design import reference_design, CAPTURE_BETA
+    od = oracle.OracleDesign(reference_design(cells=64, capture_mode=CAPTURE_BETA))
+    th = np.array([0.5, 0.3, 0.8, -0.2, -0.3, 0.0])
+    a = oracle.burn_cells(od, 1, 0, 5, th, 64)
+    b = oracle.burn_cells(od, 2, 0, 5, th, 64)
+    c = oracle.burn_cells(od, 1, 0, 5, th, 64)
+    np.testing.assert_array_equal(a["capture"], c["capture"])
+    assert np.mean(a["capture"] != b["capture"]) > 0.95
