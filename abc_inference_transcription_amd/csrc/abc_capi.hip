@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -476,7 +476,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
-                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt,
+                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt, &c->dxs,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -1353,35 +1353,51 @@ int data_run(abc_ctx* c, int64_t G, int64_t n, const uint32_t* u, const uint32_t
              int32_t J, int64_t np, int64_t nch, int32_t n_boot, uint64_t seed, double* out) {
   int rc;
   const int64_t K1 = 4 * nc + 3 * J;
-  const int64_t per_gene = (int64_t)n_boot * K1 * (int64_t)sizeof(double);
-  int64_t chunk = (1ll << 30) / per_gene;
-  if (chunk < 1) chunk = 1;
-  if (chunk > G) chunk = G;
-  if ((rc = ensure(c, c->dboot, (size_t)(chunk * per_gene)))) return rc;
-  if ((rc = ensure(c, c->dpt, sizeof(double) * (size_t)(chunk * K1)))) return rc;
-  if ((rc = ensure(c, c->dflag, sizeof(unsigned long long)))) return rc;
-  unsigned long long* flag = P<unsigned long long>(c->dflag);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, flag, (int64_t)1);
-  const int32_t* m = P<int32_t>(c->dmeta);
-  // families of at most ABC_DATA_SORTED_MAX cells: per-cell draw counts in LDS,
-  // then sums over the cells in group order; larger ones: sums per draw
+  // families of at most ABC_DATA_SORTED_MAX cells: each gene's cells packed in
+  // group order, per-cell draw counts in LDS, then sums over the cells in
+  // group order; larger ones: sums per draw
   int64_t mmax = n;
   if (np > mmax) mmax = np;
   if (nch > mmax) mmax = nch;
   const bool sorted = mmax <= ABC_DATA_SORTED_MAX;
   const int64_t total = np + nch + 2 * n;
+  const int64_t slots = (int64_t)ABC_DATA_WG *
+                        ((np + ABC_DATA_WG - 1) / ABC_DATA_WG + (nch + ABC_DATA_WG - 1) / ABC_DATA_WG +
+                         2 * ((n + ABC_DATA_WG - 1) / ABC_DATA_WG));
+  const int64_t per_gene = (int64_t)n_boot * K1 * (int64_t)sizeof(double) +
+                           (sorted ? slots * (int64_t)sizeof(unsigned long long) : 0);
+  int64_t chunk = (1ll << 30) / per_gene;
+  if (chunk < 1) chunk = 1;
+  if (sorted && chunk > 65535) chunk = 65535;  // genes are data_pack_kernel's grid y
+  if (chunk > G) chunk = G;
+  if ((rc = ensure(c, c->dboot, (size_t)(chunk * n_boot * K1) * sizeof(double)))) return rc;
+  if ((rc = ensure(c, c->dpt, sizeof(double) * (size_t)(chunk * K1)))) return rc;
+  if ((rc = ensure(c, c->dflag, sizeof(unsigned long long)))) return rc;
+  if (sorted && slots > 0 && (rc = ensure(c, c->dxs, sizeof(unsigned long long) * (size_t)(chunk * slots))))
+    return rc;
+  unsigned long long* flag = P<unsigned long long>(c->dflag);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, flag, (int64_t)1);
+  const int32_t* m = P<int32_t>(c->dmeta);
   const int32_t* perm = P<int32_t>(c->dsort);
   for (int64_t g0 = 0; g0 < G; g0 += chunk) {
     const int64_t cnt = G - g0 < chunk ? G - g0 : chunk;
     if (sorted) {
+      if (slots > 0)
+        hipLaunchKernelGGL(data_pack_kernel, dim3((unsigned)(slots / ABC_DATA_WG), (unsigned)cnt),
+                         dim3(ABC_DATA_WG), 0, c->stream, n, u, l, g0, m + 2 * n, np,
+                         m + 2 * n + np, nch, perm, perm + total, slots,
+                         P<unsigned long long>(c->dxs), reinterpret_cast<unsigned int*>(flag));
+      HIPCHK(c, hipGetLastError());
       const int spb = ABC_DATA_SETS_PER_WG;
-      const unsigned nby = (unsigned)((n_boot + 1 + spb - 1) / spb);
-      hipLaunchKernelGGL(data_stats_sorted_kernel, dim3((unsigned)cnt, nby), dim3(ABC_DATA_WG),
+      const int nsb = (n_boot + 1 + spb - 1) / spb;
+      const int64_t items = cnt * nsb;
+      hipLaunchKernelGGL(data_stats_sorted_kernel, dim3((unsigned)((items + 7) / 8 * 8)),
+                         dim3(ABC_DATA_WG),
                          (size_t)((mmax + 1) / 2 > 0 ? (mmax + 1) / 2 : 1) * sizeof(uint32_t),
-                         c->stream, n, u, l, g0, m + 2 * n, np, m + 2 * n + np, nch, perm,
-                         perm + total, nc, J, P<double>(c->dw), n_boot, spb, (uint32_t)seed,
+                         c->stream, cnt, nsb, g0, P<unsigned long long>(c->dxs), slots, n, np, nch,
+                         nc, J, P<double>(c->dw), n_boot, spb, (uint32_t)seed,
                          (uint32_t)(seed >> 32) ^ 0x2545F491u, P<double>(c->dpt),
-                         P<double>(c->dboot), reinterpret_cast<unsigned int*>(flag));
+                         P<double>(c->dboot));
       HIPCHK(c, hipGetLastError());
       hipLaunchKernelGGL(data_se_kernel, dim3(grid1(cnt * K1, 256)), dim3(256), 0, c->stream, G,
                          g0, cnt, nc, J, n_boot, P<double>(c->dpt), P<double>(c->dboot), out);

@@ -28,9 +28,10 @@
 // data_stats_kernel below does this with one workgroup per gene and the
 // per-draw atomics just described; it serves families of more than
 // ABC_DATA_SORTED_MAX cells.  Smaller families (the reference data's 5,422
-// cells) take data_stats_sorted_kernel, which counts the draws per cell
-// first and spreads a gene's sets over several workgroups, and
-// data_se_kernel; both give the same integers and so the same bits.
+// cells) take data_pack_kernel (each gene's cells packed in group order),
+// data_stats_sorted_kernel, which counts the draws per cell first and spreads
+// a gene's sets over several workgroups, and data_se_kernel; both paths give
+// the same integers and so the same bits.
 #include "abc_device.h"
 #include "abc_kernels.h"
 
@@ -267,38 +268,77 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_kernel(
   if (tid == 0 && S.bad) atomicOr(bad_out, 1u);
 }
 
-// The same statistics for genes whose families hold at most
-// ABC_DATA_SORTED_MAX cells, with the sets spread over workgroups: workgroup
-// (x, y) takes gene g0 + x and sets y * spb .. y * spb + spb - 1 of 1 + n_boot
-// (set 0: the data themselves, into pt; set 1 + b: resample b, into boot).
-// Each resample first counts how often it draws each cell of a family
-// (16-bit halves of 32-bit LDS words, atomics at random addresses; a count
-// never exceeds the family's size), then every lane walks a contiguous slice
-// of the family's cells in group order (perm, with the group of each
-// position in grp, -1 for a cell in no condition), sums count-weighted values
-// in registers and adds them to the group's sums where its slice changes
-// group.  The integer sums are those of data_stats_kernel, regrouped, so the
-// statistics are identical.  perm and grp hold the four families one after
-// another: pulse (np), chase (nch), all cells (n) for the ratios and all
-// cells (n) for the correlations, whose groups are cond * nc + cluster.
-// data_se_kernel then forms the standard errors.
-__global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
+// Each family's cells of one gene as one 64-bit word per cell: bits 0-19 u,
+// 20-39 l, 40-55 the cell's position in the family (its draw counter), 56-63
+// its group + 1 (0: a cell in no group, or padding).  Per gene the families
+// follow one another (pulse, chase, all cells by condition, all cells by
+// (condition, cluster)), each in ceil(m / 256) * 256 slots, transposed: slot
+// j * 256 + t holds sorted position t * ceil(m / 256) + j, so that lane t of
+// data_stats_sorted_kernel walks a contiguous run of the sorted cells while
+// the 256 lanes read 256 consecutive words.  A count of 2^20 or more in a
+// cell with a group sets *bad_out (the same cells the data's own set sums).
+__global__ __launch_bounds__(256) void data_pack_kernel(
     int64_t n, const uint32_t* __restrict__ u, const uint32_t* __restrict__ l, int64_t g0,
     const int32_t* __restrict__ pulse, int64_t np, const int32_t* __restrict__ chase, int64_t nch,
-    const int32_t* __restrict__ perm, const int32_t* __restrict__ grp, int nc, int J,
-    const double* __restrict__ w_point, int n_boot, int spb, uint32_t k0, uint32_t k1,
-    double* __restrict__ pt, double* __restrict__ boot, unsigned int* __restrict__ bad_out) {
+    const int32_t* __restrict__ perm, const int32_t* __restrict__ grp, int64_t slots,
+    unsigned long long* __restrict__ xs, unsigned int* __restrict__ bad_out) {
+  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (s >= slots) return;
+  const int64_t gl = blockIdx.y, g = g0 + gl;
+  const int64_t c0 = (np + 255) / 256, c1 = (nch + 255) / 256, c2 = (n + 255) / 256;
+  int64_t f, at, m, base, cf;
+  if (s < 256 * c0) { f = 0; at = s; m = np; base = 0; cf = c0; }
+  else if (s < 256 * (c0 + c1)) { f = 1; at = s - 256 * c0; m = nch; base = np; cf = c1; }
+  else if (s < 256 * (c0 + c1 + c2)) { f = 2; at = s - 256 * (c0 + c1); m = n; base = np + nch; cf = c2; }
+  else { f = 3; at = s - 256 * (c0 + c1 + c2); m = n; base = np + nch + n; cf = c2; }
+  const int64_t k = (at & 255) * cf + (at >> 8);
+  unsigned long long word = 0ull;
+  if (k < m) {
+    const int gk = grp[base + k];
+    if (gk >= 0) {
+      const int32_t pos = perm[base + k];
+      const int64_t i = f == 0 ? pulse[pos] : (f == 1 ? chase[pos] : pos);
+      const uint32_t x0 = u[g * n + i], x1 = l[g * n + i];
+      if ((x0 | x1) >= (1u << 20)) atomicOr(bad_out, 1u);
+      else
+        word = (unsigned long long)x0 | (unsigned long long)x1 << 20 |
+               (unsigned long long)pos << 40 | (unsigned long long)(gk + 1) << 56;
+    }
+  }
+  xs[gl * slots + s] = word;
+}
+
+// The same statistics for genes whose families hold at most
+// ABC_DATA_SORTED_MAX cells, with the sets spread over workgroups: gene g0 +
+// gl, sets sb * spb .. sb * spb + spb - 1 of 1 + n_boot (set 0: the data
+// themselves, into pt; set 1 + b: resample b, into boot).  Workgroup L runs
+// item (L % 8) * (gridDim.x / 8) + L / 8 of genes x nsb (gl = item / nsb, sb
+// = item % nsb): workgroups go round-robin to the eight XCDs, so a gene's nsb
+// workgroups share one XCD's L2 for its packed cells.
+// Each resample first counts how often it draws each cell of a family
+// (16-bit halves of 32-bit LDS words, atomics at random addresses; a count
+// never exceeds the family's size), then every lane walks a contiguous run
+// of the family's cells in group order (data_pack_kernel's words), sums
+// count-weighted values in registers and adds them to the group's sums
+// where its run changes group.  The integer sums are those of
+// data_stats_kernel, regrouped, so the statistics are identical.
+// data_se_kernel then forms the standard errors.
+__global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
+    int64_t genes, int nsb, int64_t g0, const unsigned long long* __restrict__ xs, int64_t slots,
+    int64_t n, int64_t np, int64_t nch, int nc, int J, const double* __restrict__ w_point,
+    int n_boot, int spb, uint32_t k0, uint32_t k1, double* __restrict__ pt,
+    double* __restrict__ boot) {
   extern __shared__ uint32_t mult_sh[];  // two 16-bit counters per word
   __shared__ DataLds S;
-  const int64_t gl = blockIdx.x;  // gene within this launch
+  const int64_t item = (int64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  if (item >= genes * nsb) return;  // the whole workgroup: padding to a multiple of 8
+  const int64_t gl = item / nsb;  // gene within this launch
   const int64_t g = g0 + gl;
   const int tid = threadIdx.x;
-  const uint32_t* ug = u + g * n;
-  const uint32_t* lg = l + g * n;
+  const unsigned long long* xg = xs + gl * slots;
   const int K1 = 4 * nc + 3 * J;
   double* bg = boot + (size_t)gl * (size_t)n_boot * (size_t)K1;
-  if (tid == 0) S.bad = 0u;
-  const int s_lo = (int)blockIdx.y * spb;
+  const int s_lo = (int)(item - gl * nsb) * spb;
   const int s_hi = s_lo + spb < n_boot + 1 ? s_lo + spb : n_boot + 1;
   for (int set = s_lo; set < s_hi; ++set) {
     const int b = set - 1;
@@ -307,10 +347,10 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
       else if (i < 2 * 3 * MC + 3 * MJ) (&S.rat[0][0])[i - 2 * 3 * MC] = 0ull;
       else (&S.cor[0][0][0])[i - 2 * 3 * MC - 3 * MJ] = 0ull;
     }
+    int64_t fb = 0;  // the family's first slot
     for (int f = 0; f < 4; ++f) {
       const int64_t m = f == 0 ? np : (f == 1 ? nch : n);
-      const int64_t base = f == 0 ? 0 : (f == 1 ? np : (f == 2 ? np + nch : np + nch + n));
-      const int32_t* list = f == 0 ? pulse : (f == 1 ? chase : nullptr);
+      const int64_t cf = (m + ABC_DATA_WG - 1) / ABC_DATA_WG;
       if (b >= 0) {
         for (int64_t i = tid; 2 * i < m; i += ABC_DATA_WG) mult_sh[i] = 0u;
         __syncthreads();
@@ -325,9 +365,6 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
         }
       }
       __syncthreads();
-      const int64_t chunk = (m + ABC_DATA_WG - 1) / ABC_DATA_WG;
-      const int64_t k_lo = (int64_t)tid * chunk;
-      const int64_t k_hi = k_lo + chunk < m ? k_lo + chunk : m;
       int cur = -1;
       unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
       auto flush = [&]() {
@@ -350,47 +387,43 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
           atomicAdd(&S.cor[5][j][c], a5);
         }
       };
-      for (int64_t k = k_lo; k < k_hi; ++k) {
-        const int gk = grp[base + k];
+      const unsigned long long* xf = xg + fb + tid;
+      for (int64_t j = 0; j < cf; ++j) {
+        const unsigned long long v = xf[j * ABC_DATA_WG];
+        const int gk = (int)(v >> 56) - 1;
         if (gk != cur) {
           flush();
           cur = gk;
           a0 = a1 = a2 = a3 = a4 = a5 = 0ull;
         }
         if (gk < 0) continue;
-        const int32_t pos = perm[base + k];
+        const uint32_t pos = (uint32_t)(v >> 40) & 0xFFFFu;
         const uint32_t mult = b < 0 ? 1u : (mult_sh[pos >> 1] >> (16 * (pos & 1))) & 0xFFFFu;
         if (mult == 0u) continue;
-        const int64_t i = list ? list[pos] : pos;
-        const uint32_t x0 = ug[i], x1 = lg[i];
-        if ((x0 | x1) >= (1u << 20)) {
-          atomicOr(&S.bad, 1u);
-          continue;
-        }
+        const unsigned long long x0 = v & 0xFFFFFull, x1 = (v >> 20) & 0xFFFFFull;
         const unsigned long long mm = mult;
         a0 += mm;
         if (f < 2) {
-          const unsigned long long x = (unsigned long long)x0 + x1;
+          const unsigned long long x = x0 + x1;
           a1 += mm * x;
           a2 += mm * (x * x);
         } else {
-          const unsigned long long x = x0, y = x1;
-          a1 += mm * x;
-          a2 += mm * y;
+          a1 += mm * x0;
+          a2 += mm * x1;
           if (f == 3) {
-            a3 += mm * (x * x);
-            a4 += mm * (y * y);
-            a5 += mm * (x * y);
+            a3 += mm * (x0 * x0);
+            a4 += mm * (x1 * x1);
+            a5 += mm * (x0 * x1);
           }
         }
       }
       flush();
       __syncthreads();
+      fb += cf * ABC_DATA_WG;
     }
     finish_set(S, tid, b, nc, J, w_point, b < 0 ? pt + (size_t)gl * K1 : bg + (size_t)b * K1);
     __syncthreads();
   }
-  if (tid == 0 && S.bad) atomicOr(bad_out, 1u);
 }
 
 // Standard errors of data_stats_sorted_kernel's sets: one lane per (gene,

@@ -159,13 +159,15 @@ __global__ void data_stats_kernel(int64_t G, int64_t g0, int64_t n, const uint32
                                   uint32_t k1, double* boot, double* out, unsigned int* bad_out);
 #define ABC_DATA_SORTED_MAX 8192  // cells per family for data_stats_sorted_kernel
 #define ABC_DATA_SETS_PER_WG 8    // sets per workgroup of data_stats_sorted_kernel
-__global__ void data_stats_sorted_kernel(int64_t n, const uint32_t* u, const uint32_t* l,
-                                         int64_t g0, const int32_t* pulse, int64_t np,
-                                         const int32_t* chase, int64_t nch, const int32_t* perm,
-                                         const int32_t* grp, int nc, int J,
+__global__ void data_pack_kernel(int64_t n, const uint32_t* u, const uint32_t* l, int64_t g0,
+                                 const int32_t* pulse, int64_t np, const int32_t* chase,
+                                 int64_t nch, const int32_t* perm, const int32_t* grp,
+                                 int64_t slots, unsigned long long* xs, unsigned int* bad_out);
+__global__ void data_stats_sorted_kernel(int64_t genes, int nsb, int64_t g0,
+                                         const unsigned long long* xs, int64_t slots, int64_t n,
+                                         int64_t np, int64_t nch, int nc, int J,
                                          const double* w_point, int n_boot, int spb, uint32_t k0,
-                                         uint32_t k1, double* pt, double* boot,
-                                         unsigned int* bad_out);
+                                         uint32_t k1, double* pt, double* boot);
 __global__ void data_se_kernel(int64_t G, int64_t g0, int64_t genes, int nc, int J, int n_boot,
                                const double* pt, const double* boot, double* out);
 // Model probabilities with bootstrap bounds (scripts/model_probs.jl)
