@@ -1368,7 +1368,6 @@ int data_run(abc_ctx* c, int64_t G, int64_t n, const uint32_t* u, const uint32_t
                            (sorted ? slots * (int64_t)sizeof(unsigned long long) : 0);
   int64_t chunk = (1ll << 30) / per_gene;
   if (chunk < 1) chunk = 1;
-  if (sorted && chunk > 65535) chunk = 65535;  // genes are data_pack_kernel's grid y
   if (chunk > G) chunk = G;
   if ((rc = ensure(c, c->dboot, (size_t)(chunk * n_boot * K1) * sizeof(double)))) return rc;
   if ((rc = ensure(c, c->dpt, sizeof(double) * (size_t)(chunk * K1)))) return rc;
@@ -1383,10 +1382,11 @@ int data_run(abc_ctx* c, int64_t G, int64_t n, const uint32_t* u, const uint32_t
     const int64_t cnt = G - g0 < chunk ? G - g0 : chunk;
     if (sorted) {
       if (slots > 0)
-        hipLaunchKernelGGL(data_pack_kernel, dim3((unsigned)(slots / ABC_DATA_WG), (unsigned)cnt),
-                         dim3(ABC_DATA_WG), 0, c->stream, n, u, l, g0, m + 2 * n, np,
-                         m + 2 * n + np, nch, perm, perm + total, slots,
-                         P<unsigned long long>(c->dxs), reinterpret_cast<unsigned int*>(flag));
+        hipLaunchKernelGGL(data_pack_kernel,
+                           dim3((unsigned)((cnt * (slots / ABC_DATA_WG) + 7) / 8 * 8)),
+                           dim3(ABC_DATA_WG), 0, c->stream, n, u, l, g0, m + 2 * n, np,
+                           m + 2 * n + np, nch, perm, perm + total, slots, cnt,
+                           P<unsigned long long>(c->dxs), reinterpret_cast<unsigned int*>(flag));
       HIPCHK(c, hipGetLastError());
       const int spb = ABC_DATA_SETS_PER_WG;
       const int nsb = (n_boot + 1 + spb - 1) / spb;

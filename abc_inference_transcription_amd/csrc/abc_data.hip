@@ -45,6 +45,14 @@ ABC_DEV int64_t pos_of(const u4& w, int h, int64_t m) {
   return (int64_t)__umul64hi(u, (uint64_t)m);
 }
 
+// the same for m < 2^32: with u = hi 2^32 + lo, floor(u m / 2^64) =
+// floor((hi m + floor(lo m / 2^32)) / 2^32), since the low word of lo m
+// cannot carry the sum past a multiple of 2^64
+ABC_DEV uint32_t pos_of32(const u4& w, int h, uint32_t m) {
+  const uint32_t lo = h == 0 ? w.x : w.z, hi = h == 0 ? w.y : w.w;
+  return (uint32_t)(((uint64_t)hi * m + __umulhi(lo, m)) >> 32);
+}
+
 // (n Sxy - Sx Sy) / (n (n - 1)) from exact 128-bit products
 ABC_DEV double cov_int(int64_t n, uint64_t Sx, uint64_t Sy, uint64_t Sxy) {
   const uint64_t ahi = __umul64hi((uint64_t)n, Sxy), alo = (uint64_t)n * Sxy;
@@ -277,14 +285,19 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_kernel(
 // data_stats_sorted_kernel walks a contiguous run of the sorted cells while
 // the 256 lanes read 256 consecutive words.  A count of 2^20 or more in a
 // cell with a group sets *bad_out (the same cells the data's own set sums).
+// Workgroup L packs block (item % (slots / 256)) of gene item / (slots /
+// 256), item = (L % 8) * (gridDim.x / 8) + L / 8, so that one XCD reads
+// each gene's counts.
 __global__ __launch_bounds__(256) void data_pack_kernel(
     int64_t n, const uint32_t* __restrict__ u, const uint32_t* __restrict__ l, int64_t g0,
     const int32_t* __restrict__ pulse, int64_t np, const int32_t* __restrict__ chase, int64_t nch,
     const int32_t* __restrict__ perm, const int32_t* __restrict__ grp, int64_t slots,
-    unsigned long long* __restrict__ xs, unsigned int* __restrict__ bad_out) {
-  const int64_t s = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (s >= slots) return;
-  const int64_t gl = blockIdx.y, g = g0 + gl;
+    int64_t genes, unsigned long long* __restrict__ xs, unsigned int* __restrict__ bad_out) {
+  const int64_t item = (int64_t)(blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+  const int64_t nblk = slots / 256;
+  if (item >= genes * nblk) return;
+  const int64_t gl = item / nblk, g = g0 + gl;
+  const int64_t s = (item - gl * nblk) * 256 + threadIdx.x;
   const int64_t c0 = (np + 255) / 256, c1 = (nch + 255) / 256, c2 = (n + 255) / 256;
   int64_t f, at, m, base, cf;
   if (s < 256 * c0) { f = 0; at = s; m = np; base = 0; cf = c0; }
@@ -356,11 +369,11 @@ __global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
         __syncthreads();
         for (int64_t q = tid; 2 * q < m; q += ABC_DATA_WG) {
           const u4 w = philox((uint32_t)q, (uint32_t)b, (uint32_t)g, (uint32_t)f, k0, k1);
-          const int64_t p0 = pos_of(w, 0, m);
-          atomicAdd(&mult_sh[p0 >> 1], 1u << (16 * (int)(p0 & 1)));
+          const uint32_t p0 = pos_of32(w, 0, (uint32_t)m);
+          atomicAdd(&mult_sh[p0 >> 1], 1u << (16 * (p0 & 1)));
           if (2 * q + 1 < m) {
-            const int64_t p1 = pos_of(w, 1, m);
-            atomicAdd(&mult_sh[p1 >> 1], 1u << (16 * (int)(p1 & 1)));
+            const uint32_t p1 = pos_of32(w, 1, (uint32_t)m);
+            atomicAdd(&mult_sh[p1 >> 1], 1u << (16 * (p1 & 1)));
           }
         }
       }

@@ -162,7 +162,8 @@ __global__ void data_stats_kernel(int64_t G, int64_t g0, int64_t n, const uint32
 __global__ void data_pack_kernel(int64_t n, const uint32_t* u, const uint32_t* l, int64_t g0,
                                  const int32_t* pulse, int64_t np, const int32_t* chase,
                                  int64_t nch, const int32_t* perm, const int32_t* grp,
-                                 int64_t slots, unsigned long long* xs, unsigned int* bad_out);
+                                 int64_t slots, int64_t genes, unsigned long long* xs,
+                                 unsigned int* bad_out);
 __global__ void data_stats_sorted_kernel(int64_t genes, int nsb, int64_t g0,
                                          const unsigned long long* xs, int64_t slots, int64_t n,
                                          int64_t np, int64_t nch, int nc, int J,
