@@ -67,15 +67,6 @@ ABC_DEV double cov_int(int64_t n, uint64_t Sx, uint64_t Sy, uint64_t Sxy) {
   return -(u128_to_f64(hi, lo) / den);
 }
 
-ABC_DEV double wcov(const double* x, const double* y, const double* w, int n) {
-  double mx = w[0] * x[0], my = w[0] * y[0];
-  for (int i = 1; i < n; ++i) mx = mx + w[i] * x[i];
-  for (int i = 1; i < n; ++i) my = my + w[i] * y[i];
-  double acc = w[0] * ((x[0] - mx) * (y[0] - my));
-  for (int i = 1; i < n; ++i) acc = acc + w[i] * ((x[i] - mx) * (y[i] - my));
-  return acc;
-}
-
 constexpr int MC = ABC_DATA_MAX_CLUSTERS, MJ = ABC_DATA_MAX_CONDS;
 
 struct DataLds {
@@ -114,37 +105,57 @@ ABC_DEV void finish_set(const DataLds& S, int tid, int b, int nc, int J,
     }
     st[4 * nc + j] = r;
   } else if (tid < 2 * nc + 2 * J) {
+    // one condition: the per-cluster means and weights are formed again from
+    // the sums in a second pass instead of being held in arrays, whose
+    // registers would bound every kernel that finishes sets
     const int j = tid - 2 * nc - J;
-    double m1[MC], m2[MC], v1[MC], v2[MC], c12[MC], w[MC];
-    bool any_cov = false;
     int64_t tot = 0;
     for (int c = 0; c < nc; ++c) tot += (int64_t)S.cor[0][j][c];
-    for (int c = 0; c < nc; ++c) {
+    auto cluster = [&](int c, double& m1, double& m2, double& w) {
       const int64_t cnt = (int64_t)S.cor[0][j][c];
-      if (cnt == 0) {
-        m1[c] = m2[c] = v1[c] = v2[c] = c12[c] = 0.0;
-      } else {
+      m1 = cnt == 0 ? 0.0 : (double)S.cor[1][j][c] / (double)cnt;
+      m2 = cnt == 0 ? 0.0 : (double)S.cor[2][j][c] / (double)cnt;
+      w = b < 0 ? w_point[c + (int64_t)nc * j] : (tot > 0 ? (double)cnt / (double)tot : 0.0);
+      return cnt;
+    };
+    // weighted within-cluster moments and the weighted means of the cluster
+    // means (weighted_cov's first two sums)
+    bool any_cov = false;
+    double s1 = 0.0, s2 = 0.0, s12 = 0.0, mx1 = 0.0, mx2 = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      double m1, m2, w;
+      const int64_t cnt = cluster(c, m1, m2, w);
+      double v1 = 0.0, v2 = 0.0, c12 = 0.0;
+      if (cnt > 0) {
         const uint64_t su = S.cor[1][j][c], sl = S.cor[2][j][c];
-        m1[c] = (double)su / (double)cnt;
-        m2[c] = (double)sl / (double)cnt;
-        v1[c] = cov_int(cnt, su, su, S.cor[3][j][c]);
-        v2[c] = cov_int(cnt, sl, sl, S.cor[4][j][c]);
-        c12[c] = cov_int(cnt, su, sl, S.cor[5][j][c]);
+        v1 = cov_int(cnt, su, su, S.cor[3][j][c]);
+        v2 = cov_int(cnt, sl, sl, S.cor[4][j][c]);
+        c12 = cov_int(cnt, su, sl, S.cor[5][j][c]);
       }
-      if (!(c12[c] == 0.0)) any_cov = true;
-      if (b < 0) w[c] = w_point[c + (int64_t)nc * j];
-      else w[c] = tot > 0 ? (double)cnt / (double)tot : 0.0;
+      if (!(c12 == 0.0)) any_cov = true;
+      if (c == 0) {
+        s1 = w * v1; s2 = w * v2; s12 = w * c12; mx1 = w * m1; mx2 = w * m2;
+      } else {
+        s1 = s1 + w * v1; s2 = s2 + w * v2; s12 = s12 + w * c12;
+        mx1 = mx1 + w * m1; mx2 = mx2 + w * m2;
+      }
     }
-    double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
-    for (int c = 1; c < nc; ++c) {
-      s1 = s1 + w[c] * v1[c];
-      s2 = s2 + w[c] * v2[c];
-      s12 = s12 + w[c] * c12[c];
+    // weighted_cov(m1, m1), (m2, m2), (m1, m2): the weighted products of deviations
+    double q11 = 0.0, q22 = 0.0, q12 = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      double m1, m2, w;
+      cluster(c, m1, m2, w);
+      const double d1 = m1 - mx1, d2 = m2 - mx2;
+      if (c == 0) {
+        q11 = w * (d1 * d1); q22 = w * (d2 * d2); q12 = w * (d1 * d2);
+      } else {
+        q11 = q11 + w * (d1 * d1); q22 = q22 + w * (d2 * d2); q12 = q12 + w * (d1 * d2);
+      }
     }
-    const double tv1 = s1 + wcov(m1, m1, w, nc), tv2 = s2 + wcov(m2, m2, w, nc);
+    const double tv1 = s1 + q11, tv2 = s2 + q22;
     const bool nz = !(tv1 == 0.0) && !(tv2 == 0.0);
     st[4 * nc + J + j] = (any_cov && nz) ? s12 / sqrt(fabs(tv1 * tv2)) : 0.0;
-    st[4 * nc + 2 * J + j] = nz ? wcov(m1, m2, w, nc) / sqrt(fabs(tv1 * tv2)) : 0.0;
+    st[4 * nc + 2 * J + j] = nz ? q12 / sqrt(fabs(tv1 * tv2)) : 0.0;
   }
 }
 
@@ -335,8 +346,10 @@ __global__ __launch_bounds__(256) void data_pack_kernel(
 // count-weighted values in registers and adds them to the group's sums
 // where its run changes group.  The integer sums are those of
 // data_stats_kernel, regrouped, so the statistics are identical.
-// data_se_kernel then forms the standard errors.
-__global__ __launch_bounds__(ABC_DATA_WG) void data_stats_sorted_kernel(
+// data_se_kernel then forms the standard errors.  Eight waves per SIMD
+// (64 VGPRs): the few values spilled are per set, not per cell or draw.
+__global__ __launch_bounds__(ABC_DATA_WG) __attribute__((amdgpu_waves_per_eu(8))) void
+data_stats_sorted_kernel(
     int64_t genes, int nsb, int64_t g0, const unsigned long long* __restrict__ xs, int64_t slots,
     int64_t n, int64_t np, int64_t nch, int nc, int J, const double* __restrict__ w_point,
     int n_boot, int spb, uint32_t k0, uint32_t k1, double* __restrict__ pt,
