@@ -56,7 +56,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs, selb;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -476,7 +476,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->stats, &c->moments, &c->superset, &c->edata, &c->ese, &c->err,
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
-                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt, &c->dxs,
+                    &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt, &c->dxs, &c->selb,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -1250,10 +1250,25 @@ int32_t abc_model_probs(abc_ctx* c, int64_t G, int32_t M, const int64_t* counts,
   if ((rc = ensure(c, c->selo, sizeof(double) * 3 * gm))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->selc.p, counts, sizeof(int64_t) * gm, hipMemcpyHostToDevice, c->stream));
   double* o = P<double>(c->selo);
-  hipLaunchKernelGGL(model_prob_kernel, dim3((unsigned)G), dim3(ABC_SEL_WG), 0, c->stream, G, M,
-                     P<int64_t>(c->selc), (uint32_t)seed, (uint32_t)(seed >> 32) ^ 0x5851F42Du,
-                     n_boot, alpha, o, o + gm, o + 2 * gm);
-  HIPCHK(c, hipGetLastError());
+  // the resampled shares of a launch's genes stay within 256 MiB
+  const int64_t per_gene = (int64_t)M * n_boot * (int64_t)sizeof(double);
+  int64_t chunk = (256ll << 20) / per_gene;
+  if (chunk > G) chunk = G;
+  if ((rc = ensure(c, c->selb, (size_t)(chunk * per_gene)))) return rc;
+  const int rpb = ABC_SEL_BOOT_PER_WG;
+  const unsigned parts = (unsigned)((n_boot + rpb - 1) / rpb);
+  for (int64_t g0 = 0; g0 < G; g0 += chunk) {
+    const int64_t cnt = G - g0 < chunk ? G - g0 : chunk;
+    hipLaunchKernelGGL(model_prob_kernel, dim3((unsigned)cnt, parts), dim3(ABC_SEL_WG), 0,
+                       c->stream, G, g0, M, P<int64_t>(c->selc), (uint32_t)seed,
+                       (uint32_t)(seed >> 32) ^ 0x5851F42Du, n_boot, rpb, P<double>(c->selb), o,
+                       o + gm, o + 2 * gm);
+    HIPCHK(c, hipGetLastError());
+    hipLaunchKernelGGL(model_prob_bounds_kernel, dim3((unsigned)cnt), dim3(ABC_SEL_WG), 0,
+                       c->stream, G, g0, M, P<int64_t>(c->selc), n_boot, alpha,
+                       P<double>(c->selb), o, o + gm, o + 2 * gm);
+    HIPCHK(c, hipGetLastError());
+  }
   HIPCHK(c, hipMemcpyAsync(prob, o, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(lb, o + gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(ub, o + 2 * gm, sizeof(double) * gm, hipMemcpyDeviceToHost, c->stream));

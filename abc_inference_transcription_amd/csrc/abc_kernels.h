@@ -175,9 +175,13 @@ __global__ void data_se_kernel(int64_t G, int64_t g0, int64_t genes, int nc, int
 #define ABC_SEL_WG 256
 #define ABC_SEL_MAX_GROUPS 8
 #define ABC_SEL_MAX_BOOT 256
-__global__ void model_prob_kernel(int64_t G, int M, const int64_t* counts, uint32_t k0,
-                                  uint32_t k1, int n_boot, double alpha, double* prob, double* lb,
-                                  double* ub);
+#define ABC_SEL_BOOT_PER_WG 8  // resamples per workgroup of model_prob_kernel
+__global__ void model_prob_kernel(int64_t G, int64_t g0, int M, const int64_t* counts,
+                                  uint32_t k0, uint32_t k1, int n_boot, int rpb, double* shares,
+                                  double* prob, double* lb, double* ub);
+__global__ void model_prob_bounds_kernel(int64_t G, int64_t g0, int M, const int64_t* counts,
+                                         int n_boot, double alpha, const double* shares,
+                                         double* prob, double* lb, double* ub);
 __global__ void stats_kernel(const DesignDev* D, const uint32_t* cu, const uint32_t* cl,
                              const uint8_t* cb, int64_t n, double* stats_ref, double* moments,
                              double* superset);

@@ -1,6 +1,6 @@
 """abc_model_probs (get_model_probs of scripts/model_probs.jl, per gene)
 against the oracle's replay of the same bootstrap: bit-exact probabilities
-and bounds, for 2, 3 and 5 model groups, the reference's 100 resamples and
+and bounds, for 2, 3, 5 and 8 model groups, the reference's 100 resamples and
 other counts, empty and single-group genes, and large accepted sets."""
 import numpy as np
 import pytest
@@ -25,7 +25,8 @@ def _counts(rng, G, M):
 
 
 @pytest.mark.parametrize("M,n_boot,alpha,seed", [(2, 100, 0.95, 0), (3, 100, 0.95, 7),
-                                                  (5, 256, 0.9, 11), (2, 1, 0.95, 3)])
+                                                  (5, 256, 0.9, 11), (2, 1, 0.95, 3),
+                                                  (8, 100, 0.95, 5)])
 def test_model_probs_bit_exact(oracle, eng, M, n_boot, alpha, seed):
     c = _counts(np.random.default_rng(M * 100 + n_boot), 300, M)
     got = eng.model_probs(c, seed=seed, n_boot=n_boot, alpha=alpha)
