@@ -385,7 +385,7 @@ def model_probs_leg(eng, stream, a):
     return res
 
 
-DATA_OPS_PER_DRAW = 50  # half a Philox4x32-10 block (30), a 64 x 64 multiply-high, checks, LDS atomics
+DATA_OPS_PER_DRAW = 50  # half a Philox4x32-10 block (30), a multiply-high, the draw count, the cell's sums
 
 
 def data_stats_leg(eng, stream, a):

@@ -137,3 +137,23 @@ def test_data_stats_large_family_path(oracle, eng):
     got = eng.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
     want = oracle.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
     assert _same_bits(got, want)
+
+
+def test_data_stats_whole_lanes(oracle, eng):
+    """Families whose sizes are multiples of 256 (every lane of the packed
+    layout holds cells, no padding) and one of fewer than 256 cells (at most
+    one cell per lane)."""
+    rng = np.random.default_rng(21)
+    n, G = 768, 4
+    age = rng.integers(0, NC, n)
+    cond = rng.integers(0, J, n)
+    u = rng.poisson(3.0, (n, G))
+    l = rng.poisson(1.0, (n, G))
+    pulse = np.sort(rng.choice(n, 512, replace=False))
+    chase = np.sort(rng.choice(n, 200, replace=False))
+    w = rng.random((NC, J))
+    w /= w.sum(axis=0)
+    print(f"\n[data_stats] whole lanes n={n} pulse=512 chase=200", flush=True)
+    got = eng.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=17, seed=6)
+    want = oracle.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=17, seed=6)
+    assert _same_bits(got, want)
