@@ -157,7 +157,7 @@ __global__ void data_stats_kernel(int64_t G, int64_t g0, int64_t n, const uint32
                                   const int32_t* pulse, int64_t np, const int32_t* chase,
                                   int64_t nch, const double* w_point, int n_boot, uint32_t k0,
                                   uint32_t k1, double* boot, double* out, unsigned int* bad_out);
-#define ABC_DATA_SORTED_MAX 8192  // cells per family for data_stats_sorted_kernel
+#define ABC_DATA_SORTED_MAX 16384  // cells per family for data_stats_sorted_kernel (32 KiB of draw counters)
 #define ABC_DATA_SETS_PER_WG 8    // sets per workgroup of data_stats_sorted_kernel
 __global__ void data_pack_kernel(int64_t n, const uint32_t* u, const uint32_t* l, int64_t g0,
                                  const int32_t* pulse, int64_t np, const int32_t* chase,

@@ -129,14 +129,16 @@ def test_data_stats_rejects_bad_arguments(eng):
 
 
 def test_data_stats_large_family_path(oracle, eng):
-    """Families of more than 8,192 cells take the per-draw path
+    """Families of more than 16,384 cells take the per-draw path
     (data_stats_kernel); smaller ones count draws per cell first
-    (data_stats_sorted_kernel).  Both equal the CPU restatement."""
-    u, l, age, cond, pulse, chase, w = synthetic(8, n=9000, G=3)
-    print("\n[data_stats] 9,000 cells (per-draw path)", flush=True)
-    got = eng.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
-    want = oracle.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=12, seed=4)
-    assert _same_bits(got, want)
+    (data_stats_sorted_kernel), up to 16,384 cells per family.  Both equal
+    the CPU restatement."""
+    for n, B in ((16500, 6), (12000, 10)):
+        u, l, age, cond, pulse, chase, w = synthetic(8, n=n, G=2)
+        print(f"\n[data_stats] {n} cells", flush=True)
+        got = eng.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=B, seed=4)
+        want = oracle.data_stats(u, l, age, cond, NC, J, pulse, chase, w, n_boot=B, seed=4)
+        assert _same_bits(got, want), n
 
 
 def test_data_stats_whole_lanes(oracle, eng):
