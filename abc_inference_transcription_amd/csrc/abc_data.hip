@@ -3,8 +3,8 @@
 // get_ccp_se, get_ratios, get_ratio_se, get_correlations,
 // get_correlation_se, weighted_cov, get_summary_stats).
 //
-// One workgroup per gene.  From the integer counts u, l of the n cells (age
-// cluster 0..nc-1, condition 0..J-1 or -1) it forms the statistics of the
+// For each gene, from the integer counts u, l of the n cells (age cluster
+// 0..nc-1, condition 0..J-1 or -1), the kernels form the statistics of the
 // data themselves, then of n_boot resamples with replacement: four
 // independent ones per resample as in the reference -- the pulse cells
 // (mean and Fano factor of u + l per cluster), the chase cells (the same),
