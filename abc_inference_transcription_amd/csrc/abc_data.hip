@@ -347,7 +347,7 @@ __global__ __launch_bounds__(256) void data_pack_kernel(
 // where its run changes group.  The integer sums are those of
 // data_stats_kernel, regrouped, so the statistics are identical.
 // data_se_kernel then forms the standard errors.  Eight waves per SIMD
-// (64 VGPRs): the few values spilled are per set, not per cell or draw.
+// (64 VGPRs): the values spilled are kept per set, outside the draw and cell loops.
 __global__ __launch_bounds__(ABC_DATA_WG) __attribute__((amdgpu_waves_per_eu(8))) void
 data_stats_sorted_kernel(
     int64_t genes, int nsb, int64_t g0, const unsigned long long* __restrict__ xs, int64_t slots,
@@ -374,6 +374,9 @@ data_stats_sorted_kernel(
       else (&S.cor[0][0][0])[i - 2 * 3 * MC - 3 * MJ] = 0ull;
     }
     int64_t fb = 0;  // the family's first slot
+    // unrolled: each family's cell walk is compiled with its own sums and no
+    // branch on the family per cell
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
       const int64_t m = f == 0 ? np : (f == 1 ? nch : n);
       const int64_t cf = (m + ABC_DATA_WG - 1) / ABC_DATA_WG;
