@@ -87,6 +87,8 @@ def _declare(L):
         "abc_ssa_time": (I32, [V, I32, V, V, V]),
         "abc_set_ssa_blocks": (I32, [V, I32]),
         "abc_get_ssa_blocks": (I32, [V]),
+        "abc_set_ssa_claim": (I32, [V, I32]),
+        "abc_get_ssa_claim": (I32, [V]),
         "abc_autotune_ssa": (I32, [V, I32, V, U64, I64, I64, V, V]),
         "abc_graph_begin": (I32, [V]),
         "abc_graph_end": (I32, [V, C.POINTER(V)]),
@@ -138,6 +140,7 @@ def _declare_optional(L):
         "abc_features_dev": (I32, [I32, I64, V, I32, V, V]),
         "abc_targets_dev": (I32, [I64, V, I32, V, V, V, V]),
         "abc_screen_dev": (I32, [V, V, I64, V, V, V, D, D, V, V]),
+        "abc_screen_score_dev": (I32, [V, V, I64, V, V, V, D, V]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):

@@ -15,7 +15,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libabc.so")
-SOURCES = ["abc_kernels.hip", "abc_capi.hip", "abc_gemm.hip", "abc_mlp.hip", "abc_moments.hip", "abc_select.hip", "abc_data.hip"]
+SOURCES = ["abc_kernels.hip", "abc_capi.hip", "abc_gemm.hip", "abc_accept.hip", "abc_mlp.hip", "abc_moments.hip", "abc_select.hip", "abc_data.hip"]
 HEADERS = ["abc_device.h", "abc_kernels.h", "abc_logtab.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -36,6 +36,10 @@ struct AcceptState {
   int64_t k = 0;
   int64_t n_candidates = 0, n_screened = 0;
   unsigned long long cand_cap = 0;
+  // top-k (mode 1): per-gene pools of capg = k + n_sub entries in device
+  // memory; batches are screened n_sub rows at a time
+  bool pooled = false;
+  int64_t n_sub = 0, capg = 0;
   std::vector<double> thr_host;                                // per-gene threshold
   std::vector<std::vector<std::pair<double, int64_t>>> kept;   // per-gene (err, index)
 };
@@ -48,6 +52,7 @@ struct abc_ctx {
   bool own_stream = true;
   int num_cu = 0;
   int ssa_blocks = 0;        // 0 = occupancy default
+  int ssa_claim = 1;         // SSA work items a lane claims per queue atomic
   int ssa_blocks_default = 0;
   int burn_blocks = 0;       // occupancy default of burn_kernel
   DesignDev hd{};
@@ -56,7 +61,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs, selb;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, pool, pooln, poolb, presc, srow, apidx, thrlow, spass, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs, selb;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -67,6 +72,9 @@ struct abc_ctx {
   int ring_used = 0;
   double burn_ms_acc = 0.0, ssa_ms_acc = 0.0;
   int64_t ssa_launches = 0;
+  // bumped whenever ensure() replaces a buffer: graphs captured before hold
+  // the old address and must not replay
+  uint64_t gen = 0;
   // stream capture (abc_graph_begin .. abc_graph_end)
   bool capturing = false;
   bool capture_failed = false;
@@ -75,6 +83,8 @@ struct abc_ctx {
 
 struct abc_graph {
   hipGraphExec_t exec = nullptr;
+  const abc_ctx* ctx = nullptr;  // the context whose stream was captured
+  uint64_t gen = 0;              // its buffer generation at capture
 };
 
 namespace {
@@ -119,6 +129,7 @@ int ensure(abc_ctx* c, DevBuf& b, size_t bytes) {
     hipFree(b.p);
     b.p = nullptr;
     b.bytes = 0;
+    c->gen++;
   }
   size_t want = bytes + bytes / 4;
   if (hipMalloc(&b.p, want) != hipSuccess) {
@@ -148,64 +159,150 @@ int check_range(abc_ctx* c, int model, int64_t first, int64_t n) {
 
 int grid1(int64_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
-// buffers of the rates, the queue order and the item offsets of n particles
-int ensure_order(abc_ctx* c, int64_t n) {
-  int rc;
+// grid of the kernels that loop over particles one workgroup at a time: one
+// workgroup per particle up to 2^20 (a grid stays far below 2^32 lanes)
+unsigned pgrid(int64_t n) { return (unsigned)std::min<int64_t>(n, (int64_t)1 << 20); }
+
+// Device memory one simulation of n particles works in: per-particle rates,
+// queue order and item offsets, each cell's burn-in record, sub-lane offsets
+// and observed counts, and the counters.  Filled from the context's buffers
+// (ws_from_ctx) or carved from one block of memory (ws_carve).
+struct SimWs {
+  PRates* rates = nullptr;
+  float* cost = nullptr;
+  unsigned* hist = nullptr;             // 2 * ABC_NBUCKET: histogram, cursors
+  int32_t* order = nullptr;
+  int64_t* bsum = nullptr;
+  int64_t* off = nullptr;               // n + 1
+  CellStart* cellrec = nullptr;         // n * C
+  int32_t* coff = nullptr;              // n * (C + 1)
+  int64_t* pitems = nullptr;
+  uint32_t* cu = nullptr;               // n * C, rounded up to whole u64 words
+  uint32_t* cl = nullptr;
+  uint8_t* cb = nullptr;                // n * C
+  unsigned long long* react = nullptr;
+  unsigned long long* queue = nullptr;  // 8 words: burn-in and SSA work queues
+  unsigned long long* tally = nullptr;  // 2 words: burn-in draws
+  unsigned long long* steps = nullptr;  // n: draws per particle when the caller keeps none
+  bool timed = true;                    // burn-in / SSA events into the timing ring
+};
+
+int64_t cell_words(int64_t n, int C) { return (n * (int64_t)C + 1) / 2; }
+
+// Bytes of a carved workspace for n particles of C cells (each array aligned
+// to 256 bytes); with base != NULL the pointers of w are set into it.
+size_t ws_carve(char* base, int64_t n, int C, SimWs* w) {
+  const size_t nn = (size_t)n, cells = (size_t)n * (size_t)C;
   const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
-  if ((rc = ensure(c, c->rates, sizeof(PRates) * n))) return rc;
-  if ((rc = ensure(c, c->cost, sizeof(float) * n))) return rc;
+  size_t at = 0;
+  auto take = [&](size_t bytes) -> char* {
+    char* q = base ? base + at : nullptr;
+    at += (std::max<size_t>(bytes, 16) + 255) / 256 * 256;
+    return q;
+  };
+  SimWs v;
+  v.rates = reinterpret_cast<PRates*>(take(sizeof(PRates) * nn));
+  v.cost = reinterpret_cast<float*>(take(sizeof(float) * nn));
+  v.hist = reinterpret_cast<unsigned*>(take(sizeof(unsigned) * 2 * ABC_NBUCKET));
+  v.order = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn));
+  v.bsum = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (size_t)nblk));
+  v.off = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * (nn + 1)));
+  v.cellrec = reinterpret_cast<CellStart*>(take(sizeof(CellStart) * cells));
+  v.coff = reinterpret_cast<int32_t*>(take(sizeof(int32_t) * nn * (size_t)(C + 1)));
+  v.pitems = reinterpret_cast<int64_t*>(take(sizeof(int64_t) * nn));
+  v.cu = reinterpret_cast<uint32_t*>(take(sizeof(uint64_t) * (size_t)cell_words(n, C)));
+  v.cl = reinterpret_cast<uint32_t*>(take(sizeof(uint64_t) * (size_t)cell_words(n, C)));
+  v.cb = reinterpret_cast<uint8_t*>(take(cells));
+  v.react = reinterpret_cast<unsigned long long*>(take(sizeof(uint64_t) * nn));
+  v.queue = reinterpret_cast<unsigned long long*>(take(64));
+  v.tally = reinterpret_cast<unsigned long long*>(take(16));
+  v.steps = reinterpret_cast<unsigned long long*>(take(sizeof(uint64_t) * nn));
+  if (base && w) *w = v;
+  return at;
+}
+
+// The context's buffers, grown to n particles.  The burn-in tally is created
+// cleared once (abc_ssa_time reads and clears it).
+int ws_from_ctx(abc_ctx* c, int64_t n, SimWs* w) {
+  int rc;
+  const int C = c->hd.C;
+  const size_t nn = (size_t)n, cells = (size_t)n * (size_t)C;
+  const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
+  if ((rc = ensure(c, c->rates, sizeof(PRates) * nn))) return rc;
+  if ((rc = ensure(c, c->cost, sizeof(float) * nn))) return rc;
   if ((rc = ensure(c, c->hist, sizeof(unsigned) * 2 * ABC_NBUCKET))) return rc;
-  if ((rc = ensure(c, c->order, sizeof(int32_t) * n))) return rc;
-  if ((rc = ensure(c, c->bsum, sizeof(int64_t) * nblk))) return rc;
-  if ((rc = ensure(c, c->off, sizeof(int64_t) * (n + 1)))) return rc;
+  if ((rc = ensure(c, c->order, sizeof(int32_t) * nn))) return rc;
+  if ((rc = ensure(c, c->bsum, sizeof(int64_t) * (size_t)nblk))) return rc;
+  if ((rc = ensure(c, c->off, sizeof(int64_t) * (nn + 1)))) return rc;
+  if ((rc = ensure(c, c->cellrec, sizeof(CellStart) * cells))) return rc;
+  if ((rc = ensure(c, c->coff, sizeof(int32_t) * nn * (size_t)(C + 1)))) return rc;
+  if ((rc = ensure(c, c->pitems, sizeof(int64_t) * nn))) return rc;
+  if ((rc = ensure(c, c->cu, sizeof(uint64_t) * (size_t)cell_words(n, C)))) return rc;
+  if ((rc = ensure(c, c->cl, sizeof(uint64_t) * (size_t)cell_words(n, C)))) return rc;
+  if ((rc = ensure(c, c->cb, cells))) return rc;
+  if ((rc = ensure(c, c->react, sizeof(uint64_t) * nn))) return rc;
+  if ((rc = ensure(c, c->queue, 64))) return rc;
+  if ((rc = ensure(c, c->steps, sizeof(uint64_t) * nn))) return rc;
+  if (!c->tally.p) {
+    if (c->capturing) return fail(c, ABC_E_STATE, "simulate once before capturing it");
+    if ((rc = ensure(c, c->tally, 16))) return rc;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                       P<unsigned long long>(c->tally), (int64_t)2);
+    HIPCHK(c, hipGetLastError());
+  }
+  w->rates = P<PRates>(c->rates);
+  w->cost = P<float>(c->cost);
+  w->hist = P<unsigned>(c->hist);
+  w->order = P<int32_t>(c->order);
+  w->bsum = P<int64_t>(c->bsum);
+  w->off = P<int64_t>(c->off);
+  w->cellrec = P<CellStart>(c->cellrec);
+  w->coff = P<int32_t>(c->coff);
+  w->pitems = P<int64_t>(c->pitems);
+  w->cu = P<uint32_t>(c->cu);
+  w->cl = P<uint32_t>(c->cl);
+  w->cb = P<uint8_t>(c->cb);
+  w->react = P<unsigned long long>(c->react);
+  w->queue = P<unsigned long long>(c->queue);
+  w->tally = P<unsigned long long>(c->tally);
+  w->steps = P<unsigned long long>(c->steps);
+  w->timed = true;
   return ABC_OK;
 }
 
 // rates of n particles of one model into rates/cost from index a on
-void launch_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev, int64_t a) {
+void launch_rates(abc_ctx* c, const SimWs& w, int model, int64_t n, const double* theta_dev,
+                  int64_t a) {
   hipLaunchKernelGGL(rates_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, c->dd, model, n,
-                     theta_dev, P<PRates>(c->rates) + a, P<float>(c->cost) + a);
+                     theta_dev, w.rates + a, w.cost + a);
 }
 
 // heaviest-first queue order of all n particles (bucket sort of the cost)
-int launch_order(abc_ctx* c, int64_t n) {
-  unsigned* hist = P<unsigned>(c->hist);
-  unsigned* cursor = hist + ABC_NBUCKET;
+int launch_order(abc_ctx* c, const SimWs& w, int64_t n) {
+  unsigned* cursor = w.hist + ABC_NBUCKET;
   hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(256), 0, c->stream,
-                     P<unsigned long long>(c->hist), (int64_t)(ABC_NBUCKET / 2));
+                     reinterpret_cast<unsigned long long*>(w.hist), (int64_t)(ABC_NBUCKET / 2));
   hipLaunchKernelGGL(bucket_hist_kernel, dim3(std::min(grid1(n, 256), 1024)), dim3(256), 0,
-                     c->stream, P<float>(c->cost), n, hist);
-  hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(ABC_NBUCKET), 0, c->stream, hist, cursor);
+                     c->stream, w.cost, n, w.hist);
+  hipLaunchKernelGGL(bucket_scan_kernel, dim3(1), dim3(ABC_NBUCKET), 0, c->stream, w.hist, cursor);
   hipLaunchKernelGGL(bucket_scatter_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
-                     P<float>(c->cost), n, cursor, P<int32_t>(c->order));
+                     w.cost, n, cursor, w.order);
   HIPCHK(c, hipGetLastError());
   return ABC_OK;
 }
 
-// rates + heaviest-first queue order; theta_dev: n x n_par
-int stage_rates(abc_ctx* c, int model, int64_t n, const double* theta_dev) {
-  int rc;
-  if ((rc = ensure_order(c, n))) return rc;
-  launch_rates(c, model, n, theta_dev, 0);
-  return launch_order(c, n);
-}
-
 // after the burn-in: each cell's sub-lanes -> per-particle cell offsets and
 // item counts -> the item offsets of the particles in queue order
-int stage_items(abc_ctx* c, int64_t n) {
-  int rc;
+int stage_items(abc_ctx* c, const SimWs& w, int64_t n) {
   const int64_t nblk = (n + ABC_SCAN_BLOCK - 1) / ABC_SCAN_BLOCK;
   const int C = c->hd.C;
-  if ((rc = ensure(c, c->coff, sizeof(int32_t) * (size_t)n * (size_t)(C + 1)))) return rc;
-  if ((rc = ensure(c, c->pitems, sizeof(int64_t) * (size_t)n))) return rc;
-  hipLaunchKernelGGL(cell_items_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, n, C,
-                     P<CellStart>(c->cellrec), P<int32_t>(c->coff), P<int64_t>(c->pitems));
-  hipLaunchKernelGGL(items_scan_kernel, dim3((unsigned)nblk), dim3(256), 0, c->stream,
-                     P<int64_t>(c->pitems), P<int32_t>(c->order), n, P<int64_t>(c->off),
-                     P<int64_t>(c->bsum));
-  hipLaunchKernelGGL(bsum_scan_kernel, dim3(1), dim3(256), 0, c->stream, P<int64_t>(c->bsum), nblk);
-  hipLaunchKernelGGL(add_bsum_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
-                     P<int64_t>(c->off), n, P<int64_t>(c->bsum));
+  hipLaunchKernelGGL(cell_items_kernel, dim3(pgrid(n)), dim3(256), 0, c->stream, n, C, w.cellrec,
+                     w.coff, w.pitems);
+  hipLaunchKernelGGL(items_scan_kernel, dim3((unsigned)nblk), dim3(256), 0, c->stream, w.pitems,
+                     w.order, n, w.off, w.bsum);
+  hipLaunchKernelGGL(bsum_scan_kernel, dim3(1), dim3(256), 0, c->stream, w.bsum, nblk);
+  hipLaunchKernelGGL(add_bsum_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, w.off, n,
+                     w.bsum);
   HIPCHK(c, hipGetLastError());
   return ABC_OK;
 }
@@ -225,43 +322,27 @@ int drain_ring(abc_ctx* c) {
   return ABC_OK;
 }
 
-// cells of n particles (rates and queue order staged): burn-in, sub-lane
+// cells of n particles (rates and queue order staged in w): burn-in, sub-lane
 // offsets, the window SSA.  Particle p is global particle pidx[p] (first + p
-// without pidx) with model tag mtags[p] (mtag_all without mtags).  Cell counts go to
-// cu / cl / cb, Philox draws per particle to steps.
-int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t mtag_all,
-                  const uint32_t* mtags, const int64_t* pidx_dev, unsigned long long* steps) {
+// without pidx) with model tag mtags[p] (mtag_all without mtags).  Cell
+// counts go to w.cu / w.cl / w.cb, Philox draws per particle to steps.
+int simulate_core(abc_ctx* c, const SimWs& w, uint64_t seed, int64_t n, int64_t first,
+                  uint32_t mtag_all, const uint32_t* mtags, const int64_t* pidx_dev,
+                  unsigned long long* steps) {
   int rc;
-  const size_t cells = (size_t)n * (size_t)c->hd.C;
-  const int64_t cell_words = (int64_t)((cells + 1) / 2);  // u32 counts cleared as u64 words
-  if ((rc = ensure(c, c->cu, sizeof(uint64_t) * cell_words))) return rc;
-  if ((rc = ensure(c, c->cl, sizeof(uint64_t) * cell_words))) return rc;
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(cell_words, 256)), dim3(256), 0, c->stream,
-                     P<unsigned long long>(c->cu), cell_words);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(cell_words, 256)), dim3(256), 0, c->stream,
-                     P<unsigned long long>(c->cl), cell_words);
-  if ((rc = ensure(c, c->cb, cells))) return rc;
-  if ((rc = ensure(c, c->cellrec, sizeof(CellStart) * cells))) return rc;
-  if ((rc = ensure(c, c->react, sizeof(uint64_t) * n))) return rc;
-  if ((rc = ensure(c, c->queue, 64))) return rc;
-  if (!c->tally.p) {
-    // created cleared, outside any capture; abc_ssa_time clears it again
-    if (c->capturing)
-      return fail(c, ABC_E_STATE, "simulate once before capturing it");
-    if ((rc = ensure(c, c->tally, 16))) return rc;
-    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
-                       P<unsigned long long>(c->tally), (int64_t)2);
-  }
+  const int64_t words = cell_words(n, c->hd.C);  // u32 counts cleared as u64 words
   // counters cleared by a kernel (no memset nodes, so the sequence can be captured)
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(words, 256)), dim3(256), 0, c->stream,
+                     reinterpret_cast<unsigned long long*>(w.cu), words);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(words, 256)), dim3(256), 0, c->stream,
+                     reinterpret_cast<unsigned long long*>(w.cl), words);
   hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, steps, n);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream,
-                     P<unsigned long long>(c->react), n);
-  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
-                     P<unsigned long long>(c->queue), (int64_t)8);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1(n, 256)), dim3(256), 0, c->stream, w.react, n);
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, w.queue, (int64_t)8);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, record(c, c->ev[1]));
   hipEvent_t* trio = nullptr;
-  if (!c->capturing) {
+  if (!c->capturing && w.timed) {
     if (c->ring_used == abc_ctx::kRing) {
       if ((rc = drain_ring(c))) return rc;
     }
@@ -270,24 +351,40 @@ int simulate_core(abc_ctx* c, uint64_t seed, int64_t n, int64_t first, uint32_t 
   }
   const int blocks = c->ssa_blocks > 0 ? c->ssa_blocks : c->ssa_blocks_default;
   hipLaunchKernelGGL(burn_kernel, dim3(c->burn_blocks), dim3(SSA_WG), 0, c->stream, c->dd,
-                     P<PRates>(c->rates), P<int32_t>(c->order), n, (uint32_t)first, key0(seed),
-                     key1(seed), mtag_all, mtags, c->d_betas, c->d_goff,
-                     P<unsigned long long>(c->queue),
-                     P<CellStart>(c->cellrec), steps, P<unsigned long long>(c->react), pidx_dev,
-                     P<unsigned long long>(c->tally));
+                     w.rates, w.order, n, (uint32_t)first, key0(seed), key1(seed), mtag_all,
+                     mtags, c->d_betas, c->d_goff, w.queue, w.cellrec, steps, w.react, pidx_dev,
+                     w.tally);
   HIPCHK(c, hipGetLastError());
-  if ((rc = stage_items(c, n))) return rc;
+  if ((rc = stage_items(c, w, n))) return rc;
   if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
-  hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd,
-                     P<PRates>(c->rates), P<int32_t>(c->order), P<int64_t>(c->off), n,
-                     (uint32_t)first, key0(seed), key1(seed), mtag_all, mtags,
-                     P<CellStart>(c->cellrec),
-                     P<int32_t>(c->coff), P<unsigned long long>(c->queue) + 1, P<uint32_t>(c->cu),
-                     P<uint32_t>(c->cl), P<uint8_t>(c->cb), steps, P<unsigned long long>(c->react),
-                     pidx_dev);
+  hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd, w.rates,
+                     w.order, w.off, n, (uint32_t)first, key0(seed), key1(seed), mtag_all, mtags,
+                     w.cellrec, w.coff, w.queue + 1, w.cu, w.cl, w.cb, steps, w.react, pidx_dev,
+                     (uint32_t)c->ssa_claim);
   HIPCHK(c, hipGetLastError());
   if (trio) HIPCHK(c, hipEventRecord(trio[2], c->stream));
   HIPCHK(c, record(c, c->ev[2]));
+  return ABC_OK;
+}
+
+// One model's particles through rates, queue order, burn-in, SSA and the
+// statistics, in workspace w.
+int simulate_ws(abc_ctx* c, const SimWs& w, int model, uint64_t seed, int64_t first, int64_t n,
+                const double* theta_dev, double* stats_dev, double* moments_dev,
+                double* superset_dev, int64_t* steps_dev, const int64_t* pidx_dev) {
+  int rc;
+  HIPCHK(c, record(c, c->ev[0]));
+  launch_rates(c, w, model, n, theta_dev, 0);
+  if ((rc = launch_order(c, w, n))) return rc;
+  unsigned long long* steps =
+      steps_dev ? reinterpret_cast<unsigned long long*>(steps_dev) : w.steps;
+  if ((rc = simulate_core(c, w, seed, n, first, mtag(model), nullptr, pidx_dev, steps)))
+    return rc;
+  hipLaunchKernelGGL(stats_kernel, dim3(pgrid(n)), dim3(256), 0, c->stream, c->dd, w.cu, w.cl,
+                     w.cb, n, stats_dev, moments_dev, superset_dev);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, record(c, c->ev[3]));
+  c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
   return ABC_OK;
 }
 
@@ -298,22 +395,10 @@ int simulate_dev(abc_ctx* c, int model, uint64_t seed, int64_t first, int64_t n,
   if ((rc = check_range(c, model, first, n))) return rc;
   if (n == 0) return ABC_OK;
   if (n > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
-  HIPCHK(c, record(c, c->ev[0]));
-  if ((rc = stage_rates(c, model, n, theta_dev))) return rc;
-  unsigned long long* steps = reinterpret_cast<unsigned long long*>(steps_dev);
-  if (!steps) {
-    if ((rc = ensure(c, c->steps, sizeof(uint64_t) * n))) return rc;
-    steps = P<unsigned long long>(c->steps);
-  }
-  if ((rc = simulate_core(c, seed, n, first, mtag(model), nullptr, pidx_dev, steps)))
-    return rc;
-  hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n), dim3(256), 0, c->stream, c->dd,
-                     P<uint32_t>(c->cu), P<uint32_t>(c->cl), P<uint8_t>(c->cb), n, stats_dev,
-                     moments_dev, superset_dev);
-  HIPCHK(c, hipGetLastError());
-  HIPCHK(c, record(c, c->ev[3]));
-  c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
-  return ABC_OK;
+  SimWs w;
+  if ((rc = ws_from_ctx(c, n, &w))) return rc;
+  return simulate_ws(c, w, model, seed, first, n, theta_dev, stats_dev, moments_dev,
+                     superset_dev, steps_dev, pidx_dev);
 }
 
 int make_spec(abc_ctx* c, int K, const int32_t* parts, int n_parts, double clamp, ErrSpec* E) {
@@ -477,6 +562,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
                     &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt, &c->dxs, &c->selb,
+                    &c->pool, &c->pooln, &c->poolb, &c->presc, &c->srow, &c->apidx, &c->thrlow, &c->spass,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -529,6 +615,16 @@ int32_t abc_set_ssa_blocks(abc_ctx* c, int32_t blocks) {
   c->ssa_blocks = blocks;
   return ABC_OK;
 }
+
+int32_t abc_set_ssa_claim(abc_ctx* c, int32_t claim) {
+  if (!c) return ABC_E_ARG;
+  if (claim != 1 && claim != 2 && claim != 4 && claim != 8)
+    return fail(c, ABC_E_ARG, "claim must be 1, 2, 4 or 8");
+  c->ssa_claim = claim;
+  return ABC_OK;
+}
+
+int32_t abc_get_ssa_claim(const abc_ctx* c) { return c ? c->ssa_claim : ABC_E_ARG; }
 
 int32_t abc_get_ssa_blocks(const abc_ctx* c) {
   if (!c) return ABC_E_ARG;
@@ -591,29 +687,29 @@ int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* mode
   if (total == 0) return ABC_OK;
   if (total > 0x7FFFFFFF) return fail(c, ABC_E_ARG, "at most 2^31 - 1 particles per call");
   HIPCHK(c, record(c, c->ev[0]));
-  if ((rc = ensure_order(c, total))) return rc;
+  SimWs w;
+  if ((rc = ws_from_ctx(c, total, &w))) return rc;
   if ((rc = ensure(c, c->mpidx, sizeof(int64_t) * total))) return rc;
   if ((rc = ensure(c, c->mk1, sizeof(uint32_t) * total))) return rc;
-  if ((rc = ensure(c, c->steps, sizeof(uint64_t) * total))) return rc;
   for (int m = 0; m < n_models; ++m) {
     if (n[m] == 0) continue;
-    launch_rates(c, models[m], n[m], theta_dev[m], a[m]);
+    launch_rates(c, w, models[m], n[m], theta_dev[m], a[m]);
     hipLaunchKernelGGL(fill_ids_kernel, dim3(grid1(n[m], 256)), dim3(256), 0, c->stream, n[m],
                        first[m], mtag(models[m]), P<int64_t>(c->mpidx) + a[m],
                        P<uint32_t>(c->mk1) + a[m]);
   }
   HIPCHK(c, hipGetLastError());
-  if ((rc = launch_order(c, total))) return rc;
-  unsigned long long* steps = P<unsigned long long>(c->steps);
-  if ((rc = simulate_core(c, seed, total, 0, 0u, P<uint32_t>(c->mk1), P<int64_t>(c->mpidx),
+  if ((rc = launch_order(c, w, total))) return rc;
+  unsigned long long* steps = w.steps;
+  if ((rc = simulate_core(c, w, seed, total, 0, 0u, P<uint32_t>(c->mk1), P<int64_t>(c->mpidx),
                           steps)))
     return rc;
   const size_t C = (size_t)c->hd.C;
   for (int m = 0; m < n_models; ++m) {
     if (n[m] == 0) continue;
     const size_t o = (size_t)a[m] * C;
-    hipLaunchKernelGGL(stats_kernel, dim3((unsigned)n[m]), dim3(256), 0, c->stream, c->dd,
-                       P<uint32_t>(c->cu) + o, P<uint32_t>(c->cl) + o, P<uint8_t>(c->cb) + o, n[m],
+    hipLaunchKernelGGL(stats_kernel, dim3(pgrid(n[m])), dim3(256), 0, c->stream, c->dd,
+                       w.cu + o, w.cl + o, w.cb + o, n[m],
                        stats_dev[m], moments_dev ? moments_dev[m] : nullptr,
                        superset_dev ? superset_dev[m] : nullptr);
     if (steps_dev && steps_dev[m])
@@ -732,9 +828,52 @@ struct GeneHit {
 
 int64_t pad_to(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
+// Top-k batches (mode 1): screened, rescored and selected per gene entirely
+// on the device, n_sub rows at a time, with no host wait.  pidx_host (n
+// global indices) is staged through device memory; without it row i is
+// particle first_index + i.
+int screen_batch_pooled(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index,
+                        const int64_t* pidx_host) {
+  AcceptState& A = c->acc;
+  const int grid = (int)std::min<int64_t>(A.G, (int64_t)c->num_cu * 8);
+  for (int64_t off = 0; off < n; off += A.n_sub) {
+    const int64_t nb = std::min(A.n_sub, n - off);
+    const int64_t npad = pad_to(nb, 64);
+    const int64_t* pidx_dev = nullptr;
+    if (pidx_host) {
+      HIPCHK(c, hipMemcpyAsync(c->apidx.p, pidx_host + off, sizeof(int64_t) * nb,
+                               hipMemcpyHostToDevice, c->stream));
+      pidx_dev = P<int64_t>(c->apidx);
+    }
+    hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
+                       stats_dev + off, nb, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah),
+                       P<__bf16>(c->al), P<uint8_t>(c->pflag), P<double>(c->srow));
+    hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
+                       dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), nb, A.Kp,
+                       P<__bf16>(c->bh), P<__bf16>(c->bl), A.G, P<double>(c->cg),
+                       P<uint8_t>(c->gflag), P<uint8_t>(c->pflag), P<double>(c->thr),
+                       A.margin_rel, A.E.clamp, (Candidate*)nullptr,
+                       (unsigned long long*)nullptr, 0ull, (float*)nullptr, (float*)nullptr,
+                       P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln));
+    hipLaunchKernelGGL(pool_rescore_kernel, dim3(grid), dim3(256), 0, c->stream,
+                       P<double>(c->srow), nb, P<double>(c->edata), P<double>(c->ese), A.G, A.E,
+                       P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln),
+                       P<uint32_t>(c->poolb), first_index + off, pidx_dev);
+    hipLaunchKernelGGL(pool_select_kernel, dim3(grid), dim3(256), 0, c->stream,
+                       P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln),
+                       P<uint32_t>(c->poolb), A.G, A.k, P<double>(c->thr),
+                       P<unsigned long long>(c->presc));
+    HIPCHK(c, hipGetLastError());
+    if (pidx_host) HIPCHK(c, hipStreamSynchronize(c->stream));  // apidx is reused
+  }
+  A.n_screened += n;
+  return ABC_OK;
+}
+
 int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index,
                  const int64_t* pidx_host = nullptr) {
   AcceptState& A = c->acc;
+  if (A.pooled) return screen_batch_pooled(c, n, stats_dev, first_index, pidx_host);
   int rc;
   const int64_t npad = pad_to(n, 64);
   if ((rc = ensure(c, c->ah, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
@@ -742,8 +881,8 @@ int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_i
   if ((rc = ensure(c, c->pflag, npad))) return rc;
   if ((rc = ensure(c, c->ccount, 64))) return rc;
   hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
-                     stats_dev, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
-                     P<uint8_t>(c->pflag));
+                     stats_dev, n, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
+                     P<uint8_t>(c->pflag), (double*)nullptr);
   HIPCHK(c, hipGetLastError());
   unsigned long long cap = std::max<unsigned long long>(A.cand_cap, 1ull << 16);
   for (;;) {
@@ -755,7 +894,8 @@ int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_i
                        P<__bf16>(c->bh), P<__bf16>(c->bl), A.G, P<double>(c->cg),
                        P<uint8_t>(c->gflag), P<uint8_t>(c->pflag), P<double>(c->thr),
                        A.margin_rel, A.E.clamp, P<Candidate>(c->cand),
-                       P<unsigned long long>(c->ccount), cap, (float*)nullptr, (float*)nullptr);
+                       P<unsigned long long>(c->ccount), cap, (float*)nullptr, (float*)nullptr,
+                       (PoolEntry*)nullptr, (int64_t)0, (uint32_t*)nullptr);
     HIPCHK(c, hipGetLastError());
     hipLaunchKernelGGL(rescore_kernel, dim3(grid1((int64_t)cap, 256)), dim3(256), 0, c->stream,
                        stats_dev, n, P<double>(c->edata), P<double>(c->ese), A.G, A.E,
@@ -834,10 +974,38 @@ int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, c
   A.margin_rel = margin_rel;
   A.n_candidates = 0;
   A.n_screened = 0;
-  A.kept.assign(G, {});
+  A.kept.assign(mode == 1 ? 0 : G, {});
   A.thr_host.assign(std::max<int64_t>(G, 1), mode == 0 ? eps : HUGE_VAL);
+  A.pooled = mode == 1;
   A.active = true;
   if (G == 0) return ABC_OK;
+  if (A.pooled) {
+    // pools of k + n_sub entries per gene: n_sub rows per screened block, as
+    // many as about 1 GiB of pools allows (256..65536)
+    if (k > (int64_t)1 << 30) return fail(c, ABC_E_ARG, "k must be <= 2^30 in top-k mode");
+    const int64_t budget = (int64_t)1 << 30;
+    int64_t ns = budget / (16 * G) - k;
+    ns = std::max<int64_t>(256, std::min<int64_t>(65536, ns)) / 64 * 64;
+    A.n_sub = ns;
+    A.capg = k + ns;
+    const int64_t npad = pad_to(ns, 64);
+    if ((rc = ensure(c, c->pool, sizeof(PoolEntry) * (size_t)G * (size_t)A.capg))) return rc;
+    if ((rc = ensure(c, c->pooln, 8 * (size_t)((G + 1) / 2)))) return rc;
+    if ((rc = ensure(c, c->poolb, 8 * (size_t)((G + 1) / 2)))) return rc;
+    if ((rc = ensure(c, c->presc, 8))) return rc;
+    if ((rc = ensure(c, c->srow, sizeof(double) * (size_t)ns * K))) return rc;
+    if ((rc = ensure(c, c->apidx, sizeof(int64_t) * (size_t)ns))) return rc;
+    if ((rc = ensure(c, c->ah, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+    if ((rc = ensure(c, c->al, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
+    if ((rc = ensure(c, c->pflag, npad))) return rc;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1((G + 1) / 2, 256)), dim3(256), 0, c->stream,
+                       P<unsigned long long>(c->pooln), (G + 1) / 2);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(grid1((G + 1) / 2, 256)), dim3(256), 0, c->stream,
+                       P<unsigned long long>(c->poolb), (G + 1) / 2);
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream,
+                       P<unsigned long long>(c->presc), (int64_t)1);
+    HIPCHK(c, hipGetLastError());
+  }
   if ((rc = ensure(c, c->edata, sizeof(double) * G * K))) return rc;
   if ((rc = ensure(c, c->ese, sizeof(double) * G * K))) return rc;
   if ((rc = ensure(c, c->bh, sizeof(uint16_t) * A.Gpad * 2 * A.Kp))) return rc;
@@ -858,11 +1026,41 @@ int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, c
   return ABC_OK;
 }
 
+namespace {
+
+int screen_run(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev, const float* mean_dev,
+               const float* sd_dev, double z, double margin, uint8_t* pass_dev, float* score_dev,
+               int64_t* n_pass);
+
+}  // namespace
+
 int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
                        const float* mean_dev, const float* sd_dev, double z, double margin,
                        uint8_t* pass_dev, int64_t* n_pass) {
   if (!c || !mlp || n < 0 || (n > 0 && (!x_dev || !mean_dev || !sd_dev || !pass_dev)))
     return ABC_E_ARG;
+  return screen_run(c, mlp, n, x_dev, mean_dev, sd_dev, z, margin, pass_dev, nullptr, n_pass);
+}
+
+int32_t abc_screen_score_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
+                             const float* mean_dev, const float* sd_dev, double margin,
+                             float* score_dev) {
+  if (!c || !mlp || n < 0 || (n > 0 && (!x_dev || !mean_dev || !sd_dev || !score_dev)))
+    return ABC_E_ARG;
+  if (n == 0) return ABC_OK;
+  int rc;
+  if ((rc = ensure(c, c->spass, (size_t)n))) return rc;
+  return screen_run(c, mlp, n, x_dev, mean_dev, sd_dev, 0.0, margin, P<uint8_t>(c->spass),
+                    score_dev, nullptr);
+}
+
+}  // extern "C"
+
+namespace {
+
+int screen_run(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev, const float* mean_dev,
+               const float* sd_dev, double z, double margin, uint8_t* pass_dev, float* score_dev,
+               int64_t* n_pass) {
   NO_CAPTURE(c);
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
@@ -877,7 +1075,7 @@ int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
   hipLaunchKernelGGL(screen_kernel, dim3((unsigned)((n + 15) / 16)), dim3(256), 0, c->stream,
                      P<float>(c->sy), n, A.E.K, mean_dev, sd_dev, (float)z, P<float>(c->wf),
                      P<double>(c->edata), A.G, P<double>(c->thr), (float)margin, A.E.clamp,
-                     pass_dev);
+                     pass_dev, score_dev);
   HIPCHK(c, hipGetLastError());
   if (n_pass) {
     std::vector<uint8_t> h(n);
@@ -890,9 +1088,14 @@ int32_t abc_screen_dev(abc_ctx* c, abc_mlp* mlp, int64_t n, const float* x_dev,
   return ABC_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int32_t abc_accept_add_dev(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_index) {
   if (!c || n < 0 || (n > 0 && !stats_dev) || first_index < 0) return ABC_E_ARG;
-  NO_CAPTURE(c);
+  if (c->capturing && !c->acc.pooled)
+    return fail(c, ABC_E_STATE, "only top-k acceptance batches can be captured");
   if (!c->acc.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (n == 0 || c->acc.G == 0) return ABC_OK;
   return screen_batch(c, n, stats_dev, first_index);
@@ -913,9 +1116,19 @@ int32_t abc_accept_set_thresholds(abc_ctx* c, const double* thr) {
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   for (int64_t g = 0; g < A.G; ++g) A.thr_host[g] = std::min(A.thr_host[g], thr[g]);
-  if (A.G > 0)
+  if (A.G > 0 && A.pooled) {
+    // the device thresholds may already be lower (k-th errors found so far)
+    int rc;
+    if ((rc = ensure(c, c->thrlow, sizeof(double) * A.G))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->thrlow.p, thr, sizeof(double) * A.G, hipMemcpyHostToDevice,
+                             c->stream));
+    hipLaunchKernelGGL(thr_min_kernel, dim3(grid1(A.G, 256)), dim3(256), 0, c->stream,
+                       P<double>(c->thr), P<double>(c->thrlow), A.G);
+    HIPCHK(c, hipGetLastError());
+  } else if (A.G > 0) {
     HIPCHK(c, hipMemcpyAsync(c->thr.p, A.thr_host.data(), sizeof(double) * A.G,
                              hipMemcpyHostToDevice, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return ABC_OK;
 }
@@ -939,6 +1152,36 @@ int32_t abc_accept_end(abc_ctx* c, int64_t* counts, int64_t* idx, double* err_ou
   AcceptState& A = c->acc;
   if (!A.active) return fail(c, ABC_E_ARG, "abc_accept_begin was not called");
   if (A.G > 0 && !counts) return ABC_E_ARG;
+  if (A.pooled && A.G > 0) {
+    // each gene's kept entries: sorted by (error, index) on the device, then copied
+    if (A.k <= ABC_POOL_SORT_MAX) {
+      const int grid = (int)std::min<int64_t>(A.G, (int64_t)c->num_cu * 8);
+      hipLaunchKernelGGL(pool_sort_kernel, dim3(grid), dim3(256), 0, c->stream,
+                         P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln), A.G);
+      HIPCHK(c, hipGetLastError());
+    }
+    std::vector<uint32_t> cnt(A.G);
+    unsigned long long nres = 0;
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->pooln.p, sizeof(uint32_t) * A.G,
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&nres, c->presc.p, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    uint32_t mx = 0;
+    for (uint32_t v : cnt) mx = std::max(mx, v);
+    std::vector<PoolEntry> h((size_t)A.G * mx);
+    if (mx)
+      HIPCHK(c, hipMemcpy2D(h.data(), sizeof(PoolEntry) * mx, c->pool.p,
+                            sizeof(PoolEntry) * (size_t)A.capg, sizeof(PoolEntry) * mx, A.G,
+                            hipMemcpyDeviceToHost));
+    A.kept.assign(A.G, {});
+    for (int64_t g = 0; g < A.G; ++g) {
+      auto& v = A.kept[g];
+      v.reserve(cnt[g]);
+      for (uint32_t q = 0; q < cnt[g]; ++q)
+        v.push_back({h[(size_t)g * mx + q].err, h[(size_t)g * mx + q].idx});
+    }
+    A.n_candidates = (int64_t)nres;
+  }
   int64_t total = 0;
   for (int64_t g = 0; g < A.G; ++g) {
     auto& v = A.kept[g];
@@ -989,15 +1232,16 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
   if ((rc = ensure(c, c->err, sizeof(float) * 2 * n * G))) return rc;
   HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
-                     P<double>(c->stats), n, K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
-                     P<uint8_t>(c->pflag));
+                     P<double>(c->stats), n, n, K, A.Kp, npad, P<__bf16>(c->ah),
+                     P<__bf16>(c->al), P<uint8_t>(c->pflag), (double*)nullptr);
   float* ap = P<float>(c->err);
   float* bp = ap + n * G;
   hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
                      dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), n, A.Kp,
                      P<__bf16>(c->bh), P<__bf16>(c->bl), G, P<double>(c->cg), P<uint8_t>(c->gflag),
                      P<uint8_t>(c->pflag), P<double>(c->thr), 0.0, clamp, (Candidate*)nullptr,
-                     (unsigned long long*)nullptr, 0ull, ap, bp);
+                     (unsigned long long*)nullptr, 0ull, ap, bp, (PoolEntry*)nullptr, (int64_t)0,
+                     (uint32_t*)nullptr);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(approx, ap, sizeof(float) * n * G, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(bound, bp, sizeof(float) * n * G, hipMemcpyDeviceToHost, c->stream));
@@ -1550,6 +1794,8 @@ int32_t abc_graph_end(abc_ctx* c, abc_graph** g) {
                : fail(c, ABC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   }
   if (*g && (*g)->exec) {
+    (*g)->ctx = c;
+    (*g)->gen = c->gen;
     // same topology with new arguments (other particles, seeds or buffers):
     // update the executable graph in place
     hipGraphNode_t bad = nullptr;
@@ -1569,12 +1815,20 @@ int32_t abc_graph_end(abc_ctx* c, abc_graph** g) {
   if (ei != hipSuccess) return fail(c, ABC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   if (!*g) *g = new abc_graph;
   (*g)->exec = exec;
+  (*g)->ctx = c;
+  (*g)->gen = c->gen;
   return ABC_OK;
 }
 
 int32_t abc_graph_launch(abc_ctx* c, abc_graph* g) {
   if (!c || !g || !g->exec) return ABC_E_ARG;
   NO_CAPTURE(c);
+  if (g->ctx != c)
+    return fail(c, ABC_E_STATE, "the graph was captured on another context");
+  if (g->gen != c->gen)
+    return fail(c, ABC_E_STATE,
+                "a device buffer the graph uses was reallocated by a larger call after the "
+                "capture: capture the graph again");
   HIPCHK(c, hipGraphLaunch(g->exec, c->stream));
   return ABC_OK;
 }
@@ -1635,6 +1889,8 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
   };
   double ms = 0.0, best = 1e300;
   int bestb = c->ssa_blocks_default;
+  const int saved_claim = c->ssa_claim;
+  c->ssa_claim = 1;
   rc = run(c->ssa_blocks_default, &ms);  // warm-up (code load, buffer growth)
   for (size_t q = 0; rc == ABC_OK && q < cand.size(); ++q) {
     rc = run(cand[q], &ms);
@@ -1643,6 +1899,18 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
       bestb = cand[q];
     }
   }
+  // then the work items a lane claims per queue atomic, on that grid
+  int bestc = 1;
+  for (int cl : {2, 4}) {
+    if (rc != ABC_OK) break;
+    c->ssa_claim = cl;
+    rc = run(bestb, &ms);
+    if (rc == ABC_OK && ms < best) {
+      best = ms;
+      bestc = cl;
+    }
+  }
+  c->ssa_claim = rc == ABC_OK ? bestc : saved_claim;
   c->ssa_ms_acc = acc0;
   c->burn_ms_acc = bacc0;
   c->ssa_launches = l0;

@@ -77,13 +77,16 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(const double* __restrict
   gflag[g] = bad ? 1 : 0;
 }
 
-// One thread per particle: A rows [s | s^2] split into bf16 hi/lo.
+// One thread per particle: A rows [s | s^2] split into bf16 hi/lo, and
+// (srow != NULL) the statistics row-major for the FP64 rescoring.  Row i of
+// the n rows is stats[i + ld * k] (a block of rows of a larger batch).
 __global__ __launch_bounds__(256) void particle_prep_kernel(const double* __restrict__ stats,
-                                                            int64_t n, int K, int Kp,
+                                                            int64_t n, int64_t ld, int K, int Kp,
                                                             int64_t npad,
                                                             bf16* __restrict__ Ah,
                                                             bf16* __restrict__ Al,
-                                                            uint8_t* __restrict__ pflag) {
+                                                            uint8_t* __restrict__ pflag,
+                                                            double* __restrict__ srow) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npad) return;
   bf16* ah = Ah + i * 2 * Kp;
@@ -92,7 +95,8 @@ __global__ __launch_bounds__(256) void particle_prep_kernel(const double* __rest
   bool bad = false;
   for (int k = 0; k < Kp; ++k) {
     if (i < n && k < K) {
-      const double s = stats[i + n * k];
+      const double s = stats[i + ld * k];
+      if (srow) srow[i * K + k] = s;
       if (!(fabs(s) <= SCREEN_LIMIT)) bad = true;
       const double sc = bad ? 0.0 : s;
       split_bf16(sc, ah + k, al + k);
@@ -115,7 +119,8 @@ __global__ __launch_bounds__(256) void gemm_screen_kernel(
     const double* __restrict__ cg, const uint8_t* __restrict__ gflag,
     const uint8_t* __restrict__ pflag, const double* __restrict__ thr, double margin_rel,
     double clamp, Candidate* __restrict__ out, unsigned long long* __restrict__ count,
-    unsigned long long cap, float* __restrict__ approx_out, float* __restrict__ bound_out) {
+    unsigned long long cap, float* __restrict__ approx_out, float* __restrict__ bound_out,
+    PoolEntry* __restrict__ pool, int64_t capg, uint32_t* __restrict__ pool_n) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t i0 = (int64_t)blockIdx.x * 64 + (wave >> 1) * 32;
@@ -157,17 +162,23 @@ __global__ __launch_bounds__(256) void gemm_screen_kernel(
   };
   for (int ch = 0; ch < nchunk; ++ch) chunk(accC, ch * 32 + kq);
   for (int ch = 0; ch < nchunk; ++ch) chunk(accS, Kp + ch * 32 + kq);
-  // epilogue: bounds, thresholds, candidate emission
+  // epilogue: bounds, thresholds, candidate emission.  A pair is a candidate
+  // unless its lower bound provably exceeds the gene's threshold; a
+  // non-finite approximation or bound (FP32 overflow) makes it one.
 #pragma unroll
-  for (int rb = 0; rb < 2; ++rb)
+  for (int cb = 0; cb < 2; ++cb) {
+    const int64_t g = g0 + cb * 16 + (lane & 15);
+    const bool gok = g < G;
+    const double c = gok ? cg[g] : 0.0;
+    const double t = gok ? thr[g] : 0.0;
+    const bool gbad = gok && gflag[g] != 0;
+    unsigned cmask = 0;  // bit rb * 4 + q: row i0 + rb * 16 + (lane >> 4) * 4 + q
 #pragma unroll
-    for (int cb = 0; cb < 2; ++cb)
+    for (int rb = 0; rb < 2; ++rb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int64_t i = i0 + rb * 16 + (lane >> 4) * 4 + q;
-        const int64_t g = g0 + cb * 16 + (lane & 15);
-        if (i >= n || g >= G) continue;
-        const double c = cg[g];
+        if (!gok || i >= n) continue;
         const double cross = (double)accC[rb][cb][q];
         const double sq = (double)accS[rb][cb][q];
         const double approx = (c + cross) + sq;
@@ -178,17 +189,44 @@ __global__ __launch_bounds__(256) void gemm_screen_kernel(
           approx_out[i + n * g] = (float)approx;
           bound_out[i + n * g] = (float)bound;
         }
-        if (!out) continue;
-        const double t = thr[g];
         double lo = approx - bound;
         if (t < 1e300) lo -= margin_rel * fabs(t);
         if (lo > clamp) lo = clamp;
-        const bool must = pflag[i] != 0 || gflag[g] != 0;
-        if (must || lo <= t) {
-          const unsigned long long slot = atomicAdd(count, 1ull);
-          if (slot < cap) out[slot] = Candidate{approx, i, (int32_t)g, 0};
-        }
+        const bool must = pflag[i] != 0 || gbad;
+        if (must || !(lo > t)) cmask |= 1u << (rb * 4 + q);
       }
+    if (pool) {
+      // lanes l, l + 16, l + 32, l + 48 hold the same gene: one atomic per
+      // gene and wave reserves their pool slots
+      const int col = lane & 15, rgrp = lane >> 4;
+      const unsigned cnt = (unsigned)__popc(cmask);
+      const unsigned c0 = __shfl(cnt, col), c1 = __shfl(cnt, col + 16),
+                     c2 = __shfl(cnt, col + 32), c3 = __shfl(cnt, col + 48);
+      const unsigned tot = c0 + c1 + c2 + c3;
+      const unsigned pre = (rgrp > 0 ? c0 : 0u) + (rgrp > 1 ? c1 : 0u) + (rgrp > 2 ? c2 : 0u);
+      unsigned base = 0;
+      if (rgrp == 0 && tot > 0) base = atomicAdd(pool_n + g, tot);
+      base = __shfl(base, col);
+      int64_t slot = (int64_t)base + pre;
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        if (!(cmask & (1u << bit))) continue;
+        const int64_t i = i0 + (bit >> 2) * 16 + (lane >> 4) * 4 + (bit & 3);
+        if (slot < capg) pool[g * capg + slot] = PoolEntry{0.0, i};
+        slot++;
+      }
+    } else if (out) {
+#pragma unroll
+      for (int bit = 0; bit < 8; ++bit) {
+        if (!(cmask & (1u << bit))) continue;
+        const int64_t i = i0 + (bit >> 2) * 16 + (lane >> 4) * 4 + (bit & 3);
+        const double cross = (double)accC[bit >> 2][cb][bit & 3];
+        const double sq = (double)accS[bit >> 2][cb][bit & 3];
+        const unsigned long long s = atomicAdd(count, 1ull);
+        if (s < cap) out[s] = Candidate{(c + cross) + sq, i, (int32_t)g, 0};
+      }
+    }
+  }
 }
 
 // FP64 rescoring of candidates in the reference operation order

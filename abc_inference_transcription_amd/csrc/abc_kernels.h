@@ -38,6 +38,9 @@ struct DesignDev {
 // Per-particle rates (1/h) for each age bin.
 struct __align__(16) PRates {
   double kon[ABC_MAX_BINS], koff[ABC_MAX_BINS], amax[ABC_MAX_BINS], gam[ABC_MAX_BINS];
+  // 1 / kon, 1 / koff, 1 / (gam + splicing), 1 / gam per bin (the same IEEE
+  // quotients the event loops would form at every bin edge)
+  double ikon[ABC_MAX_BINS], ikoff[ABC_MAX_BINS], iku[ABC_MAX_BINS], ikt[ABC_MAX_BINS];
   double bs;                       // splicing rate
   double brate;                    // mean over the bins of alpha_max * P(gene on)
   uint64_t thr_lam;                // floor(lambda * 2^32): labelling threshold
@@ -63,6 +66,14 @@ struct ErrSpec {
   int parts[ABC_MAX_PARTS];
   double clamp;
 };
+
+// One entry of a gene's acceptance pool (abc_accept.hip): the FP64 error
+// and the global 0-based particle index (the batch row until rescored).
+struct PoolEntry {
+  double err;
+  int64_t idx;
+};
+#define ABC_POOL_SORT_MAX 2048  // entries per gene sorted on the device
 
 struct Candidate {
   double err;
@@ -105,7 +116,7 @@ __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* o
                            unsigned long long* queue,
                            uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,
                            unsigned long long* steps_out, unsigned long long* react_out,
-                           const int64_t* pidx);
+                           const int64_t* pidx, uint32_t claim);
 // Moment model (the reference simulator, scripts/model.jl): one call's
 // configuration, passed by value as a kernel argument (about 1.9 kB).
 #define ABC_MOM_MAX_AGES ABC_MAX_BINS
@@ -200,19 +211,33 @@ __global__ void gene_prep_kernel(const double* data, const double* se, int64_t G
 // surrogate screen (abc_mlp.hip)
 __global__ void screen_kernel(const float* y, int64_t n, int K, const float* mean, const float* sd,
                               float z, const float* Wf, const double* data, int64_t G,
-                              const double* thr, float margin, double clamp, uint8_t* pass);
+                              const double* thr, float margin, double clamp, uint8_t* pass, float* score);
 struct abc_mlp;
 extern "C" int32_t abc_mlp_forward_dev(abc_mlp* p, int64_t n, const float* x_dev, float* y_dev);
 extern "C" int32_t abc_mlp_synchronize(abc_mlp* p);
 extern "C" int32_t abc_mlp_n_stats(const abc_mlp* p);
-__global__ void particle_prep_kernel(const double* stats, int64_t n, int K, int Kp, int64_t npad,
-                                     __bf16* Ah, __bf16* Al, uint8_t* pflag);
+__global__ void particle_prep_kernel(const double* stats, int64_t n, int64_t ld, int K, int Kp,
+                                     int64_t npad,
+                                     __bf16* Ah, __bf16* Al, uint8_t* pflag, double* srow);
 __global__ void gemm_screen_kernel(const __bf16* Ah, const __bf16* Al, int64_t n, int Kp,
                                    const __bf16* Bh, const __bf16* Bl, int64_t G, const double* cg,
                                    const uint8_t* gflag, const uint8_t* pflag, const double* thr,
                                    double margin_rel, double clamp, Candidate* out,
                                    unsigned long long* count, unsigned long long cap,
-                                   float* approx_out, float* bound_out);
+                                   float* approx_out, float* bound_out, PoolEntry* pool,
+                                   int64_t capg, uint32_t* pool_n);
+// per-gene top-k pools (abc_accept.hip)
+__global__ void pool_rescore_kernel(const double* srow, int64_t n, const double* data,
+                                    const double* se, int64_t G, ErrSpec E, PoolEntry* pool,
+                                    int64_t capg, const uint32_t* pool_n,
+                                    const uint32_t* pool_base, int64_t first_index,
+                                    const int64_t* pidx);
+__global__ void pool_select_kernel(PoolEntry* pool, int64_t capg, uint32_t* pool_n,
+                                   uint32_t* pool_base, int64_t G, int64_t k, double* thr,
+                                   unsigned long long* n_rescored);
+__global__ void pool_sort_kernel(PoolEntry* pool, int64_t capg, const uint32_t* pool_n,
+                                 int64_t G);
+__global__ void thr_min_kernel(double* thr, const double* lower, int64_t G);
 __global__ void rescore_kernel(const double* stats, int64_t n, const double* data,
                                const double* se, int64_t G, ErrSpec E, Candidate* cand,
                                const unsigned long long* count, unsigned long long cap);

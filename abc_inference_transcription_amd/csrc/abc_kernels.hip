@@ -132,6 +132,12 @@ __global__ __launch_bounds__(256) void rates_kernel(const DesignDev* __restrict_
     r.gam[j] = ex10(theta[i + n * (L.o_ga + (L.n_ga > 1 ? j : 0))]);
   }
   r.bs = ex10(theta[i + n * L.o_spl]);
+  for (int j = 0; j < nb; ++j) {
+    r.ikon[j] = 1.0 / r.kon[j];
+    r.ikoff[j] = 1.0 / r.koff[j];
+    r.iku[j] = 1.0 / (r.gam[j] + r.bs);
+    r.ikt[j] = 1.0 / r.gam[j];
+  }
   const double lam = ex10(theta[i + n * L.o_lam]);
   r.thr_lam = (uint64_t)(lam * 4294967296.0);
   double pi;
@@ -269,21 +275,20 @@ __global__ __launch_bounds__(256) void cell_items_kernel(int64_t n, int C,
                                                          int32_t* __restrict__ coff,
                                                          int64_t* __restrict__ pitems) {
   __shared__ int32_t s[256];
-  const int64_t p = blockIdx.x;
-  if (p >= n) return;
   const int t = threadIdx.x;
   const int per = (C + 255) / 256;
   const int c0 = t * per, c1 = min(C, c0 + per);
-  const CellStart* rp = rec + (size_t)p * (size_t)C;
-  int32_t acc = 0;
-  for (int c = c0; c < c1; ++c) acc += 1 << rp[c].msh;
-  s[t] = acc;
-  __syncthreads();
-  for (int d = 1; d < 256; d <<= 1) {
-    const int32_t x = t >= d ? s[t - d] : 0;
+  for (int64_t p = blockIdx.x; p < n; p += gridDim.x) {
+    const CellStart* rp = rec + (size_t)p * (size_t)C;
+    int32_t acc = 0;
+    for (int c = c0; c < c1; ++c) acc += 1 << rp[c].msh;
+    s[t] = acc;
     __syncthreads();
-    s[t] += x;
-    __syncthreads();
+    for (int d = 1; d < 256; d <<= 1) {
+      const int32_t x = t >= d ? s[t - d] : 0;
+      __syncthreads();
+      s[t] += x;
+      __syncthreads();
   }
   int32_t run = t ? s[t - 1] : 0;
   int32_t* co = coff + (size_t)p * (size_t)(C + 1);
@@ -294,6 +299,8 @@ __global__ __launch_bounds__(256) void cell_items_kernel(int64_t n, int C,
   if (t == 255) {
     co[C] = s[255];
     pitems[p] = s[255];
+  }
+  __syncthreads();  // s[] is reused by the next particle
   }
 }
 
@@ -555,8 +562,8 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
   const int nb = D->nb, J = D->J, C = D->C;
   const unsigned long long total = (unsigned long long)n * (unsigned long long)C;
   unsigned long long lane_steps = 0;  // this lane's burn-in draws, added to *burn_tally at exit
-  __shared__ LogEntry ltab[ABC_LOG_N];
-  load_log_table(ltab);
+  __shared__ LogEntry32 ltab32[ABC_LOG_N];
+  load_log32_table(ltab32);
   __syncthreads();
   for (;;) {
     const unsigned long long q = atomicAdd(queue, 1ull);
@@ -581,9 +588,9 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     double t = grid_time(0, nbB, T, nb), cs = t, gk = grid_time(knext, nbB, T, nb), tg = 0.0;
     double lu = 0.0, lt = 0.0;
     bool g = cs0.g, gneed = true;
-    double ikon = 1.0 / rp->kon[0], ikoff = 1.0 / rp->koff[0];
+    double ikon = rp->ikon[0], ikoff = rp->ikoff[0];
     double atx = rp->amax[0] * bc, gam = rp->gam[0], bound = D->bound[0];
-    double iku = 1.0 / (gam + bs), ikt = 1.0 / gam, abnd = atx / bound;
+    double iku = rp->iku[0], ikt = rp->ikt[0], abnd = atx / bound;
     unsigned long long st = 0, re = 0;
     for (;;) {
       double tb = age;
@@ -592,7 +599,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
       if (gneed) {
         const u4 v = draw(gctr++, 0u, cw, pid, k0, k1);
         st++;
-        tg = t + (-ln(u01_32(v.x), ltab)) * (g ? ikoff : ikon);
+        tg = t + (double)exp1_32(v.x, ltab32) * (g ? ikoff : ikon);
         gneed = false;
       }
       // births at rate atx (T + a) / bound_j (size scaling) or atx, a = t - cycle start
@@ -616,20 +623,21 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
       if (tb == gk) {
         k = knext;
         knext += step;
-        j = (int)(k % nb);
+        j += step;  // k mod nb (step is 1 or nb)
+        if (j >= nb) j -= nb;
         if (j == 0) {
           cs = tb;
           lu = lu * 0.5;  // division halves the Poisson means
           lt = lt * 0.5;
         }
         if (step == 1) {
-          ikon = 1.0 / rp->kon[j];
-          ikoff = 1.0 / rp->koff[j];
+          ikon = rp->ikon[j];
+          ikoff = rp->ikoff[j];
           atx = rp->amax[j] * bc;
           gam = rp->gam[j];
           bound = D->bound[j];
-          iku = 1.0 / (gam + bs);
-          ikt = 1.0 / gam;
+          iku = rp->iku[j];
+          ikt = rp->ikt[j];
           abnd = atx / bound;
         }
         if (gvar) gneed = true;  // the gene's rate changed: restart its clock
@@ -702,12 +710,15 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                      uint8_t* __restrict__ out_bin,
                                                      unsigned long long* __restrict__ steps_out,
                                                      unsigned long long* __restrict__ react_out,
-                                                     const int64_t* __restrict__ pidx) {
+                                                     const int64_t* __restrict__ pidx,
+                                                     uint32_t claim) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
   const unsigned long long total = (unsigned long long)off[n];
   __shared__ LogEntry ltab[ABC_LOG_N];
+  __shared__ LogEntry32 ltab32[ABC_LOG_N];
   load_log_table(ltab);
+  load_log32_table(ltab32);
   __syncthreads();
 
   uint32_t cw = 0;  // counter word of the current item: cell | model << 24
@@ -718,7 +729,8 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
   uint64_t gctr = 0, mctr = 0, thr_lam = 0;
   int64_t k = 0, knext = 0, nbB = 0;
   double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, we = 0.0, cs = 0.0;
-  double ikon = 0.0, ikoff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0, share = 1.0;
+  double ikon = 0.0, ikoff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0;
+  double ishare = 1.0;  // 1 / (the cell's sub-lanes), a power of two
   bool g = false, in_win = false, div = false;
   int div_s = 0;
   uint32_t div_rem = 0, div_keep = 0;
@@ -728,10 +740,17 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
   bool gneed = false, pend = false;
   double erem = 0.0;
   uint32_t py = 0, pz = 0, pw = 0;
+  // next item of the lane's claim: claim (a power of two) consecutive items
+  // per queue atomic, a launch shape only (items are independent).  The
+  // queue advances by whole claims, so a claim is used up exactly when qnext
+  // reaches a multiple of it.
+  unsigned long long qnext = 0;
 
   for (;;) {
     if (!have) {
-      const unsigned long long q = atomicAdd(queue, 1ull);
+      if ((qnext & (unsigned long long)(claim - 1u)) == 0ull)
+        qnext = atomicAdd(queue, (unsigned long long)claim);
+      const unsigned long long q = qnext++;
       if (q >= total) break;
       // particle rank: largest r with off[r] <= q
       int64_t lo = 0, hi = n;
@@ -755,7 +774,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       cw = (uint32_t)c | (mtags ? mtags[p] : mtag_all);
       sstream = 1u + (uint32_t)sub;
       const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
-      share = (double)(1 << rc->msh);
+      ishare = ldexp(1.0, -(int)rc->msh);  // x / M == x * ishare exactly
       st = 0;
       re = 0;
       mctr = 0;
@@ -770,8 +789,8 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         // the sub-lane's unlabelled molecules at the window start
         WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, cw, pid, k0, k1, st};
         const double lu0 = rc->lu, ls = rc->lt - lu0;
-        n0 = poisson(src, lu0 / share, ltab);
-        n1 = poisson(src, ls > 0.0 ? ls / share : 0.0, ltab);
+        n0 = poisson(src, lu0 * ishare, ltab);
+        n1 = poisson(src, ls > 0.0 ? ls * ishare : 0.0, ltab);
       }
       const CellStart r0 = *rc;
       n2 = n3 = 0;
@@ -788,7 +807,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       nbB = (int64_t)rp->B * nb;
       k = r0.k;
       knext = k + step;
-      j = (int)(k % nb);
+      j = (int)((uint64_t)k % (uint64_t)nb);
       cs = grid_time(k - j, nbB, T, nb);
       gk = grid_time(knext, nbB, T, nb);
       t = age - D->pulse[cond] - D->chase[cond];
@@ -796,9 +815,9 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       g = r0.g != 0;
       tg = r0.tg;
       gneed = r0.gneed != 0;
-      ikon = 1.0 / rp->kon[j];  // gene clock: waiting time = E * (1 / rate)
-      ikoff = 1.0 / rp->koff[j];
-      atx = (rp->amax[j] * bc) / share;
+      ikon = rp->ikon[j];  // gene clock: waiting time = E * (1 / rate)
+      ikoff = rp->ikoff[j];
+      atx = (rp->amax[j] * bc) * ishare;
       gam = rp->gam[j];
       bound = D->bound[j];
       in_win = true;
@@ -855,7 +874,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       }
       continue;
     }
-    const double E = -ln(u01_32(w.x), ltab);
+    const double E = (double)exp1_32(w.x, ltab32);
     if (kind == 2) {
       tg = t + E * (g ? ikoff : ikon);
       gneed = false;
@@ -922,7 +941,8 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (tb == gk) {
       k = knext;
       knext += step;
-      j = (int)(k % nb);
+      j += step;  // k mod nb (step is 1 or nb)
+      if (j >= nb) j -= nb;
       if (j == 0) {
         cs = tb;
         div_s = 0;
@@ -935,9 +955,9 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       }
       if (step == 1) {
         const PRates* rp = R + p;
-        ikon = 1.0 / rp->kon[j];
-        ikoff = 1.0 / rp->koff[j];
-        atx = (rp->amax[j] * bc) / share;
+        ikon = rp->ikon[j];
+        ikoff = rp->ikoff[j];
+        atx = (rp->amax[j] * bc) * ishare;
         gam = rp->gam[j];
         bound = D->bound[j];
       }
@@ -1012,7 +1032,6 @@ __global__ __launch_bounds__(256) void stats_kernel(const DesignDev* __restrict_
                                                     double* __restrict__ moments,
                                                     double* __restrict__ superset) {
   const int nb = D->nb, J = D->J, C = D->C, NBJ = nb * J;
-  const int64_t i = blockIdx.x;
   __shared__ unsigned int s_n[ABC_MAX_BINS * ABC_MAX_CONDS];
   __shared__ unsigned long long s_su[ABC_MAX_BINS * ABC_MAX_CONDS], s_sl[ABC_MAX_BINS * ABC_MAX_CONDS];
   __shared__ unsigned long long s_uu[2][ABC_MAX_BINS * ABC_MAX_CONDS];
@@ -1021,119 +1040,122 @@ __global__ __launch_bounds__(256) void stats_kernel(const DesignDev* __restrict_
   __shared__ double s_m1[ABC_MAX_BINS * ABC_MAX_CONDS], s_m2[ABC_MAX_BINS * ABC_MAX_CONDS];
   __shared__ double s_v1[ABC_MAX_BINS * ABC_MAX_CONDS], s_v2[ABC_MAX_BINS * ABC_MAX_CONDS];
   __shared__ double s_c12[ABC_MAX_BINS * ABC_MAX_CONDS];
-  for (int q = threadIdx.x; q < NBJ; q += blockDim.x) {
-    s_n[q] = 0; s_su[q] = 0; s_sl[q] = 0;
-    s_uu[0][q] = s_uu[1][q] = 0; s_ll[0][q] = s_ll[1][q] = 0; s_ul[0][q] = s_ul[1][q] = 0;
-  }
-  __syncthreads();
-  const uint32_t* u = cu + (size_t)i * C;
-  const uint32_t* l = cl + (size_t)i * C;
-  const uint8_t* bn = cb + (size_t)i * C;
-  for (int c = threadIdx.x; c < C; c += blockDim.x) {
-    if (bn[c] >= nb) continue;  // never produced by the SSA; keeps LDS indexing in range
-    const int idx = bn[c] + nb * (c % J);
-    const unsigned long long a = u[c], b = l[c];
-    atomicAdd(&s_n[idx], 1u);
-    atomicAdd(&s_su[idx], a);
-    atomicAdd(&s_sl[idx], b);
-    add128(&s_uu[0][idx], &s_uu[1][idx], a * a);
-    add128(&s_ll[0][idx], &s_ll[1][idx], b * b);
-    add128(&s_ul[0][idx], &s_ul[1][idx], a * b);
-  }
-  __syncthreads();
-  for (int idx = threadIdx.x; idx < NBJ; idx += blockDim.x) {
-    const uint64_t nn = s_n[idx];
-    const U128 uu{s_uu[1][idx], s_uu[0][idx]}, ll{s_ll[1][idx], s_ll[0][idx]},
-        ul{s_ul[1][idx], s_ul[0][idx]};
-    double mt = 0.0, fft = 0.0, m1 = 0.0, m2 = 0.0, v1 = 0.0, v2 = 0.0, c12 = 0.0;
-    if (nn > 0) {
-      const double q = (double)nn;
-      m1 = (double)s_su[idx] / q;
-      m2 = (double)s_sl[idx] / q;
-      v1 = var_of(nn, s_su[idx], uu);
-      v2 = var_of(nn, s_sl[idx], ll);
-      c12 = cov_of(nn, s_su[idx], s_sl[idx], ul);
-      const uint64_t St = s_su[idx] + s_sl[idx];
-      const U128 Stt = add(add(uu, add(ul, ul)), ll);
-      mt = (double)St / q;
-      const double vt = var_of(nn, St, Stt);
-      fft = vt / (mt + (mt == 0.0 ? 0.0001 : 0.0));
+  // one particle per workgroup at a time (grid-stride over particles)
+  for (int64_t i = blockIdx.x; i < n; i += gridDim.x) {
+    for (int q = threadIdx.x; q < NBJ; q += blockDim.x) {
+      s_n[q] = 0; s_su[q] = 0; s_sl[q] = 0;
+      s_uu[0][q] = s_uu[1][q] = 0; s_ll[0][q] = s_ll[1][q] = 0; s_ul[0][q] = s_ul[1][q] = 0;
     }
-    s_m1[idx] = m1; s_m2[idx] = m2; s_v1[idx] = v1; s_v2[idx] = v2; s_c12[idx] = c12;
-    if (moments) {
-      double* mo = moments + i;
-      const int64_t base = 6 * (int64_t)idx;
-      mo[n * (base + 0)] = (double)nn;
-      mo[n * (base + 1)] = m1;
-      mo[n * (base + 2)] = m2;
-      mo[n * (base + 3)] = v1;
-      mo[n * (base + 4)] = c12;
-      mo[n * (base + 5)] = v2;
-      if (nn == 0) {
-        for (int qq = 0; qq < 6; ++qq) mo[n * (base + qq)] = 0.0;
-      }
+    __syncthreads();
+    const uint32_t* u = cu + (size_t)i * C;
+    const uint32_t* l = cl + (size_t)i * C;
+    const uint8_t* bn = cb + (size_t)i * C;
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+      if (bn[c] >= nb) continue;  // never produced by the SSA; keeps LDS indexing in range
+      const int idx = bn[c] + nb * (c % J);
+      const unsigned long long a = u[c], b = l[c];
+      atomicAdd(&s_n[idx], 1u);
+      atomicAdd(&s_su[idx], a);
+      atomicAdd(&s_sl[idx], b);
+      add128(&s_uu[0][idx], &s_uu[1][idx], a * a);
+      add128(&s_ll[0][idx], &s_ll[1][idx], b * b);
+      add128(&s_ul[0][idx], &s_ul[1][idx], a * b);
     }
-    if (superset) {
-      superset[i + n * idx] = mt;
-      superset[i + n * (NBJ + idx)] = fft;
-    }
-  }
-  __syncthreads();
-  if (!stats_ref) return;
-  const int tid = threadIdx.x;
-  if (tid < 2 * nb) {
-    const int grp = tid / nb, b = tid % nb;
-    uint64_t nn = 0, St = 0;
-    U128 Stt{0, 0};
-    for (int jj = 0; jj < J; ++jj) {
-      if (D->is_chase[jj] != grp) continue;
-      const int idx = b + nb * jj;
-      nn += s_n[idx];
-      St += s_su[idx] + s_sl[idx];
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < NBJ; idx += blockDim.x) {
+      const uint64_t nn = s_n[idx];
       const U128 uu{s_uu[1][idx], s_uu[0][idx]}, ll{s_ll[1][idx], s_ll[0][idx]},
           ul{s_ul[1][idx], s_ul[0][idx]};
-      Stt = add(Stt, add(add(uu, add(ul, ul)), ll));
+      double mt = 0.0, fft = 0.0, m1 = 0.0, m2 = 0.0, v1 = 0.0, v2 = 0.0, c12 = 0.0;
+      if (nn > 0) {
+        const double q = (double)nn;
+        m1 = (double)s_su[idx] / q;
+        m2 = (double)s_sl[idx] / q;
+        v1 = var_of(nn, s_su[idx], uu);
+        v2 = var_of(nn, s_sl[idx], ll);
+        c12 = cov_of(nn, s_su[idx], s_sl[idx], ul);
+        const uint64_t St = s_su[idx] + s_sl[idx];
+        const U128 Stt = add(add(uu, add(ul, ul)), ll);
+        mt = (double)St / q;
+        const double vt = var_of(nn, St, Stt);
+        fft = vt / (mt + (mt == 0.0 ? 0.0001 : 0.0));
+      }
+      s_m1[idx] = m1; s_m2[idx] = m2; s_v1[idx] = v1; s_v2[idx] = v2; s_c12[idx] = c12;
+      if (moments) {
+        double* mo = moments + i;
+        const int64_t base = 6 * (int64_t)idx;
+        mo[n * (base + 0)] = (double)nn;
+        mo[n * (base + 1)] = m1;
+        mo[n * (base + 2)] = m2;
+        mo[n * (base + 3)] = v1;
+        mo[n * (base + 4)] = c12;
+        mo[n * (base + 5)] = v2;
+        if (nn == 0) {
+          for (int qq = 0; qq < 6; ++qq) mo[n * (base + qq)] = 0.0;
+        }
+      }
+      if (superset) {
+        superset[i + n * idx] = mt;
+        superset[i + n * (NBJ + idx)] = fft;
+      }
     }
-    double mean = 0.0, ff = 0.0;
-    if (nn > 0) {
-      mean = (double)St / (double)nn;
-      const double v = var_of(nn, St, Stt);
-      ff = v / (mean + (mean == 0.0 ? 0.0001 : 0.0));
+    __syncthreads();
+    const int tid = threadIdx.x;
+    if (stats_ref && tid < 2 * nb) {
+      const int grp = tid / nb, b = tid % nb;
+      uint64_t nn = 0, St = 0;
+      U128 Stt{0, 0};
+      for (int jj = 0; jj < J; ++jj) {
+        if (D->is_chase[jj] != grp) continue;
+        const int idx = b + nb * jj;
+        nn += s_n[idx];
+        St += s_su[idx] + s_sl[idx];
+        const U128 uu{s_uu[1][idx], s_uu[0][idx]}, ll{s_ll[1][idx], s_ll[0][idx]},
+            ul{s_ul[1][idx], s_ul[0][idx]};
+        Stt = add(Stt, add(add(uu, add(ul, ul)), ll));
+      }
+      double mean = 0.0, ff = 0.0;
+      if (nn > 0) {
+        mean = (double)St / (double)nn;
+        const double v = var_of(nn, St, Stt);
+        ff = v / (mean + (mean == 0.0 ? 0.0001 : 0.0));
+      }
+      stats_ref[i + n * ((2 * grp) * nb + b)] = mean;
+      stats_ref[i + n * ((2 * grp + 1) * nb + b)] = ff;
+    } else if (stats_ref && tid >= 64 && tid < 64 + J) {
+      const int jj = tid - 64;
+      uint64_t nn = 0, su = 0, sl = 0;
+      bool anynz = false;
+      for (int b = 0; b < nb; ++b) {
+        const int idx = b + nb * jj;
+        nn += s_n[idx]; su += s_su[idx]; sl += s_sl[idx];
+        if (s_c12[idx] != 0.0) anynz = true;
+      }
+      double ratio = 0.0;
+      if (nn > 0) {
+        const double a1 = (double)su / (double)nn, a2 = (double)sl / (double)nn;
+        if (a1 + a2 > 0.0) ratio = a2 / (a1 + a2);
+      }
+      const double* w = D->weights + nb * jj;
+      const double* m1 = s_m1 + nb * jj;
+      const double* m2 = s_m2 + nb * jj;
+      const double* v1 = s_v1 + nb * jj;
+      const double* v2 = s_v2 + nb * jj;
+      const double* c12 = s_c12 + nb * jj;
+      double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
+      for (int b = 1; b < nb; ++b) s1 = s1 + w[b] * v1[b];
+      for (int b = 1; b < nb; ++b) s2 = s2 + w[b] * v2[b];
+      for (int b = 1; b < nb; ++b) s12 = s12 + w[b] * c12[b];
+      const double tv1 = s1 + wcov(m1, m1, w, nb);
+      const double tv2 = s2 + wcov(m2, m2, w, nb);
+      double mc = 0.0, cm = 0.0;
+      if (anynz && tv1 != 0.0 && tv2 != 0.0) mc = s12 / sqrt(fabs(tv1 * tv2));
+      if (tv1 != 0.0 && tv2 != 0.0) cm = wcov(m1, m2, w, nb) / sqrt(fabs(tv1 * tv2));
+      stats_ref[i + n * (4 * nb + jj)] = ratio;
+      stats_ref[i + n * (4 * nb + J + jj)] = mc;
+      stats_ref[i + n * (4 * nb + 2 * J + jj)] = cm;
     }
-    stats_ref[i + n * ((2 * grp) * nb + b)] = mean;
-    stats_ref[i + n * ((2 * grp + 1) * nb + b)] = ff;
-  } else if (tid >= 64 && tid < 64 + J) {
-    const int jj = tid - 64;
-    uint64_t nn = 0, su = 0, sl = 0;
-    bool anynz = false;
-    for (int b = 0; b < nb; ++b) {
-      const int idx = b + nb * jj;
-      nn += s_n[idx]; su += s_su[idx]; sl += s_sl[idx];
-      if (s_c12[idx] != 0.0) anynz = true;
-    }
-    double ratio = 0.0;
-    if (nn > 0) {
-      const double a1 = (double)su / (double)nn, a2 = (double)sl / (double)nn;
-      if (a1 + a2 > 0.0) ratio = a2 / (a1 + a2);
-    }
-    const double* w = D->weights + nb * jj;
-    const double* m1 = s_m1 + nb * jj;
-    const double* m2 = s_m2 + nb * jj;
-    const double* v1 = s_v1 + nb * jj;
-    const double* v2 = s_v2 + nb * jj;
-    const double* c12 = s_c12 + nb * jj;
-    double s1 = w[0] * v1[0], s2 = w[0] * v2[0], s12 = w[0] * c12[0];
-    for (int b = 1; b < nb; ++b) s1 = s1 + w[b] * v1[b];
-    for (int b = 1; b < nb; ++b) s2 = s2 + w[b] * v2[b];
-    for (int b = 1; b < nb; ++b) s12 = s12 + w[b] * c12[b];
-    const double tv1 = s1 + wcov(m1, m1, w, nb);
-    const double tv2 = s2 + wcov(m2, m2, w, nb);
-    double mc = 0.0, cm = 0.0;
-    if (anynz && tv1 != 0.0 && tv2 != 0.0) mc = s12 / sqrt(fabs(tv1 * tv2));
-    if (tv1 != 0.0 && tv2 != 0.0) cm = wcov(m1, m2, w, nb) / sqrt(fabs(tv1 * tv2));
-    stats_ref[i + n * (4 * nb + jj)] = ratio;
-    stats_ref[i + n * (4 * nb + J + jj)] = mc;
-    stats_ref[i + n * (4 * nb + 2 * J + jj)] = cm;
+    __syncthreads();  // the LDS sums are cleared for the next particle
   }
 }
 

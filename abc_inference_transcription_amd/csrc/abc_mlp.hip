@@ -324,18 +324,23 @@ __global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y
                                                      const float* __restrict__ Wf,
                                                      const double* __restrict__ data, int64_t G,
                                                      const double* __restrict__ thr, float margin,
-                                                     double clamp, uint8_t* __restrict__ pass) {
+                                                     double clamp, uint8_t* __restrict__ pass,
+                                                     float* __restrict__ score) {
   constexpr int PT = 16;
   __shared__ float e1[PT][ABC_MAX_STATS / 2], c2[PT][ABC_MAX_STATS / 2];
   __shared__ float c3[PT][ABC_MAX_STATS / 2], c4[PT][ABC_MAX_STATS / 2];
-  __shared__ int hit[PT];
+  __shared__ int hit[PT], bad[PT];
+  __shared__ float s_sum[4][PT], s_max[4][PT];  // per wave: expected pairs, largest margin
   const double gx[7] = {-3.7504397177257425, -2.366759410734541, -1.1544053947399682, 0.0,
                         1.1544053947399682, 2.366759410734541, 3.7504397177257425};
   const double gw[7] = {0.000548268855972217, 0.03075712396758652, 0.2401231786050127,
                         0.45714285714285724, 0.2401231786050127, 0.03075712396758652,
                         0.000548268855972217};
   const int64_t i0 = (int64_t)blockIdx.x * PT;
-  if (threadIdx.x < PT) hit[threadIdx.x] = (i0 + threadIdx.x < n) ? 0 : 1;
+  if (threadIdx.x < PT) {
+    hit[threadIdx.x] = (i0 + threadIdx.x < n) ? 0 : 1;
+    bad[threadIdx.x] = 0;
+  }
   __syncthreads();
   for (int q = threadIdx.x; q < PT * K; q += blockDim.x) {
     const int p = q / K, k = q % K;
@@ -356,7 +361,7 @@ __global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y
         v3 += gw[r] * e2 * e;
         v4 += gw[r] * e2 * e2;
       }
-      if (!(fabs(m1) < 1e30) || !(v4 < 1e300)) hit[p] = 1;
+      if (!(fabs(m1) < 1e30) || !(v4 < 1e300)) hit[p] = bad[p] = 1;
     }
     e1[p][k] = (float)m1;
     c2[p][k] = (float)v2;
@@ -364,10 +369,19 @@ __global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y
     c4[p][k] = (float)v4;
   }
   __syncthreads();
+  // per particle over this thread's genes: sum of P(err <= t) under the
+  // Gaussian approximation, and the largest standardised margin (t - E) / sd
+  float psum[PT], pmax[PT];
+#pragma unroll
+  for (int p = 0; p < PT; ++p) { psum[p] = 0.f; pmax[p] = -HUGE_VALF; }
   for (int64_t g = threadIdx.x; g < G; g += blockDim.x) {
     const double tg = thr[g];
     if (tg >= clamp) {  // every clamped error ties at the threshold: nobody may be skipped
-      for (int p = 0; p < PT; ++p) hit[p] = 1;
+      for (int p = 0; p < PT; ++p) {
+        hit[p] = 1;
+        psum[p] += 1.f;
+        pmax[p] = HUGE_VALF;
+      }
       continue;
     }
     const float t = (float)(tg * (1.0 + (double)margin));
@@ -387,11 +401,41 @@ __global__ __launch_bounds__(256) void screen_kernel(const float* __restrict__ y
       }
     }
 #pragma unroll
-    for (int p = 0; p < PT; ++p)
-      if (em[p] - z * sqrtf(ev[p]) <= t) hit[p] = 1;  // benign race: every writer stores 1
+    for (int p = 0; p < PT; ++p) {
+      const float sd = sqrtf(ev[p]);
+      if (em[p] - z * sd <= t) hit[p] = 1;  // benign race: every writer stores 1
+      if (score) {
+        const float d = t - em[p];
+        const float u = sd > 0.f ? d / sd : (d >= 0.f ? HUGE_VALF : -HUGE_VALF);
+        psum[p] += normcdff(u);
+        pmax[p] = fmaxf(pmax[p], u);
+      }
+    }
+  }
+  if (score) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int p = 0; p < PT; ++p) {
+      float a = psum[p], b = pmax[p];
+      for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o);
+        b = fmaxf(b, __shfl_xor(b, o));
+      }
+      if (lane == 0) { s_sum[wave][p] = a; s_max[wave][p] = b; }
+    }
   }
   __syncthreads();
-  if (threadIdx.x < PT && i0 + threadIdx.x < n) pass[i0 + threadIdx.x] = (uint8_t)hit[threadIdx.x];
+  if (threadIdx.x < PT && i0 + threadIdx.x < n) {
+    const int p = threadIdx.x;
+    pass[i0 + p] = (uint8_t)hit[p];
+    if (score) {
+      // a prediction that is not finite passes whatever the threshold
+      float a = 0.f, b = -HUGE_VALF;
+      for (int w = 0; w < 4; ++w) { a += s_sum[w][p]; b = fmaxf(b, s_max[w][p]); }
+      score[2 * (i0 + p)] = bad[p] ? HUGE_VALF : a;
+      score[2 * (i0 + p) + 1] = bad[p] ? HUGE_VALF : b;
+    }
+  }
 }
 
 struct abc_mlp {

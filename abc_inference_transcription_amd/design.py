@@ -52,6 +52,21 @@ def reference_design(cells: int = 550, capture_mode: int = CAPTURE_NONE, betas=N
                 splice_log10=0.0, split_target=32768.0, split_max=1024)
 
 
+def reference_shape_design(betas, cells: int = 5445, seed: int = 5) -> dict:
+    """The reference's design at its own shape, with its shipped capture
+    efficiencies (data/capture_efficiencies.txt, one per cell of the data;
+    wrapper.jl:36).  The raw data that tie each of those cells to an age
+    cluster and to the pulse or chase experiments are not shipped, so each
+    efficiency gets a seeded uniform age bin and pulse/chase group.  5,445
+    cells = 99 per (age, condition), the size of the reference data (5,422
+    cells) spread over its 55 groups."""
+    betas = np.asarray(betas, np.float64)
+    rng = np.random.default_rng(seed)
+    return reference_design(cells=cells, capture_mode=CAPTURE_EMPIRICAL, betas=betas,
+                            beta_bin=rng.integers(0, 5, len(betas)),
+                            beta_is_chase=rng.integers(0, 2, len(betas)))
+
+
 def config_design(cells: int = 2500, prior_mode: int = PRIOR_CONFIG,
                   capture_mode: int = CAPTURE_BETA, burn_eps: float = 5e-3) -> dict:
     """The benchmark workload: T = 21 h, theta ~ U[0,1), five equal bins, nine

@@ -257,6 +257,12 @@ class Engine:
     def ssa_blocks(self) -> int:
         return self.L.abc_get_ssa_blocks(self.ctx)
 
+    def set_ssa_claim(self, claim: int):
+        self._chk(self.L.abc_set_ssa_claim(self.ctx, claim), "set_ssa_claim")
+
+    def ssa_claim(self) -> int:
+        return self.L.abc_get_ssa_claim(self.ctx)
+
     def autotune(self, models=(1, 2, 3, 4, 5), n: int = 256, seed: int = 0, first: int = 0):
         """Pick the fastest persistent-SSA grid for this workload (results do not
         depend on it).  Returns (workgroups, ms of the SSA over the sample)."""

@@ -90,7 +90,8 @@ def make_genes(eng, G, seed=0, first=3_000_000_000):
 
 def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
         z=4.0, margin=0.0,
-        train_steps=1500, batch=8192, lr=1e-3, seed=0, log=print):
+        train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
+        cal_recall=0.995, log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -215,15 +216,131 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
                               eval_sets, eval_steps, eval_first, n_eval, n_all, plain_pairs,
                               t_plain, seed, dev, log))
     out["by_z"] = by_z
-    best = [r for r in by_z if r["recall"] >= 0.99]
-    pick = max(best, key=lambda r: r["speedup_accepted_per_s"]) if best else by_z[0]
+    if n_cal and mode == 0:
+        out["by_rule"] = _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal,
+                                           cal_recall, eval_sets, eval_steps, eval_first,
+                                           n_eval, n_all, plain_pairs, t_plain, seed, dev, log)
+        by_z = by_z + out["by_rule"]
+    if ceiling_reps and mode == 0:
+        out["ceiling"] = acceptance_ceiling(
+            eng, data, se, parts, eps, {m: eval_sets[m][0] for m in MODELS},
+            {m: eval_first(m) for m in MODELS}, reps=ceiling_reps, log=log)
+        log(f"[screened] ceiling: {out['ceiling']}")
+    # the headline: calibrated rules when there are any (their thresholds never
+    # saw the evaluation particles), else the best z of the grid
+    pool = out.get("by_rule") or by_z
+    best = [r for r in pool if r["recall"] >= 0.99]
+    pick = max(best, key=lambda r: r["speedup_accepted_per_s"]) if best else pool[0]
     out.update(pick)
     out.update(genes=G, train_particles=n_train * len(MODELS), k=k)
     return out
 
 
+def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,
+                      eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev,
+                      log):
+    """The screen's two scores (expected accepting genes; largest standardised
+    margin) on a calibration set of exactly simulated particles fix, per rule,
+    the largest pass threshold whose recall of the calibration set's accepted
+    pairs is at least cal_recall.  Each rule then screens the held-out
+    evaluation particles with that threshold (timed like the z rule), so the
+    reported recall is measured on particles the threshold never saw."""
+    import torch
+    data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
+                          device=dev)
+    se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
+    _lib.check(L.abc_accept_begin(eng.ctx, KS, G, data.ctypes.data_as(C.c_void_p),
+                                  se.ctypes.data_as(C.c_void_p),
+                                  parts.ctypes.data_as(C.c_void_p), 2, 10.0, 0, eps, 0, 0.02),
+               eng.ctx, "accept_begin")
+    scores, acc = [], []
+    for m in MODELS:
+        first = 2_500_000_000 + m * n_cal
+        th, sup, _ = simulate_block(eng, m, n_cal, first, seed)
+        err = torch.empty(n_cal * G, dtype=torch.float64, device=dev)
+        _lib.check(L.abc_errors_dev(eng.ctx, n_cal, KS, _ptr(sup), G, _ptr(data_d), _ptr(se_d),
+                                    parts.ctypes.data_as(C.c_void_p), len(parts), 10.0,
+                                    _ptr(err)), eng.ctx, "errors")
+        acc.append((err.view(G, n_cal) <= eps).sum(0).cpu().numpy())
+        scores.append(sur.scores(eng, sur.features(m, th)).cpu().numpy())
+    _end(eng, G, 16)
+    acc, scores = np.concatenate(acc), np.vstack(scores)
+    log(f"[screened] calibration: {len(acc)} particles, {int(acc.sum())} accepted pairs")
+    res = []
+    for col, name in ((0, "expected_genes"), (1, "max_margin")):
+        sc = scores[:, col]
+        order = np.argsort(-sc, kind="stable")
+        cum = np.cumsum(acc[order]) / max(acc.sum(), 1)
+        r = int(np.searchsorted(cum, cal_recall))
+        tau = float(sc[order[min(r, len(order) - 1)]])
+        r = _screened(eng, sur, L, data, se, parts, KS, G, 0, eps, 0, np.full(G, eps), None, 0.0,
+                      eval_sets, eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain,
+                      seed, dev, log, rule=(col, tau))
+        r.update(rule=name, threshold=tau, cal_recall_target=cal_recall,
+                 cal_pass_share=float((sc >= tau).mean()))
+        res.append(r)
+    return res
+
+
+def acceptance_ceiling(eng, data, se, parts, eps, thetas, firsts, reps=16, seed0=1,
+                       recalls=(0.99, 0.995), log=print):
+    """What any screen that sees only a particle's parameters can gain.
+
+    The plain path's acceptance of a pair (particle, gene) is random given the
+    particle's parameters: its statistics carry the sampling noise of the
+    simulated cells.  Each held-out particle (thetas[m], particles firsts[m]
+    ... of model m) is simulated again `reps` times with independent random
+    streams (seeds seed0, seed0 + 1, ...), which estimates p(i, g), the
+    probability that the pair is accepted (FP64 error <= eps).  A particle
+    carries s_i = sum_g p(i, g) expected accepted pairs; a screen that
+    passes particles in decreasing s_i reaches recall R with the smallest
+    share of particles, f(R).  1 / f(R) bounds the gain in accepted pairs per
+    unit of exact simulation at recall R; the screen's own cost only lowers it.
+    Returns f(R) and 1 / f(R) per recall, the share of particles with any
+    accepted replicate and the mean pairs per particle."""
+    import torch
+    dev = torch.device("cuda:0")
+    L = eng.L
+    G = data.shape[0]
+    KS = data.shape[1]
+    data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64, device=dev)
+    se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
+    counts = []
+    for m in sorted(thetas):
+        th = thetas[m]
+        n = th.numel() // eng.n_params(m)
+        st = torch.empty((4 * eng.nb + 3 * eng.J) * n, dtype=torch.float64, device=dev)
+        sup = torch.empty(KS * n, dtype=torch.float64, device=dev)
+        err = torch.empty(n * G, dtype=torch.float64, device=dev)
+        cnt = torch.zeros((G, n), dtype=torch.int32, device=dev)
+        for r in range(reps):
+            _lib.check(L.abc_simulate_dev(eng.ctx, m, seed0 + r, firsts[m], n, _ptr(th), _ptr(st),
+                                          None, _ptr(sup), None), eng.ctx, "simulate")
+            _lib.check(L.abc_errors_dev(eng.ctx, n, KS, _ptr(sup), G, _ptr(data_d), _ptr(se_d),
+                                        parts.ctypes.data_as(C.c_void_p), len(parts), 10.0,
+                                        _ptr(err)), eng.ctx, "errors")
+            cnt += (err.view(G, n) <= eps).to(torch.int32)
+        counts.append(cnt)
+        log(f"[ceiling] model {m}: {reps} replicates of {n} particles")
+    p = torch.cat(counts, dim=1).to(torch.float64) / reps          # G x n_all
+    s_i = p.sum(0).cpu().numpy()
+    order = np.sort(s_i)[::-1]
+    cum = np.cumsum(order) / max(order.sum(), 1e-300)
+    out = {"reps": reps, "particles": int(len(s_i)),
+           "pairs_per_particle": float(s_i.mean()),
+           "share_with_any_acceptance": float((s_i > 0).mean())}
+    for R in recalls:
+        f = (int(np.searchsorted(cum, R)) + 1) / len(s_i)
+        out[f"share_needed_{R}"] = f
+        out[f"gain_bound_{R}"] = 1.0 / f
+    return out
+
+
 def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin, eval_sets,
-              eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev, log):
+              eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev, log,
+              rule=None):
+    """One screened run over the evaluation particles: pass flags from the z
+    rule, or (rule = (score column, threshold)) score >= threshold."""
     import torch
     nb, J = eng.nb, eng.J
     torch.cuda.synchronize()
@@ -240,7 +357,11 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
         th, _ = eval_sets[m]
         ts = time.perf_counter()
         x = sur.features(m, th)
-        pas, npass = sur.screen(eng, x, z=z, margin=margin)
+        if rule is None:
+            pas, npass = sur.screen(eng, x, z=z, margin=margin)
+        else:
+            pas = (sur.scores(eng, x, margin)[:, rule[0]] >= rule[1]).to(torch.uint8)
+            npass = int(pas.sum().item())
         t_screen += time.perf_counter() - ts
         n_pass_total += npass
         work_all += int(eval_steps[m].sum().item())
@@ -269,7 +390,8 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
                recall=hit / max(len(plain_pairs), 1),
                screened_accepted_per_s=hit / t_scr,
                speedup_accepted_per_s=(hit / t_scr) / (len(plain_pairs) / t_plain))
-    log(f"[screened] z={z}: pass {n_pass_total}/{n_all}, recall {res['recall']:.4f}, "
+    log(f"[screened] {('z=%s' % z) if rule is None else ('rule %d >= %.4g' % rule)}: "
+        f"pass {n_pass_total}/{n_all}, recall {res['recall']:.4f}, "
         f"time {t_scr:.2f}s vs plain {t_plain:.2f}s")
     return res
 

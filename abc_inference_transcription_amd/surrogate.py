@@ -142,6 +142,17 @@ class Surrogate:
         p = np.ascontiguousarray(p, np.float32)
         self._chk(self.L.abc_mlp_set_params(self.h, p.ctypes.data_as(C.c_void_p)), "set_params")
 
+    def scores(self, eng, x, margin: float = 0.0):
+        """Per particle (torch float32, n x 2) against the engine's open
+        acceptance session: expected accepting genes, largest standardised
+        margin."""
+        n = x.shape[0]
+        sc = self.torch.empty((n, 2), dtype=self.torch.float32, device=self.dev)
+        rc = self.L.abc_screen_score_dev(eng.ctx, self.h, n, _ptr(x), _ptr(self.mean),
+                                         _ptr(self.sd), margin, _ptr(sc))
+        _lib.check(rc, eng.ctx, "screen_score")
+        return sc
+
     def screen(self, eng, x, z: float = 4.0, margin: float = 0.0):
         """pass flags (torch uint8) against the engine's open acceptance session."""
         n = x.shape[0]

@@ -1,12 +1,18 @@
-"""Benchmark: ABC particles/sec for the plain path (SSA + statistics + errors).
+"""Benchmark: ABC particles/sec for the plain path (SSA + statistics + errors
++ acceptance).
 
-Workload: all five models, the cell design of the SSA workload (2,500 cells
-per particle, theta ~ U[0,1), 21 h cycle, 5 age bins, 4 pulses + 5 chases,
-Beta(2,8) capture, config priors, Philox seed 0); every particle's statistics
-(the 90-value mean/Fano superset) are scored against 2,000 synthetic genes
-whose data statistics come from ground-truth particles of the same priors with
-SE = |stat| * U[0.05, 0.3].  One step = one batch of P particles per model
-through prior -> rates -> SSA -> statistics -> FP64 errors, all in HBM.
+Workload: the full ABC sweep (5e6 particles per model of the five models,
+2,000 synthetic genes, per-gene top-1,000 acceptance), sampled: each step
+takes P particles per model of that sweep.  Cells: 2,500 per particle,
+theta ~ U[0,1), 21 h cycle, 5 age bins, 4 pulses + 5 chases, Beta(2,8)
+capture, config priors, Philox seed 0.  Every particle's 90 superset
+statistics (mean and Fano factor per age bin and condition) are scored
+against genes whose data statistics come from ground-truth particles of the
+same priors with SE = |stat| * U[0.05, 0.3].  One step = one batch through
+prior -> rates -> burn-in -> SSA -> statistics -> split-BF16 MFMA error
+screen -> FP64 rescoring of the pairs within 2 % (plus the proven bound) of
+each gene's threshold -> on-device per-gene top-1,000 selection, all in HBM;
+the timed region ends with the sweep's accepted sets on the host.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--particles P]
 
@@ -30,6 +36,52 @@ METRIC = "ABC particles/sec (SSA+stats+errors); accepted/sec surrogate vs plain 
 HBM_PEAK = 8000.0       # GB/s
 
 
+def host_info(threads):
+    """The host the CPU baseline runs on: core counts, CPU model and clock, and
+    the oracle's compiler and flags."""
+    import subprocess
+    info = {"threads_used": threads, "nproc": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        info["cgroup_cpu_max"] = open("/sys/fs/cgroup/cpu.max").read().strip()
+    except OSError:
+        info["cgroup_cpu_max"] = None
+    model, mhz = None, None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model is None:
+                model = line.split(":", 1)[1].strip()
+            if line.startswith("cpu MHz") and mhz is None:
+                mhz = float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    info["cpu_model"], info["cpu_mhz"] = model, mhz
+    try:
+        info["compiler"] = subprocess.run(["gcc", "--version"], capture_output=True,
+                                          text=True).stdout.splitlines()[0]
+    except (OSError, IndexError):
+        info["compiler"] = None
+    flags = None
+    for line in open(os.path.join(ROOT, "oracle", "Makefile")):
+        if line.startswith("CFLAGS"):
+            flags = line.split("=", 1)[1].strip().rstrip("\\").strip()
+    info["flags"] = flags
+    return info
+
+
+def cpu_threads():
+    """Every CPU this process may run on: the affinity set, bounded by a
+    cgroup CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -37,6 +89,11 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=4096, help="particles per model per step")
     ap.add_argument("--genes", type=int, default=2000)
+    ap.add_argument("--topk", type=int, default=1000, help="accepted particles per gene")
+    ap.add_argument("--margin", type=float, default=0.02,
+                    help="FP64 rescoring band around each gene's threshold (relative)")
+    ap.add_argument("--sweep-per-model", type=int, default=5_000_000,
+                    help="particles per model of the sweep the steps sample")
     ap.add_argument("--cells", type=int, default=2500)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -46,6 +103,11 @@ def parse():
     ap.add_argument("--sur-train", type=int, default=48000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
     ap.add_argument("--sur-steps", type=int, default=6000, help="Adam steps (batch 8192)")
+    ap.add_argument("--sur-cal", type=int, default=2000,
+                    help="calibration particles per model that set the calibrated pass rules")
+    ap.add_argument("--ceiling-reps", type=int, default=16,
+                    help="independent exact replicates per held-out particle for the bound on "
+                         "what any parameter-only screen can gain (0 = skip)")
     ap.add_argument("--no-moments", action="store_true",
                     help="skip the leg of the reference's own algorithm (moment model)")
     ap.add_argument("--mom-particles", type=int, default=32768,
@@ -53,6 +115,11 @@ def parse():
     ap.add_argument("--mom-method", type=int, default=1, choices=[0, 1],
                     help="moment-model integration: 1 = gene block in closed form, "
                          "0 = Dormand-Prince on all nine moments")
+    ap.add_argument("--no-reference-shape", action="store_true",
+                    help="skip the leg at the reference's own design and 3,419 genes")
+    ap.add_argument("--ref-particles", type=int, default=2048,
+                    help="particles per model per step of the reference-shape leg")
+    ap.add_argument("--ref-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run three device copies of 256 MiB (profiler byte-counter scale)")
     return ap.parse_args()
@@ -137,7 +204,6 @@ def main():
             stats=torch.empty((4 * nb + 3 * J) * P, dtype=torch.float64, device=dev),
             sup=torch.empty(KS * P, dtype=torch.float64, device=dev),
             steps=torch.zeros(P, dtype=torch.int64, device=dev),
-            err=torch.empty(P * G, dtype=torch.float64, device=dev),
         )
     steps_acc = torch.zeros((), dtype=torch.int64, device=dev)
     # all five models' particles go through one burn-in and one SSA launch per step
@@ -150,6 +216,33 @@ def main():
     m_sup = VA(*[bufs[m]["sup"].data_ptr() for m in models])
     m_steps = VA(*[bufs[m]["steps"].data_ptr() for m in models])
 
+    data_f = np.asfortranarray(data)
+    se_f = np.asfortranarray(se)
+    S_PER = a.sweep_per_model
+
+    def begin():
+        # top-k acceptance session over the genes: per gene the k particles of
+        # smallest FP64 error (ties by index) among every particle added
+        _lib.check(L.abc_accept_begin(eng.ctx, KS, G, data_f.ctypes.data_as(C.c_void_p),
+                                      se_f.ctypes.data_as(C.c_void_p),
+                                      parts_c.ctypes.data_as(C.c_void_p), 2, 10.0, 1, 0.0,
+                                      a.topk, a.margin), eng.ctx, "accept_begin")
+
+    def end():
+        """Close the session: per-gene counts, 1-based indices, errors and the
+        number of FP64 rescorings."""
+        cap = G * a.topk + 16
+        counts = np.zeros(G, np.int64)
+        idx = np.zeros(cap, np.int64)
+        eo = np.zeros(cap)
+        nc = C.c_int64()
+        _lib.check(L.abc_accept_end(eng.ctx, counts.ctypes.data_as(C.c_void_p),
+                                    idx.ctypes.data_as(C.c_void_p), eo.ctypes.data_as(C.c_void_p),
+                                    cap, C.byref(nc)), eng.ctx, "accept_end")
+        return counts, idx, eo, nc.value
+
+    acc_ev = []
+
     def step(it, timed):
         first = it * P
         for m in models:
@@ -158,13 +251,20 @@ def main():
         _lib.check(L.abc_simulate_multi_dev(eng.ctx, nm, m_models, 0, (C.c_int64 * nm)(*([first] * nm)),
                                             m_n, m_theta, m_stats, None, m_sup, m_steps),
                    eng.ctx, "simulate")
-        for m in models:
+        if timed:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+        for q, m in enumerate(models):
             b = bufs[m]
-            _lib.check(L.abc_errors_dev(eng.ctx, P, KS, ptr(b["sup"]), G, ptr(data_d), ptr(se_d),
-                                        parts_c.ctypes.data_as(C.c_void_p), 2, 10.0, ptr(b["err"])),
-                       eng.ctx, "errors")
+            # particle `first + i` of model m is sweep particle q * S_PER + first + i
+            _lib.check(L.abc_accept_add_dev(eng.ctx, P, ptr(b["sup"]), q * S_PER + first),
+                       eng.ctx, "accept_add")
             if timed:
                 steps_acc.add_(b["steps"].sum())  # same stream: no host sync
+        if timed:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record(stream)
+            acc_ev.append((e0, e1))
 
     def ssa_time(reset):
         ms, bst, nl = (C.c_double * 2)(), C.c_int64(), C.c_int64()
@@ -180,10 +280,13 @@ def main():
         blocks, tms = eng.autotune(models=models, n=P, first=50_000_000)
         tuned = {"ssa_blocks": blocks, "sample_ssa_ms": round(tms, 3)}
         print(f"[bench] autotune: {blocks} workgroups", file=sys.stderr, flush=True)
+    begin()
     for w in range(a.warmup):
         step(10_000 + w, False)
+    end()
     torch.cuda.synchronize()
     ssa_time(1)
+    begin()
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -191,9 +294,11 @@ def main():
     for it in range(a.steps):
         step(it, True)
         print(f"[bench] step {it} enqueued", file=sys.stderr, flush=True)
+    counts, idx, eo, n_resc = end()
     ev1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    accept_ms = sum(e0.elapsed_time(e1) for e0, e1 in acc_ev)
     gpu_ms = ev0.elapsed_time(ev1)
     n_particles = a.steps * P * len(models)
     value = n_particles / (gpu_ms / 1e3)
@@ -203,23 +308,26 @@ def main():
 
     graph = None
     if not a.no_graph:
-        # the first steps again, each captured into one HIP graph (updated in
-        # place for the next step's particles) and replayed
-        g = None
+        # the first steps again, each captured into one HIP graph (its own
+        # acceptance session; the graph is updated in place for the next step's
+        # particles) and replayed; only the replays are timed
         gsteps = min(a.steps, 2)
-        ev2, ev3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        ev2.record(stream)
+        begin()
+        g, gms = None, 0.0
         for it in range(gsteps):
             g = eng.capture(lambda: step(it, False), graph=g)
+            ev2, ev3 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev2.record(stream)
             eng.replay(g)
-        ev3.record(stream)
-        torch.cuda.synchronize()
-        gms = ev2.elapsed_time(ev3)
+            ev3.record(stream)
+            torch.cuda.synchronize()
+            gms += ev2.elapsed_time(ev3)
+        end()
         eng.drop_graph(g)
         ssa_time(1)
         graph = {"value": round(gsteps * P * len(models) / (gms / 1e3), 3),
-                 "ms_per_step": round(gms / gsteps, 3), "steps": gsteps}
+                 "ms_per_step": round(gms / gsteps, 3), "steps": gsteps,
+                 "timed": "graph launches only (capture and update outside the timing)"}
         print(f"[bench] graph replay: {graph}", file=sys.stderr, flush=True)
 
     # ---- roofline of the dominant kernel (ssa_kernel, the labelling window):
@@ -243,9 +351,17 @@ def main():
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(gpu_ms / a.steps, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (prior draws, Philox seed 0; genes from ground-truth prior particles)",
-        "config": {"workload": "plain ABC: 5 models x %d particles/step, 2500 cells, SSA+stats+FP64 errors vs %d genes" % (P, G),
+        "config": {"workload": "full ABC sweep (5 models x %d particles, %d genes, top-%d per gene), "
+                               "sampled: %d of its %d particles (%d per model per step)"
+                               % (S_PER, G, a.topk, n_particles, S_PER * len(models), P),
                    "models": models, "particles_per_model_per_step": P, "cells_per_particle": a.cells,
                    "genes": G, "stats": KS, "cycle_h": 21.0, "capture": "Beta(2,8)",
+                   "acceptance": {"rule": "top-k per gene", "k": a.topk,
+                                  "rescoring_band": a.margin,
+                                  "fp64_rescorings": n_resc,
+                                  "pairs": n_particles * G,
+                                  "accepted_pairs": int(counts.sum()),
+                                  "ms_per_step": round(accept_ms / a.steps, 3)},
                    "mean_steps_per_particle": total_steps / n_particles},
         "roofline": roof,
         "wall_s": round(wall, 3),
@@ -259,6 +375,8 @@ def main():
         out["surrogate"] = surrogate_leg(eng, stream, a)
     if not a.no_moments:
         out["reference_moment_path"] = moment_leg(stream, a)
+    if not a.no_reference_shape:
+        out["reference_shape"] = reference_shape_leg(stream, a)
     out["model_probs"] = model_probs_leg(eng, stream, a)
     out["data_summary_stats"] = data_stats_leg(eng, stream, a)
     print(json.dumps(out), flush=True)
@@ -273,14 +391,16 @@ def surrogate_leg(eng, stream, a):
     sur = Surrogate(2 * eng.nb * eng.J, seed=0, stream=stream)
     res = S.run(eng, sur, G=a.genes, n_train=a.sur_train, n_eval=a.sur_eval, k=0,
                 accept_rate=4e-5, z=[4.0, 3.0, 2.5, 2.0], train_steps=a.sur_steps, batch=8192,
+                ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal,
                 log=lambda *x: print(*x, file=sys.stderr, flush=True))
     sur.close()
     keep = ("plain_accepted_per_s", "screened_accepted_per_s", "speedup_accepted_per_s",
             "recall", "ssa_fraction_avoided", "ssa_work_fraction_avoided", "plain_pairs",
             "eval_particles", "train_particles", "heldout_nll", "heldout_nll_of_mean_predictor",
             "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
-            "train_loss_last", "train_s", "genes", "z", "eps", "accept_rate_target", "by_z")
-    return {k: res[k] for k in keep}
+            "train_loss_last", "train_s", "genes", "z", "eps", "accept_rate_target", "by_z",
+            "ceiling", "by_rule", "rule", "threshold")
+    return {k: res[k] for k in keep if k in res}
 
 
 def moment_leg(stream, a):
@@ -351,6 +471,92 @@ def moment_leg(stream, a):
     if not a.no_cpu:
         res["cpu"] = moment_cpu(models, cfg, w, is_chase, cap, a.cpu_seconds, a.mom_method)
         res["gpu_vs_cpu"] = round(value / res["cpu"]["value"], 1)
+    return res
+
+
+def reference_shape_leg(stream, a):
+    """The drop-in at the reference's own shape (wrapper.jl:59-78): T = 20 h,
+    ages at the bin centres, 11 conditions, capture from the shipped
+    efficiencies, the 53 abc_sim statistics scored against all 3,419 genes of
+    data/summary_stats and accepted at the fixed epsilon 4.8
+    (scripts/accepted_particles.jl:10); 5,445 cells per particle.  Each step
+    simulates P particles of each of the five models (reference prior) in one
+    launch and adds them to one acceptance session (indices offset per model,
+    so each gene's list splits into the per-model files); the timed region
+    ends with the accepted sets on the host."""
+    import torch
+    from abc_inference_transcription_amd import _lib
+    from abc_inference_transcription_amd.design import reference_shape_design
+    from abc_inference_transcription_amd.engine import Engine
+    from abc_inference_transcription_amd.host import data_matrices
+    golden = os.path.join(ROOT, "tests", "golden")
+    betas = np.load(os.path.join(golden, "capture.npz"))["betas"]
+    data, se = data_matrices(dict(np.load(os.path.join(golden, "summary_stats_all.npz"))))
+    data, se = np.asfortranarray(data), np.asfortranarray(se)
+    G, K = data.shape
+    eng = Engine(reference_shape_design(betas), device=0)
+    eng.bind_stream(stream)
+    L = eng.L
+    dev = torch.device("cuda:0")
+    models, P, per = [1, 2, 3, 4, 5], a.ref_particles, 250_000
+    nm = len(models)
+    th = [torch.empty(eng.n_params(m) * P, dtype=torch.float64, device=dev) for m in models]
+    st = [torch.empty(K * P, dtype=torch.float64, device=dev) for _ in models]
+    VA = C.c_void_p * nm
+    m_models = (C.c_int32 * nm)(*models)
+    m_n = (C.c_int64 * nm)(*([P] * nm))
+    m_th = VA(*[t.data_ptr() for t in th])
+    m_st = VA(*[t.data_ptr() for t in st])
+    parts = np.ascontiguousarray(eng.parts, np.int32)
+    vp = lambda x: x.ctypes.data_as(C.c_void_p)
+
+    def step(it):
+        first = it * P
+        for q, m in enumerate(models):
+            _lib.check(L.abc_sample_prior_dev(eng.ctx, m, 0, first, P,
+                                              C.c_void_p(th[q].data_ptr())), eng.ctx, "prior")
+        _lib.check(L.abc_simulate_multi_dev(eng.ctx, nm, m_models, 0,
+                                            (C.c_int64 * nm)(*([first] * nm)), m_n, m_th, m_st,
+                                            None, None, None), eng.ctx, "simulate")
+        for q in range(nm):
+            _lib.check(L.abc_accept_add_dev(eng.ctx, P, C.c_void_p(st[q].data_ptr()),
+                                            q * per + first), eng.ctx, "accept_add")
+
+    def session(steps, base):
+        _lib.check(L.abc_accept_begin(eng.ctx, K, G, vp(data), vp(se), vp(parts), len(parts),
+                                      10.0, 0, 4.8, 0, 0.02), eng.ctx, "accept_begin")
+        for it in range(steps):
+            step(base + it)
+        cap = 1 << 22
+        while True:  # a short capacity leaves the session open with the counts filled in
+            counts = np.zeros(G, np.int64)
+            idx = np.zeros(cap, np.int64)
+            nc = C.c_int64()
+            rc = L.abc_accept_end(eng.ctx, vp(counts), vp(idx), None, cap, C.byref(nc))
+            if rc != -5:
+                break
+            cap = int(counts.sum())
+        _lib.check(rc, eng.ctx, "accept_end")
+        return counts, idx[:int(counts.sum())], nc.value
+
+    session(1, 1000)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    counts, idx, nres = session(a.ref_steps, 0)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    n = a.ref_steps * P * nm
+    per_model = [int(((idx - 1) // per == q).sum()) for q in range(nm)]
+    eng.close()
+    res = {"value": round(n / wall, 3), "unit": "particles/s", "particles": n,
+           "cells_per_particle": 5445, "genes": G, "stats": K, "eps": 4.8,
+           "accepted_pairs": int(counts.sum()), "accepted_pairs_per_model": per_model,
+           "genes_with_acceptance": int((counts > 0).sum()), "fp64_rescorings": nres,
+           "ms_per_step": round(wall / a.ref_steps * 1e3, 3),
+           "timing": "host clock around the session (simulation, screening, FP64 rescoring, "
+                     "per-batch merge, sorted lists on the host)"}
+    print(f"[bench] reference shape: {res['value']:.1f} particles/s, "
+          f"{res['accepted_pairs']} accepted pairs", file=sys.stderr, flush=True)
     return res
 
 
@@ -440,7 +646,7 @@ def data_stats_leg(eng, stream, a):
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
-        threads = min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
         k = 256
         t = time.perf_counter()
         O.data_stats(u[:, :k], l[:, :k], age, cond, nc, J, pulse, chase, w, n_boot=B,
@@ -459,7 +665,7 @@ def moment_cpu(models, cfg, w, is_chase, cap, budget_s, method):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from abc_inference_transcription_amd.design import reference_design
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     od = O.OracleDesign(reference_design(cells=55))
     ocfg = O.moment_cfg(cfg.ages, cfg.pulse, cfg.chase, cycle=20.0, iv=[0.0, 0.5] + [0.0] * 7,
                         method=method)
@@ -474,7 +680,7 @@ def moment_cpu(models, cfg, w, is_chase, cap, budget_s, method):
         done += 16
         it += 1
     return {"value": round(done / t_used, 3), "unit": "particles/s", "cores": threads,
-            "kind": "port", "sample": f"{done} particles (16 per model round-robin) in "
+            "kind": "port", "host": host_info(threads), "sample": f"{done} particles (16 per model round-robin) in "
                                       f"{t_used:.1f}s on {threads} threads"}
 
 
@@ -486,7 +692,7 @@ def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):
     benchmark's measured mean steps per 2,500-cell particle."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     small = dict(design, cells=100)
     od = O.OracleDesign(small)
     done_steps, done_p, t_used = 0, 0, 0.0
@@ -503,7 +709,7 @@ def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):
         it += 1
     sps = done_steps / t_used
     return {"value": round(sps / gpu_steps_per_particle, 4), "unit": "particles/s",
-            "cores": threads, "kind": "port",
+            "cores": threads, "kind": "port", "host": host_info(threads),
             "sample": f"{done_p} benchmark particles (16 per model round-robin, first of each "
                       f"model's stream) with 100 cells each: {done_steps} SSA steps in "
                       f"{t_used:.1f}s on {threads} threads = {sps:.3e} steps/s; scaled by the "

@@ -364,18 +364,37 @@ int32_t abc_screen_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float* x_dev
                        const float* mean_dev, const float* sd_dev, double z, double margin,
                        uint8_t* pass_dev, int64_t* n_pass);
 
+/* The screen's two scores per particle instead of a pass flag, for choosing
+ * the pass rule and its threshold: score[2 i] = the expected number of genes
+ * that accept particle i, sum over genes of P(err <= thr * (1 + margin))
+ * under the same Gaussian approximation of each gene's error, and
+ * score[2 i + 1] = the largest standardised margin (thr * (1 + margin) - E)
+ * / sd over the genes (abc_screen_dev passes a particle when it is >= -z).
+ * A threshold that reaches the clamp, or a prediction that is not finite,
+ * gives +inf.  score_dev: n x 2 floats, device. */
+int32_t abc_screen_score_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float* x_dev,
+                             const float* mean_dev, const float* sd_dev, double margin,
+                             float* score_dev);
+
 /* Launch shape of the persistent SSA kernel: workgroups (0 = occupancy
  * default).  Set by the autotuner. */
 int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
 int32_t abc_get_ssa_blocks(const abc_ctx* ctx);
+/* Work items (cell sub-lanes) a lane of the SSA kernel claims per atomic on
+ * its work queue: 1, 2, 4 or 8 consecutive items (default 1).  Items are
+ * independent, so this changes the schedule only.  Set by the autotuner. */
+int32_t abc_set_ssa_claim(abc_ctx* ctx, int32_t claim);
+int32_t abc_get_ssa_claim(const abc_ctx* ctx);
 
 /* Autotuner of the persistent SSA launch shape (no reference counterpart: the
  * reference runs one particle at a time, scripts/abc_simulation.jl:88).  Draws
  * n prior particles (from first) of each listed model, simulates them once per
  * candidate grid (1, 2, 3, 4, 6, 8, 12, 16 workgroups per CU, up to
- * occupancy), times ssa_kernel with HIP events and keeps the fastest grid for
- * later calls.  Outputs never depend on the grid, so tuning changes speed
- * only.  best_blocks / best_ms (optional) receive the choice and its time. */
+ * occupancy) and then, on the fastest grid, once per queue claim (2 and 4
+ * work items per atomic against 1), times ssa_kernel with HIP events and keeps
+ * the fastest shape for later calls.  Outputs never depend on the shape, so
+ * tuning changes speed only.  best_blocks / best_ms (optional) receive the
+ * grid and its time; abc_get_ssa_claim gives the claim. */
 int32_t abc_autotune_ssa(abc_ctx* ctx, int32_t n_models, const int32_t* models, uint64_t seed,
                          int64_t first, int64_t n, int32_t* best_blocks, double* best_ms);
 

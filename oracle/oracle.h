@@ -34,6 +34,7 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 double orc_u01_32(uint32_t w);
 double orc_u01_53(uint32_t a, uint32_t b);
 double orc_log(double x);
+float orc_exp1_32(uint32_t w);   /* -ln(((float)w + 0.5f) 2^-32), single precision */
 double orc_exp(double x);
 double orc_exp10(double x);
 double orc_u128_to_f64(uint64_t hi, uint64_t lo);

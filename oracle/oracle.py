@@ -67,6 +67,7 @@ def _declare(L):
     for f in ("orc_log", "orc_exp", "orc_exp10"):
         getattr(L, f).argtypes = [D]; getattr(L, f).restype = D
     L.orc_u128_to_f64.argtypes = [U64, U64]; L.orc_u128_to_f64.restype = D
+    L.orc_exp1_32.argtypes = [C.c_uint32]; L.orc_exp1_32.restype = C.c_float
     L.orc_n_params.argtypes = [C.c_int]; L.orc_n_params.restype = C.c_int
     L.orc_sample_prior.argtypes = [P, C.c_int, U64, I64, I64, P]
     L.orc_simulate.argtypes = [P, C.c_int, U64, I64, I64, P, C.c_int, P, P, P, P, P, P, P, P]
@@ -124,6 +125,11 @@ def log(x):
 
 def exp(x):
     return lib().orc_exp(float(x))
+
+
+def exp1_32(w):
+    """The single-precision waiting time -ln(((float)w + 0.5f) 2^-32)."""
+    return lib().orc_exp1_32(int(w))
 
 
 def exp10(x):

@@ -92,6 +92,30 @@ double orc_log(double x) {
   return hi + ((kd * LN2_LO + abc_log_lo[j]) + lp);
 }
 
+static inline uint32_t fbits(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static inline float bitsf(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+
+/* Standard exponential variate from one 32-bit word in single precision, the
+ * waiting times of the event loops: E = -ln(u), u = ((float)w + 0.5f) 2^-32.
+ * The same table scheme on the bits of a float: the bits of u minus
+ * ABC_LOG32_OFF select subinterval j (bits 16..22), r = z * INV32[j] - 1 by
+ * one fused multiply-add (|r| <= 2^-8), log1p(r) = r + r^2 (r/3 - 1/2),
+ * ln u = k ln2 + L32[j] + log1p(r) with ln2 as two floats.  Every step is one
+ * IEEE single-precision operation or fused multiply-add. */
+float orc_exp1_32(uint32_t w) {
+  const float u = ((float)w + 0.5f) * 0x1p-32f;
+  const uint32_t iu = fbits(u);
+  const uint32_t tmp = iu - ABC_LOG32_OFF;
+  const int j = (int)((tmp >> 16) & (uint32_t)(ABC_LOG_N - 1));
+  const int k = (int32_t)tmp >> 23;
+  const float z = bitsf(iu - (tmp & 0xFF800000u));
+  const float r = fmaf(z, abc_log32_inv[j], -1.0f);
+  const float p = fmaf(r, 0x1.555556p-2f, -0.5f);   /* 1/3, -1/2 */
+  const float lp = fmaf(p * r, r, r);
+  const float kf = (float)k;
+  return -(fmaf(kf, 0x1.62e430p-1f, abc_log32_l[j]) + fmaf(kf, -0x1.05c610p-29f, lp));
+}
+
 double orc_exp(double x) {
   if (x != x) return x;
   if (x > 709.78) return INFINITY;

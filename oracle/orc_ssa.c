@@ -418,7 +418,7 @@ static int burn_cell(const orc_design* d, const prates* r, const uint32_t key[3]
       /* waiting time = Exp(1) * (1 / rate); the reciprocal is a per-bin constant */
       draw(&s0, w);
       st++;
-      tg = t + (-orc_log(orc_u01_32(w[0]))) * (1.0 / (g ? r->koff[j] : r->kon[j]));
+      tg = t + (double)orc_exp1_32(w[0]) * (1.0 / (g ? r->koff[j] : r->kon[j]));
       gneed = 0;
     }
     /* births at rate amax_j bc (T + a) / bound_j (size scaling) or amax_j bc,
@@ -552,11 +552,11 @@ static int sim_window(const orc_design* d, const prates* r, const uint32_t key[3
       continue;
     }
     if (kind == 2) {
-      tg = t + (-orc_log(orc_u01_32(w[0]))) * (1.0 / (g ? r->koff[j] : r->kon[j]));
+      tg = t + (double)orc_exp1_32(w[0]) * (1.0 / (g ? r->koff[j] : r->kon[j]));
       gneed = 0;
       if (!pend && a0 > 0.0) continue;   /* a molecule candidate comes first */
     } else if (kind == 3) {
-      erem = -orc_log(orc_u01_32(w[0]));
+      erem = (double)orc_exp1_32(w[0]);
       pw[0] = w[1]; pw[1] = w[2]; pw[2] = w[3];
       pend = 1;
     }

@@ -7,6 +7,8 @@ Writes tests/golden/*.npz -- inputs and expected outputs only (data, no code):
                   recovered_statistics rows (scripts/recover_statistics.jl:27-68
                   output; 11 conditions x 5 ages x 5 moments).
   summary_stats.npz : the first G genes of the 14 data/summary_stats matrices.
+  summary_stats_all.npz : all 3,419 genes of the 14 matrices (the reference's
+                  own error-sweep input at its full size).
   capture.npz   : data/capture_efficiencies.txt (all 5,422 cells).
   julia_floats.txt : float tokens exactly as Julia's writedlm printed them in
                   the reference's data files (every exponent-form token up to
@@ -41,6 +43,8 @@ def main(rows=12, genes=20):
     np.savez_compressed(os.path.join(OUT, "recovered.npz"), **rec)
     ss = {k: np.loadtxt(R + f"summary_stats/{k}.txt", ndmin=2)[:genes] for k in SS}
     np.savez_compressed(os.path.join(OUT, "summary_stats.npz"), **ss)
+    full = {k: np.loadtxt(R + f"summary_stats/{k}.txt", ndmin=2) for k in SS}
+    np.savez_compressed(os.path.join(OUT, "summary_stats_all.npz"), **full)
     np.savez_compressed(os.path.join(OUT, "capture.npz"),
                         betas=np.loadtxt(R + "capture_efficiencies.txt"))
 

@@ -1,9 +1,12 @@
 """HIP-graph capture of the device-resident path and the SSA autotuner.
 
 A captured prior -> simulate -> errors sequence must give, on every replay,
-the bits of the same calls run eagerly; replays are interleaved with other
-eager work and host synchronisations, and an in-place update to other
-particles must match an eager run of those particles."""
+the bits of the same calls run eagerly and of the oracle's replay of the same
+particles; replays are interleaved with other eager work and host
+synchronisations, and an in-place update to other particles must match an
+eager run of those particles.  A graph whose buffers a larger eager call
+reallocated is refused on the host.  A captured top-k acceptance (one graph
+updated per batch) gives the eager session's sets."""
 import ctypes as C
 
 import numpy as np
@@ -55,8 +58,37 @@ class Step:
         self.steps.fill_(-1)
 
 
+def same_bits(a, b):
+    """Bit-identical where not NaN, NaN in the same places (IEEE leaves a NaN's
+    payload to the hardware, and x86 and the GPU make different ones)."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+def check_against_oracle(oracle, design, step, outs, first):
+    """Replayed theta / stats / superset / steps / errors against the oracle's
+    prior, simulation and FP64 errors of the same particles."""
+    theta, stats, sup, steps, err = outs
+    n, K, G = step.n, step.K, step.G
+    od = oracle.OracleDesign(design)
+    th = oracle.sample_prior(od, step.model, 0, first, n)
+    assert same_bits(theta.reshape(-1, n).T, th)
+    want = oracle.simulate(od, step.model, 0, first, th, cells=False)
+    assert same_bits(stats.reshape(K, n).T, want["stats_ref"])
+    assert same_bits(sup.reshape(-1, n).T, want["superset"])
+    assert np.array_equal(steps, np.asarray(want["steps"], np.int64))
+    data = step.data.cpu().numpy().reshape(K, G).T
+    se = step.se.cpu().numpy().reshape(K, G).T
+    e = oracle.errors(want["stats_ref"], data, se, step.parts)
+    assert same_bits(err.reshape(G, n).T, e)
+
+
 @pytest.mark.parametrize("dname,model", [("config", 4), ("reference", 3)])
-def test_graph_replays_match_eager(engine_factory, dname, model):
+def test_graph_replays_match_eager(engine_factory, oracle, dname, model):
     import torch
     dev = torch.device("cuda:0")
     design = config_design(cells=300) if dname == "config" else reference_design(cells=220)
@@ -81,15 +113,114 @@ def test_graph_replays_match_eager(engine_factory, dname, model):
             eng.replay(g)
             torch.cuda.synchronize()
             print(f"[graph] replay {rep}", flush=True)
-            for a, b in zip(step.outputs(), ref):
+            outs = step.outputs()
+            for a, b in zip(outs, ref):
                 np.testing.assert_array_equal(a, b)
+            if rep == 0:
+                check_against_oracle(oracle, design, step, outs, 1000)
         g = eng.capture(lambda: step(2000), graph=g)   # same sequence, other particles
         step.clear()
         eng.replay(g)
         torch.cuda.synchronize()
-        for a, b in zip(step.outputs(), ref2):
+        outs = step.outputs()
+        for a, b in zip(outs, ref2):
+            np.testing.assert_array_equal(a, b)
+        check_against_oracle(oracle, design, step, outs, 2000)
+        eng.drop_graph(g)
+
+
+def test_replay_refused_after_buffer_growth(engine_factory):
+    """A larger eager call after the capture reallocates a buffer the graph
+    holds: the replay is refused before any kernel runs, and a new capture
+    works."""
+    import torch
+    dev = torch.device("cuda:0")
+    eng = engine_factory(config_design(cells=150))
+    s = torch.cuda.Stream(device=dev)
+    eng.bind_stream(s)
+    with torch.cuda.stream(s):
+        small = Step(eng, torch, 2, 45, 7, dev)
+        small(0)
+        torch.cuda.synchronize()
+        ref = small.outputs()
+        g = eng.capture(lambda: small(0))
+        big = Step(eng, torch, 2, 100, 7, dev)
+        big(500)                                   # grows the context's buffers
+        torch.cuda.synchronize()
+        small.clear()
+        with pytest.raises(RuntimeError, match="reallocated"):
+            eng.replay(g)
+        torch.cuda.synchronize()
+        assert float(small.stats[0].item()) == -1.0   # nothing ran
+        g = eng.capture(lambda: small(0), graph=g)
+        eng.replay(g)
+        torch.cuda.synchronize()
+        for a, b in zip(small.outputs(), ref):
             np.testing.assert_array_equal(a, b)
         eng.drop_graph(g)
+
+
+def test_captured_topk_acceptance_matches_eager(engine_factory):
+    """Per batch, prior -> simulate -> top-k acceptance captured into one graph
+    (updated in place for the next batch) and replayed; the session's sets
+    equal those of the same batches run eagerly."""
+    import torch
+    dev = torch.device("cuda:0")
+    eng = engine_factory(config_design(cells=600))
+    s = torch.cuda.Stream(device=dev)
+    eng.bind_stream(s)
+    L, ctx = eng.L, eng.ctx
+    KS = 2 * eng.nb * eng.J
+    n, G, k, nbatch = 300, 40, 12, 4
+    rng = np.random.default_rng(8)
+    with torch.cuda.stream(s):
+        th = torch.empty(n * eng.n_params(3), dtype=torch.float64, device=dev)
+        st = torch.empty(n * eng.n_stats, dtype=torch.float64, device=dev)
+        sup = torch.empty(n * KS, dtype=torch.float64, device=dev)
+        # genes from the (finite) simulated statistics of other particles
+        gt = eng.simulate(3, eng.sample_prior(3, 8 * G, first=90_000), first=90_000,
+                          superset=True)["superset"]
+        gt = gt[np.all(np.isfinite(gt), axis=1)][:G]
+        assert len(gt) == G
+        data = np.asfortranarray(gt * rng.uniform(0.9, 1.1, gt.shape))
+        se = np.asfortranarray(np.abs(data) * rng.uniform(0.05, 0.3, data.shape))
+        parts = np.array([KS // 2, KS // 2], np.int32)
+        vp = lambda a: a.ctypes.data_as(C.c_void_p)
+
+        def batch(b):
+            eng._chk(L.abc_sample_prior_dev(ctx, 3, 0, b * n, n, _ptr(th)), "prior")
+            eng._chk(L.abc_simulate_dev(ctx, 3, 0, b * n, n, _ptr(th), _ptr(st), None,
+                                        _ptr(sup), None), "sim")
+            eng._chk(L.abc_accept_add_dev(ctx, n, _ptr(sup), b * n), "add")
+
+        def session(run):
+            eng._chk(L.abc_accept_begin(ctx, KS, G, vp(data), vp(se), vp(parts), 2, 10.0, 1,
+                                        0.0, k, 0.02), "begin")
+            run()
+            counts = np.zeros(G, np.int64)
+            idx = np.zeros(G * k, np.int64)
+            eo = np.zeros(G * k)
+            eng._chk(L.abc_accept_end(ctx, vp(counts), vp(idx), vp(eo), G * k, None), "end")
+            return counts, idx, eo
+
+        def eager():
+            for b in range(nbatch):
+                batch(b)
+        want = session(eager)
+        graphs = {}
+
+        def captured():
+            g = None
+            for b in range(nbatch):
+                g = eng.capture(lambda: batch(b), graph=g)
+                eng.replay(g)
+            graphs["g"] = g
+        got = session(captured)
+        torch.cuda.synchronize()
+        eng.drop_graph(graphs["g"])
+    assert np.array_equal(want[0], got[0]) and int(want[0].sum()) > 0
+    assert np.array_equal(want[1], got[1])
+    assert same_bits(want[2], got[2])
 
 
 def test_capture_refuses_host_calls_and_growth(engine_factory):
@@ -125,9 +256,15 @@ def test_autotune_changes_speed_not_results(engine_factory):
     th = eng.sample_prior(5, 40, first=77)
     before = eng.simulate(5, th, first=77, superset=True)
     blocks, ms = eng.autotune(models=(4, 5), n=64, first=10)
-    print(f"[autotune] blocks {blocks}, {ms:.2f} ms", flush=True)
+    print(f"[autotune] blocks {blocks}, claim {eng.ssa_claim()}, {ms:.2f} ms", flush=True)
     assert blocks > 0 and ms > 0
-    assert eng.ssa_blocks() == blocks
+    assert eng.ssa_blocks() == blocks and eng.ssa_claim() in (1, 2, 4)
     after = eng.simulate(5, th, first=77, superset=True)
     for k in ("stats", "superset", "steps"):
         np.testing.assert_array_equal(before[k], after[k])
+    # every queue claim gives the same bits
+    for claim in (2, 8):
+        eng.set_ssa_claim(claim)
+        again = eng.simulate(5, th, first=77, superset=True)
+        for k in ("stats", "superset", "steps"):
+            np.testing.assert_array_equal(before[k], again[k])

@@ -2,7 +2,10 @@
 
 Philox4x32-10 against the Random123 known-answer vectors (Salmon et al. 2011,
 kat_vectors: philox4x32_10); log/exp against libm within a few ulp; the
-128-bit conversion against Python's correctly rounded int -> float.
+single-precision waiting-time log against double-precision log within a few
+float ulp, its table against the mathematics it tabulates, its variates
+against Exp(1); the 128-bit conversion against Python's correctly rounded
+int -> float.
 """
 import math
 import struct
@@ -57,6 +60,77 @@ def test_log_accuracy(oracle):
     for d in (2**-52, -2**-53, 1e-10, -1e-10, 2**-9, -2**-9):
         x = 1.0 + d
         assert ulps(oracle.log(x), math.log1p(x - 1.0)) <= 1.0
+
+
+def _table32(name):
+    """A float table of the generated log header, parsed from its hex literals."""
+    import os
+    import re
+    h = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                     "abc_inference_transcription_amd", "csrc", "abc_logtab.h")
+    body = re.search(name + r"\[\d+\] = \{(.*?)\};", open(h).read(), re.S).group(1)
+    return [float.fromhex(t.strip().rstrip("f")) for t in body.split(",") if t.strip()]
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def test_exp1_32_table_is_the_mathematics():
+    # subinterval j of the float bits above 0x3F350000 has midpoint c_j;
+    # INV32[j] = float(1/c_j) (1 next to 1) and L32[j] = float(-ln INV32[j])
+    inv, L = _table32("abc_log32_inv"), _table32("abc_log32_l")
+    assert len(inv) == len(L) == 128
+    asf = lambda i: struct.unpack("<f", struct.pack("<I", i))[0]
+    one = ((0x3F800000 - 0x3F350000) >> 16) & 127
+    for j in range(128):
+        c = (asf(0x3F350000 + (j << 16)) + asf(0x3F350000 + ((j + 1) << 16))) / 2
+        want = 1.0 if j in (one - 1, one) else f32(1.0 / c)
+        assert inv[j] == want, j
+        assert abs(L[j] - (-math.log(inv[j]))) <= 0.5 * np.spacing(np.float32(abs(L[j]) or 1e-30)), j
+
+
+def test_exp1_32_accuracy(oracle):
+    # against -log of the same single-precision u in double precision; words
+    # cover every subinterval of the table in several binades, both ends of the
+    # 32-bit range and the exact / rounded boundary of float(w) at 2^24
+    rng = random.Random(4)
+    words = [0, 1, 2, 3, 2**24 - 1, 2**24, 2**24 + 1, 2**31, 2**31 + 1, 2**32 - 1, 2**32 - 128,
+             2**32 - 129, 2**32 - 200, 0x80000000 - 1]
+    words += [rng.getrandbits(32) for _ in range(20000)]
+    asf = lambda i: struct.unpack("<f", struct.pack("<I", i))[0]
+    for j in range(128):
+        for e in (0, -1, -5, -20, -31):
+            for _ in range(4):
+                u = asf(0x3F350000 + (j << 16) + rng.randrange(1 << 16)) * 2.0 ** e
+                w = int(u * 2.0 ** 32 - 0.5)
+                if 0 <= w < 2**32:
+                    words.append(w)
+    worst = 0.0
+    for w in words:
+        u = np.float32(np.float32(np.float32(w) + np.float32(0.5)) * np.float32(2.0 ** -32))
+        want = -math.log(float(u))
+        got = oracle.exp1_32(w)
+        if want == 0.0:
+            assert got == 0.0
+            continue
+        worst = max(worst, abs(got - want) / float(np.spacing(np.float32(want))))
+    assert worst <= 3.0, worst
+    assert oracle.exp1_32(0) == f32(-math.log(0.5 * 2.0 ** -32))
+    # next to u = 1 the reduction is exact: small waiting times keep their
+    # relative precision
+    for w in (2**32 - 2**9, 2**32 - 2**12, 2**32 - 2**20):
+        u = float(np.float32(np.float32(w) + np.float32(0.5)) * np.float32(2.0 ** -32))
+        assert abs(oracle.exp1_32(w) - (-math.log1p(u - 1.0))) <= 1e-6 * -math.log1p(u - 1.0)
+
+
+def test_exp1_32_is_exponential(oracle):
+    # Philox words through exp1_32: Kolmogorov-Smirnov against Exp(1)
+    from scipy import stats
+    ws = [int(x) for i in range(5000) for x in oracle.philox([i, 3, 9, 27], [5, 6])]
+    e = np.array([oracle.exp1_32(w) for w in ws])
+    assert stats.kstest(e, "expon").pvalue > 1e-3
+    assert abs(e.mean() - 1.0) < 4.0 / math.sqrt(len(e))
 
 
 def test_exp_accuracy(oracle):
