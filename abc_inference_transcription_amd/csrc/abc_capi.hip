@@ -12,13 +12,41 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/abc.h"
 #include "abc_kernels.h"
 
+// Device memory of abc_simulate_noalloc_dev: one array per GPU, alive until
+// the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (scratch_address).
+#define ABC_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+static __device__ __align__(256) unsigned char scratch[ABC_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The address of this device's scratch array (the device must be current);
+// looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  const bool cached = device >= 0 && device < kMaxDevices;
+  if (cached && g_scratch_ptr[device]) {
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+  }
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (cached) g_scratch_ptr[device] = p;
+  *out = p;
+  return hipSuccess;
+}
 
 struct DevBuf {
   void* p = nullptr;
@@ -48,6 +76,7 @@ struct AcceptState {
 
 struct abc_ctx {
   int device = 0;
+  void* scratch = nullptr;   // this device's scratch array (abc_simulate_noalloc_dev)
   hipStream_t stream = nullptr;
   bool own_stream = true;
   int num_cu = 0;
@@ -519,6 +548,11 @@ int32_t abc_create(abc_ctx** out, int32_t device, const abc_design* d) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return ABC_E_HIP; }
   c->num_cu = prop.multiProcessorCount;
+  if (scratch_address(device, &c->scratch) != hipSuccess || !c->scratch) {
+    (void)hipGetLastError();
+    delete c;
+    return ABC_E_HIP;
+  }
   int per_cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ssa_kernel, SSA_WG, 0) != hipSuccess ||
       per_cu < 1)
@@ -720,6 +754,34 @@ int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* mode
   HIPCHK(c, record(c, c->ev[3]));
   c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
   return ABC_OK;
+}
+
+int32_t abc_simulate_noalloc_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first,
+                                 int64_t n, const double* theta_dev, double* stats_dev,
+                                 double* moments_dev, double* superset_dev, int64_t* steps_dev) {
+  if (!c || (n > 0 && (!theta_dev || !stats_dev))) return ABC_E_ARG;
+  int rc;
+  if ((rc = check_range(c, model, first, n))) return rc;
+  if (n == 0) return ABC_OK;
+  const int C = c->hd.C;
+  if (ws_carve(nullptr, n, C, nullptr) > ABC_NOALLOC_SCRATCH_BYTES) {
+    int64_t lo = 0, hi = (int64_t)1 << 31;  // the largest n whose workspace fits
+    while (hi - lo > 1) {
+      const int64_t mid = lo + (hi - lo) / 2;
+      (ws_carve(nullptr, mid, C, nullptr) <= ABC_NOALLOC_SCRATCH_BYTES ? lo : hi) = mid;
+    }
+    return fail(c, ABC_E_ARG,
+                "abc_simulate_noalloc_dev takes at most " + std::to_string(lo) +
+                    " particles of " + std::to_string(C) + " cells per call (" +
+                    std::to_string(ABC_NOALLOC_SCRATCH_BYTES >> 20) + " MiB of scratch)");
+  }
+  SimWs w;
+  ws_carve(static_cast<char*>(c->scratch), n, C, &w);
+  w.timed = false;  // its burn-in draws are not in the context's tally
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, c->stream, w.tally, (int64_t)2);
+  HIPCHK(c, hipGetLastError());
+  return simulate_ws(c, w, model, seed, first, n, theta_dev, stats_dev, moments_dev,
+                     superset_dev, steps_dev, nullptr);
 }
 
 int32_t abc_simulate_idx_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t n,

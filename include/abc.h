@@ -175,6 +175,21 @@ int32_t abc_simulate_multi_dev(abc_ctx* ctx, int32_t n_models, const int32_t* mo
                                const double* const* theta_dev, double* const* stats_dev,
                                double* const* moments_dev, double* const* superset_dev,
                                int64_t* const* steps_dev);
+/* abc_simulate_dev without device allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results, bit for bit, as abc_simulate_dev; its workspace (rates, queue
+ * order, burn-in records, sub-lane offsets, cell counts, counters) is carved
+ * from one 256 MiB array per GPU held by the library until the process ends,
+ * so a call needs about 77 * cells + 400 bytes per particle: at most about
+ * 1,390 particles of 2,500 cells (ABC_E_ARG beyond the limit, with the limit
+ * in abc_last_error).  Every call on a GPU uses the same array, and each call
+ * clears what it needs cleared: calls that overlap on one GPU (two streams,
+ * two contexts or two threads) race and must be serialised by the caller.
+ * Its work is not counted by abc_ssa_time. */
+int32_t abc_simulate_noalloc_dev(abc_ctx* ctx, int32_t model, uint64_t seed, int64_t first,
+                                 int64_t n, const double* theta_dev, double* stats_dev,
+                                 double* moments_dev, double* superset_dev, int64_t* steps_dev);
 /* Simulation of an arbitrary subset of particles: row i uses the random
  * streams of global particle pidx_dev[i] (device int64, < 2^32), so its
  * results equal those of the same particle in a contiguous run. */
