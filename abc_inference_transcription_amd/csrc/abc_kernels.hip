@@ -372,8 +372,9 @@ __global__ __launch_bounds__(256) void add_bsum_kernel(int64_t* __restrict__ off
 // cell | model << 24, particle), key (seed_lo, seed_hi); stream 0 holds the
 // set-up draws and the gene clock, stream 1 + sub the molecule events and
 // division coins of sub-lane sub.
-__device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double T, int nb) {
-  return ((double)(k - nbB) * T) / (double)nb;
+// time of grid edge k (edges every h = T / nb hours; edge nbB is t = 0)
+__device__ __forceinline__ double grid_time(int64_t k, int64_t nbB, double h) {
+  return (double)(k - nbB) * h;
 }
 
 __device__ __forceinline__ u4 draw(uint64_t ctr, uint32_t stream, uint32_t cell, uint32_t pid,
@@ -450,16 +451,17 @@ __device__ __forceinline__ double logfact(double kd, const LogEntry* lt) {
 
 // Poisson(mu): inversion below 10, else the transformed rejection method with
 // squeeze (Hormann, Insurance: Math. Econ. 12, 1993), at most 1000 tries.
-__device__ uint32_t poisson(WordSrc& src, double mu, const LogEntry* lt) {
+__device__ uint32_t poisson(WordSrc& src, double mu, const LogEntry* lt, const double* rk) {
   if (!(mu > 0.0)) return 0u;
   if (mu < 10.0) {
+    // inversion: p_k = p_{k-1} * mu * RN(1/k) (rk: the table in LDS)
     const uint32_t w0 = src.next(), w1 = src.next();
     const double u = u01_53(w0, w1);
     double p = ex(-mu), F = p;
     uint32_t k = 0;
     while (u > F && k < 200u) {
       k++;
-      p = (p * mu) / (double)k;
+      p = (p * mu) * rk[k];
       F = F + p;
     }
     return k;
@@ -560,6 +562,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
                                                       unsigned long long* __restrict__ burn_tally) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
+  const double h = T / (double)nb;  // hours between rate-bin edges
   const unsigned long long total = (unsigned long long)n * (unsigned long long)C;
   unsigned long long lane_steps = 0;  // this lane's burn-in draws, added to *burn_tally at exit
   __shared__ LogEntry32 ltab32[ABC_LOG_N];
@@ -585,7 +588,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
     const int64_t nbB = (int64_t)rp->B * nb;
     int64_t k = 0, knext = step;
     int j = 0;
-    double t = grid_time(0, nbB, T, nb), cs = t, gk = grid_time(knext, nbB, T, nb), tg = 0.0;
+    double t = grid_time(0, nbB, h), cs = t, gk = grid_time(knext, nbB, h), tg = 0.0;
     double lu = 0.0, lt = 0.0;
     bool g = cs0.g, gneed = true;
     double ikon = rp->ikon[0], ikoff = rp->ikoff[0];
@@ -641,7 +644,7 @@ __global__ __launch_bounds__(SSA_WG) void burn_kernel(const DesignDev* __restric
           abnd = atx / bound;
         }
         if (gvar) gneed = true;  // the gene's rate changed: restart its clock
-        gk = grid_time(knext, nbB, T, nb);
+        gk = grid_time(knext, nbB, h);
       }
       if (tb == ws) break;
     }
@@ -714,11 +717,14 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                      uint32_t claim) {
   const double T = D->T;
   const int nb = D->nb, J = D->J, C = D->C;
+  const double h = T / (double)nb;  // hours between rate-bin edges
   const unsigned long long total = (unsigned long long)off[n];
   __shared__ LogEntry ltab[ABC_LOG_N];
   __shared__ LogEntry32 ltab32[ABC_LOG_N];
+  __shared__ double rk[ABC_RECIP_N];
   load_log_table(ltab);
   load_log32_table(ltab32);
+  for (int q = threadIdx.x; q < ABC_RECIP_N; q += blockDim.x) rk[q] = abc_recip_k[q];
   __syncthreads();
 
   uint32_t cw = 0;  // counter word of the current item: cell | model << 24
@@ -789,8 +795,8 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         // the sub-lane's unlabelled molecules at the window start
         WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, cw, pid, k0, k1, st};
         const double lu0 = rc->lu, ls = rc->lt - lu0;
-        n0 = poisson(src, lu0 * ishare, ltab);
-        n1 = poisson(src, ls > 0.0 ? ls * ishare : 0.0, ltab);
+        n0 = poisson(src, lu0 * ishare, ltab, rk);
+        n1 = poisson(src, ls > 0.0 ? ls * ishare : 0.0, ltab, rk);
       }
       const CellStart r0 = *rc;
       n2 = n3 = 0;
@@ -808,8 +814,8 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       k = r0.k;
       knext = k + step;
       j = (int)((uint64_t)k % (uint64_t)nb);
-      cs = grid_time(k - j, nbB, T, nb);
-      gk = grid_time(knext, nbB, T, nb);
+      cs = grid_time(k - j, nbB, h);
+      gk = grid_time(knext, nbB, h);
       t = age - D->pulse[cond] - D->chase[cond];
       gctr = r0.gctr;
       g = r0.g != 0;
@@ -899,7 +905,9 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       //   splicing (c1 <= rr < c2): molecule lemire(pz, n0 + n2) of the
       //     unspliced, unlabelled first;
       //   decay (rr >= c2): molecule lemire(pz, ntot) in species order.
-      const double rr = u01_32(py) * a0;
+      // u01_32(py) * a0 with the power-of-two scale on a0: the same single
+      // rounding of the same exact product, one instruction less
+      const double rr = ((double)py + 0.5) * (a0 * 0x1p-32);
       const bool tx = rr < c1;
       const bool spl = !tx && rr < c2;
       const bool dec = !tx && !spl;
@@ -908,10 +916,12 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       const uint32_t x = lemire(pz, spl ? nun : ntot);
       const bool ge0 = x >= n0, ge1 = x >= n0 + n1, ge2 = x >= n0 + n1 + n2;
       re += (!tx || acc) ? 1ull : 0ull;
-      n0 = n0 + (uint32_t)(acc && !lab) - (uint32_t)(!tx && !ge0);
-      n1 = n1 + (uint32_t)(spl && !ge0) - (uint32_t)(dec && ge0 && !ge1);
-      n2 = n2 + (uint32_t)(acc && lab) - (uint32_t)(spl && ge0) - (uint32_t)(dec && ge1 && !ge2);
-      n3 = n3 + (uint32_t)(spl && ge0) - (uint32_t)(dec && ge2);
+      // each species gains one, loses one or stays (the conditions exclude
+      // each other): one select chain and one add per species
+      n0 += (acc && !lab) ? 1u : ((!tx && !ge0) ? 0xFFFFFFFFu : 0u);
+      n1 += (spl && !ge0) ? 1u : ((dec && ge0 && !ge1) ? 0xFFFFFFFFu : 0u);
+      n2 += (acc && lab) ? 1u : (((spl && ge0) || (dec && ge1 && !ge2)) ? 0xFFFFFFFFu : 0u);
+      n3 += (spl && ge0) ? 1u : ((dec && ge2) ? 0xFFFFFFFFu : 0u);
       continue;
     }
     if (pend) {
@@ -962,7 +972,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
         bound = D->bound[j];
       }
       if (gvar) gneed = true;  // the gene's rate changed: restart its clock (memoryless)
-      gk = grid_time(knext, nbB, T, nb);
+      gk = grid_time(knext, nbB, h);
     }
     if (in_win && tb == we) in_win = false;
     tb = age;

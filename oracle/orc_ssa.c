@@ -235,7 +235,8 @@ static void draw(cell_stream* s, uint32_t w[4]) {
 }
 
 static inline double grid_t(int64_t k, int64_t nbB, double T, int nb) {
-  return ((double)(k - nbB) * T) / (double)nb;
+  const double h = T / (double)nb;   /* hours between rate-bin edges */
+  return (double)(k - nbB) * h;
 }
 
 /* Exponential waiting time with rate a from one word. */
@@ -306,7 +307,7 @@ static uint32_t poisson(cell_stream* s, wordbuf* b, double mu, int64_t* st) {
     uint32_t k = 0;
     while (u > F && k < 200u) {
       k++;
-      p = (p * mu) / (double)k;
+      p = (p * mu) * abc_recip_k[k];   /* RN(1/k), generated table */
       F = F + p;
     }
     return k;

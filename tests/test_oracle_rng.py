@@ -90,6 +90,14 @@ def test_exp1_32_table_is_the_mathematics():
         assert abs(L[j] - (-math.log(inv[j]))) <= 0.5 * np.spacing(np.float32(abs(L[j]) or 1e-30)), j
 
 
+def test_poisson_reciprocal_table_is_exact():
+    # the inversion's p_k = p_{k-1} * mu * RN(1/k): entry k must be the
+    # correctly rounded quotient 1/k
+    rk = _table32("abc_recip_k")
+    assert len(rk) == 201 and rk[0] == 0.0
+    assert all(rk[k] == 1.0 / k for k in range(1, 201))
+
+
 def test_exp1_32_accuracy(oracle):
     # against -log of the same single-precision u in double precision; words
     # cover every subinterval of the table in several binades, both ends of the
