@@ -91,7 +91,7 @@ def make_genes(eng, G, seed=0, first=3_000_000_000):
 def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
-        cal_recall=0.995, log=print):
+        cal_recall=0.995, direct_steps=0, log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -142,9 +142,11 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
         errs = []
         data_d = torch.tensor(data.ravel(order="F"), dtype=torch.float64, device=dev)
         se_d = torch.tensor(se.ravel(order="F"), dtype=torch.float64, device=dev)
+        ne = min(n_train, 40_000)   # the first particles of each model: 4e8 pairs at most
         for s in ts_raw:
-            e = torch.empty(n_train * G, dtype=torch.float64, device=dev)
-            _lib.check(L.abc_errors_dev(eng.ctx, n_train, KS, _ptr(s), G, _ptr(data_d), _ptr(se_d),
+            e = torch.empty(ne * G, dtype=torch.float64, device=dev)
+            blk = s.view(KS, n_train)[:, :ne].contiguous()
+            _lib.check(L.abc_errors_dev(eng.ctx, ne, KS, _ptr(blk), G, _ptr(data_d), _ptr(se_d),
                                         parts.ctypes.data_as(C.c_void_p), 2, 10.0, _ptr(e)),
                        eng.ctx, "errors")
             errs.append(e)
@@ -155,6 +157,11 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
         out["accept_rate_target"] = accept_rate
         log(f"[screened] epsilon {eps:.4g} accepts {accept_rate:g} of training pairs")
     out["eps"] = eps if mode == 0 else None
+    direct = None
+    if direct_steps and mode == 0:
+        direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, n_train,
+                               direct_steps, batch, lr, seed, dev, log)
+        out["direct_train_s"] = direct.train_s
 
     # ---- exhaustive plain path on the evaluation set (timed)
     def eval_first(m):
@@ -219,7 +226,8 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     if n_cal and mode == 0:
         out["by_rule"] = _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal,
                                            cal_recall, eval_sets, eval_steps, eval_first,
-                                           n_eval, n_all, plain_pairs, t_plain, seed, dev, log)
+                                           n_eval, n_all, plain_pairs, t_plain, seed, dev, log,
+                                           direct=direct)
         by_z = by_z + out["by_rule"]
     if ceiling_reps and mode == 0:
         out["ceiling"] = acceptance_ceiling(
@@ -233,12 +241,65 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     pick = max(best, key=lambda r: r["speedup_accepted_per_s"]) if best else pool[0]
     out.update(pick)
     out.update(genes=G, train_particles=n_train * len(MODELS), k=k)
+    if direct is not None:
+        direct.net.close()
     return out
+
+
+def _accepting_genes(eng, L, sup, n, data_d, se_d, parts, KS, G, eps, dev, chunk=50_000):
+    """Per particle (statistics sup: n x KS on the device), the number of genes
+    whose FP64 error is <= eps (torch int32, device)."""
+    import torch
+    out = torch.empty(n, dtype=torch.int32, device=dev)
+    err = torch.empty(min(n, chunk) * G, dtype=torch.float64, device=dev)
+    for a in range(0, n, chunk):
+        b = min(n, a + chunk)
+        # rows a..b-1 of the column-major n x KS matrix: a block with leading dimension n
+        blk = sup.view(KS, n)[:, a:b].contiguous()
+        _lib.check(L.abc_errors_dev(eng.ctx, b - a, KS, _ptr(blk), G, _ptr(data_d), _ptr(se_d),
+                                    parts.ctypes.data_as(C.c_void_p), len(parts), 10.0,
+                                    _ptr(err)), eng.ctx, "errors")
+        out[a:b] = (err[:(b - a) * G].view(G, b - a) <= eps).sum(0).to(torch.int32)
+    return out
+
+
+class _Direct:
+    """A network trained on the number of genes that accept each training
+    particle (one noisy draw per particle): with the Gaussian NLL its mean
+    output estimates E[accepting genes | parameters], the quantity the ideal
+    screen ranks particles by.  Trained for one gene set."""
+
+    def __init__(self, net, scale, train_s):
+        self.net, self.scale, self.train_s = net, scale, train_s
+
+    def score(self, x):
+        return self.net.predict(x)[:, 0] * self.scale
+
+
+def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, n_train, steps, batch,
+                  lr, seed, dev, log):
+    import torch
+    from .surrogate import Surrogate
+    data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
+                          device=dev)
+    se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
+    cnt = torch.cat([_accepting_genes(eng, L, s, n_train, data_d, se_d, parts, KS, G, eps, dev)
+                     for s in ts_raw]).to(torch.float32)
+    scale = float(max(cnt.std().item(), 1e-3))
+    net = Surrogate(1, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17, stream=sur.stream)
+    t0 = time.perf_counter()
+    losses = net.train(torch.cat(xs), (cnt / scale).view(-1, 1).contiguous(), steps=steps,
+                       batch=batch, lr=lr, seed=seed + 1)
+    train_s = time.perf_counter() - t0
+    log(f"[screened] direct network: {int((cnt > 0).sum().item())} of {len(cnt)} training "
+        f"particles accepted by some gene; loss {losses[:10].mean():.3f} -> "
+        f"{losses[-50:].mean():.3f} ({train_s:.1f}s)")
+    return _Direct(net, scale, train_s)
 
 
 def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,
                       eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev,
-                      log):
+                      log, direct=None):
     """The screen's two scores (expected accepting genes; largest standardised
     margin) on a calibration set of exactly simulated particles fix, per rule,
     the largest pass threshold whose recall of the calibration set's accepted
@@ -262,12 +323,20 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
                                     parts.ctypes.data_as(C.c_void_p), len(parts), 10.0,
                                     _ptr(err)), eng.ctx, "errors")
         acc.append((err.view(G, n_cal) <= eps).sum(0).cpu().numpy())
-        scores.append(sur.scores(eng, sur.features(m, th)).cpu().numpy())
+        x = sur.features(m, th)
+        sc = sur.scores(eng, x)
+        if direct is not None:
+            sc = torch.cat([sc, direct.score(x).view(-1, 1)], dim=1)
+        scores.append(sc.cpu().numpy())
     _end(eng, G, 16)
     acc, scores = np.concatenate(acc), np.vstack(scores)
     log(f"[screened] calibration: {len(acc)} particles, {int(acc.sum())} accepted pairs")
     res = []
-    for col, name in ((0, "expected_genes"), (1, "max_margin")):
+    rules = [(0, "expected_genes", lambda m, x: sur.scores(eng, x)[:, 0]),
+             (1, "max_margin", lambda m, x: sur.scores(eng, x)[:, 1])]
+    if direct is not None:
+        rules.append((2, "direct_expected_genes", lambda m, x: direct.score(x)))
+    for col, name, fn in rules:
         sc = scores[:, col]
         order = np.argsort(-sc, kind="stable")
         cum = np.cumsum(acc[order]) / max(acc.sum(), 1)
@@ -275,7 +344,7 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
         tau = float(sc[order[min(r, len(order) - 1)]])
         r = _screened(eng, sur, L, data, se, parts, KS, G, 0, eps, 0, np.full(G, eps), None, 0.0,
                       eval_sets, eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain,
-                      seed, dev, log, rule=(col, tau))
+                      seed, dev, log, rule=(fn, tau))
         r.update(rule=name, threshold=tau, cal_recall_target=cal_recall,
                  cal_pass_share=float((sc >= tau).mean()))
         res.append(r)
@@ -340,7 +409,8 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
               eval_steps, eval_first, n_eval, n_all, plain_pairs, t_plain, seed, dev, log,
               rule=None):
     """One screened run over the evaluation particles: pass flags from the z
-    rule, or (rule = (score column, threshold)) score >= threshold."""
+    rule, or (rule = (score function of (model, features), threshold)) score >=
+    threshold."""
     import torch
     nb, J = eng.nb, eng.J
     torch.cuda.synchronize()
@@ -360,7 +430,7 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
         if rule is None:
             pas, npass = sur.screen(eng, x, z=z, margin=margin)
         else:
-            pas = (sur.scores(eng, x, margin)[:, rule[0]] >= rule[1]).to(torch.uint8)
+            pas = (rule[0](m, x) >= rule[1]).to(torch.uint8)
             npass = int(pas.sum().item())
         t_screen += time.perf_counter() - ts
         n_pass_total += npass
@@ -390,7 +460,7 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
                recall=hit / max(len(plain_pairs), 1),
                screened_accepted_per_s=hit / t_scr,
                speedup_accepted_per_s=(hit / t_scr) / (len(plain_pairs) / t_plain))
-    log(f"[screened] {('z=%s' % z) if rule is None else ('rule %d >= %.4g' % rule)}: "
+    log(f"[screened] {('z=%s' % z) if rule is None else ('score >= %.4g' % rule[1])}: "
         f"pass {n_pass_total}/{n_all}, recall {res['recall']:.4f}, "
         f"time {t_scr:.2f}s vs plain {t_plain:.2f}s")
     return res

@@ -105,6 +105,8 @@ def parse():
     ap.add_argument("--sur-steps", type=int, default=6000, help="Adam steps (batch 8192)")
     ap.add_argument("--sur-cal", type=int, default=2000,
                     help="calibration particles per model that set the calibrated pass rules")
+    ap.add_argument("--sur-direct-steps", type=int, default=3000,
+                    help="Adam steps of the network trained on accepting-gene counts (0 = skip)")
     ap.add_argument("--ceiling-reps", type=int, default=16,
                     help="independent exact replicates per held-out particle for the bound on "
                          "what any parameter-only screen can gain (0 = skip)")
@@ -391,14 +393,14 @@ def surrogate_leg(eng, stream, a):
     sur = Surrogate(2 * eng.nb * eng.J, seed=0, stream=stream)
     res = S.run(eng, sur, G=a.genes, n_train=a.sur_train, n_eval=a.sur_eval, k=0,
                 accept_rate=4e-5, z=[4.0, 3.0, 2.5, 2.0], train_steps=a.sur_steps, batch=8192,
-                ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal,
+                ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal, direct_steps=a.sur_direct_steps,
                 log=lambda *x: print(*x, file=sys.stderr, flush=True))
     sur.close()
     keep = ("plain_accepted_per_s", "screened_accepted_per_s", "speedup_accepted_per_s",
             "recall", "ssa_fraction_avoided", "ssa_work_fraction_avoided", "plain_pairs",
             "eval_particles", "train_particles", "heldout_nll", "heldout_nll_of_mean_predictor",
             "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
-            "train_loss_last", "train_s", "genes", "z", "eps", "accept_rate_target", "by_z",
+            "train_loss_last", "train_s", "direct_train_s", "genes", "z", "eps", "accept_rate_target", "by_z",
             "ceiling", "by_rule", "rule", "threshold")
     return {k: res[k] for k in keep if k in res}
 

@@ -20,7 +20,7 @@ def screened():
     sur = Surrogate(90, max_batch=8192, seed=5)
     res = S.run(eng, sur, G=2000, n_train=12000, n_eval=2000, k=0, accept_rate=4e-5,
                 z=[3.0, 2.5], train_steps=2500, batch=8192, n_cal=2000, cal_recall=0.995,
-                log=lambda *a: print(*a, flush=True))
+                direct_steps=1500, log=lambda *a: print(*a, flush=True))
     print({k: v for k, v in res.items() if k not in ("by_z", "by_rule")}, flush=True)
     yield eng, sur, res
     sur.close()
@@ -37,7 +37,8 @@ def test_screened_abc_recall_and_savings(screened):
     # rules calibrated on other particles keep the held-out recall and skip
     # exact work: what the trained network gains over simulating every particle
     rules = res["by_rule"]
-    assert {r["rule"] for r in rules} == {"expected_genes", "max_margin"}
+    assert {r["rule"] for r in rules} == {"expected_genes", "max_margin",
+                                          "direct_expected_genes"}
     best = max(rules, key=lambda r: r["ssa_work_fraction_avoided"] if r["recall"] >= 0.98 else -1)
     assert best["recall"] >= 0.98, rules
     assert best["ssa_work_fraction_avoided"] > 0.2, rules
