@@ -388,7 +388,7 @@ int simulate_core(abc_ctx* c, const SimWs& w, uint64_t seed, int64_t n, int64_t 
   if (trio) HIPCHK(c, hipEventRecord(trio[1], c->stream));
   hipLaunchKernelGGL(ssa_kernel, dim3(blocks), dim3(SSA_WG), 0, c->stream, c->dd, w.rates,
                      w.order, w.off, n, (uint32_t)first, key0(seed), key1(seed), mtag_all, mtags,
-                     w.cellrec, w.coff, w.queue + 1, w.cu, w.cl, w.cb, steps, w.react, pidx_dev,
+                     w.cellrec, w.coff, w.queue + 1, w.cu, w.cl, w.cb, steps, pidx_dev,
                      (uint32_t)c->ssa_claim);
   HIPCHK(c, hipGetLastError());
   if (trio) HIPCHK(c, hipEventRecord(trio[2], c->stream));

@@ -110,7 +110,9 @@ ABC_DEV void load_log32_table(LogEntry32* t) {
 // as two floats.  These are the waiting times of the event loops, added to
 // their FP64 clocks; about 18 instructions against 38 for the FP64 log.
 ABC_DEV float exp1_32(uint32_t w, const LogEntry32* tab) {
-  const float u = ((float)w + 0.5f) * 0x1p-32f;
+  // one rounding of ((float)w + 1/2) 2^-32, the same bits as the sum then the
+  // exact power-of-two scaling (checked for every 32-bit w)
+  const float u = fmaf((float)w, 0x1p-32f, 0x1p-33f);
   const uint32_t iu = __float_as_uint(u);
   const uint32_t tmp = iu - ABC_LOG32_OFF;
   const int j = (int)((tmp >> 16) & (uint32_t)(ABC_LOG_N - 1));

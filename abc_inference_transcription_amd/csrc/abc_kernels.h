@@ -115,7 +115,7 @@ __global__ void ssa_kernel(const DesignDev* D, const PRates* R, const int32_t* o
                            const int32_t* coff,
                            unsigned long long* queue,
                            uint32_t* out_u, uint32_t* out_l, uint8_t* out_bin,
-                           unsigned long long* steps_out, unsigned long long* react_out,
+                           unsigned long long* steps_out,
                            const int64_t* pidx, uint32_t claim);
 // Moment model (the reference simulator, scripts/model.jl): one call's
 // configuration, passed by value as a kernel argument (about 1.9 kB).

@@ -425,13 +425,11 @@ __device__ __forceinline__ void burn_accum(double& lu, double& lt, bool g, doubl
 struct WordSrc {
   u4 w;
   int i;  // next word; 4 = empty
-  uint64_t& ctr;
+  uint64_t& ctr;  // Philox blocks drawn so far on this stream
   uint32_t stream, cell, pid, k0, k1;
-  unsigned long long& st;
   __device__ uint32_t next() {
     if (i == 4) {
       w = draw(ctr++, stream, cell, pid, k0, k1);
-      st++;
       i = 0;
     }
     const uint32_t v = i == 0 ? w.x : (i == 1 ? w.y : (i == 2 ? w.z : w.w));
@@ -712,7 +710,6 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
                                                      uint32_t* __restrict__ out_l,
                                                      uint8_t* __restrict__ out_bin,
                                                      unsigned long long* __restrict__ steps_out,
-                                                     unsigned long long* __restrict__ react_out,
                                                      const int64_t* __restrict__ pidx,
                                                      uint32_t claim) {
   const double T = D->T;
@@ -732,15 +729,22 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
   bool have = false;
   int p = 0, c = 0, j = 0, step = 1, scaling = 0, gvar = 0;
   uint32_t pid = 0, sstream = 1, n0 = 0, n1 = 0, n2 = 0, n3 = 0;
-  uint64_t gctr = 0, mctr = 0, thr_lam = 0;
+  // Philox blocks drawn on the gene stream (continued from the burn-in, which
+  // had drawn gctr0) and on the item's molecule stream: every step draws one
+  // block on exactly one of them, so the item's steps are mctr + gctr - gctr0
+  uint64_t gctr = 0, gctr0 = 0, mctr = 0;
+  // labelled when pw <= thr_m1 = floor(lambda 2^32) - 1 (lab_on: lambda > 0)
+  uint32_t thr_m1 = 0, shi = 0;  // shi: the molecule stream's counter word, stream << 16
+  bool lab_on = false;
   int64_t k = 0, knext = 0, nbB = 0;
-  double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, we = 0.0, cs = 0.0;
+  // tcs = T - (start of the current cycle): the size-scaling test compares
+  // with t + tcs = T + (time since the cycle start)
+  double t = 0.0, tb = 0.0, gk = 0.0, tg = 0.0, age = 0.0, we = 0.0, tcs = 0.0;
   double ikon = 0.0, ikoff = 0.0, atx = 0.0, gam = 0.0, bs = 0.0, bc = 1.0, bound = 0.0;
   double ishare = 1.0;  // 1 / (the cell's sub-lanes), a power of two
   bool g = false, in_win = false, div = false;
   int div_s = 0;
   uint32_t div_rem = 0, div_keep = 0;
-  unsigned long long st = 0, re = 0;
   // gene clock to (re)draw; pending molecule candidate (its residual Exp(1)
   // hazard and unused words)
   bool gneed = false, pend = false;
@@ -779,10 +783,9 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       cw = (uint32_t)c | (mtags ? mtags[p] : mtag_all);
       sstream = 1u + (uint32_t)sub;
+      shi = sstream << 16;
       const CellStart* rc = rec + ((size_t)p * (size_t)C + (size_t)c);
       ishare = ldexp(1.0, -(int)rc->msh);  // x / M == x * ishare exactly
-      st = 0;
-      re = 0;
       mctr = 0;
       // every loop-carried value is reset here, so none stays live across the
       // sampler below
@@ -793,7 +796,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       py = pz = pw = 0;
       {
         // the sub-lane's unlabelled molecules at the window start
-        WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, cw, pid, k0, k1, st};
+        WordSrc src{u4{0, 0, 0, 0}, 4, mctr, sstream, cw, pid, k0, k1};
         const double lu0 = rc->lu, ls = rc->lt - lu0;
         n0 = poisson(src, lu0 * ishare, ltab, rk);
         n1 = poisson(src, ls > 0.0 ? ls * ishare : 0.0, ltab, rk);
@@ -808,16 +811,18 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       step = rp->step;
       scaling = rp->scaling;
       gvar = rp->gene_varies;
-      thr_lam = rp->thr_lam;
+      lab_on = rp->thr_lam != 0ull;
+      thr_m1 = (uint32_t)(rp->thr_lam - 1ull);
       bs = rp->bs;
       nbB = (int64_t)rp->B * nb;
       k = r0.k;
       knext = k + step;
       j = (int)((uint64_t)k % (uint64_t)nb);
-      cs = grid_time(k - j, nbB, h);
+      tcs = T - grid_time(k - j, nbB, h);
       gk = grid_time(knext, nbB, h);
       t = age - D->pulse[cond] - D->chase[cond];
       gctr = r0.gctr;
+      gctr0 = gctr;
       g = r0.g != 0;
       tg = r0.tg;
       gneed = r0.gneed != 0;
@@ -848,14 +853,16 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     // candidate (stream 1 + sub).  A candidate pre-empted by the gene clock or
     // a boundary stays pending: its words are unused and its Exp(1) hazard,
     // less what the interval used up, is again Exp(1) (memoryless).
-    const int kind = div ? 1 : (gneed ? 2 : ((!pend && a0 > 0.0) ? 3 : 0));
-    const bool mol = kind == 1 || kind == 3;
-    const u4 w = draw(mol ? mctr : gctr, mol ? sstream : 0u, cw, pid, k0, k1);
-    if (kind != 0) {
-      st++;
-      if (mol) mctr++; else gctr++;
-    }
-    if (kind == 1) {
+    // (division coins, else the gene clock, else a new candidate, else no draw)
+    const bool kgen = !div && gneed;
+    const bool kmol = !div && !gneed && !pend && a0 > 0.0;
+    const bool mol = div || kmol;
+    const uint64_t ctr = mol ? mctr : gctr;
+    const u4 w = philox((uint32_t)ctr, ((uint32_t)(ctr >> 32) & 0xFFFFu) | (mol ? shi : 0u), cw,
+                        pid, k0, k1);
+    mctr += mol ? 1ull : 0ull;
+    gctr += kgen ? 1ull : 0ull;
+    if (div) {
       // binomial(count, 1/2) partition of species div_s, 128 coin flips per block
       const uint32_t m = div_rem < 128u ? div_rem : 128u;
       uint32_t cnt = __popc(m >= 32u ? w.x : (w.x & ((1u << m) - 1u)));
@@ -881,11 +888,11 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       continue;
     }
     const double E = (double)exp1_32(w.x, ltab32);
-    if (kind == 2) {
+    if (kgen) {
       tg = t + E * (g ? ikoff : ikon);
       gneed = false;
       if (!pend && a0 > 0.0) continue;  // a molecule candidate comes first
-    } else if (kind == 3) {
+    } else if (kmol) {
       erem = E;
       py = w.y;
       pz = w.z;
@@ -911,11 +918,10 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       const bool tx = rr < c1;
       const bool spl = !tx && rr < c2;
       const bool dec = !tx && !spl;
-      const bool acc = tx && (!scaling || u01_32(pz) * bound < T + (t - cs));
-      const bool lab = in_win && (uint64_t)pw < thr_lam;
+      const bool acc = tx && (!scaling || u01_32(pz) * bound < t + tcs);
+      const bool lab = in_win && lab_on && pw <= thr_m1;
       const uint32_t x = lemire(pz, spl ? nun : ntot);
       const bool ge0 = x >= n0, ge1 = x >= n0 + n1, ge2 = x >= n0 + n1 + n2;
-      re += (!tx || acc) ? 1ull : 0ull;
       // each species gains one, loses one or stays (the conditions exclude
       // each other): one select chain and one add per species
       n0 += (acc && !lab) ? 1u : ((!tx && !ge0) ? 0xFFFFFFFFu : 0u);
@@ -933,7 +939,6 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (tg < tb) {
       t = tg;
       g = !g;
-      re++;
       gneed = true;
       continue;
     }
@@ -943,8 +948,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       const size_t o = (size_t)p * (size_t)C + (size_t)c;
       atomicAdd(out_u + o, n0 + n1);
       atomicAdd(out_l + o, n2 + n3);
-      atomicAdd(steps_out + p, st);
-      atomicAdd(react_out + p, re);
+      atomicAdd(steps_out + p, (unsigned long long)(mctr + (gctr - gctr0)));
       have = false;
       continue;
     }
@@ -954,7 +958,7 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
       j += step;  // k mod nb (step is 1 or nb)
       if (j >= nb) j -= nb;
       if (j == 0) {
-        cs = tb;
+        tcs = T - tb;
         div_s = 0;
         while (div_s < 4 && (div_s == 0 ? n0 : (div_s == 1 ? n1 : (div_s == 2 ? n2 : n3))) == 0u) div_s++;
         if (div_s < 4) {

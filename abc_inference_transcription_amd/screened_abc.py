@@ -114,12 +114,13 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     log(f"[screened] genes ready ({time.perf_counter() - t0:.1f}s)")
 
     # ---- training set (streams 1e9 + ...), disjoint from the evaluation set
-    xs, ts_raw = [], []
+    xs, ts_raw, ts_steps = [], [], []
     t0 = time.perf_counter()
     for m in MODELS:
-        th, sup, _ = simulate_block(eng, m, n_train, 1_000_000_000 + m * n_train, seed)
+        th, sup, stp = simulate_block(eng, m, n_train, 1_000_000_000 + m * n_train, seed)
         xs.append(sur.features(m, th))
         ts_raw.append(sup)
+        ts_steps.append(stp)
     torch.cuda.synchronize()
     out["train_simulate_s"] = time.perf_counter() - t0
     stats_np = np.vstack([s.view(KS, n_train).t().cpu().numpy() for s in ts_raw])
@@ -159,8 +160,8 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     out["eps"] = eps if mode == 0 else None
     direct = None
     if direct_steps and mode == 0:
-        direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, n_train,
-                               direct_steps, batch, lr, seed, dev, log)
+        direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps,
+                               n_train, direct_steps, batch, lr, seed, dev, log)
         out["direct_train_s"] = direct.train_s
 
     # ---- exhaustive plain path on the evaluation set (timed)
@@ -265,19 +266,27 @@ def _accepting_genes(eng, L, sup, n, data_d, se_d, parts, KS, G, eps, dev, chunk
 
 class _Direct:
     """A network trained on the number of genes that accept each training
-    particle (one noisy draw per particle): with the Gaussian NLL its mean
-    output estimates E[accepting genes | parameters], the quantity the ideal
-    screen ranks particles by.  Trained for one gene set."""
+    particle (one noisy draw per particle) and on the log of its exact
+    simulation steps: with the Gaussian NLL its mean outputs estimate
+    E[accepting genes | parameters] and E[log steps | parameters].  Passing
+    particles in decreasing expected count keeps the most accepted pairs per
+    particle simulated; in decreasing count per expected step, the most per
+    unit of exact work.  Trained for one gene set."""
 
-    def __init__(self, net, scale, train_s):
-        self.net, self.scale, self.train_s = net, scale, train_s
+    def __init__(self, net, scale, lmu, lsd, train_s):
+        self.net, self.scale, self.lmu, self.lsd, self.train_s = net, scale, lmu, lsd, train_s
 
     def score(self, x):
         return self.net.predict(x)[:, 0] * self.scale
 
+    def per_work(self, x):
+        y = self.net.predict(x)
+        import torch
+        return (y[:, 0] * self.scale) * torch.exp(-(y[:, 1] * self.lsd + self.lmu))
 
-def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, n_train, steps, batch,
-                  lr, seed, dev, log):
+
+def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps, n_train, steps,
+                  batch, lr, seed, dev, log):
     import torch
     from .surrogate import Surrogate
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
@@ -286,15 +295,17 @@ def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, n_train,
     cnt = torch.cat([_accepting_genes(eng, L, s, n_train, data_d, se_d, parts, KS, G, eps, dev)
                      for s in ts_raw]).to(torch.float32)
     scale = float(max(cnt.std().item(), 1e-3))
-    net = Surrogate(1, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17, stream=sur.stream)
+    lw = torch.log(torch.cat(ts_steps).to(torch.float64).clamp(min=1.0)).to(torch.float32)
+    lmu, lsd = float(lw.mean().item()), float(max(lw.std().item(), 1e-3))
+    tgt = torch.stack([cnt / scale, (lw - lmu) / lsd], dim=1).contiguous()
+    net = Surrogate(2, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17, stream=sur.stream)
     t0 = time.perf_counter()
-    losses = net.train(torch.cat(xs), (cnt / scale).view(-1, 1).contiguous(), steps=steps,
-                       batch=batch, lr=lr, seed=seed + 1)
+    losses = net.train(torch.cat(xs), tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1)
     train_s = time.perf_counter() - t0
     log(f"[screened] direct network: {int((cnt > 0).sum().item())} of {len(cnt)} training "
         f"particles accepted by some gene; loss {losses[:10].mean():.3f} -> "
         f"{losses[-50:].mean():.3f} ({train_s:.1f}s)")
-    return _Direct(net, scale, train_s)
+    return _Direct(net, scale, lmu, lsd, train_s)
 
 
 def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,
@@ -326,7 +337,8 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
         x = sur.features(m, th)
         sc = sur.scores(eng, x)
         if direct is not None:
-            sc = torch.cat([sc, direct.score(x).view(-1, 1)], dim=1)
+            sc = torch.cat([sc, direct.score(x).view(-1, 1), direct.per_work(x).view(-1, 1)],
+                           dim=1)
         scores.append(sc.cpu().numpy())
     _end(eng, G, 16)
     acc, scores = np.concatenate(acc), np.vstack(scores)
@@ -336,6 +348,7 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
              (1, "max_margin", lambda m, x: sur.scores(eng, x)[:, 1])]
     if direct is not None:
         rules.append((2, "direct_expected_genes", lambda m, x: direct.score(x)))
+        rules.append((3, "direct_genes_per_step", lambda m, x: direct.per_work(x)))
     for col, name, fn in rules:
         sc = scores[:, col]
         order = np.argsort(-sc, kind="stable")
@@ -363,10 +376,14 @@ def acceptance_ceiling(eng, data, se, parts, eps, thetas, firsts, reps=16, seed0
     probability that the pair is accepted (FP64 error <= eps).  A particle
     carries s_i = sum_g p(i, g) expected accepted pairs; a screen that
     passes particles in decreasing s_i reaches recall R with the smallest
-    share of particles, f(R).  1 / f(R) bounds the gain in accepted pairs per
-    unit of exact simulation at recall R; the screen's own cost only lowers it.
-    Returns f(R) and 1 / f(R) per recall, the share of particles with any
-    accepted replicate and the mean pairs per particle."""
+    share of particles, f(R).  Particles differ in cost (w_i, their mean exact
+    simulation steps), and the time of the screened path follows the steps
+    simulated: passing particles in decreasing s_i / w_i reaches recall R with
+    the least work, a share g(R) of the exhaustive path's steps.  1 / g(R)
+    bounds the gain in accepted pairs per second at recall R (1 / f(R) when
+    every particle costs the same); the screen's own cost only lowers it.
+    Returns f(R), g(R) and their inverses per recall, the share of particles
+    with any accepted replicate and the mean pairs per particle."""
     import torch
     dev = torch.device("cuda:0")
     L = eng.L
@@ -374,7 +391,7 @@ def acceptance_ceiling(eng, data, se, parts, eps, thetas, firsts, reps=16, seed0
     KS = data.shape[1]
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64, device=dev)
     se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
-    counts = []
+    counts, works = [], []
     for m in sorted(thetas):
         th = thetas[m]
         n = th.numel() // eng.n_params(m)
@@ -382,26 +399,36 @@ def acceptance_ceiling(eng, data, se, parts, eps, thetas, firsts, reps=16, seed0
         sup = torch.empty(KS * n, dtype=torch.float64, device=dev)
         err = torch.empty(n * G, dtype=torch.float64, device=dev)
         cnt = torch.zeros((G, n), dtype=torch.int32, device=dev)
+        stp = torch.empty(n, dtype=torch.int64, device=dev)
+        wsum = torch.zeros(n, dtype=torch.float64, device=dev)
         for r in range(reps):
             _lib.check(L.abc_simulate_dev(eng.ctx, m, seed0 + r, firsts[m], n, _ptr(th), _ptr(st),
-                                          None, _ptr(sup), None), eng.ctx, "simulate")
+                                          None, _ptr(sup), _ptr(stp)), eng.ctx, "simulate")
+            wsum += stp.to(torch.float64)
             _lib.check(L.abc_errors_dev(eng.ctx, n, KS, _ptr(sup), G, _ptr(data_d), _ptr(se_d),
                                         parts.ctypes.data_as(C.c_void_p), len(parts), 10.0,
                                         _ptr(err)), eng.ctx, "errors")
             cnt += (err.view(G, n) <= eps).to(torch.int32)
         counts.append(cnt)
+        works.append(wsum / reps)
         log(f"[ceiling] model {m}: {reps} replicates of {n} particles")
     p = torch.cat(counts, dim=1).to(torch.float64) / reps          # G x n_all
     s_i = p.sum(0).cpu().numpy()
-    order = np.sort(s_i)[::-1]
-    cum = np.cumsum(order) / max(order.sum(), 1e-300)
+    w_i = torch.cat(works).cpu().numpy()
     out = {"reps": reps, "particles": int(len(s_i)),
            "pairs_per_particle": float(s_i.mean()),
            "share_with_any_acceptance": float((s_i > 0).mean())}
-    for R in recalls:
-        f = (int(np.searchsorted(cum, R)) + 1) / len(s_i)
-        out[f"share_needed_{R}"] = f
-        out[f"gain_bound_{R}"] = 1.0 / f
+    # by expected pairs (fewest particles), and by expected pairs per exact
+    # step (least exact work: the greedy order of the fractional knapsack)
+    for key, order in (("", np.argsort(-s_i, kind="stable")),
+                       ("work_", np.argsort(-(s_i / np.maximum(w_i, 1.0)), kind="stable"))):
+        cum = np.cumsum(s_i[order]) / max(s_i.sum(), 1e-300)
+        wcum = np.cumsum(w_i[order]) / max(w_i.sum(), 1e-300)
+        for R in recalls:
+            r = int(np.searchsorted(cum, R))
+            f = (r + 1) / len(s_i) if not key else float(wcum[r])
+            out[f"{key}share_needed_{R}"] = f
+            out[f"{key}gain_bound_{R}"] = 1.0 / f
     return out
 
 

@@ -107,8 +107,9 @@ static double ratio_of(int64_t cnt, uint64_t Su, uint64_t Sl) {
 /* get_cond_specific_correlations for condition j with weights w[nc] */
 static void corr_stats(const corr_acc* a, int j, int nc, const double* w, double* mean_corr,
                        double* corr_mean) {
-  double m1[ORC_DATA_MAX_CLUSTERS], m2[ORC_DATA_MAX_CLUSTERS], v1[ORC_DATA_MAX_CLUSTERS],
-      v2[ORC_DATA_MAX_CLUSTERS], c12[ORC_DATA_MAX_CLUSTERS];
+  double m1[ORC_DATA_MAX_CLUSTERS] = {0}, m2[ORC_DATA_MAX_CLUSTERS] = {0},
+         v1[ORC_DATA_MAX_CLUSTERS] = {0}, v2[ORC_DATA_MAX_CLUSTERS] = {0},
+         c12[ORC_DATA_MAX_CLUSTERS] = {0};
   int any_cov = 0;
   for (int c = 0; c < nc; ++c) {
     const int64_t n = a->cnt[j][c];

@@ -569,9 +569,10 @@ static int sim_window(const orc_design* d, const prates* r, const uint32_t key[3
       if (rr < c1) {
         int acc = 1;
         if (r->scaling) {
-          /* accept with probability (T + a) / bound_j, a = time since cycle start */
+          /* accept with probability (T + a) / bound_j, a = time since cycle start,
+           * formed as t + (T - cycle start) */
           double cs = grid_t(k - j, nbB, T, nb);
-          acc = orc_u01_32(pw[1]) * r->bound[j] < T + (t - cs);
+          acc = orc_u01_32(pw[1]) * r->bound[j] < t + (T - cs);
         }
         if (acc) {
           re++;
@@ -766,8 +767,8 @@ void orc_cell_stats(const orc_design* d, const uint32_t* u, const uint32_t* l,
   /* ratios (data_summary_statistics.jl:61-77) and correlations (110-147) */
   for (int j = 0; j < J; ++j) {
     uint64_t nn = 0, su = 0, sl = 0;
-    double m1[ORC_MAX_BINS], m2[ORC_MAX_BINS], v1[ORC_MAX_BINS], v2[ORC_MAX_BINS],
-        c12[ORC_MAX_BINS];
+    double m1[ORC_MAX_BINS] = {0}, m2[ORC_MAX_BINS] = {0}, v1[ORC_MAX_BINS] = {0},
+           v2[ORC_MAX_BINS] = {0}, c12[ORC_MAX_BINS] = {0};
     int anynz = 0;
     for (int b = 0; b < nb; ++b) {
       int idx = b + nb * j;
