@@ -83,6 +83,7 @@ def _declare(L):
         "abc_simulate_dev": (I32, [V, I32, U64, I64, I64, V, V, V, V, V]),
         "abc_errors_dev": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V]),
         "abc_simulate_multi_dev": (I32, [V, I32, V, U64, V, V, V, V, V, V, V]),
+        "abc_simulate_multi_idx_dev": (I32, [V, I32, V, U64, V, V, V, V, V, V, V]),
         "abc_last_timing": (I32, [V, V]),
         "abc_ssa_time": (I32, [V, I32, V, V, V]),
         "abc_set_ssa_blocks": (I32, [V, I32]),
@@ -130,6 +131,8 @@ def _declare_optional(L):
         "abc_mlp_destroy": (I32, [V]),
         "abc_mlp_last_error": (C.c_char_p, [V]),
         "abc_mlp_n_params": (I64, [V]),
+        "abc_mlp_set_count_outputs": (I32, [V, I32]),
+        "abc_mlp_set_tile_rows": (I32, [V, I32]),
         "abc_mlp_n_stats": (I32, [V]),
         "abc_mlp_set_stream": (I32, [V, V]),
         "abc_mlp_synchronize": (I32, [V]),

@@ -700,20 +700,28 @@ int32_t abc_simulate_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first
                       steps_dev);
 }
 
-int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* models, uint64_t seed,
-                               const int64_t* first, const int64_t* n,
-                               const double* const* theta_dev, double* const* stats_dev,
-                               double* const* moments_dev, double* const* superset_dev,
-                               int64_t* const* steps_dev) {
-  if (!c) return ABC_E_ARG;
-  if (n_models < 1 || n_models > 16 || !models || !first || !n || !theta_dev || !stats_dev)
-    return fail(c, ABC_E_ARG, "need 1..16 models with first, n, theta and stats");
+}  // extern "C"
+
+namespace {
+
+// The particles of several models through one burn-in and one SSA launch:
+// model m's particles are first[m] + i, or pidx_dev[m][i] when pidx_dev is
+// given (then first is not read).
+int simulate_multi(abc_ctx* c, int32_t n_models, const int32_t* models, uint64_t seed,
+                   const int64_t* first, const int64_t* const* pidx_dev, const int64_t* n,
+                   const double* const* theta_dev, double* const* stats_dev,
+                   double* const* moments_dev, double* const* superset_dev,
+                   int64_t* const* steps_dev) {
+  if (n_models < 1 || n_models > 16 || !models || !(first || pidx_dev) || !n || !theta_dev ||
+      !stats_dev)
+    return fail(c, ABC_E_ARG, "need 1..16 models with first (or pidx), n, theta and stats");
   int rc;
   int64_t total = 0, a[17];
   for (int m = 0; m < n_models; ++m) {
-    if ((rc = check_range(c, models[m], first[m], n[m]))) return rc;
-    if (n[m] > 0 && (!theta_dev[m] || !stats_dev[m]))
-      return fail(c, ABC_E_ARG, "theta and stats are required for every model with particles");
+    if ((rc = check_range(c, models[m], pidx_dev ? 0 : first[m], n[m]))) return rc;
+    if (n[m] > 0 && (!theta_dev[m] || !stats_dev[m] || (pidx_dev && !pidx_dev[m])))
+      return fail(c, ABC_E_ARG,
+                  "theta and stats (and pidx) are required for every model with particles");
     a[m] = total;
     total += n[m];
   }
@@ -729,8 +737,8 @@ int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* mode
     if (n[m] == 0) continue;
     launch_rates(c, w, models[m], n[m], theta_dev[m], a[m]);
     hipLaunchKernelGGL(fill_ids_kernel, dim3(grid1(n[m], 256)), dim3(256), 0, c->stream, n[m],
-                       first[m], mtag(models[m]), P<int64_t>(c->mpidx) + a[m],
-                       P<uint32_t>(c->mk1) + a[m]);
+                       pidx_dev ? (int64_t)0 : first[m], pidx_dev ? pidx_dev[m] : nullptr,
+                       mtag(models[m]), P<int64_t>(c->mpidx) + a[m], P<uint32_t>(c->mk1) + a[m]);
   }
   HIPCHK(c, hipGetLastError());
   if ((rc = launch_order(c, w, total))) return rc;
@@ -754,6 +762,33 @@ int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* mode
   HIPCHK(c, record(c, c->ev[3]));
   c->timed[0] = c->timed[1] = c->timed[2] = !c->capturing;
   return ABC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t abc_simulate_multi_dev(abc_ctx* c, int32_t n_models, const int32_t* models, uint64_t seed,
+                               const int64_t* first, const int64_t* n,
+                               const double* const* theta_dev, double* const* stats_dev,
+                               double* const* moments_dev, double* const* superset_dev,
+                               int64_t* const* steps_dev) {
+  if (!c) return ABC_E_ARG;
+  if (!first) return fail(c, ABC_E_ARG, "first is required");
+  return simulate_multi(c, n_models, models, seed, first, nullptr, n, theta_dev, stats_dev,
+                        moments_dev, superset_dev, steps_dev);
+}
+
+int32_t abc_simulate_multi_idx_dev(abc_ctx* c, int32_t n_models, const int32_t* models,
+                                   uint64_t seed, const int64_t* n,
+                                   const int64_t* const* pidx_dev,
+                                   const double* const* theta_dev, double* const* stats_dev,
+                                   double* const* moments_dev, double* const* superset_dev,
+                                   int64_t* const* steps_dev) {
+  if (!c) return ABC_E_ARG;
+  if (!pidx_dev) return fail(c, ABC_E_ARG, "pidx is required");
+  return simulate_multi(c, n_models, models, seed, nullptr, pidx_dev, n, theta_dev, stats_dev,
+                        moments_dev, superset_dev, steps_dev);
 }
 
 int32_t abc_simulate_noalloc_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first,
@@ -1963,7 +1998,7 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
   }
   // then the work items a lane claims per queue atomic, on that grid
   int bestc = 1;
-  for (int cl : {2, 4}) {
+  for (int cl : {2, 4, 8}) {
     if (rc != ABC_OK) break;
     c->ssa_claim = cl;
     rc = run(bestb, &ms);

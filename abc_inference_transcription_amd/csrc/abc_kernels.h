@@ -96,8 +96,8 @@ __global__ void items_scan_kernel(const int64_t* pitems, const int32_t* order, i
                                   int64_t* off, int64_t* bsum);
 __global__ void bsum_scan_kernel(int64_t* bsum, int64_t nblk);
 __global__ void add_bsum_kernel(int64_t* off, int64_t n, const int64_t* bsum);
-__global__ void fill_ids_kernel(int64_t n, int64_t first, uint32_t mtag, int64_t* pidx,
-                                uint32_t* mtags);
+__global__ void fill_ids_kernel(int64_t n, int64_t first, const int64_t* src, uint32_t mtag,
+                                int64_t* pidx, uint32_t* mtags);
 __global__ void copy_u64_kernel(const unsigned long long* src, unsigned long long* dst,
                                 int64_t n);
 __global__ void cell_items_kernel(int64_t n, int C, const CellStart* rec, int32_t* coff,

@@ -248,14 +248,15 @@ __global__ __launch_bounds__(256) void bucket_scatter_kernel(const float* __rest
 }
 
 // Identity of the particles of one model in a multi-model batch: global
-// particle index first + i and the model's tag (model << 24) for the cell
-// word of its counters.
-__global__ __launch_bounds__(256) void fill_ids_kernel(int64_t n, int64_t first, uint32_t mtag,
-                                                       int64_t* __restrict__ pidx,
+// particle index src[i] (first + i without src) and the model's tag
+// (model << 24) for the cell word of its counters.
+__global__ __launch_bounds__(256) void fill_ids_kernel(int64_t n, int64_t first,
+                                                       const int64_t* __restrict__ src,
+                                                       uint32_t mtag, int64_t* __restrict__ pidx,
                                                        uint32_t* __restrict__ mtags) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) {
-    pidx[i] = first + i;
+    pidx[i] = src ? src[i] : first + i;
     mtags[i] = mtag;
   }
 }
@@ -755,31 +756,62 @@ __global__ __launch_bounds__(SSA_WG) __attribute__((amdgpu_waves_per_eu(4))) voi
   // queue advances by whole claims, so a claim is used up exactly when qnext
   // reaches a multiple of it.
   unsigned long long qnext = 0;
+  // where the lane's last item lies, so the next item of the same claim (the
+  // next queue position) is found by one step instead of two searches: queue
+  // rank of its particle, item within the particle, the particle's items,
+  // the end of the cell's sub-lanes (item offsets within the particle) and
+  // the sub-lane
+  int32_t wr = 0, wrem = 0, witems = 0, wcend = 0, wsub = 0;
 
   for (;;) {
     if (!have) {
-      if ((qnext & (unsigned long long)(claim - 1u)) == 0ull)
-        qnext = atomicAdd(queue, (unsigned long long)claim);
+      const bool fresh = (qnext & (unsigned long long)(claim - 1u)) == 0ull;
+      if (fresh) qnext = atomicAdd(queue, (unsigned long long)claim);
       const unsigned long long q = qnext++;
       if (q >= total) break;
-      // particle rank: largest r with off[r] <= q
-      int64_t lo = 0, hi = n;
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if ((unsigned long long)off[mid] <= q) lo = mid; else hi = mid;
+      if (fresh) {
+        // particle rank: largest r with off[r] <= q
+        int64_t lo = 0, hi = n;
+        while (hi - lo > 1) {
+          const int64_t mid = (lo + hi) >> 1;
+          if ((unsigned long long)off[mid] <= q) lo = mid; else hi = mid;
+        }
+        wr = (int32_t)lo;
+        p = order[lo];
+        wrem = (int32_t)(q - (unsigned long long)off[lo]);
+        witems = (int32_t)(off[lo + 1] - off[lo]);
+        // cell: largest c with coff[c] <= wrem among the particle's C cells
+        const int32_t* co = coff + (size_t)p * (size_t)(C + 1);
+        int clo = 0, chi = C;
+        while (chi - clo > 1) {
+          const int mid = (clo + chi) >> 1;
+          if (co[mid] <= wrem) clo = mid; else chi = mid;
+        }
+        c = clo;
+        wsub = wrem - co[c];
+        wcend = co[c + 1];
+      } else if (++wrem < witems) {
+        // the next item of the same particle: the next sub-lane, or the next
+        // cell's first (every cell has at least one sub-lane)
+        if (wrem < wcend) {
+          wsub++;
+        } else {
+          c++;
+          wsub = 0;
+          wcend = coff[(size_t)p * (size_t)(C + 1) + (size_t)(c + 1)];
+        }
+      } else {
+        // the first item of the next particle in queue order
+        wr++;
+        p = order[wr];
+        wrem = 0;
+        witems = (int32_t)(off[wr + 1] - off[wr]);
+        c = 0;
+        wsub = 0;
+        wcend = coff[(size_t)p * (size_t)(C + 1) + 1];
       }
-      p = order[lo];
       const PRates* rp = R + p;
-      // cell: largest c with coff[c] <= rem among the particle's C cells
-      const int32_t rem = (int32_t)(q - (unsigned long long)off[lo]);
-      const int32_t* co = coff + (size_t)p * (size_t)(C + 1);
-      int clo = 0, chi = C;
-      while (chi - clo > 1) {
-        const int mid = (clo + chi) >> 1;
-        if (co[mid] <= rem) clo = mid; else chi = mid;
-      }
-      c = clo;
-      const int sub = rem - co[c];
+      const int sub = wsub;
       pid = pidx ? (uint32_t)pidx[p] : first + (uint32_t)p;
       cw = (uint32_t)c | (mtags ? mtags[p] : mtag_all);
       sstream = 1u + (uint32_t)sub;

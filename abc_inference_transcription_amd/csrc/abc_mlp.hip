@@ -28,15 +28,16 @@ namespace {
 
 enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2 };
 
-constexpr int BM = 128, BK = 32, LDSK = BK + 8;  // +8 bf16 pad against bank conflicts
+constexpr int BK = 32, LDSK = BK + 8;  // +8 bf16 pad against bank conflicts
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
-// 256 threads, workgroup tile 128 x BN, wave tile 64 x BN/2 (2 x 2 waves).
-// blockIdx.z selects a K chunk [z*K, (z+1)*K) of rows with leading dimension
-// ld (split-K: EPI_F32 then writes its partial tile to Cf + z*M*N).  The next
-// K tile is prefetched into registers while the MFMAs run on the current one.
-template <int BN, int EPI>
+// 256 threads, workgroup tile BM x BN (BM 128 or 64), wave tile BM/2 x BN/2
+// (2 x 2 waves).  blockIdx.z selects a K chunk [z*K, (z+1)*K) of rows with
+// leading dimension ld (split-K: EPI_F32 then writes its partial tile to
+// Cf + z*M*N).  The next K tile is prefetched into registers while the MFMAs
+// run on the current one.
+template <int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A,
                                                       const bf16* __restrict__ B, int M, int N,
                                                       int K, int ld, const float* __restrict__ bias,
@@ -46,6 +47,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
                                                       const float* __restrict__ Zprev,
                                                       float* __restrict__ Zout) {
   constexpr int WN = BN / 2, NCB = WN / 16, NBQ = BN / 64;
+  constexpr int WM = BM / 2, NRB = WM / 16, NAQ = BM / 64;
   __shared__ bf16 As[BM][LDSK];
   __shared__ bf16 Bs[BN][LDSK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -53,16 +55,16 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   const int kbeg = blockIdx.z * K, kend = kbeg + K;
   if (EPI == EPI_F32) Cf += (size_t)blockIdx.z * M * N;
-  f32x4 acc[4][NCB];
+  f32x4 acc[NRB][NCB];
 #pragma unroll
-  for (int a = 0; a < 4; ++a)
+  for (int a = 0; a < NRB; ++a)
 #pragma unroll
     for (int b = 0; b < NCB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   // this thread's slice of a K tile: rows r = c >> 2, 8 columns at (c & 3) * 8
-  bf16x8 ra[2], rb[NBQ];
+  bf16x8 ra[NAQ], rb[NBQ];
   auto load = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < NAQ; ++q) {
       const int c = tid + q * 256;
       ra[q] = *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + (c >> 2)) * ld + k0 + (c & 3) * 8);
     }
@@ -75,7 +77,7 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
   load(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < NAQ; ++q) {
       const int c = tid + q * 256;
       *reinterpret_cast<bf16x8*>(&As[c >> 2][(c & 3) * 8]) = ra[q];
     }
@@ -86,27 +88,27 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
     }
     __syncthreads();
     if (k0 + BK < kend) load(k0 + BK);  // in flight while the MFMAs below run
-    bf16x8 af[4], bfr[NCB];
+    bf16x8 af[NRB], bfr[NCB];
 #pragma unroll
-    for (int r4 = 0; r4 < 4; ++r4)
-      af[r4] = *reinterpret_cast<const bf16x8*>(&As[wm * 64 + r4 * 16 + (lane & 15)][(lane >> 4) * 8]);
+    for (int r4 = 0; r4 < NRB; ++r4)
+      af[r4] = *reinterpret_cast<const bf16x8*>(&As[wm * WM + r4 * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
       bfr[cb] = *reinterpret_cast<const bf16x8*>(&Bs[wn * WN + cb * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
-    for (int r4 = 0; r4 < 4; ++r4)
+    for (int r4 = 0; r4 < NRB; ++r4)
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb)
         acc[r4][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r4], bfr[cb], acc[r4][cb], 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int r4 = 0; r4 < 4; ++r4)
+  for (int r4 = 0; r4 < NRB; ++r4)
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm * 64 + r4 * 16 + (lane >> 4) * 4 + q;
+        const int m = m0 + wm * WM + r4 * 16 + (lane >> 4) * 4 + q;
         const int n = n0 + wn * WN + cb * 16 + (lane & 15);
         float v = acc[r4][cb][q];
         const size_t o = (size_t)m * N + n;
@@ -151,9 +153,12 @@ __global__ void cast_input_kernel(const float* __restrict__ x, int Bn, int Bpad,
 }
 
 // Gaussian negative log-likelihood per element, mean over B*ns.
-// O row: [mu (ns) | logvar (ns) | pad]; dO and dO^T in bf16.
+// O row: [mu (ns) | logvar (ns) | pad]; dO and dO^T in bf16.  The first n_cnt
+// statistics are counts: their mu slot is a log-rate o (clipped to [-30, 30],
+// no gradient outside) with the Poisson negative log-likelihood exp(o) - t o
+// (log t! dropped), and their logvar slot is unused.
 __global__ void nll_kernel(const float* __restrict__ O, const float* __restrict__ T, int Bn,
-                           int Bpad, int ns, int out_pad, bf16* __restrict__ dO,
+                           int Bpad, int ns, int n_cnt, int out_pad, bf16* __restrict__ dO,
                            bf16* __restrict__ dOT, float* __restrict__ loss) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.0f;
@@ -169,7 +174,14 @@ __global__ void nll_kernel(const float* __restrict__ O, const float* __restrict_
       const float r = T[(size_t)b * ns + s] - mu;
       const float inv = __expf(-lv);
       const float scale = 1.0f / ((float)Bn * (float)ns);
-      if (k < ns) {
+      if (s < n_cnt) {
+        if (k < ns) {
+          const float o = fminf(fmaxf(mu, -30.0f), 30.0f);
+          const float lam = __expf(o), y = T[(size_t)b * ns + s];
+          g = (mu < -30.0f || mu > 30.0f) ? 0.0f : (lam - y) * scale;
+          l = (lam - y * o) * scale;
+        }
+      } else if (k < ns) {
         g = -r * inv * scale;
         l = 0.5f * (lv + r * r * inv) * scale;
       } else {
@@ -443,6 +455,8 @@ struct abc_mlp {
   hipStream_t stream = nullptr;
   bool own_stream = true;
   int n_in = 0, in_pad = 0, width = 0, depth = 0, n_out = 0, out_pad = 0, ns = 0, max_batch = 0;
+  int n_cnt = 0;  // leading count statistics (Poisson loss on a log-rate output)
+  int tile_rows = 0;  // GEMM workgroup rows: 0 = by shape, else 64 or 128
   int64_t n_par = 0, n_bf = 0;
   std::vector<LayerDesc> layers;
   std::vector<int64_t> bf_off;
@@ -479,10 +493,17 @@ int mfail(abc_mlp* p, int code, const std::string& s) {
 // Workgroups of a gemm grid; splitting K fills the machine when M x N is small
 // (the weight gradients: M, N = layer widths, K = batch).
 int splitk_for(int M, int N, int K) {
-  const int tiles = (M / BM) * (N / (N % 128 == 0 ? 128 : 64));
+  const int tiles = (M / 128) * (N / (N % 128 == 0 ? 128 : 64));
   int S = 1;
   while (tiles * S < 256 && K % (2 * S * BK) == 0 && K / (2 * S) >= 4 * BK) S *= 2;
   return S;
+}
+
+// Workgroup tile rows: 64 when 128-row tiles would leave fewer than two
+// workgroups per CU (one wave per SIMD cannot hide the load and LDS latency).
+int tile_rows(int M, int N, int S) {
+  const int tiles = (M / 128) * (N / (N % 128 == 0 ? 128 : 64)) * S;
+  return (tiles < 512 && M % 64 == 0) ? 64 : 128;
 }
 
 template <int EPI>
@@ -495,11 +516,18 @@ void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const f
     if ((size_t)S * M * N > p->splitk_cap) S = 1;
     if (S > 1) out = p->splitk_ws;
   }
-  if (N % 128 == 0) {
-    hipLaunchKernelGGL((gemm_nt_kernel<128, EPI>), dim3(M / BM, N / 128, S), dim3(256), 0,
+  const int bm = p->tile_rows ? p->tile_rows : tile_rows(M, N, S);
+  if (N % 128 == 0 && bm == 128) {
+    hipLaunchKernelGGL((gemm_nt_kernel<128, 128, EPI>), dim3(M / 128, N / 128, S), dim3(256), 0,
+                       p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
+  } else if (N % 128 == 0) {
+    hipLaunchKernelGGL((gemm_nt_kernel<64, 128, EPI>), dim3(M / 64, N / 128, S), dim3(256), 0,
+                       p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
+  } else if (bm == 128) {
+    hipLaunchKernelGGL((gemm_nt_kernel<128, 64, EPI>), dim3(M / 128, N / 64, S), dim3(256), 0,
                        p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
   } else {
-    hipLaunchKernelGGL((gemm_nt_kernel<64, EPI>), dim3(M / BM, N / 64, S), dim3(256), 0,
+    hipLaunchKernelGGL((gemm_nt_kernel<64, 64, EPI>), dim3(M / 64, N / 64, S), dim3(256), 0,
                        p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
   }
   if (S > 1)
@@ -532,8 +560,8 @@ int grad(abc_mlp* p, int64_t Bn, const float* x_dev, const float* t_dev) {
   const int Bpad = (int)((Bn + 127) / 128 * 128);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, p->stream, p->loss, (int64_t)1);
   hipLaunchKernelGGL(nll_kernel, dim3(g1((int64_t)Bpad * p->out_pad, 256)), dim3(256), 0,
-                     p->stream, p->O, t_dev, (int)Bn, Bpad, p->ns, p->out_pad, p->dA, p->dAT,
-                     p->loss);
+                     p->stream, p->O, t_dev, (int)Bn, Bpad, p->ns, p->n_cnt, p->out_pad, p->dA,
+                     p->dAT, p->loss);
   // dA / dAT hold dZ of the current layer (output layer first)
   bf16 *cur = p->dA, *curT = p->dAT, *nxt = p->dB, *nxtT = p->dBT;
   for (int l = p->depth; l >= 0; --l) {
@@ -623,7 +651,7 @@ int32_t abc_mlp_create(abc_mlp** out, int32_t device, int32_t n_in, int32_t widt
   A(&p->dB, 2 * Bp * dmax); A(&p->dBT, 2 * Bp * dmax);
   A(&p->loss, 64);
   // split-K partials: at most 256 tiles of 128 x 128 floats in flight
-  p->splitk_cap = (size_t)256 * BM * 128;
+  p->splitk_cap = (size_t)256 * 128 * 128;
   A(&p->splitk_ws, 4 * p->splitk_cap);
   if (!ok) { abc_mlp_destroy(p); return ABC_E_OOM; }
   std::vector<float> zeros(off, 0.0f);
@@ -660,6 +688,18 @@ int32_t abc_mlp_destroy(abc_mlp* p) {
 const char* abc_mlp_last_error(const abc_mlp* p) { return p ? p->msg.c_str() : "null model"; }
 
 int64_t abc_mlp_n_params(const abc_mlp* p) { return p ? p->n_par : ABC_E_ARG; }
+
+int32_t abc_mlp_set_tile_rows(abc_mlp* p, int32_t rows) {
+  if (!p || (rows != 0 && rows != 64 && rows != 128)) return ABC_E_ARG;
+  p->tile_rows = rows;
+  return ABC_OK;
+}
+
+int32_t abc_mlp_set_count_outputs(abc_mlp* p, int32_t n_counts) {
+  if (!p || n_counts < 0 || n_counts > p->ns) return ABC_E_ARG;
+  p->n_cnt = n_counts;
+  return ABC_OK;
+}
 
 int32_t abc_mlp_set_stream(abc_mlp* p, void* s) {
   if (!p) return ABC_E_ARG;

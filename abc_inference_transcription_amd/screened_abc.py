@@ -17,9 +17,16 @@
    from below (mean - z * sd of the error under the predicted moments), and
    only particles that can pass some gene are simulated exactly and accepted
    in FP64 -> A_screened.
+6. Calibrated rules (n_cal > 0): scores of particles (expected accepting
+   genes and largest margin from the statistics surrogate; with
+   direct_steps > 0, a second network's expected accepting genes and expected
+   genes per unit of exact cost) with pass thresholds set on a separate,
+   exactly simulated calibration set at the target recall, then applied to
+   the held-out particles like the z rule.
 Reports recall |A_plain & A_screened| / |A_plain|, the fraction of exact
 simulation work avoided and accepted pairs per second of both paths, per z
-(``by_z``) and for the fastest z that keeps recall >= 0.99.
+(``by_z``), per calibrated rule (``by_rule``) and for the fastest setting
+that keeps recall >= 0.99.
 All compute runs in libabc.so; torch only holds device buffers.
 """
 from __future__ import annotations
@@ -54,6 +61,33 @@ def simulate_block(eng, model, n, first, seed=0, with_steps=False):
     _lib.check(L.abc_simulate_dev(eng.ctx, model, seed, first, n, _ptr(th), _ptr(st), None,
                                   _ptr(sup), _ptr(steps)), eng.ctx, "simulate")
     return th, sup, steps
+
+
+def simulate_models(eng, models, n, firsts, seed=0):
+    """Prior draws of n particles per model (particles firsts[m] ... of model
+    m) and their exact simulation in one multi-model call: m -> (theta,
+    superset n x K column-major, steps), all on the device."""
+    import torch
+    dev = torch.device("cuda:0")
+    L = eng.L
+    KS = 2 * eng.nb * eng.J
+    out = {}
+    for m in models:
+        th = torch.empty(eng.n_params(m) * n, dtype=torch.float64, device=dev)
+        _lib.check(L.abc_sample_prior_dev(eng.ctx, m, seed, firsts[m], n, _ptr(th)), eng.ctx,
+                   "prior")
+        out[m] = (th, torch.empty((4 * eng.nb + 3 * eng.J) * n, dtype=torch.float64, device=dev),
+                  torch.empty(KS * n, dtype=torch.float64, device=dev),
+                  torch.zeros(n, dtype=torch.int64, device=dev))
+    k = len(models)
+    VA = C.c_void_p * k
+    _lib.check(L.abc_simulate_multi_dev(
+        eng.ctx, k, (C.c_int32 * k)(*models), seed, (C.c_int64 * k)(*[firsts[m] for m in models]),
+        (C.c_int64 * k)(*([n] * k)), VA(*[out[m][0].data_ptr() for m in models]),
+        VA(*[out[m][1].data_ptr() for m in models]), None,
+        VA(*[out[m][2].data_ptr() for m in models]), VA(*[out[m][3].data_ptr() for m in models])),
+        eng.ctx, "simulate_multi")
+    return {m: (th, sup, stp) for m, (th, _, sup, stp) in out.items()}
 
 
 def make_genes(eng, G, seed=0, first=3_000_000_000):
@@ -91,7 +125,7 @@ def make_genes(eng, G, seed=0, first=3_000_000_000):
 def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
-        cal_recall=0.995, direct_steps=0, log=print):
+        cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.2, log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -161,7 +195,8 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     direct = None
     if direct_steps and mode == 0:
         direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps,
-                               n_train, direct_steps, batch, lr, seed, dev, log)
+                               n_train, direct_steps, batch, lr, seed, dev, log, direct_loss,
+                               fixed_share)
         out["direct_train_s"] = direct.train_s
 
     # ---- exhaustive plain path on the evaluation set (timed)
@@ -175,8 +210,11 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
                                   parts.ctypes.data_as(C.c_void_p), 2, 10.0, mode, eps, k, 0.02),
                eng.ctx, "accept_begin")
     eval_sets, eval_steps = {}, {}
+    sims = simulate_models(eng, MODELS, n_eval, {m: eval_first(m) for m in MODELS}, seed)
+    torch.cuda.synchronize()
+    t_plain_sim = time.perf_counter() - t0
     for m in MODELS:
-        th, sup, steps = simulate_block(eng, m, n_eval, eval_first(m), seed)
+        th, sup, steps = sims[m]
         eval_sets[m] = (th, sup)
         eval_steps[m] = steps
         _lib.check(L.abc_accept_add_dev(eng.ctx, n_eval, _ptr(sup), eval_first(m)), eng.ctx, "add")
@@ -186,9 +224,11 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     thr = (np.array([max(e) if len(e) else 10.0 for (_, e) in A_plain]) if mode == 1
            else np.full(G, eps))
     plain_pairs = {(g, int(i)) for g, (idx, _) in enumerate(A_plain) for i in idx}
-    out.update(plain_time_s=t_plain, plain_pairs=len(plain_pairs), eval_particles=n_all,
+    out.update(plain_time_s=t_plain, plain_simulate_time_s=t_plain_sim,
+               plain_pairs=len(plain_pairs), eval_particles=n_all,
                plain_accepted_per_s=len(plain_pairs) / t_plain)
-    log(f"[screened] plain: {len(plain_pairs)} accepted pairs in {t_plain:.2f}s")
+    log(f"[screened] plain: {len(plain_pairs)} accepted pairs in {t_plain:.2f}s (exact "
+        f"simulation {t_plain_sim:.3f}s)")
 
     # ---- held-out accuracy of the surrogate against the simulator (per statistic,
     # in standardised asinh units; the targets carry the sampling noise of the
@@ -208,6 +248,8 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
         zsq.append((((t - mu) / sd) ** 2).mean(0))
     err_rms = torch.sqrt(torch.stack(err_rms).mean(0)).cpu().numpy()
     cover = torch.stack(cover).mean(0).cpu().numpy()
+    out["heldout_rmse_std_units"] = [round(float(v), 5) for v in err_rms]
+    out["heldout_coverage_2sigma_per_stat"] = [round(float(v), 4) for v in cover]
     out["heldout_rmse_std_units_median"] = float(np.median(err_rms))
     out["heldout_rmse_std_units_max"] = float(err_rms.max())
     out["heldout_coverage_2sigma"] = float(cover.mean())
@@ -267,26 +309,35 @@ def _accepting_genes(eng, L, sup, n, data_d, se_d, parts, KS, G, eps, dev, chunk
 class _Direct:
     """A network trained on the number of genes that accept each training
     particle (one noisy draw per particle) and on the log of its exact
-    simulation steps: with the Gaussian NLL its mean outputs estimate
-    E[accepting genes | parameters] and E[log steps | parameters].  Passing
-    particles in decreasing expected count keeps the most accepted pairs per
-    particle simulated; in decreasing count per expected step, the most per
-    unit of exact work.  Trained for one gene set."""
+    simulation steps: its outputs estimate E[accepting genes | parameters]
+    (Poisson NLL on a log-rate output, or the Gaussian NLL's mean) and
+    E[log steps | parameters] (Gaussian NLL).  Passing particles in decreasing
+    expected count keeps the most accepted pairs per particle simulated; in
+    decreasing count per unit of expected cost, the most per unit of exact
+    work.  A particle's cost is its steps plus c0, the fixed cost of a
+    particle in step units (its cells' set-up, burn-in records, queue refills
+    and statistics, which do not scale with its steps).  Trained for one gene
+    set."""
 
-    def __init__(self, net, scale, lmu, lsd, train_s):
-        self.net, self.scale, self.lmu, self.lsd, self.train_s = net, scale, lmu, lsd, train_s
+    def __init__(self, net, scale, lmu, lsd, train_s, poisson, c0):
+        self.net, self.scale, self.lmu, self.lsd = net, scale, lmu, lsd
+        self.train_s, self.poisson, self.c0 = train_s, poisson, c0
+
+    def _count(self, y):
+        import torch
+        return torch.exp(y[:, 0]) if self.poisson else y[:, 0] * self.scale
 
     def score(self, x):
-        return self.net.predict(x)[:, 0] * self.scale
+        return self._count(self.net.predict(x))
 
     def per_work(self, x):
-        y = self.net.predict(x)
         import torch
-        return (y[:, 0] * self.scale) * torch.exp(-(y[:, 1] * self.lsd + self.lmu))
+        y = self.net.predict(x)
+        return self._count(y) / (torch.exp(y[:, 1] * self.lsd + self.lmu) + self.c0)
 
 
 def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps, n_train, steps,
-                  batch, lr, seed, dev, log):
+                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.2):
     import torch
     from .surrogate import Surrogate
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
@@ -294,18 +345,22 @@ def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps
     se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
     cnt = torch.cat([_accepting_genes(eng, L, s, n_train, data_d, se_d, parts, KS, G, eps, dev)
                      for s in ts_raw]).to(torch.float32)
-    scale = float(max(cnt.std().item(), 1e-3))
+    poisson = loss == "poisson"
+    scale = 1.0 if poisson else float(max(cnt.std().item(), 1e-3))
     lw = torch.log(torch.cat(ts_steps).to(torch.float64).clamp(min=1.0)).to(torch.float32)
     lmu, lsd = float(lw.mean().item()), float(max(lw.std().item(), 1e-3))
+    c0 = fixed_share * float(torch.cat(ts_steps).to(torch.float64).mean().item())
     tgt = torch.stack([cnt / scale, (lw - lmu) / lsd], dim=1).contiguous()
     net = Surrogate(2, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17, stream=sur.stream)
+    if poisson:
+        net.set_count_outputs(1)
     t0 = time.perf_counter()
     losses = net.train(torch.cat(xs), tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1)
     train_s = time.perf_counter() - t0
     log(f"[screened] direct network: {int((cnt > 0).sum().item())} of {len(cnt)} training "
         f"particles accepted by some gene; loss {losses[:10].mean():.3f} -> "
         f"{losses[-50:].mean():.3f} ({train_s:.1f}s)")
-    return _Direct(net, scale, lmu, lsd, train_s)
+    return _Direct(net, scale, lmu, lsd, train_s, poisson, c0)
 
 
 def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,
@@ -447,41 +502,57 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
                                   parts.ctypes.data_as(C.c_void_p), 2, 10.0, mode, eps, k, 0.02),
                eng.ctx, "accept_begin")
     _lib.check(L.abc_accept_set_thresholds(eng.ctx, thr.ctypes.data_as(C.c_void_p)), eng.ctx, "thr")
-    n_pass_total = 0
+    # the screen over every model's particles, then one exact simulation of
+    # all the particles that pass (one burn-in and one SSA launch)
     t_screen = 0.0
-    work_all, work_pass = 0, 0
+    passed = {}
+    ts = time.perf_counter()
     for m in MODELS:
         th, _ = eval_sets[m]
-        ts = time.perf_counter()
         x = sur.features(m, th)
         if rule is None:
-            pas, npass = sur.screen(eng, x, z=z, margin=margin)
+            pas, _ = sur.screen(eng, x, z=z, margin=margin)
         else:
             pas = (rule[0](m, x) >= rule[1]).to(torch.uint8)
-            npass = int(pas.sum().item())
-        t_screen += time.perf_counter() - ts
-        n_pass_total += npass
-        work_all += int(eval_steps[m].sum().item())
-        work_pass += int(eval_steps[m][pas.bool()].sum().item())
-        if npass == 0:
-            continue
-        keep = torch.nonzero(pas, as_tuple=False).flatten()
+        passed[m] = pas
+    keeps = {m: torch.nonzero(passed[m], as_tuple=False).flatten() for m in MODELS}
+    npass = {m: int(keeps[m].numel()) for m in MODELS}
+    t_screen += time.perf_counter() - ts
+    ms = [m for m in MODELS if npass[m] > 0]
+    bufs = {}
+    for m in ms:
+        th, _ = eval_sets[m]
         npar = eng.n_params(m)
-        th_keep = th.view(npar, n_eval)[:, keep].contiguous().view(-1)
-        pidx = (keep + eval_first(m)).to(torch.int64)
-        st = torch.empty((4 * nb + 3 * J) * npass, dtype=torch.float64, device=dev)
-        sup = torch.empty(KS * npass, dtype=torch.float64, device=dev)
-        _lib.check(L.abc_simulate_idx_dev(eng.ctx, m, seed, npass, _ptr(pidx), _ptr(th_keep),
-                                          _ptr(st), None, _ptr(sup), None), eng.ctx, "simulate_idx")
-        pidx_h = np.ascontiguousarray(pidx.cpu().numpy())
-        _lib.check(L.abc_accept_add_idx_dev(eng.ctx, npass, _ptr(sup),
+        th_keep = th.view(npar, n_eval)[:, keeps[m]].contiguous().view(-1)
+        pidx = (keeps[m] + eval_first(m)).to(torch.int64)
+        st = torch.empty((4 * nb + 3 * J) * npass[m], dtype=torch.float64, device=dev)
+        sup = torch.empty(KS * npass[m], dtype=torch.float64, device=dev)
+        bufs[m] = (pidx, th_keep, st, sup)
+    t_sim0 = time.perf_counter()
+    if ms:
+        q = len(ms)
+        VA = C.c_void_p * q
+        _lib.check(L.abc_simulate_multi_idx_dev(
+            eng.ctx, q, (C.c_int32 * q)(*ms), seed, (C.c_int64 * q)(*[npass[m] for m in ms]),
+            VA(*[bufs[m][0].data_ptr() for m in ms]), VA(*[bufs[m][1].data_ptr() for m in ms]),
+            VA(*[bufs[m][2].data_ptr() for m in ms]), None,
+            VA(*[bufs[m][3].data_ptr() for m in ms]), None), eng.ctx, "simulate_multi_idx")
+    torch.cuda.synchronize()
+    t_sim = time.perf_counter() - t_sim0
+    for m in ms:
+        pidx_h = np.ascontiguousarray(bufs[m][0].cpu().numpy())
+        _lib.check(L.abc_accept_add_idx_dev(eng.ctx, npass[m], _ptr(bufs[m][3]),
                                             pidx_h.ctypes.data_as(C.c_void_p)), eng.ctx, "add_idx")
     A_scr = _end(eng, G, G * max(k, 16))
     t_scr = time.perf_counter() - t0
+    n_pass_total = sum(npass.values())
+    work_all = sum(int(eval_steps[m].sum().item()) for m in MODELS)
+    work_pass = sum(int(eval_steps[m][passed[m].bool()].sum().item()) for m in MODELS)
     scr_pairs = {(g, int(i)) for g, (idx, e) in enumerate(A_scr) for i, ee in zip(idx, e)
                  if ee <= thr[g]}
     hit = len(plain_pairs & scr_pairs)
-    res = dict(z=z, screened_time_s=t_scr, screen_time_s=t_screen, screened_pass=n_pass_total,
+    res = dict(z=z, screened_time_s=t_scr, screen_time_s=t_screen, simulate_time_s=t_sim,
+               screened_pass=n_pass_total,
                ssa_fraction_avoided=1.0 - n_pass_total / n_all,
                ssa_work_fraction_avoided=1.0 - work_pass / max(work_all, 1),
                recall=hit / max(len(plain_pairs), 1),
@@ -489,7 +560,8 @@ def _screened(eng, sur, L, data, se, parts, KS, G, mode, eps, k, thr, z, margin,
                speedup_accepted_per_s=(hit / t_scr) / (len(plain_pairs) / t_plain))
     log(f"[screened] {('z=%s' % z) if rule is None else ('score >= %.4g' % rule[1])}: "
         f"pass {n_pass_total}/{n_all}, recall {res['recall']:.4f}, "
-        f"time {t_scr:.2f}s vs plain {t_plain:.2f}s")
+        f"time {t_scr:.2f}s (screen {t_screen:.3f}s, exact simulation {t_sim:.3f}s) vs plain "
+        f"{t_plain:.2f}s")
     return res
 
 

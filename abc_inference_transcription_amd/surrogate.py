@@ -60,6 +60,11 @@ class Surrogate:
             self.L.abc_mlp_destroy(self.h)
             self.h = None
 
+    def set_count_outputs(self, n_counts: int):
+        """The first n_counts statistics become counts (Poisson NLL on a
+        log-rate output: predict()'s mean column holds log E[count])."""
+        self._chk(self.L.abc_mlp_set_count_outputs(self.h, n_counts), "set_count_outputs")
+
     @property
     def n_params(self) -> int:
         return self.L.abc_mlp_n_params(self.h)
@@ -67,11 +72,20 @@ class Surrogate:
     def _s(self):
         return C.c_void_p(self.stream.cuda_stream) if self.stream is not None else None
 
+    def _after_torch(self):
+        """Order the next network call after the torch work that made its
+        inputs.  Unless bound to torch's current stream, the network runs on
+        a stream of its own that does not wait for torch's."""
+        cur = self.torch.cuda.current_stream(self.dev)
+        if self.stream is None or self.stream.cuda_stream != cur.cuda_stream:
+            cur.synchronize()
+
     # ---------------------------------------------------------------- data
     def features(self, model: int, theta_dev):
         """theta_dev: torch float64 holding n x n_params column-major."""
         n = theta_dev.numel() // (6 if model in (1, 2) else 10)
         x = self.torch.empty((n, self.n_in), dtype=self.torch.float32, device=self.dev)
+        self._after_torch()
         rc = self.L.abc_features_dev(model, n, _ptr(theta_dev), self.n_bins, _ptr(x), self._s())
         self._chk(rc, "features")
         return x
@@ -85,6 +99,7 @@ class Surrogate:
 
     def targets(self, stats_dev, n: int):
         t = self.torch.empty((n, self.n_stats), dtype=self.torch.float32, device=self.dev)
+        self._after_torch()
         rc = self.L.abc_targets_dev(n, _ptr(stats_dev), self.n_stats, _ptr(self.mean),
                                     _ptr(self.sd), _ptr(t), self._s())
         self._chk(rc, "targets")
@@ -105,6 +120,7 @@ class Surrogate:
             if pos + batch > n:
                 perm = torch.randperm(n, device=self.dev, generator=g)
                 xs, ts = x[perm].contiguous(), t[perm].contiguous()
+                self._after_torch()
                 pos = 0
             xb, tb = xs[pos:pos + batch], ts[pos:pos + batch]
             pos += batch
@@ -121,6 +137,7 @@ class Surrogate:
     def predict(self, x):
         y = self.torch.empty((x.shape[0], 2 * self.n_stats), dtype=self.torch.float32,
                              device=self.dev)
+        self._after_torch()
         self._chk(self.L.abc_mlp_forward_dev(self.h, x.shape[0], _ptr(x), _ptr(y)), "forward")
         self._chk(self.L.abc_mlp_synchronize(self.h), "sync")
         return y
@@ -128,6 +145,7 @@ class Surrogate:
     def grad(self, x, t):
         g = np.zeros(self.n_params, np.float32)
         loss = C.c_float()
+        self._after_torch()
         rc = self.L.abc_mlp_grad_dev(self.h, x.shape[0], _ptr(x), _ptr(t),
                                      g.ctypes.data_as(C.c_void_p), C.byref(loss))
         self._chk(rc, "grad")
@@ -148,6 +166,7 @@ class Surrogate:
         margin."""
         n = x.shape[0]
         sc = self.torch.empty((n, 2), dtype=self.torch.float32, device=self.dev)
+        self._after_torch()
         rc = self.L.abc_screen_score_dev(eng.ctx, self.h, n, _ptr(x), _ptr(self.mean),
                                          _ptr(self.sd), margin, _ptr(sc))
         _lib.check(rc, eng.ctx, "screen_score")
@@ -158,6 +177,7 @@ class Surrogate:
         n = x.shape[0]
         pas = self.torch.zeros(n, dtype=self.torch.uint8, device=self.dev)
         npass = C.c_int64()
+        self._after_torch()
         rc = self.L.abc_screen_dev(eng.ctx, self.h, n, _ptr(x), _ptr(self.mean), _ptr(self.sd),
                                    z, margin, _ptr(pas), C.byref(npass))
         _lib.check(rc, eng.ctx, "screen")

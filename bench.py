@@ -100,14 +100,16 @@ def parse():
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-default SSA grid")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
-    ap.add_argument("--sur-train", type=int, default=48000, help="training particles per model")
-    ap.add_argument("--sur-eval", type=int, default=2000, help="held-out particles per model")
-    ap.add_argument("--sur-steps", type=int, default=6000, help="Adam steps (batch 8192)")
-    ap.add_argument("--sur-cal", type=int, default=2000,
+    ap.add_argument("--sur-train", type=int, default=100000, help="training particles per model")
+    ap.add_argument("--sur-eval", type=int, default=10000, help="held-out particles per model")
+    ap.add_argument("--sur-steps", type=int, default=10000, help="Adam steps (batch 8192)")
+    ap.add_argument("--sur-cal", type=int, default=10000,
                     help="calibration particles per model that set the calibrated pass rules")
-    ap.add_argument("--sur-direct-steps", type=int, default=3000,
+    ap.add_argument("--sur-cal-recall", type=float, default=0.992,
+                    help="recall of the calibration set's accepted pairs each rule keeps")
+    ap.add_argument("--sur-direct-steps", type=int, default=8000,
                     help="Adam steps of the network trained on accepting-gene counts (0 = skip)")
-    ap.add_argument("--ceiling-reps", type=int, default=16,
+    ap.add_argument("--ceiling-reps", type=int, default=8,
                     help="independent exact replicates per held-out particle for the bound on "
                          "what any parameter-only screen can gain (0 = skip)")
     ap.add_argument("--no-moments", action="store_true",
@@ -367,7 +369,8 @@ def main():
                    "mean_steps_per_particle": total_steps / n_particles},
         "roofline": roof,
         "wall_s": round(wall, 3),
-        "ssa_launch": {"workgroups": eng.ssa_blocks(), "wg_lanes": 256, "autotune": tuned},
+        "ssa_launch": {"workgroups": eng.ssa_blocks(), "wg_lanes": 256,
+                       "items_per_claim": eng.ssa_claim(), "autotune": tuned},
         "graph_replay": graph,
     }
 
@@ -392,15 +395,17 @@ def surrogate_leg(eng, stream, a):
     from abc_inference_transcription_amd.surrogate import Surrogate
     sur = Surrogate(2 * eng.nb * eng.J, seed=0, stream=stream)
     res = S.run(eng, sur, G=a.genes, n_train=a.sur_train, n_eval=a.sur_eval, k=0,
-                accept_rate=4e-5, z=[4.0, 3.0, 2.5, 2.0], train_steps=a.sur_steps, batch=8192,
-                ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal, direct_steps=a.sur_direct_steps,
+                accept_rate=4e-5, z=[3.0, 2.0], train_steps=a.sur_steps, batch=8192,
+                ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal, cal_recall=a.sur_cal_recall,
+                direct_steps=a.sur_direct_steps,
                 log=lambda *x: print(*x, file=sys.stderr, flush=True))
     sur.close()
     keep = ("plain_accepted_per_s", "screened_accepted_per_s", "speedup_accepted_per_s",
             "recall", "ssa_fraction_avoided", "ssa_work_fraction_avoided", "plain_pairs",
             "eval_particles", "train_particles", "heldout_nll", "heldout_nll_of_mean_predictor",
             "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
-            "train_loss_last", "train_s", "direct_train_s", "genes", "z", "eps", "accept_rate_target", "by_z",
+            "train_loss_last", "train_s", "direct_train_s", "train_simulate_s",
+            "heldout_rmse_std_units", "heldout_rmse_std_units_max", "genes", "z", "eps", "accept_rate_target", "by_z",
             "ceiling", "by_rule", "rule", "threshold")
     return {k: res[k] for k in keep if k in res}
 

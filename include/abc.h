@@ -175,6 +175,16 @@ int32_t abc_simulate_multi_dev(abc_ctx* ctx, int32_t n_models, const int32_t* mo
                                const double* const* theta_dev, double* const* stats_dev,
                                double* const* moments_dev, double* const* superset_dev,
                                int64_t* const* steps_dev);
+/* abc_simulate_multi_dev over a subset of each model's particles (no reference
+ * counterpart; the particles a screen passes): model m's row i uses the random
+ * streams of global particle pidx_dev[m][i] (device int64, < 2^32), so its
+ * results equal those of abc_simulate_idx_dev per model, bit for bit. */
+int32_t abc_simulate_multi_idx_dev(abc_ctx* ctx, int32_t n_models, const int32_t* models,
+                                   uint64_t seed, const int64_t* n,
+                                   const int64_t* const* pidx_dev,
+                                   const double* const* theta_dev, double* const* stats_dev,
+                                   double* const* moments_dev, double* const* superset_dev,
+                                   int64_t* const* steps_dev);
 /* abc_simulate_dev without device allocation (no reference counterpart; for
  * callers that cannot allocate device memory: code under HIP graph capture,
  * serving loops, harnesses that forbid hipMalloc).  Same arguments and
@@ -346,6 +356,14 @@ int32_t abc_mlp_create(abc_mlp** out, int32_t device, int32_t n_in, int32_t widt
 int32_t abc_mlp_destroy(abc_mlp* mlp);
 const char* abc_mlp_last_error(const abc_mlp* mlp);
 int64_t abc_mlp_n_params(const abc_mlp* mlp);
+/* Make the first n_counts statistics counts: their mean output is a log-rate
+ * o and their loss the Poisson negative log-likelihood exp(o) - t o (o clipped
+ * to [-30, 30]); their log-variance output is unused.  Default 0. */
+int32_t abc_mlp_set_count_outputs(abc_mlp* mlp, int32_t n_counts);
+/* Rows of the GEMM workgroup tile: 0 (default) picks 64 when 128-row tiles
+ * would give fewer than two workgroups per CU, else 128; 64 or 128 force it.
+ * A launch shape only: the results do not depend on it. */
+int32_t abc_mlp_set_tile_rows(abc_mlp* mlp, int32_t rows);
 int32_t abc_mlp_n_stats(const abc_mlp* mlp);
 int32_t abc_mlp_set_stream(abc_mlp* mlp, void* hip_stream);
 int32_t abc_mlp_synchronize(abc_mlp* mlp);
@@ -396,8 +414,10 @@ int32_t abc_screen_score_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float*
 int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
 int32_t abc_get_ssa_blocks(const abc_ctx* ctx);
 /* Work items (cell sub-lanes) a lane of the SSA kernel claims per atomic on
- * its work queue: 1, 2, 4 or 8 consecutive items (default 1).  Items are
- * independent, so this changes the schedule only.  Set by the autotuner. */
+ * its work queue: 1, 2, 4 or 8 consecutive items (default 1).  The first item
+ * of a claim is located by binary search, the others by a step from the
+ * previous one.  Items are independent, so this changes the schedule only.
+ * Set by the autotuner. */
 int32_t abc_set_ssa_claim(abc_ctx* ctx, int32_t claim);
 int32_t abc_get_ssa_claim(const abc_ctx* ctx);
 
@@ -405,7 +425,7 @@ int32_t abc_get_ssa_claim(const abc_ctx* ctx);
  * reference runs one particle at a time, scripts/abc_simulation.jl:88).  Draws
  * n prior particles (from first) of each listed model, simulates them once per
  * candidate grid (1, 2, 3, 4, 6, 8, 12, 16 workgroups per CU, up to
- * occupancy) and then, on the fastest grid, once per queue claim (2 and 4
+ * occupancy) and then, on the fastest grid, once per queue claim (2, 4 and 8
  * work items per atomic against 1), times ssa_kernel with HIP events and keeps
  * the fastest shape for later calls.  Outputs never depend on the shape, so
  * tuning changes speed only.  best_blocks / best_ms (optional) receive the

@@ -4,7 +4,11 @@ Same architecture and parameter layout as libabc's abc_mlp: per layer
 W (out_pad x in_pad, row-major) then b (out_pad); hidden layers SiLU; the
 output row is [mean (ns) | log-variance (ns) | padding]; the loss is the mean
 Gaussian negative log-likelihood over the batch and the ns statistics,
-0.5 * (lv + (t - mu)^2 exp(-lv)) with lv clipped to [-12, 12].
+0.5 * (lv + (t - mu)^2 exp(-lv)) with lv clipped to [-12, 12].  The first
+n_counts statistics may instead be counts: their mean slot holds a log-rate
+o, clipped to [-30, 30], with the Poisson negative log-likelihood
+exp(o) - t o (log t! dropped), and their log-variance slot is unused (no
+loss, zero gradient).
 Used to check the GPU gradients; pinned itself by finite differences.
 """
 import numpy as np
@@ -53,7 +57,7 @@ def forward(theta, x, n_in, width, depth, ns):
     return out, zs, hs
 
 
-def loss_and_grad(theta, x, t, n_in, width, depth, ns):
+def loss_and_grad(theta, x, t, n_in, width, depth, ns, n_counts=0):
     layers, n_par, in_pad, out_pad = layout(n_in, width, depth, ns)
     theta = np.asarray(theta, np.float64)
     Ws, bs = unpack(theta, layers)
@@ -65,11 +69,20 @@ def loss_and_grad(theta, x, t, n_in, width, depth, ns):
     r = t - mu
     inv = np.exp(-lv)
     scale = 1.0 / (B * ns)
-    loss = float(np.sum(0.5 * (lv + r * r * inv)) * scale)
+    el = 0.5 * (lv + r * r * inv)
     dO = np.zeros_like(out)
     dO[:, :ns] = -r * inv * scale
     dO[:, ns:2 * ns] = np.where((lv_raw < -12.0) | (lv_raw > 12.0), 0.0,
                                 0.5 * (1.0 - r * r * inv) * scale)
+    c = n_counts
+    if c:
+        o_raw = mu[:, :c]
+        o = np.clip(o_raw, -30.0, 30.0)
+        lam = np.exp(o)
+        el[:, :c] = lam - t[:, :c] * o
+        dO[:, :c] = np.where((o_raw < -30.0) | (o_raw > 30.0), 0.0, (lam - t[:, :c]) * scale)
+        dO[:, ns:ns + c] = 0.0
+    loss = float(np.sum(el) * scale)
     grad = np.zeros(n_par)
     d = dO
     for l in range(depth, -1, -1):

@@ -258,13 +258,31 @@ def test_autotune_changes_speed_not_results(engine_factory):
     blocks, ms = eng.autotune(models=(4, 5), n=64, first=10)
     print(f"[autotune] blocks {blocks}, claim {eng.ssa_claim()}, {ms:.2f} ms", flush=True)
     assert blocks > 0 and ms > 0
-    assert eng.ssa_blocks() == blocks and eng.ssa_claim() in (1, 2, 4)
+    assert eng.ssa_blocks() == blocks and eng.ssa_claim() in (1, 2, 4, 8)
     after = eng.simulate(5, th, first=77, superset=True)
     for k in ("stats", "superset", "steps"):
         np.testing.assert_array_equal(before[k], after[k])
     # every queue claim gives the same bits
-    for claim in (2, 8):
+    for claim in (2, 4, 8):
         eng.set_ssa_claim(claim)
         again = eng.simulate(5, th, first=77, superset=True)
         for k in ("stats", "superset", "steps"):
             np.testing.assert_array_equal(before[k], again[k])
+
+
+def test_claim_walk_over_split_cells(engine_factory, oracle):
+    """With a small split target most cells run as several sub-lanes, so a
+    lane's claim steps across sub-lanes, cells and particles; every claim
+    gives the bits of claim 1 and of the oracle."""
+    design = dict(config_design(cells=120))
+    design["split_target"] = 64.0
+    eng = engine_factory(design)
+    th = eng.sample_prior(4, 24, first=300)
+    want = oracle.simulate(oracle.OracleDesign(design), 4, 0, 300, th, cells=False)
+    for claim in (1, 2, 4, 8):
+        eng.set_ssa_claim(claim)
+        got = eng.simulate(4, th, first=300, superset=True)
+        print(f"[claim] {claim}", flush=True)
+        assert same_bits(got["superset"], want["superset"]), claim
+        assert same_bits(got["stats"], want["stats_ref"]), claim
+        assert np.array_equal(got["steps"], np.asarray(want["steps"], np.int64)), claim

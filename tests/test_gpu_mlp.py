@@ -45,6 +45,37 @@ def test_mlp_gradient_matches_fp64_oracle(torch):
     s.close()
 
 
+def test_mlp_count_outputs_gradient_matches_oracle(torch):
+    """Poisson negative log-likelihood on the first statistics' log-rate
+    outputs (the rest Gaussian): loss and gradients against the float64
+    oracle."""
+    import mlp_ref as M
+    s = make(torch)
+    nc = 3
+    s.set_count_outputs(nc)
+    rng = np.random.default_rng(7)
+    B = 512
+    x = rng.normal(0, 0.7, (B, s.n_in)).astype(np.float32)
+    t = rng.normal(0, 1, (B, 90)).astype(np.float32)
+    t[:, :nc] = rng.poisson(1.5, (B, nc))
+    loss, g = s.grad(torch.tensor(x, device=s.dev), torch.tensor(t, device=s.dev))
+    theta = s.get_params().astype(np.float64)
+    loss_ref, g_ref = M.loss_and_grad(theta, x.astype(np.float64), t.astype(np.float64),
+                                      s.n_in, 512, 4, 90, n_counts=nc)
+    assert abs(loss - loss_ref) < 2e-2 * abs(loss_ref)
+    layers, _, _, _ = M.layout(s.n_in, 512, 4, 90)
+    for (i, o, wo, bo) in layers:
+        for a, b in ((g[wo:wo + i * o], g_ref[wo:wo + i * o]), (g[bo:bo + o], g_ref[bo:bo + o])):
+            nrm = np.linalg.norm(b)
+            if nrm == 0:
+                assert np.linalg.norm(a) == 0
+                continue
+            rel = np.linalg.norm(a - b) / nrm
+            cos = float(a @ b) / (np.linalg.norm(a) * nrm)
+            assert rel < 6e-2 and cos > 0.998, (wo, rel, cos)
+    s.close()
+
+
 def test_mlp_forward_matches_oracle(torch):
     import mlp_ref as M
     s = make(torch, max_batch=256)
@@ -72,4 +103,25 @@ def test_mlp_training_reduces_nll(torch):
     y = s.predict(xd[:1024]).cpu().numpy()
     assert np.median(y[:, 90:]) < -2.5
     assert np.mean(np.abs(y[:, :90] - mu[:1024])) < 0.15
+    s.close()
+
+
+def test_gemm_tile_rows_change_speed_not_results(torch):
+    """64- and 128-row workgroup tiles accumulate every output in the same
+    order: identical forward outputs and identical parameters after training."""
+    rng = np.random.default_rng(9)
+    n = 4096
+    x = torch.tensor(rng.uniform(-1, 1, (n, 23)).astype(np.float32), device="cuda:0")
+    t = torch.tensor(rng.normal(0, 1, (n, 90)).astype(np.float32), device="cuda:0")
+    outs = []
+    for rows in (128, 64):
+        s = make(torch, max_batch=2048)
+        assert s.L.abc_mlp_set_tile_rows(s.h, rows) == 0
+        s.train(x, t, steps=12, batch=2048)
+        outs.append((s.get_params(), s.predict(x).cpu().numpy()))
+        s.close()
+    assert np.array_equal(outs[0][0].view(np.int32), outs[1][0].view(np.int32))
+    assert np.array_equal(outs[0][1].view(np.int32), outs[1][1].view(np.int32))
+    s = make(torch)
+    assert s.L.abc_mlp_set_tile_rows(s.h, 96) != 0
     s.close()
