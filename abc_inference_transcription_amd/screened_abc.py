@@ -125,7 +125,8 @@ def make_genes(eng, G, seed=0, first=3_000_000_000):
 def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
-        cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.2, log=print):
+        cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.3, direct_nets=2,
+        log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -196,7 +197,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     if direct_steps and mode == 0:
         direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps,
                                n_train, direct_steps, batch, lr, seed, dev, log, direct_loss,
-                               fixed_share)
+                               fixed_share, direct_nets)
         out["direct_train_s"] = direct.train_s
 
     # ---- exhaustive plain path on the evaluation set (timed)
@@ -285,7 +286,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     out.update(pick)
     out.update(genes=G, train_particles=n_train * len(MODELS), k=k)
     if direct is not None:
-        direct.net.close()
+        direct.close()
     return out
 
 
@@ -316,28 +317,39 @@ class _Direct:
     decreasing count per unit of expected cost, the most per unit of exact
     work.  A particle's cost is its steps plus c0, the fixed cost of a
     particle in step units (its cells' set-up, burn-in records, queue refills
-    and statistics, which do not scale with its steps).  Trained for one gene
-    set."""
+    and statistics, which do not scale with its steps).  Several networks
+    (trained from different initial weights and batch orders) average their
+    outputs.  Trained for one gene set."""
 
-    def __init__(self, net, scale, lmu, lsd, train_s, poisson, c0):
-        self.net, self.scale, self.lmu, self.lsd = net, scale, lmu, lsd
+    def __init__(self, nets, scale, lmu, lsd, train_s, poisson, c0):
+        self.nets, self.scale, self.lmu, self.lsd = nets, scale, lmu, lsd
         self.train_s, self.poisson, self.c0 = train_s, poisson, c0
+
+    def close(self):
+        for n in self.nets:
+            n.close()
+
+    def _y(self, x):
+        y = self.nets[0].predict(x)[:, :2]
+        for n in self.nets[1:]:
+            y = y + n.predict(x)[:, :2]
+        return y / len(self.nets)
 
     def _count(self, y):
         import torch
         return torch.exp(y[:, 0]) if self.poisson else y[:, 0] * self.scale
 
     def score(self, x):
-        return self._count(self.net.predict(x))
+        return self._count(self._y(x))
 
     def per_work(self, x):
         import torch
-        y = self.net.predict(x)
+        y = self._y(x)
         return self._count(y) / (torch.exp(y[:, 1] * self.lsd + self.lmu) + self.c0)
 
 
 def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps, n_train, steps,
-                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.2):
+                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.3, n_nets=2):
     import torch
     from .surrogate import Surrogate
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
@@ -351,16 +363,21 @@ def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps
     lmu, lsd = float(lw.mean().item()), float(max(lw.std().item(), 1e-3))
     c0 = fixed_share * float(torch.cat(ts_steps).to(torch.float64).mean().item())
     tgt = torch.stack([cnt / scale, (lw - lmu) / lsd], dim=1).contiguous()
-    net = Surrogate(2, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17, stream=sur.stream)
-    if poisson:
-        net.set_count_outputs(1)
+    x_all = torch.cat(xs)
+    nets = []
     t0 = time.perf_counter()
-    losses = net.train(torch.cat(xs), tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1)
+    for q in range(n_nets):
+        net = Surrogate(2, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17 + 1000 * q,
+                        stream=sur.stream)
+        if poisson:
+            net.set_count_outputs(1)
+        losses = net.train(x_all, tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1 + 1000 * q)
+        nets.append(net)
     train_s = time.perf_counter() - t0
     log(f"[screened] direct network: {int((cnt > 0).sum().item())} of {len(cnt)} training "
         f"particles accepted by some gene; loss {losses[:10].mean():.3f} -> "
         f"{losses[-50:].mean():.3f} ({train_s:.1f}s)")
-    return _Direct(net, scale, lmu, lsd, train_s, poisson, c0)
+    return _Direct(nets, scale, lmu, lsd, train_s, poisson, c0)
 
 
 def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,

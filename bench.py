@@ -99,6 +99,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the occupancy-default SSA grid")
+    ap.add_argument("--ssa-claim", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="SSA work items per queue claim (0 = the autotuner's or the default)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
     ap.add_argument("--sur-train", type=int, default=100000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=10000, help="held-out particles per model")
@@ -283,7 +285,10 @@ def main():
         # which run fastest at low occupancy, and would mislead the choice
         blocks, tms = eng.autotune(models=models, n=P, first=50_000_000)
         tuned = {"ssa_blocks": blocks, "sample_ssa_ms": round(tms, 3)}
-        print(f"[bench] autotune: {blocks} workgroups", file=sys.stderr, flush=True)
+        print(f"[bench] autotune: {blocks} workgroups, {eng.ssa_claim()} items per claim",
+              file=sys.stderr, flush=True)
+    if a.ssa_claim:
+        eng.set_ssa_claim(a.ssa_claim)
     begin()
     for w in range(a.warmup):
         step(10_000 + w, False)
