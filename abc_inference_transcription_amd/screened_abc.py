@@ -125,7 +125,7 @@ def make_genes(eng, G, seed=0, first=3_000_000_000):
 def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4e-5,
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
-        cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.3, direct_nets=2,
+        cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.5, direct_nets=2,
         log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
@@ -342,14 +342,20 @@ class _Direct:
     def score(self, x):
         return self._count(self._y(x))
 
-    def per_work(self, x):
+    def per_work(self, x, other=None, alpha=0.75):
+        """Expected accepting genes per unit of expected cost; with `other`
+        (another estimate of the expected genes, the statistics surrogate's),
+        the genes are the weighted geometric mean count^alpha other^(1-alpha)."""
         import torch
         y = self._y(x)
-        return self._count(y) / (torch.exp(y[:, 1] * self.lsd + self.lmu) + self.c0)
+        cnt = self._count(y)
+        if other is not None:
+            cnt = cnt ** alpha * other.clamp(min=1e-12) ** (1.0 - alpha)
+        return cnt / (torch.exp(y[:, 1] * self.lsd + self.lmu) + self.c0)
 
 
 def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps, n_train, steps,
-                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.3, n_nets=2):
+                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.5, n_nets=2):
     import torch
     from .surrogate import Surrogate
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
@@ -409,8 +415,8 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
         x = sur.features(m, th)
         sc = sur.scores(eng, x)
         if direct is not None:
-            sc = torch.cat([sc, direct.score(x).view(-1, 1), direct.per_work(x).view(-1, 1)],
-                           dim=1)
+            sc = torch.cat([sc, direct.score(x).view(-1, 1), direct.per_work(x).view(-1, 1),
+                            direct.per_work(x, sc[:, 0]).view(-1, 1)], dim=1)
         scores.append(sc.cpu().numpy())
     _end(eng, G, 16)
     acc, scores = np.concatenate(acc), np.vstack(scores)
@@ -421,6 +427,8 @@ def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recal
     if direct is not None:
         rules.append((2, "direct_expected_genes", lambda m, x: direct.score(x)))
         rules.append((3, "direct_genes_per_step", lambda m, x: direct.per_work(x)))
+        rules.append((4, "blended_genes_per_step",
+                      lambda m, x: direct.per_work(x, sur.scores(eng, x)[:, 0])))
     for col, name, fn in rules:
         sc = scores[:, col]
         order = np.argsort(-sc, kind="stable")

@@ -409,7 +409,8 @@ def surrogate_leg(eng, stream, a):
             "recall", "ssa_fraction_avoided", "ssa_work_fraction_avoided", "plain_pairs",
             "eval_particles", "train_particles", "heldout_nll", "heldout_nll_of_mean_predictor",
             "heldout_coverage_2sigma", "heldout_z_rms", "heldout_rmse_std_units_median",
-            "train_loss_last", "train_s", "direct_train_s", "train_simulate_s",
+            "train_loss_last", "train_s", "direct_train_s", "train_simulate_s", "plain_time_s",
+            "plain_simulate_time_s", "heldout_coverage_2sigma_per_stat",
             "heldout_rmse_std_units", "heldout_rmse_std_units_max", "genes", "z", "eps", "accept_rate_target", "by_z",
             "ceiling", "by_rule", "rule", "threshold")
     return {k: res[k] for k in keep if k in res}

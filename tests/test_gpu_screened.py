@@ -38,7 +38,8 @@ def test_screened_abc_recall_and_savings(screened):
     # exact work: what the trained network gains over simulating every particle
     rules = res["by_rule"]
     assert {r["rule"] for r in rules} == {"expected_genes", "max_margin",
-                                          "direct_expected_genes", "direct_genes_per_step"}
+                                          "direct_expected_genes", "direct_genes_per_step",
+                                          "blended_genes_per_step"}
     best = max(rules, key=lambda r: r["ssa_work_fraction_avoided"] if r["recall"] >= 0.98 else -1)
     assert best["recall"] >= 0.98, rules
     assert best["ssa_work_fraction_avoided"] > 0.2, rules
