@@ -81,7 +81,7 @@ struct abc_ctx {
   bool own_stream = true;
   int num_cu = 0;
   int ssa_blocks = 0;        // 0 = occupancy default
-  int ssa_claim = 1;         // SSA work items a lane claims per queue atomic
+  int ssa_claim = 2;         // SSA work items a lane claims per queue atomic
   int ssa_blocks_default = 0;
   int burn_blocks = 0;       // occupancy default of burn_kernel
   DesignDev hd{};
@@ -1950,7 +1950,7 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
   int np_max = 0;
   for (int q = 0; q < n_models; ++q) np_max = std::max(np_max, abc_layout_n_params(models[q], nb));
   if ((rc = ensure(c, c->theta, sizeof(double) * (size_t)n * np_max * n_models))) return rc;
-  if ((rc = ensure(c, c->stats, sizeof(double) * (size_t)n * K))) return rc;
+  if ((rc = ensure(c, c->stats, sizeof(double) * (size_t)n * K * n_models))) return rc;
   double* theta = P<double>(c->theta);
   for (int q = 0; q < n_models; ++q)
     if ((rc = abc_sample_prior_dev(c, models[q], seed, first, n, theta + (size_t)q * n * np_max)))
@@ -1972,12 +1972,21 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   const int saved = c->ssa_blocks;
+  // every trial is one multi-model call (all models' particles in one burn-in
+  // and one SSA launch), the shape of a batch of abc_simulate_multi_dev
+  std::vector<int64_t> firsts(n_models, first), ns(n_models, n);
+  std::vector<const double*> th_m(n_models);
+  std::vector<double*> st_m(n_models);
+  for (int q = 0; q < n_models; ++q) {
+    th_m[q] = theta + (size_t)q * n * np_max;
+    st_m[q] = P<double>(c->stats) + (size_t)q * n * K;
+  }
   auto run = [&](int blocks, double* ms) -> int {
     c->ssa_blocks = blocks;
     const double a = c->ssa_ms_acc;
-    for (int q = 0; q < n_models; ++q) {
-      const int r = simulate_dev(c, models[q], seed, first, n, theta + (size_t)q * n * np_max,
-                                 P<double>(c->stats), nullptr, nullptr, nullptr);
+    {
+      const int r = simulate_multi(c, n_models, models, seed, firsts.data(), nullptr, ns.data(),
+                                   th_m.data(), st_m.data(), nullptr, nullptr, nullptr);
       if (r) return r;
     }
     const int r = drain_ring(c);

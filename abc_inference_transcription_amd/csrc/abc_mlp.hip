@@ -22,6 +22,7 @@
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace {
@@ -105,11 +106,15 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
 #pragma unroll
   for (int r4 = 0; r4 < NRB; ++r4)
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb)
+    for (int cb = 0; cb < NCB; ++cb) {
+      // this lane's 4 outputs are rows m..m+3 of column n: one 8-byte store
+      // of the transposed copy (rows consecutive in C^T)
+      const int mb = m0 + wm * WM + r4 * 16 + (lane >> 4) * 4;
+      const int n = n0 + wn * WN + cb * 16 + (lane & 15);
+      bf16x4 tv;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int m = m0 + wm * WM + r4 * 16 + (lane >> 4) * 4 + q;
-        const int n = n0 + wn * WN + cb * 16 + (lane & 15);
+        const int m = mb + q;
         float v = acc[r4][cb][q];
         const size_t o = (size_t)m * N + n;
         if (EPI == EPI_F32) {
@@ -120,15 +125,17 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
           Zout[o] = z;
           const bf16 h = (bf16)(z * sigm(z));
           Cb[o] = h;
-          CbT[(size_t)n * M + m] = h;
+          tv[q] = h;
         } else {
           const float z = Zprev[o];
           const float s = sigm(z);
           const bf16 d = (bf16)(v * (s * (1.0f + z * (1.0f - s))));
           Cb[o] = d;
-          CbT[(size_t)n * M + m] = d;
+          tv[q] = d;
         }
       }
+      if (EPI != EPI_F32) *reinterpret_cast<bf16x4*>(CbT + (size_t)n * M + mb) = tv;
+    }
 }
 
 // C[i] = sum over the S split-K partials P[s][i], in the order s = 0, 1, ...
@@ -157,13 +164,22 @@ __global__ void cast_input_kernel(const float* __restrict__ x, int Bn, int Bpad,
 // statistics are counts: their mu slot is a log-rate o (clipped to [-30, 30],
 // no gradient outside) with the Poisson negative log-likelihood exp(o) - t o
 // (log t! dropped), and their logvar slot is unused.
-__global__ void nll_kernel(const float* __restrict__ O, const float* __restrict__ T, int Bn,
-                           int Bpad, int ns, int n_cnt, int out_pad, bf16* __restrict__ dO,
-                           bf16* __restrict__ dOT, float* __restrict__ loss) {
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per 64 x 64 tile of (sample b, output k): the gradient is
+// written row-major to dO with k fastest, and through an LDS transpose to dO^T
+// with b fastest, so both stores are coalesced; one loss atomic per tile.
+__global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O,
+                                                  const float* __restrict__ T, int Bn, int Bpad,
+                                                  int ns, int n_cnt, int out_pad,
+                                                  bf16* __restrict__ dO, bf16* __restrict__ dOT,
+                                                  float* __restrict__ loss) {
+  __shared__ bf16 tile[64][64 + 2];
+  __shared__ float red[256];
+  const int b0 = blockIdx.x * 64, k0 = blockIdx.y * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 columns x 4 rows per pass
+  const float scale = 1.0f / ((float)Bn * (float)ns);
   float l = 0.0f;
-  if (q < (int64_t)Bpad * out_pad) {
-    const int b = (int)(q / out_pad), k = (int)(q % out_pad);
+  for (int rr = ty; rr < 64; rr += 4) {
+    const int b = b0 + rr, k = k0 + tx;
     float g = 0.0f;
     if (b < Bn && k < 2 * ns) {
       const int s = k < ns ? k : k - ns;
@@ -171,32 +187,33 @@ __global__ void nll_kernel(const float* __restrict__ O, const float* __restrict_
       const float lv_raw = O[(size_t)b * out_pad + ns + s];
       const bool clipped = lv_raw < -12.0f || lv_raw > 12.0f;
       const float lv = fminf(fmaxf(lv_raw, -12.0f), 12.0f);
-      const float r = T[(size_t)b * ns + s] - mu;
+      const float y = T[(size_t)b * ns + s];
+      const float r = y - mu;
       const float inv = __expf(-lv);
-      const float scale = 1.0f / ((float)Bn * (float)ns);
       if (s < n_cnt) {
         if (k < ns) {
           const float o = fminf(fmaxf(mu, -30.0f), 30.0f);
-          const float lam = __expf(o), y = T[(size_t)b * ns + s];
+          const float lam = __expf(o);
           g = (mu < -30.0f || mu > 30.0f) ? 0.0f : (lam - y) * scale;
-          l = (lam - y * o) * scale;
+          l += (lam - y * o) * scale;
         }
       } else if (k < ns) {
         g = -r * inv * scale;
-        l = 0.5f * (lv + r * r * inv) * scale;
+        l += 0.5f * (lv + r * r * inv) * scale;
       } else {
         g = clipped ? 0.0f : 0.5f * (1.0f - r * r * inv) * scale;
       }
     }
-    dO[q] = (bf16)g;
-    dOT[(size_t)k * Bpad + b] = (bf16)g;
+    const bf16 gb = (bf16)g;
+    dO[(size_t)b * out_pad + k] = gb;
+    tile[rr][tx] = gb;
   }
-  // block reduction of the loss
-  __shared__ float red[256];
+  __syncthreads();
+  for (int kk = ty; kk < 64; kk += 4) dOT[(size_t)(k0 + kk) * Bpad + b0 + tx] = tile[tx][kk];
   red[threadIdx.x] = l;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+  for (int q = 128; q > 0; q >>= 1) {
+    if ((int)threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicAdd(loss, red[0]);
@@ -559,9 +576,9 @@ int grad(abc_mlp* p, int64_t Bn, const float* x_dev, const float* t_dev) {
   if (rc) return rc;
   const int Bpad = (int)((Bn + 127) / 128 * 128);
   hipLaunchKernelGGL(zero_f32_kernel, dim3(1), dim3(64), 0, p->stream, p->loss, (int64_t)1);
-  hipLaunchKernelGGL(nll_kernel, dim3(g1((int64_t)Bpad * p->out_pad, 256)), dim3(256), 0,
-                     p->stream, p->O, t_dev, (int)Bn, Bpad, p->ns, p->n_cnt, p->out_pad, p->dA,
-                     p->dAT, p->loss);
+  hipLaunchKernelGGL(nll_kernel, dim3(Bpad / 64, p->out_pad / 64), dim3(256), 0, p->stream,
+                     p->O, t_dev, (int)Bn, Bpad, p->ns, p->n_cnt, p->out_pad, p->dA, p->dAT,
+                     p->loss);
   // dA / dAT hold dZ of the current layer (output layer first)
   bf16 *cur = p->dA, *curT = p->dAT, *nxt = p->dB, *nxtT = p->dBT;
   for (int l = p->depth; l >= 0; --l) {

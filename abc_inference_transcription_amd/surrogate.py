@@ -38,17 +38,20 @@ class Surrogate:
         if rc != 0:
             raise RuntimeError(f"abc_mlp_create failed ({rc})")
         self.h = h
-        self.stream = stream
+        self.stream = None
         if stream is not None:
-            self._chk(self.L.abc_mlp_set_stream(h, C.c_void_p(stream.cuda_stream)), "set_stream")
+            self.bind_stream(stream)
         self.mean = torch.zeros(n_stats, dtype=torch.float32, device=self.dev)
         self.sd = torch.ones(n_stats, dtype=torch.float32, device=self.dev)
 
     def bind_stream(self, torch_stream):
-        """Launch every later call on this (non-null) torch stream."""
+        """Launch every later call on this torch stream.  The null stream
+        cannot be shared (the library then keeps a stream of its own), so
+        binding it leaves the network unbound and every call orders itself
+        against torch's stream by synchronising."""
         self._chk(self.L.abc_mlp_set_stream(self.h, C.c_void_p(torch_stream.cuda_stream)),
                   "set_stream")
-        self.stream = torch_stream
+        self.stream = torch_stream if torch_stream.cuda_stream != 0 else None
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -79,6 +82,13 @@ class Surrogate:
         cur = self.torch.cuda.current_stream(self.dev)
         if self.stream is None or self.stream.cuda_stream != cur.cuda_stream:
             cur.synchronize()
+
+    def _before_torch(self):
+        """Wait for the network's queued work before torch reuses memory it
+        reads (only needed when the network has a stream of its own)."""
+        cur = self.torch.cuda.current_stream(self.dev)
+        if self.stream is None or self.stream.cuda_stream != cur.cuda_stream:
+            self._chk(self.L.abc_mlp_synchronize(self.h), "sync")
 
     # ---------------------------------------------------------------- data
     def features(self, model: int, theta_dev):
@@ -118,6 +128,10 @@ class Surrogate:
         perm, pos = None, n
         for s in range(steps):
             if pos + batch > n:
+                # the previous epoch's shuffled copies may still be read by
+                # steps queued on the network's stream: let them finish before
+                # torch may reuse that memory
+                self._before_torch()
                 perm = torch.randperm(n, device=self.dev, generator=g)
                 xs, ts = x[perm].contiguous(), t[perm].contiguous()
                 self._after_torch()

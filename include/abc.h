@@ -414,7 +414,8 @@ int32_t abc_screen_score_dev(abc_ctx* ctx, abc_mlp* mlp, int64_t n, const float*
 int32_t abc_set_ssa_blocks(abc_ctx* ctx, int32_t blocks);
 int32_t abc_get_ssa_blocks(const abc_ctx* ctx);
 /* Work items (cell sub-lanes) a lane of the SSA kernel claims per atomic on
- * its work queue: 1, 2, 4 or 8 consecutive items (default 1).  The first item
+ * its work queue: 1, 2, 4 or 8 consecutive items (default 2, the fastest on
+ * the configured workload).  The first item
  * of a claim is located by binary search, the others by a step from the
  * previous one.  Items are independent, so this changes the schedule only.
  * Set by the autotuner. */
@@ -423,7 +424,8 @@ int32_t abc_get_ssa_claim(const abc_ctx* ctx);
 
 /* Autotuner of the persistent SSA launch shape (no reference counterpart: the
  * reference runs one particle at a time, scripts/abc_simulation.jl:88).  Draws
- * n prior particles (from first) of each listed model, simulates them once per
+ * n prior particles (from first) of each listed model, simulates them (all
+ * models in one multi-model call, as abc_simulate_multi_dev) once per
  * candidate grid (1, 2, 3, 4, 6, 8, 12, 16 workgroups per CU, up to
  * occupancy) and then, on the fastest grid, once per queue claim (2, 4 and 8
  * work items per atomic against 1), times ssa_kernel with HIP events and keeps

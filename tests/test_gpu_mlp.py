@@ -125,3 +125,32 @@ def test_gemm_tile_rows_change_speed_not_results(torch):
     s = make(torch)
     assert s.L.abc_mlp_set_tile_rows(s.h, 96) != 0
     s.close()
+
+
+def test_training_is_reproducible_across_epochs_and_streams(torch):
+    """Several epochs (each reshuffles into new torch buffers) give the same
+    parameters on the network's own stream, bound to torch's null stream, and
+    bound to a torch side stream: no step reads a buffer torch has reused."""
+    rng = np.random.default_rng(12)
+    n = 4096
+    x = torch.tensor(rng.uniform(-1, 1, (n, 23)).astype(np.float32), device="cuda:0")
+    t = torch.tensor(rng.normal(0, 1, (n, 90)).astype(np.float32), device="cuda:0")
+    side = torch.cuda.Stream(device="cuda:0")
+    params = []
+    for mode in ("own", "null", "side"):
+        s = make(torch, max_batch=1024)
+        if mode == "null":
+            s.bind_stream(torch.cuda.default_stream(torch.device("cuda:0")))
+        if mode == "side":
+            s.bind_stream(side)
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                s.train(x, t, steps=20, batch=1024)
+        else:
+            s.train(x, t, steps=20, batch=1024)
+        torch.cuda.synchronize()
+        params.append(s.get_params())
+        s.close()
+    assert np.all(np.isfinite(params[0]))
+    for p in params[1:]:
+        assert np.array_equal(params[0].view(np.int32), p.view(np.int32))
