@@ -161,9 +161,11 @@ __global__ void cast_input_kernel(const float* __restrict__ x, int Bn, int Bpad,
 
 // Gaussian negative log-likelihood per element, mean over B*ns.
 // O row: [mu (ns) | logvar (ns) | pad]; dO and dO^T in bf16.  The first n_cnt
-// statistics are counts: their mu slot is a log-rate o (clipped to [-30, 30],
-// no gradient outside) with the Poisson negative log-likelihood exp(o) - t o
-// (log t! dropped), and their logvar slot is unused.
+// statistics are counts: their mu slot is a log-rate o with the Poisson
+// negative log-likelihood exp(o) - t o (log t! dropped) on [-30, 8], continued
+// linearly outside (the gradient there is the bound's, exp(bound) - t, so an
+// output that strays is pulled back and exp never overflows), and their
+// logvar slot is unused.
 // One workgroup per 64 x 64 tile of (sample b, output k): the gradient is
 // written row-major to dO with k fastest, and through an LDS transpose to dO^T
 // with b fastest, so both stores are coalesced; one loss atomic per tile.
@@ -192,10 +194,10 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O,
       const float inv = __expf(-lv);
       if (s < n_cnt) {
         if (k < ns) {
-          const float o = fminf(fmaxf(mu, -30.0f), 30.0f);
+          const float o = fminf(fmaxf(mu, -30.0f), 8.0f);
           const float lam = __expf(o);
-          g = (mu < -30.0f || mu > 30.0f) ? 0.0f : (lam - y) * scale;
-          l += (lam - y * o) * scale;
+          g = (lam - y) * scale;
+          l += ((lam - y * o) + (lam - y) * (mu - o)) * scale;
         }
       } else if (k < ns) {
         g = -r * inv * scale;

@@ -377,6 +377,11 @@ def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps
                         stream=sur.stream)
         if poisson:
             net.set_count_outputs(1)
+            # start the log-rate at the mean count: small first gradients
+            w = net.get_params()
+            out_pad = (2 * 2 + 127) // 128 * 128
+            w[len(w) - out_pad] = np.log(max(float(cnt.mean().item()), 1e-3))
+            net.set_params(w)
         losses = net.train(x_all, tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1 + 1000 * q)
         nets.append(net)
     train_s = time.perf_counter() - t0

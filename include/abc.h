@@ -357,8 +357,9 @@ int32_t abc_mlp_destroy(abc_mlp* mlp);
 const char* abc_mlp_last_error(const abc_mlp* mlp);
 int64_t abc_mlp_n_params(const abc_mlp* mlp);
 /* Make the first n_counts statistics counts: their mean output is a log-rate
- * o and their loss the Poisson negative log-likelihood exp(o) - t o (o clipped
- * to [-30, 30]); their log-variance output is unused.  Default 0. */
+ * o and their loss the Poisson negative log-likelihood exp(o) - t o on
+ * [-30, 8], continued linearly outside; their log-variance output is unused.
+ * Default 0. */
 int32_t abc_mlp_set_count_outputs(abc_mlp* mlp, int32_t n_counts);
 /* Rows of the GEMM workgroup tile: 0 (default) picks 64 when 128-row tiles
  * would give fewer than two workgroups per CU, else 128; 64 or 128 force it.

@@ -6,9 +6,9 @@ output row is [mean (ns) | log-variance (ns) | padding]; the loss is the mean
 Gaussian negative log-likelihood over the batch and the ns statistics,
 0.5 * (lv + (t - mu)^2 exp(-lv)) with lv clipped to [-12, 12].  The first
 n_counts statistics may instead be counts: their mean slot holds a log-rate
-o, clipped to [-30, 30], with the Poisson negative log-likelihood
-exp(o) - t o (log t! dropped), and their log-variance slot is unused (no
-loss, zero gradient).
+o with the Poisson negative log-likelihood exp(o) - t o (log t! dropped) on
+[-30, 8], continued linearly outside, and their log-variance slot is unused
+(no loss, zero gradient).
 Used to check the GPU gradients; pinned itself by finite differences.
 """
 import numpy as np
@@ -77,10 +77,10 @@ def loss_and_grad(theta, x, t, n_in, width, depth, ns, n_counts=0):
     c = n_counts
     if c:
         o_raw = mu[:, :c]
-        o = np.clip(o_raw, -30.0, 30.0)
+        o = np.clip(o_raw, -30.0, 8.0)
         lam = np.exp(o)
-        el[:, :c] = lam - t[:, :c] * o
-        dO[:, :c] = np.where((o_raw < -30.0) | (o_raw > 30.0), 0.0, (lam - t[:, :c]) * scale)
+        el[:, :c] = (lam - t[:, :c] * o) + (lam - t[:, :c]) * (o_raw - o)
+        dO[:, :c] = (lam - t[:, :c]) * scale
         dO[:, ns:ns + c] = 0.0
     loss = float(np.sum(el) * scale)
     grad = np.zeros(n_par)

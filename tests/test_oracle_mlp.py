@@ -32,8 +32,9 @@ def test_mlp_oracle_gradient_matches_finite_differences():
 
 def test_mlp_oracle_count_outputs():
     """Count statistics (Poisson NLL on a log-rate output): the loss equals
-    scipy's Poisson log-pmf up to the log t! constant, and the gradient
-    matches central finite differences."""
+    scipy's Poisson log-pmf up to the log t! constant inside [-30, 8] and its
+    tangent at the bound outside, and the gradient matches central finite
+    differences."""
     import mlp_ref as M
     from scipy.special import gammaln
     from scipy.stats import poisson
@@ -50,8 +51,12 @@ def test_mlp_oracle_count_outputs():
     t[:, :nc] = rng.poisson(2.0, (B, nc))
     loss, g = M.loss_and_grad(theta, x, t, n_in, width, depth, ns, n_counts=nc)
     out, _, _ = M.forward(theta, x, n_in, width, depth, ns)
-    lam = np.exp(out[:, :nc])
-    pois = -poisson.logpmf(t[:, :nc], lam) - gammaln(t[:, :nc] + 1)
+    # inside [-30, 8] the Poisson NLL itself; outside, its tangent at the bound
+    o = out[:, :nc]
+    ob = np.clip(o, -30.0, 8.0)
+    assert np.any(o != ob) and np.any(o == ob)   # both branches are exercised
+    pois = (-poisson.logpmf(t[:, :nc], np.exp(ob)) - gammaln(t[:, :nc] + 1)
+            + (np.exp(ob) - t[:, :nc]) * (o - ob))
     lv = np.clip(out[:, ns + nc:2 * ns], -12, 12)
     gau = 0.5 * (lv + (t[:, nc:] - out[:, nc:ns]) ** 2 * np.exp(-lv))
     assert abs(loss - (pois.sum() + gau.sum()) / (B * ns)) < 1e-12 * max(1.0, abs(loss))
