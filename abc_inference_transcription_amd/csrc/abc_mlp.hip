@@ -141,11 +141,18 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
 // C[i] = sum over the S split-K partials P[s][i], in the order s = 0, 1, ...
 __global__ void splitk_reduce_kernel(const float* __restrict__ P, int S, int64_t MN,
                                      float* __restrict__ C) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // four consecutive outputs per thread (MN is a multiple of 4)
+  const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 4;
   if (i >= MN) return;
-  float v = P[i];
-  for (int s = 1; s < S; ++s) v += P[(size_t)s * MN + i];
-  C[i] = v;
+  float4 v = *reinterpret_cast<const float4*>(P + i);
+  for (int s = 1; s < S; ++s) {
+    const float4 w = *reinterpret_cast<const float4*>(P + (size_t)s * MN + i);
+    v.x += w.x;
+    v.y += w.y;
+    v.z += w.z;
+    v.w += w.w;
+  }
+  *reinterpret_cast<float4*>(C + i) = v;
 }
 
 // x (B x n_in FP32, row-major) -> X (B x in_pad bf16) and X^T (in_pad x B).
@@ -221,19 +228,23 @@ __global__ __launch_bounds__(256) void nll_kernel(const float* __restrict__ O,
   if (threadIdx.x == 0) atomicAdd(loss, red[0]);
 }
 
-// bias gradient: row sums of dZ^T (rows x Bpad bf16), one workgroup per row
-__global__ void rowsum_kernel(const bf16* __restrict__ dZT, int Bpad, float* __restrict__ db) {
+// bias gradient: row sums of dZ^T (rows x Bpad bf16), one workgroup per row:
+// 16-byte loads (8 values), then a wave reduction and one across the 4 waves
+__global__ __launch_bounds__(256) void rowsum_kernel(const bf16* __restrict__ dZT, int Bpad,
+                                                     float* __restrict__ db) {
   const int r = blockIdx.x;
+  const bf16x8* row = reinterpret_cast<const bf16x8*>(dZT + (size_t)r * Bpad);
   float s = 0.0f;
-  for (int b = threadIdx.x; b < Bpad; b += blockDim.x) s += (float)dZT[(size_t)r * Bpad + b];
-  __shared__ float red[256];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int q = 128; q > 0; q >>= 1) {
-    if ((int)threadIdx.x < q) red[threadIdx.x] += red[threadIdx.x + q];
-    __syncthreads();
+  for (int v = threadIdx.x; v < Bpad / 8; v += blockDim.x) {
+    const bf16x8 x = row[v];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) s += (float)x[e];
   }
-  if (threadIdx.x == 0) db[r] = red[0];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) db[r] = (red[0] + red[1]) + (red[2] + red[3]);
 }
 
 struct LayerDesc {
@@ -550,7 +561,7 @@ void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const f
                        p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
   }
   if (S > 1)
-    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(g1((int64_t)M * N, 256)), dim3(256), 0,
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(g1((int64_t)M * N / 4, 256)), dim3(256), 0,
                        p->stream, p->splitk_ws, S, (int64_t)M * N, Cf);
 }
 
