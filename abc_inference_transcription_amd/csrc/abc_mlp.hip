@@ -523,7 +523,9 @@ int mfail(abc_mlp* p, int code, const std::string& s) {
 // Workgroups of a gemm grid; splitting K fills the machine when M x N is small
 // (the weight gradients: M, N = layer widths, K = batch).
 int splitk_for(int M, int N, int K) {
-  const int tiles = (M / 128) * (N / (N % 128 == 0 ? 128 : 64));
+  // counted in 64-row tiles, so the split does not depend on the tile rows
+  // chosen below (results do not depend on them either)
+  const int tiles = (M / 64) * (N / (N % 128 == 0 ? 128 : 64));
   int S = 1;
   while (tiles * S < 256 && K % (2 * S * BK) == 0 && K / (2 * S) >= 4 * BK) S *= 2;
   return S;
