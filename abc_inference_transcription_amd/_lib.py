@@ -133,6 +133,7 @@ def _declare_optional(L):
         "abc_mlp_n_params": (I64, [V]),
         "abc_mlp_set_count_outputs": (I32, [V, I32]),
         "abc_mlp_set_tile_rows": (I32, [V, I32]),
+        "abc_mlp_set_tile_cols": (I32, [V, I32]),
         "abc_mlp_n_stats": (I32, [V]),
         "abc_mlp_set_stream": (I32, [V, V]),
         "abc_mlp_synchronize": (I32, [V]),

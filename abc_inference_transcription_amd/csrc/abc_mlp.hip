@@ -29,15 +29,19 @@ namespace {
 
 enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2 };
 
-constexpr int BK = 32, LDSK = BK + 8;  // +8 bf16 pad against bank conflicts
+// K per pipeline stage (two 32-deep MFMA steps) and the LDS row length
+// (+8 bf16 of padding against bank conflicts)
+constexpr int BK = 64, LDSK = BK + 8;
 
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // 256 threads, workgroup tile BM x BN (BM 128 or 64), wave tile BM/2 x BN/2
 // (2 x 2 waves).  blockIdx.z selects a K chunk [z*K, (z+1)*K) of rows with
 // leading dimension ld (split-K: EPI_F32 then writes its partial tile to
-// Cf + z*M*N).  The next K tile is prefetched into registers while the MFMAs
-// run on the current one.
+// Cf + z*M*N).  K advances in stages of BK = 64 through two LDS buffers: while
+// the MFMAs read one stage, the next is loaded from global memory into
+// registers and stored to the other buffer, one barrier per stage.  Every
+// output sums its K in order, 32 at a time, whatever BM.
 template <int BM, int BN, int EPI>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A,
                                                       const bf16* __restrict__ B, int M, int N,
@@ -47,10 +51,10 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
                                                       bf16* __restrict__ CbT,
                                                       const float* __restrict__ Zprev,
                                                       float* __restrict__ Zout) {
-  constexpr int WN = BN / 2, NCB = WN / 16, NBQ = BN / 64;
-  constexpr int WM = BM / 2, NRB = WM / 16, NAQ = BM / 64;
-  __shared__ bf16 As[BM][LDSK];
-  __shared__ bf16 Bs[BN][LDSK];
+  constexpr int WN = BN / 2, NCB = WN / 16, NBQ = BN / 32;
+  constexpr int WM = BM / 2, NRB = WM / 16, NAQ = BM / 32;
+  __shared__ bf16 As[2][BM][LDSK];
+  __shared__ bf16 Bs[2][BN][LDSK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
@@ -61,47 +65,60 @@ __global__ __launch_bounds__(256) void gemm_nt_kernel(const bf16* __restrict__ A
   for (int a = 0; a < NRB; ++a)
 #pragma unroll
     for (int b = 0; b < NCB; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // this thread's slice of a K tile: rows r = c >> 2, 8 columns at (c & 3) * 8
+  // this thread's slice of a stage: rows c >> 3, 8 columns at (c & 7) * 8
   bf16x8 ra[NAQ], rb[NBQ];
   auto load = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < NAQ; ++q) {
       const int c = tid + q * 256;
-      ra[q] = *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + (c >> 2)) * ld + k0 + (c & 3) * 8);
+      ra[q] = *reinterpret_cast<const bf16x8*>(A + (size_t)(m0 + (c >> 3)) * ld + k0 + (c & 7) * 8);
     }
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
       const int c = tid + q * 256;
-      rb[q] = *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + (c >> 2)) * ld + k0 + (c & 3) * 8);
+      rb[q] = *reinterpret_cast<const bf16x8*>(B + (size_t)(n0 + (c >> 3)) * ld + k0 + (c & 7) * 8);
     }
   };
-  load(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+  auto stash = [&](int buf) {
 #pragma unroll
     for (int q = 0; q < NAQ; ++q) {
       const int c = tid + q * 256;
-      *reinterpret_cast<bf16x8*>(&As[c >> 2][(c & 3) * 8]) = ra[q];
+      *reinterpret_cast<bf16x8*>(&As[buf][c >> 3][(c & 7) * 8]) = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < NBQ; ++q) {
       const int c = tid + q * 256;
-      *reinterpret_cast<bf16x8*>(&Bs[c >> 2][(c & 3) * 8]) = rb[q];
+      *reinterpret_cast<bf16x8*>(&Bs[buf][c >> 3][(c & 7) * 8]) = rb[q];
     }
-    __syncthreads();
-    if (k0 + BK < kend) load(k0 + BK);  // in flight while the MFMAs below run
-    bf16x8 af[NRB], bfr[NCB];
+  };
+  load(kbeg);
+  stash(0);
+  __syncthreads();
+  int buf = 0;
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) load(k0 + BK);  // in flight while the MFMAs below run
 #pragma unroll
-    for (int r4 = 0; r4 < NRB; ++r4)
-      af[r4] = *reinterpret_cast<const bf16x8*>(&As[wm * WM + r4 * 16 + (lane & 15)][(lane >> 4) * 8]);
+    for (int ks = 0; ks < BK / 32; ++ks) {
+      bf16x8 af[NRB], bfr[NCB];
 #pragma unroll
-    for (int cb = 0; cb < NCB; ++cb)
-      bfr[cb] = *reinterpret_cast<const bf16x8*>(&Bs[wn * WN + cb * 16 + (lane & 15)][(lane >> 4) * 8]);
-#pragma unroll
-    for (int r4 = 0; r4 < NRB; ++r4)
+      for (int r4 = 0; r4 < NRB; ++r4)
+        af[r4] = *reinterpret_cast<const bf16x8*>(
+            &As[buf][wm * WM + r4 * 16 + (lane & 15)][ks * 32 + (lane >> 4) * 8]);
 #pragma unroll
       for (int cb = 0; cb < NCB; ++cb)
-        acc[r4][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r4], bfr[cb], acc[r4][cb], 0, 0, 0);
+        bfr[cb] = *reinterpret_cast<const bf16x8*>(
+            &Bs[buf][wn * WN + cb * 16 + (lane & 15)][ks * 32 + (lane >> 4) * 8]);
+#pragma unroll
+      for (int r4 = 0; r4 < NRB; ++r4)
+#pragma unroll
+        for (int cb = 0; cb < NCB; ++cb)
+          acc[r4][cb] =
+              __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[r4], bfr[cb], acc[r4][cb], 0, 0, 0);
+    }
+    if (more) stash(buf ^ 1);  // the other buffer: its last readers passed the barrier below
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
   for (int r4 = 0; r4 < NRB; ++r4)
@@ -487,6 +504,7 @@ struct abc_mlp {
   int n_in = 0, in_pad = 0, width = 0, depth = 0, n_out = 0, out_pad = 0, ns = 0, max_batch = 0;
   int n_cnt = 0;  // leading count statistics (Poisson loss on a log-rate output)
   int tile_rows = 0;  // GEMM workgroup rows: 0 = by shape, else 64 or 128
+  int tile_cols = 0;  // GEMM workgroup columns: 0 = by shape, else 64 or 128
   int64_t n_par = 0, n_bf = 0;
   std::vector<LayerDesc> layers;
   std::vector<int64_t> bf_off;
@@ -538,6 +556,12 @@ int tile_rows(int M, int N, int S) {
   return (tiles < 512 && M % 64 == 0) ? 64 : 128;
 }
 
+// Workgroup tile columns (N % 128 == 0): 64 when 128-column tiles would leave
+// fewer than four workgroups per CU.
+int tile_cols(int M, int N, int S, int bm) {
+  return (M / bm) * (N / 128) * S < 1024 ? 64 : 128;
+}
+
 template <int EPI>
 void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const float* bias,
           float* Cf, bf16* Cb, bf16* CbT, const float* Zprev, float* Zout) {
@@ -549,10 +573,11 @@ void gemm(abc_mlp* p, const bf16* A, const bf16* B, int M, int N, int K, const f
     if (S > 1) out = p->splitk_ws;
   }
   const int bm = p->tile_rows ? p->tile_rows : tile_rows(M, N, S);
-  if (N % 128 == 0 && bm == 128) {
+  const int bn = N % 128 != 0 ? 64 : (p->tile_cols ? p->tile_cols : tile_cols(M, N, S, bm));
+  if (bn == 128 && bm == 128) {
     hipLaunchKernelGGL((gemm_nt_kernel<128, 128, EPI>), dim3(M / 128, N / 128, S), dim3(256), 0,
                        p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
-  } else if (N % 128 == 0) {
+  } else if (bn == 128) {
     hipLaunchKernelGGL((gemm_nt_kernel<64, 128, EPI>), dim3(M / 64, N / 128, S), dim3(256), 0,
                        p->stream, A, B, M, N, K / S, K, bias, out, Cb, CbT, Zprev, Zout);
   } else if (bm == 128) {
@@ -724,6 +749,12 @@ int64_t abc_mlp_n_params(const abc_mlp* p) { return p ? p->n_par : ABC_E_ARG; }
 int32_t abc_mlp_set_tile_rows(abc_mlp* p, int32_t rows) {
   if (!p || (rows != 0 && rows != 64 && rows != 128)) return ABC_E_ARG;
   p->tile_rows = rows;
+  return ABC_OK;
+}
+
+int32_t abc_mlp_set_tile_cols(abc_mlp* p, int32_t cols) {
+  if (!p || (cols != 0 && cols != 64 && cols != 128)) return ABC_E_ARG;
+  p->tile_cols = cols;
   return ABC_OK;
 }
 

@@ -102,14 +102,14 @@ def parse():
     ap.add_argument("--ssa-claim", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="SSA work items per queue claim (0 = the autotuner's or the default)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
-    ap.add_argument("--sur-train", type=int, default=100000, help="training particles per model")
+    ap.add_argument("--sur-train", type=int, default=200000, help="training particles per model")
     ap.add_argument("--sur-eval", type=int, default=10000, help="held-out particles per model")
-    ap.add_argument("--sur-steps", type=int, default=10000, help="Adam steps (batch 8192)")
+    ap.add_argument("--sur-steps", type=int, default=15000, help="Adam steps (batch 8192)")
     ap.add_argument("--sur-cal", type=int, default=10000,
                     help="calibration particles per model that set the calibrated pass rules")
     ap.add_argument("--sur-cal-recall", type=float, default=0.992,
                     help="recall of the calibration set's accepted pairs each rule keeps")
-    ap.add_argument("--sur-direct-steps", type=int, default=8000,
+    ap.add_argument("--sur-direct-steps", type=int, default=12000,
                     help="Adam steps of the network trained on accepting-gene counts (0 = skip)")
     ap.add_argument("--ceiling-reps", type=int, default=8,
                     help="independent exact replicates per held-out particle for the bound on "

@@ -365,6 +365,10 @@ int32_t abc_mlp_set_count_outputs(abc_mlp* mlp, int32_t n_counts);
  * would give fewer than two workgroups per CU, else 128; 64 or 128 force it.
  * A launch shape only: the results do not depend on it. */
 int32_t abc_mlp_set_tile_rows(abc_mlp* mlp, int32_t rows);
+/* Columns of the GEMM workgroup tile for outputs of a multiple of 128
+ * columns: 0 (default) picks 64 when 128-column tiles would give fewer than
+ * four workgroups per CU, else 128; 64 or 128 force it.  A launch shape only. */
+int32_t abc_mlp_set_tile_cols(abc_mlp* mlp, int32_t cols);
 int32_t abc_mlp_n_stats(const abc_mlp* mlp);
 int32_t abc_mlp_set_stream(abc_mlp* mlp, void* hip_stream);
 int32_t abc_mlp_synchronize(abc_mlp* mlp);

@@ -107,21 +107,24 @@ def test_mlp_training_reduces_nll(torch):
 
 
 def test_gemm_tile_rows_change_speed_not_results(torch):
-    """64- and 128-row workgroup tiles accumulate every output in the same
-    order: identical forward outputs and identical parameters after training."""
+    """64- and 128-row and -column workgroup tiles accumulate every output in
+    the same order: identical forward outputs and identical parameters after
+    training."""
     rng = np.random.default_rng(9)
     n = 4096
     x = torch.tensor(rng.uniform(-1, 1, (n, 23)).astype(np.float32), device="cuda:0")
     t = torch.tensor(rng.normal(0, 1, (n, 90)).astype(np.float32), device="cuda:0")
     outs = []
-    for rows in (128, 64):
+    for rows, cols in ((128, 128), (64, 128), (64, 64), (128, 64), (0, 0)):
         s = make(torch, max_batch=2048)
         assert s.L.abc_mlp_set_tile_rows(s.h, rows) == 0
+        assert s.L.abc_mlp_set_tile_cols(s.h, cols) == 0
         s.train(x, t, steps=12, batch=2048)
         outs.append((s.get_params(), s.predict(x).cpu().numpy()))
         s.close()
-    assert np.array_equal(outs[0][0].view(np.int32), outs[1][0].view(np.int32))
-    assert np.array_equal(outs[0][1].view(np.int32), outs[1][1].view(np.int32))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0][0].view(np.int32), o[0].view(np.int32))
+        assert np.array_equal(outs[0][1].view(np.int32), o[1].view(np.int32))
     s = make(torch)
     assert s.L.abc_mlp_set_tile_rows(s.h, 96) != 0
     s.close()
