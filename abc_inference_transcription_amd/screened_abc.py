@@ -126,7 +126,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
         cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.5, direct_nets=2,
-        log=print):
+        ceiling_particles=0, log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -274,9 +274,12 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
                                            direct=direct)
         by_z = by_z + out["by_rule"]
     if ceiling_reps and mode == 0:
+        # the first ceiling_particles of each model's held-out set (0: all)
+        nc = min(n_eval, ceiling_particles) if ceiling_particles > 0 else n_eval
+        th_c = {m: eval_sets[m][0].view(-1, n_eval)[:, :nc].contiguous().view(-1) for m in MODELS}
         out["ceiling"] = acceptance_ceiling(
-            eng, data, se, parts, eps, {m: eval_sets[m][0] for m in MODELS},
-            {m: eval_first(m) for m in MODELS}, reps=ceiling_reps, log=log)
+            eng, data, se, parts, eps, th_c, {m: eval_first(m) for m in MODELS},
+            reps=ceiling_reps, log=log)
         log(f"[screened] ceiling: {out['ceiling']}")
     # the headline: calibrated rules when there are any (their thresholds never
     # saw the evaluation particles), else the best z of the grid

@@ -103,9 +103,9 @@ def parse():
                     help="SSA work items per queue claim (0 = the autotuner's or the default)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay of the steps")
     ap.add_argument("--sur-train", type=int, default=200000, help="training particles per model")
-    ap.add_argument("--sur-eval", type=int, default=10000, help="held-out particles per model")
+    ap.add_argument("--sur-eval", type=int, default=40000, help="held-out particles per model")
     ap.add_argument("--sur-steps", type=int, default=15000, help="Adam steps (batch 8192)")
-    ap.add_argument("--sur-cal", type=int, default=10000,
+    ap.add_argument("--sur-cal", type=int, default=40000,
                     help="calibration particles per model that set the calibrated pass rules")
     ap.add_argument("--sur-cal-recall", type=float, default=0.992,
                     help="recall of the calibration set's accepted pairs each rule keeps")
@@ -114,6 +114,9 @@ def parse():
     ap.add_argument("--ceiling-reps", type=int, default=8,
                     help="independent exact replicates per held-out particle for the bound on "
                          "what any parameter-only screen can gain (0 = skip)")
+    ap.add_argument("--ceiling-particles", type=int, default=10000,
+                    help="held-out particles per model (the first ones) the replicates cover "
+                         "(0 = all)")
     ap.add_argument("--no-moments", action="store_true",
                     help="skip the leg of the reference's own algorithm (moment model)")
     ap.add_argument("--mom-particles", type=int, default=32768,
@@ -402,7 +405,7 @@ def surrogate_leg(eng, stream, a):
     res = S.run(eng, sur, G=a.genes, n_train=a.sur_train, n_eval=a.sur_eval, k=0,
                 accept_rate=4e-5, z=[3.0, 2.0], train_steps=a.sur_steps, batch=8192,
                 ceiling_reps=a.ceiling_reps, n_cal=a.sur_cal, cal_recall=a.sur_cal_recall,
-                direct_steps=a.sur_direct_steps,
+                direct_steps=a.sur_direct_steps, ceiling_particles=a.ceiling_particles,
                 log=lambda *x: print(*x, file=sys.stderr, flush=True))
     sur.close()
     keep = ("plain_accepted_per_s", "screened_accepted_per_s", "speedup_accepted_per_s",
