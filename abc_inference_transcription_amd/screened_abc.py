@@ -503,18 +503,37 @@ def acceptance_ceiling(eng, data, se, parts, eps, thetas, firsts, reps=16, seed0
     p = torch.cat(counts, dim=1).to(torch.float64) / reps          # G x n_all
     s_i = p.sum(0).cpu().numpy()
     w_i = torch.cat(works).cpu().numpy()
-    out = {"reps": reps, "particles": int(len(s_i)),
+    out = {"reps": reps}
+    out.update(ceiling_shares(s_i, w_i, recalls))
+    return out
+
+
+def ceiling_shares(s_i, w_i, recalls=(0.99, 0.995)):
+    """From expected accepted pairs s_i and costs w_i (>= 1 each) per
+    particle: the smallest share of the particles whose pairs reach each
+    recall R (particles in decreasing s_i), and the share of the work in
+    decreasing s_i / w_i, the greedy order of the fractional knapsack: no
+    subset of particles reaching R costs less than that share, so its
+    inverse bounds the gain in pairs per unit of work."""
+    s_i = np.asarray(s_i, np.float64)
+    w_i = np.maximum(np.asarray(w_i, np.float64), 1.0)
+    out = {"particles": int(len(s_i)),
            "pairs_per_particle": float(s_i.mean()),
            "share_with_any_acceptance": float((s_i > 0).mean())}
-    # by expected pairs (fewest particles), and by expected pairs per exact
-    # step (least exact work: the greedy order of the fractional knapsack)
+    total = max(float(s_i.sum()), 1e-300)
     for key, order in (("", np.argsort(-s_i, kind="stable")),
-                       ("work_", np.argsort(-(s_i / np.maximum(w_i, 1.0)), kind="stable"))):
-        cum = np.cumsum(s_i[order]) / max(s_i.sum(), 1e-300)
-        wcum = np.cumsum(w_i[order]) / max(w_i.sum(), 1e-300)
+                       ("work_", np.argsort(-(s_i / w_i), kind="stable"))):
+        cum = np.cumsum(s_i[order]) / total
+        wcum = np.cumsum(w_i[order]) / w_i.sum()
         for R in recalls:
-            r = int(np.searchsorted(cum, R))
-            f = (r + 1) / len(s_i) if not key else float(wcum[r])
+            r = min(int(np.searchsorted(cum, R)), len(s_i) - 1)
+            if key:
+                # the fractional share: the last particle only in part
+                need = R * total - (float(np.sum(s_i[order[:r]])))
+                part = min(max(need / s_i[order[r]], 0.0), 1.0) if s_i[order[r]] > 0 else 1.0
+                f = (float(wcum[r - 1]) if r > 0 else 0.0) + part * w_i[order[r]] / w_i.sum()
+            else:
+                f = (r + 1) / len(s_i)
             out[f"{key}share_needed_{R}"] = f
             out[f"{key}gain_bound_{R}"] = 1.0 / f
     return out
