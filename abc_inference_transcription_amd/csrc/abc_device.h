@@ -126,30 +126,43 @@ ABC_DEV float exp1_32(uint32_t w, const LogEntry32* tab) {
   return -(fmaf(kf, 0x1.62e430p-1f, e.l) + fmaf(kf, -0x1.05c610p-29f, lp));
 }
 
-ABC_DEV double ex(double x) {
-  if (x != x) return x;
-  if (x > 709.78) return __longlong_as_double(0x7ff0000000000000ll);
+// v_fma_f64 with its own destination: the compiler's two-address form
+// (v_fmac_f64) writes over the addend and so copies a constant addend first
+ABC_DEV double fma3(double a, double b, double c) {
+  double d;
+  asm volatile("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+}
+
+// e^x for x <= 709.78 (or NaN): Cody-Waite reduction x = k ln2 + r with
+// ln2 split in two, the degree-13 Taylor polynomial of e^r, then the scaling
+// by 2^k as one ldexp (exact, or one rounding into the subnormal range)
+ABC_DEV double ex_core(double x) {
   if (x < -745.2) return 0.0;
   const double kd = floor(x * 0x1.71547652b82fep+0 + 0.5);
   const int k = (int)kd;
   const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
   double p = 0x1.6124613a86d09p-33;
-  p = fma(p, r, 0x1.1eed8eff8d898p-29);
-  p = fma(p, r, 0x1.ae64567f544e4p-26);
-  p = fma(p, r, 0x1.27e4fb7789f5cp-22);
-  p = fma(p, r, 0x1.71de3a556c734p-19);
-  p = fma(p, r, 0x1.a01a01a01a01ap-16);
-  p = fma(p, r, 0x1.a01a01a01a01ap-13);
-  p = fma(p, r, 0x1.6c16c16c16c17p-10);
-  p = fma(p, r, 0x1.1111111111111p-7);
-  p = fma(p, r, 0x1.5555555555555p-5);
-  p = fma(p, r, 0x1.5555555555555p-3);
+  p = fma3(p, r, 0x1.1eed8eff8d898p-29);
+  p = fma3(p, r, 0x1.ae64567f544e4p-26);
+  p = fma3(p, r, 0x1.27e4fb7789f5cp-22);
+  p = fma3(p, r, 0x1.71de3a556c734p-19);
+  p = fma3(p, r, 0x1.a01a01a01a01ap-16);
+  p = fma3(p, r, 0x1.a01a01a01a01ap-13);
+  p = fma3(p, r, 0x1.6c16c16c16c17p-10);
+  p = fma3(p, r, 0x1.1111111111111p-7);
+  p = fma3(p, r, 0x1.5555555555555p-5);
+  p = fma3(p, r, 0x1.5555555555555p-3);
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  if (k < -1000) return (p * __longlong_as_double((int64_t)(k + 1000 + 1023) << 52)) * 0x1p-1000;
-  if (k > 1000) return (p * __longlong_as_double((int64_t)(k - 1000 + 1023) << 52)) * 0x1p1000;
-  return p * __longlong_as_double((int64_t)(k + 1023) << 52);
+  return ldexp(p, k);
+}
+
+ABC_DEV double ex(double x) {
+  if (x != x) return x;
+  if (x > 709.78) return __longlong_as_double(0x7ff0000000000000ll);
+  return ex_core(x);
 }
 
 ABC_DEV double ex10(double x) { return ex(x * 0x1.26bb1bbb55516p+1); }

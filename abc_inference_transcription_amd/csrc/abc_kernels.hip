@@ -410,7 +410,8 @@ __device__ __forceinline__ void seg_int(double k, double ik, double D, double e,
 __device__ __forceinline__ void burn_accum(double& lu, double& lt, bool g, double c0, double c1,
                                            double ku, double iku, double kt, double ikt, double D) {
   if (!(D > 0.0)) return;
-  const double eu = ex(-(ku * D)), et = ex(-(kt * D));
+  // arguments <= 0 (D > 0, rates > 0): no NaN or overflow test
+  const double eu = ex_core(-(ku * D)), et = ex_core(-(kt * D));
   lu = lu * eu;
   lt = lt * et;
   if (g) {

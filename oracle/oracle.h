@@ -36,6 +36,7 @@ double orc_u01_53(uint32_t a, uint32_t b);
 double orc_log(double x);
 float orc_exp1_32(uint32_t w);   /* -ln(((float)w + 0.5f) 2^-32), single precision */
 double orc_exp(double x);
+int64_t orc_exp_scaling_mismatches(double lo, double hi, int64_t n, uint64_t seed);
 double orc_exp10(double x);
 double orc_u128_to_f64(uint64_t hi, uint64_t lo);
 

@@ -67,6 +67,8 @@ def _declare(L):
     for f in ("orc_log", "orc_exp", "orc_exp10"):
         getattr(L, f).argtypes = [D]; getattr(L, f).restype = D
     L.orc_u128_to_f64.argtypes = [U64, U64]; L.orc_u128_to_f64.restype = D
+    L.orc_exp_scaling_mismatches.argtypes = [D, D, I64, U64]
+    L.orc_exp_scaling_mismatches.restype = I64
     L.orc_exp1_32.argtypes = [C.c_uint32]; L.orc_exp1_32.restype = C.c_float
     L.orc_n_params.argtypes = [C.c_int]; L.orc_n_params.restype = C.c_int
     L.orc_sample_prior.argtypes = [P, C.c_int, U64, I64, I64, P]
@@ -125,6 +127,12 @@ def log(x):
 
 def exp(x):
     return lib().orc_exp(float(x))
+
+
+def exp_scaling_mismatches(lo, hi, n, seed=1):
+    """x in [lo, hi] where scaling e^r by 2^k with one ldexp (the device
+    code) gives other bits than the oracle's two-step scaling."""
+    return int(lib().orc_exp_scaling_mismatches(float(lo), float(hi), int(n), int(seed)))
 
 
 def exp1_32(w):

@@ -116,10 +116,9 @@ float orc_exp1_32(uint32_t w) {
   return -(fmaf(kf, 0x1.62e430p-1f, abc_log32_l[j]) + fmaf(kf, -0x1.05c610p-29f, lp));
 }
 
-double orc_exp(double x) {
-  if (x != x) return x;
-  if (x > 709.78) return INFINITY;
-  if (x < -745.2) return 0.0;
+/* e^r of the reduction x = k ln2 + r (ln2 in two parts): the degree-13
+   Taylor polynomial, evaluated by Horner's rule in FMAs */
+static double exp_reduced(double x, int* k_out) {
   double kd = floor(x * INV_LN2 + 0.5);
   int k = (int)kd;
   double r = (x - kd * LN2_HI) - kd * LN2_LO;
@@ -137,7 +136,18 @@ double orc_exp(double x) {
   p = fma(p, r, 0.5);                           /* 1/2!  */
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
-  /* scale by 2^k in two exact steps so subnormal results stay exact-ish */
+  *k_out = k;
+  return p;
+}
+
+double orc_exp(double x) {
+  if (x != x) return x;
+  if (x > 709.78) return INFINITY;
+  if (x < -745.2) return 0.0;
+  int k;
+  double p = exp_reduced(x, &k);
+  /* scale by 2^k in two steps: the first exact, the second rounding once
+     where the result is subnormal */
   if (k < -1000) {
     double s1 = bitsd((uint64_t)(k + 1000 + 1023) << 52);
     return (p * s1) * 0x1p-1000;
@@ -147,6 +157,30 @@ double orc_exp(double x) {
     return (p * s1) * 0x1p1000;
   }
   return p * bitsd((uint64_t)(k + 1023) << 52);
+}
+
+/* The device code scales by 2^k with one ldexp, which is exact or rounds
+   once into the subnormal range like the second step above.  Returns how
+   many of n pseudo-random x in [lo, hi] (xorshift64 from seed) give
+   different bits under the two scalings. */
+int64_t orc_exp_scaling_mismatches(double lo, double hi, int64_t n, uint64_t seed) {
+  uint64_t s = seed ? seed : 1u;
+  int64_t bad = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    s ^= s << 13;
+    s ^= s >> 7;
+    s ^= s << 17;
+    const double x = lo + (hi - lo) * ((double)(s >> 11) * 0x1p-53);
+    if (x < -745.2 || x > 709.78) continue;
+    int k;
+    const double p = exp_reduced(x, &k);
+    const double a = orc_exp(x), b = ldexp(p, k);
+    uint64_t ua, ub;
+    memcpy(&ua, &a, 8);
+    memcpy(&ub, &b, 8);
+    bad += ua != ub;
+  }
+  return bad;
 }
 
 double orc_exp10(double x) { return orc_exp(x * LN10); }

@@ -153,6 +153,17 @@ def test_exp_accuracy(oracle):
     assert oracle.exp10(0.0) == 1.0
 
 
+def test_exp_scaling_by_one_ldexp(oracle):
+    """The device exp scales by 2^k with one ldexp; the oracle in two steps.
+    Bit for bit the same over the subnormal results, the normal range and
+    small arguments."""
+    assert oracle.exp_scaling_mismatches(-745.2, -700.0, 2_000_000, 3) == 0
+    assert oracle.exp_scaling_mismatches(-760.0, 710.0, 2_000_000, 5) == 0
+    assert oracle.exp_scaling_mismatches(-1e-3, 1e-3, 500_000, 7) == 0
+    # the subnormal range is reached: e^-745 ~ 5e-324, e^-708.4 ~ 2.2e-308
+    assert 0.0 < oracle.exp(-745.0) < 2.3e-308 and oracle.exp(-708.5) < 2.3e-308
+
+
 def test_u128_correct_rounding(oracle):
     rng = random.Random(3)
     vals = [0, 1, 2**53 - 1, 2**53, 2**53 + 1, 2**53 + 3, 2**64 - 1, 2**64, 2**127 + 2**74,
