@@ -136,33 +136,42 @@ ABC_DEV double fma3(double a, double b, double c) {
 
 // e^x for x <= 709.78 (or NaN): Cody-Waite reduction x = k ln2 + r with
 // ln2 split in two, the degree-13 Taylor polynomial of e^r, then the scaling
-// by 2^k as one ldexp (exact, or one rounding into the subnormal range)
-ABC_DEV double ex_core(double x) {
+// by 2^k as one ldexp (exact, or one rounding into the subnormal range).
+// F: the fused multiply-add of the Horner chain (fma, or fma3 where the
+// constants may stay in registers)
+template <double (*F)(double, double, double)>
+ABC_DEV double ex_core_with(double x) {
   if (x < -745.2) return 0.0;
   const double kd = floor(x * 0x1.71547652b82fep+0 + 0.5);
   const int k = (int)kd;
   const double r = (x - kd * 0x1.62e42fee00000p-1) - kd * 0x1.a39ef35793c76p-33;
   double p = 0x1.6124613a86d09p-33;
-  p = fma3(p, r, 0x1.1eed8eff8d898p-29);
-  p = fma3(p, r, 0x1.ae64567f544e4p-26);
-  p = fma3(p, r, 0x1.27e4fb7789f5cp-22);
-  p = fma3(p, r, 0x1.71de3a556c734p-19);
-  p = fma3(p, r, 0x1.a01a01a01a01ap-16);
-  p = fma3(p, r, 0x1.a01a01a01a01ap-13);
-  p = fma3(p, r, 0x1.6c16c16c16c17p-10);
-  p = fma3(p, r, 0x1.1111111111111p-7);
-  p = fma3(p, r, 0x1.5555555555555p-5);
-  p = fma3(p, r, 0x1.5555555555555p-3);
+  p = F(p, r, 0x1.1eed8eff8d898p-29);
+  p = F(p, r, 0x1.ae64567f544e4p-26);
+  p = F(p, r, 0x1.27e4fb7789f5cp-22);
+  p = F(p, r, 0x1.71de3a556c734p-19);
+  p = F(p, r, 0x1.a01a01a01a01ap-16);
+  p = F(p, r, 0x1.a01a01a01a01ap-13);
+  p = F(p, r, 0x1.6c16c16c16c17p-10);
+  p = F(p, r, 0x1.1111111111111p-7);
+  p = F(p, r, 0x1.5555555555555p-5);
+  p = F(p, r, 0x1.5555555555555p-3);
   p = fma(p, r, 0.5);
   p = fma(p, r, 1.0);
   p = fma(p, r, 1.0);
   return ldexp(p, k);
 }
 
+ABC_DEV double fma2(double a, double b, double c) { return fma(a, b, c); }
+
+// in loops that evaluate it every iteration: the polynomial's constants stay
+// in registers and each step is one instruction
+ABC_DEV double ex_core(double x) { return ex_core_with<fma3>(x); }
+
 ABC_DEV double ex(double x) {
   if (x != x) return x;
   if (x > 709.78) return __longlong_as_double(0x7ff0000000000000ll);
-  return ex_core(x);
+  return ex_core_with<fma2>(x);
 }
 
 ABC_DEV double ex10(double x) { return ex(x * 0x1.26bb1bbb55516p+1); }
