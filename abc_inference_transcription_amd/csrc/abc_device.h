@@ -130,7 +130,7 @@ ABC_DEV float exp1_32(uint32_t w, const LogEntry32* tab) {
 // (v_fmac_f64) writes over the addend and so copies a constant addend first
 ABC_DEV double fma3(double a, double b, double c) {
   double d;
-  asm volatile("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
   return d;
 }
 

@@ -517,6 +517,8 @@ def ceiling_shares(s_i, w_i, recalls=(0.99, 0.995)):
     inverse bounds the gain in pairs per unit of work."""
     s_i = np.asarray(s_i, np.float64)
     w_i = np.maximum(np.asarray(w_i, np.float64), 1.0)
+    if len(s_i) == 0:
+        raise ValueError("ceiling_shares: no particles")
     out = {"particles": int(len(s_i)),
            "pairs_per_particle": float(s_i.mean()),
            "share_with_any_acceptance": float((s_i > 0).mean())}
