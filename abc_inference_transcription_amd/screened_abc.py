@@ -126,7 +126,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
         z=4.0, margin=0.0,
         train_steps=1500, batch=8192, lr=1e-3, seed=0, ceiling_reps=0, n_cal=0,
         cal_recall=0.995, direct_steps=0, direct_loss="poisson", fixed_share=0.5, direct_nets=2,
-        ceiling_particles=0, log=print):
+        ceiling_particles=0, direct_aux=(1.5, 3.0), log=print):
     """Returns a dict of metrics (see module docstring)."""
     import torch
     dev = torch.device("cuda:0")
@@ -197,7 +197,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     if direct_steps and mode == 0:
         direct = _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps,
                                n_train, direct_steps, batch, lr, seed, dev, log, direct_loss,
-                               fixed_share, direct_nets)
+                               fixed_share, direct_nets, direct_aux)
         out["direct_train_s"] = direct.train_s
 
     # ---- exhaustive plain path on the evaluation set (timed)
@@ -324,19 +324,22 @@ class _Direct:
     (trained from different initial weights and batch orders) average their
     outputs.  Trained for one gene set."""
 
-    def __init__(self, nets, scale, lmu, lsd, train_s, poisson, c0):
+    def __init__(self, nets, scale, lmu, lsd, train_s, poisson, c0, n_aux=0):
         self.nets, self.scale, self.lmu, self.lsd = nets, scale, lmu, lsd
         self.train_s, self.poisson, self.c0 = train_s, poisson, c0
+        self.n_aux = n_aux
 
     def close(self):
         for n in self.nets:
             n.close()
 
     def _y(self, x):
-        y = self.nets[0].predict(x)[:, :2]
+        """Averaged (count output, standardised log steps) per particle."""
+        cols = [0, 1 + self.n_aux]
+        y = self.nets[0].predict(x)[:, cols]
         for n in self.nets[1:]:
-            y = y + n.predict(x)[:, :2]
-        return y / len(self.nets)
+            y = y + n.predict(x)[:, cols]
+        return (y / len(self.nets)).contiguous()
 
     def _count(self, y):
         import torch
@@ -358,40 +361,52 @@ class _Direct:
 
 
 def _train_direct(eng, sur, L, data, se, parts, KS, G, eps, xs, ts_raw, ts_steps, n_train, steps,
-                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.5, n_nets=2):
+                  batch, lr, seed, dev, log, loss="poisson", fixed_share=0.5, n_nets=2,
+                  aux=(1.5, 3.0)):
+    """Count networks: outputs (accepting genes at eps, at eps * a for each a
+    in aux, standardised log steps).  The auxiliary counts at looser
+    thresholds are denser targets (about 4x and 7x as many training
+    particles with a count at 1.5 and 3 eps) that shape the shared hidden layers; only
+    the count at eps and the steps are read (Poisson counts only)."""
     import torch
     from .surrogate import Surrogate
     data_d = torch.tensor(np.asfortranarray(data).ravel(order="F"), dtype=torch.float64,
                           device=dev)
     se_d = torch.tensor(np.asfortranarray(se).ravel(order="F"), dtype=torch.float64, device=dev)
-    cnt = torch.cat([_accepting_genes(eng, L, s, n_train, data_d, se_d, parts, KS, G, eps, dev)
-                     for s in ts_raw]).to(torch.float32)
     poisson = loss == "poisson"
+    aux = tuple(aux) if poisson else ()
+    cnts = [torch.cat([_accepting_genes(eng, L, s, n_train, data_d, se_d, parts, KS, G, eps * a,
+                                        dev) for s in ts_raw]).to(torch.float32)
+            for a in (1.0,) + aux]
+    cnt = cnts[0]
     scale = 1.0 if poisson else float(max(cnt.std().item(), 1e-3))
     lw = torch.log(torch.cat(ts_steps).to(torch.float64).clamp(min=1.0)).to(torch.float32)
     lmu, lsd = float(lw.mean().item()), float(max(lw.std().item(), 1e-3))
     c0 = fixed_share * float(torch.cat(ts_steps).to(torch.float64).mean().item())
-    tgt = torch.stack([cnt / scale, (lw - lmu) / lsd], dim=1).contiguous()
+    nc = len(cnts)
+    tgt = torch.stack([cnt / scale] + cnts[1:] + [(lw - lmu) / lsd], dim=1).contiguous()
     x_all = torch.cat(xs)
     nets = []
     t0 = time.perf_counter()
     for q in range(n_nets):
-        net = Surrogate(2, n_bins=eng.nb, max_batch=sur.max_batch, seed=seed + 17 + 1000 * q,
-                        stream=sur.stream)
+        net = Surrogate(nc + 1, n_bins=eng.nb, max_batch=sur.max_batch,
+                        seed=seed + 17 + 1000 * q, stream=sur.stream)
         if poisson:
-            net.set_count_outputs(1)
-            # start the log-rate at the mean count: small first gradients
+            net.set_count_outputs(nc)
+            # start each log-rate at its mean count: small first gradients
             w = net.get_params()
-            out_pad = (2 * 2 + 127) // 128 * 128
-            w[len(w) - out_pad] = np.log(max(float(cnt.mean().item()), 1e-3))
+            out_pad = (2 * (nc + 1) + 127) // 128 * 128
+            for i in range(nc):
+                w[len(w) - out_pad + i] = np.log(max(float(cnts[i].mean().item()), 1e-3))
             net.set_params(w)
         losses = net.train(x_all, tgt, steps=steps, batch=batch, lr=lr, seed=seed + 1 + 1000 * q)
         nets.append(net)
     train_s = time.perf_counter() - t0
     log(f"[screened] direct network: {int((cnt > 0).sum().item())} of {len(cnt)} training "
-        f"particles accepted by some gene; loss {losses[:10].mean():.3f} -> "
+        f"particles accepted by some gene (auxiliary thresholds x{list(aux)}: "
+        f"{[int((c > 0).sum().item()) for c in cnts[1:]]}); loss {losses[:10].mean():.3f} -> "
         f"{losses[-50:].mean():.3f} ({train_s:.1f}s)")
-    return _Direct(nets, scale, lmu, lsd, train_s, poisson, c0)
+    return _Direct(nets, scale, lmu, lsd, train_s, poisson, c0, n_aux=len(aux))
 
 
 def _calibrated_rules(eng, sur, L, data, se, parts, KS, G, eps, n_cal, cal_recall, eval_sets,
