@@ -551,8 +551,8 @@ def ceiling_shares(s_i, w_i, recalls=(0.99, 0.995)):
                 f = (float(wcum[r - 1]) if r > 0 else 0.0) + part * w_i[order[r]] / w_i.sum()
             else:
                 f = (r + 1) / len(s_i)
-            out[f"{key}share_needed_{R}"] = f
-            out[f"{key}gain_bound_{R}"] = 1.0 / f
+            out[f"{key}share_needed_{R}"] = float(f)
+            out[f"{key}gain_bound_{R}"] = 1.0 / float(f)
     return out
 
 
