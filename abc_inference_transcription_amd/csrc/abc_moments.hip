@@ -202,11 +202,11 @@ ABC_DEV int64_t dopri_piece(const Piece& pc, const Particle& P, const MomCfgDev&
         y[i] = yn[i];
         k1[i] = k7[i];
       }
-      double fac = err == 0.0 ? 5.0 : 0.9 * ex(-0.2 * ln(err, ltab));
+      double fac = err == 0.0 ? 5.0 : 0.9 * ex_core(-0.2 * ln(err, ltab));
       if (fac > 5.0) fac = 5.0;
       h *= fac;
     } else {
-      double fac = 0.9 * ex(-0.2 * ln(err, ltab));
+      double fac = 0.9 * ex_core(-0.2 * ln(err, ltab));
       if (fac < 0.2) fac = 0.2;
       h *= fac;
     }
@@ -289,8 +289,8 @@ ABC_DEV void gb_setup(GBlock& q, const double* p, double al1, double lam, const 
 
 // the gene block at a + s into y[0], y[3], y[4], y[5]
 ABC_DEV void gb_eval(const GBlock& q, double s, double* y) {
-  const double e1 = ex(-(q.K * s));
-  const double eg = ex(-(q.gam * s));
+  const double e1 = ex_core(-(q.K * s));
+  const double eg = ex_core(-(q.gam * s));
   const double e2 = e1 * e1;
   const double en = e1 * eg;
   double E10, E20, E1K, E2K, E12, E22;
@@ -393,11 +393,11 @@ ABC_DEV int64_t egb_piece(const Piece& pc, const Particle& P, const MomCfgDev& c
         y[i] = yn[i];
         k1[i] = k7[i];
       }
-      double fac = err == 0.0 ? 5.0 : 0.9 * ex(-0.2 * ln(err, ltab));
+      double fac = err == 0.0 ? 5.0 : 0.9 * ex_core(-0.2 * ln(err, ltab));
       if (fac > 5.0) fac = 5.0;
       h *= fac;
     } else {
-      double fac = 0.9 * ex(-0.2 * ln(err, ltab));
+      double fac = 0.9 * ex_core(-0.2 * ln(err, ltab));
       if (fac < 0.2) fac = 0.2;
       h *= fac;
     }
