@@ -127,6 +127,7 @@ def _declare_optional(L):
         "abc_simulate_idx_dev": (I32, [V, I32, U64, I64, V, V, V, V, V, V]),
         "abc_simulate_noalloc_dev": (I32, [V, I32, U64, I64, I64, V, V, V, V, V]),
         "abc_screen_errors": (I32, [V, I64, I32, V, I64, V, V, V, I32, D, V, V]),
+        "abc_math_probe_dev": (I32, [V, I32, I64, V, V]),
         "abc_mlp_create": (I32, [C.POINTER(V), I32, I32, I32, I32, I32, I32, U64]),
         "abc_mlp_destroy": (I32, [V]),
         "abc_mlp_last_error": (C.c_char_p, [V]),

@@ -692,6 +692,17 @@ int32_t abc_sample_prior(abc_ctx* c, int32_t model, uint64_t seed, int64_t first
   return ABC_OK;
 }
 
+int32_t abc_math_probe_dev(abc_ctx* c, int32_t kind, int64_t n, const double* x_dev,
+                           double* y_dev) {
+  if (!c || kind < 0 || kind > 2 || n < 0 || (n > 0 && (!x_dev || !y_dev))) return ABC_E_ARG;
+  if (n == 0) return ABC_OK;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(math_probe_kernel, dim3((unsigned)blocks), dim3(256), 0, c->stream, kind, n,
+                     x_dev, y_dev);
+  HIPCHK(c, hipGetLastError());
+  return ABC_OK;
+}
+
 int32_t abc_simulate_dev(abc_ctx* c, int32_t model, uint64_t seed, int64_t first, int64_t n,
                          const double* theta_dev, double* stats_dev, double* moments_dev,
                          double* superset_dev, int64_t* steps_dev) {

@@ -203,6 +203,7 @@ __global__ void accept_fp64_kernel(const double* stats, int64_t n, const double*
                                    double eps, int64_t first, Candidate* cand, unsigned long long* count,
                                    unsigned long long cap);
 __global__ void zero_u64_kernel(unsigned long long* p, int64_t n);
+__global__ void math_probe_kernel(int kind, int64_t n, const double* x, double* y);
 
 // screening (abc_gemm.hip)
 __global__ void gene_prep_kernel(const double* data, const double* se, int64_t G, int K, int Kp,

@@ -1263,6 +1263,22 @@ __global__ __launch_bounds__(256) void accept_fp64_kernel(const double* __restri
   }
 }
 
+// The device's elementary functions on n inputs, for validation: kind 0 =
+// ex (every kernel except the loops below), 1 = ex_core (the burn-in and the
+// moment model's loops; arguments <= 709.78), 2 = ln (LDS table)
+__global__ __launch_bounds__(256) void math_probe_kernel(int kind, int64_t n,
+                                                         const double* __restrict__ x,
+                                                         double* __restrict__ y) {
+  __shared__ LogEntry ltab[ABC_LOG_N];
+  load_log_table(ltab);
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[i];
+    y[i] = kind == 0 ? ex(v) : (kind == 1 ? ex_core(v) : ln(v, ltab));
+  }
+}
+
 __global__ void zero_u64_kernel(unsigned long long* __restrict__ p, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;

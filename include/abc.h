@@ -153,6 +153,14 @@ int32_t abc_screen_errors(abc_ctx* ctx, int64_t n, int32_t K, const double* stat
                           const double* data, const double* se, const int32_t* parts,
                           int32_t n_parts, double clamp, float* approx, float* bound);
 
+/* The device's elementary functions on n inputs (device pointers x, y), for
+ * validation against the CPU restatement: kind 0 = e^x as most kernels take
+ * it, 1 = e^x as the burn-in and moment-model loops take it (arguments up to
+ * 709.78; three-operand FMA chain), 2 = ln x (table-driven).  No counterpart
+ * in the reference: the tests use it to compare the device's bits with the
+ * oracle's, the subnormal range included. */
+int32_t abc_math_probe_dev(abc_ctx* ctx, int32_t kind, int64_t n, const double* x, double* y);
+
 /* ---- device-resident variants (benchmarks, pipelines that keep data in HBM) */
 int32_t abc_set_stream(abc_ctx* ctx, void* hip_stream);   /* NULL = own stream */
 int32_t abc_synchronize(abc_ctx* ctx);
