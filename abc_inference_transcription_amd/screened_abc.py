@@ -173,7 +173,7 @@ def run(eng, sur, G=2000, n_train=6000, n_eval=2000, k=0, eps=4.8, accept_rate=4
     # ---- acceptance threshold at the target selectivity: the FP64 errors of
     # the (already simulated) training particles against the genes give the
     # epsilon at which a fraction `accept_rate` of (particle, gene) pairs pass
-    # (config 2 accepts 1,000 of 2.5e7 particles per gene: 4e-5)
+    # (a sweep that keeps the best 1,000 of 2.5e7 particles per gene accepts 4e-5)
     if accept_rate is not None and mode == 0:
         errs = []
         data_d = torch.tensor(data.ravel(order="F"), dtype=torch.float64, device=dev)
