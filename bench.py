@@ -702,35 +702,39 @@ def moment_cpu(models, cfg, w, is_chase, cap, budget_s, method):
 
 def cpu_baseline(design, models, P, gpu_steps_per_particle, budget_s):
     """The oracle's scalar replay of the same SSA (the identical Philox steps)
-    on the host cores, over a bounded sample of the benchmark's particles:
-    the same particles and cell design but 100 cells each, so the sample stays
-    within the time budget.  steps/s is then scaled to particles/s through the
-    benchmark's measured mean steps per 2,500-cell particle."""
+    on the host cores, over a bounded sample of the benchmark's own particles
+    (the first ones of each model's stream, the benchmark's design and cells,
+    cells spread over the threads), for about budget_s seconds.  Particle
+    costs are heavy-tailed, so the sample's steps/s is scaled to particles/s
+    through the benchmark's measured mean steps per particle; the sample's
+    own particles/s is reported beside it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = cpu_threads()
-    small = dict(design, cells=100)
-    od = O.OracleDesign(small)
+    od = O.OracleDesign(design)
+    cells = int(design["cells"])
+    batch = 8
     done_steps, done_p, t_used = 0, 0, 0.0
     it = 0
     while t_used < budget_s and it < 5000:
         m = models[it % len(models)]
-        first = (it // len(models)) * 16
-        th = O.sample_prior(od, m, 0, first, 16)
+        first = (it // len(models)) * batch
+        th = O.sample_prior(od, m, 0, first, batch)
         t = time.perf_counter()
         r = O.simulate(od, m, 0, first, th, n_threads=threads, cells=False)
         t_used += time.perf_counter() - t
         done_steps += int(r["steps"].sum())
-        done_p += 16
+        done_p += batch
         it += 1
     sps = done_steps / t_used
     return {"value": round(sps / gpu_steps_per_particle, 4), "unit": "particles/s",
             "cores": threads, "kind": "port", "host": host_info(threads),
-            "sample": f"{done_p} benchmark particles (16 per model round-robin, first of each "
-                      f"model's stream) with 100 cells each: {done_steps} SSA steps in "
-                      f"{t_used:.1f}s on {threads} threads = {sps:.3e} steps/s; scaled by the "
-                      f"benchmark's mean {gpu_steps_per_particle:.4g} steps per 2500-cell particle",
-            "steps_per_s": sps}
+            "sample": f"{done_p} benchmark particles ({batch} per model round-robin, first of each "
+                      f"model's stream) with {cells} cells each: {done_steps} SSA steps in "
+                      f"{t_used:.1f}s on {threads} threads = {sps:.3e} steps/s "
+                      f"({done_p / t_used:.2f} particles/s on this sample); scaled by the "
+                      f"benchmark's mean {gpu_steps_per_particle:.4g} steps per particle",
+            "steps_per_s": sps, "sample_particles_per_s": done_p / t_used}
 
 
 if __name__ == "__main__":
