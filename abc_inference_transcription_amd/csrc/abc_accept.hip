@@ -64,22 +64,26 @@ __device__ __forceinline__ unsigned block_scan_flag(bool f, unsigned* scratch, u
 
 }  // namespace
 
-// One workgroup per gene (grid-stride over genes): the FP64 error of every
-// entry appended since the last selection, in the order of
-// compute_errors.jl:30-64 (no contraction), and its global particle index.
-// srow holds the batch's statistics row-major (n x K); an entry's index field
-// holds its batch row on entry.
+// Workgroups (blockIdx.x over genes, grid-stride; blockIdx.y of gridDim.y
+// over each gene's entries): the FP64 error of every entry appended since the
+// last selection, in the order of compute_errors.jl:30-64 (no contraction),
+// and its global particle index.  One lane per entry.  The batch's statistics
+// are read in their column-major layout (row i, stat k at scol[i + ld k]):
+// a gene's new entries come from the screen in nearly increasing row order,
+// so the lanes' loads of one statistic are mostly contiguous.  An entry's
+// index field holds its batch row on entry.
 __global__ __launch_bounds__(256) void pool_rescore_kernel(
-    const double* __restrict__ srow, int64_t n, const double* __restrict__ data,
+    const double* __restrict__ scol, int64_t ld, const double* __restrict__ data,
     const double* __restrict__ se, int64_t G, ErrSpec E, PoolEntry* __restrict__ pool,
     int64_t capg, const uint32_t* __restrict__ pool_n, const uint32_t* __restrict__ pool_base,
     int64_t first_index, const int64_t* __restrict__ pidx) {
   __shared__ double s_d[ABC_MAX_STATS], s_den[ABC_MAX_STATS];
   const int K = E.K;
   const double s2 = 0.1 * 0.1;
+  const int64_t stride = (int64_t)kWG * gridDim.y;
   for (int64_t g = blockIdx.x; g < G; g += gridDim.x) {
     const int64_t cnt = min((int64_t)pool_n[g], capg);
-    const int64_t b = pool_base[g];
+    const int64_t b = pool_base[g] + (int64_t)blockIdx.y * kWG;
     if (cnt <= b) continue;  // uniform over the workgroup
     for (int k = threadIdx.x; k < K; k += kWG) {
       const double d = data[g + G * k];
@@ -90,16 +94,16 @@ __global__ __launch_bounds__(256) void pool_rescore_kernel(
     }
     __syncthreads();
     PoolEntry* pg = pool + g * capg;
-    for (int64_t e = b + threadIdx.x; e < cnt; e += kWG) {
+    for (int64_t e = b + threadIdx.x; e < cnt; e += stride) {
       const int64_t i = pg[e].idx;
-      const double* s = srow + i * K;
+      const double* s = scol + i;
       double err = 0.0;
       int k = 0;
       for (int pp = 0; pp < E.n_parts; ++pp) {
         double acc = 0.0;
         const int end = k + E.parts[pp];
         for (; k < end; ++k) {
-          const double diff = s_d[k] - s[k];
+          const double diff = s_d[k] - s[ld * k];
           acc += (diff * diff) / s_den[k];
         }
         err += acc / (double)K;

@@ -955,7 +955,7 @@ int screen_batch_pooled(abc_ctx* c, int64_t n, const double* stats_dev, int64_t 
     }
     hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
                        stats_dev + off, nb, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah),
-                       P<__bf16>(c->al), P<uint8_t>(c->pflag), P<double>(c->srow));
+                       P<__bf16>(c->al), P<uint8_t>(c->pflag), (double*)nullptr);
     hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
                        dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), nb, A.Kp,
                        P<__bf16>(c->bh), P<__bf16>(c->bl), A.G, P<double>(c->cg),
@@ -963,8 +963,9 @@ int screen_batch_pooled(abc_ctx* c, int64_t n, const double* stats_dev, int64_t 
                        A.margin_rel, A.E.clamp, (Candidate*)nullptr,
                        (unsigned long long*)nullptr, 0ull, (float*)nullptr, (float*)nullptr,
                        P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln));
-    hipLaunchKernelGGL(pool_rescore_kernel, dim3(grid), dim3(256), 0, c->stream,
-                       P<double>(c->srow), nb, P<double>(c->edata), P<double>(c->ese), A.G, A.E,
+    // four workgroups share each gene's new entries
+    hipLaunchKernelGGL(pool_rescore_kernel, dim3(grid, 4), dim3(256), 0, c->stream,
+                       stats_dev + off, n, P<double>(c->edata), P<double>(c->ese), A.G, A.E,
                        P<PoolEntry>(c->pool), A.capg, P<uint32_t>(c->pooln),
                        P<uint32_t>(c->poolb), first_index + off, pidx_dev);
     hipLaunchKernelGGL(pool_select_kernel, dim3(grid), dim3(256), 0, c->stream,
@@ -1101,7 +1102,6 @@ int32_t abc_accept_begin(abc_ctx* c, int32_t K, int64_t G, const double* data, c
     if ((rc = ensure(c, c->pooln, 8 * (size_t)((G + 1) / 2)))) return rc;
     if ((rc = ensure(c, c->poolb, 8 * (size_t)((G + 1) / 2)))) return rc;
     if ((rc = ensure(c, c->presc, 8))) return rc;
-    if ((rc = ensure(c, c->srow, sizeof(double) * (size_t)ns * K))) return rc;
     if ((rc = ensure(c, c->apidx, sizeof(int64_t) * (size_t)ns))) return rc;
     if ((rc = ensure(c, c->ah, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;
     if ((rc = ensure(c, c->al, sizeof(uint16_t) * npad * 2 * A.Kp))) return rc;

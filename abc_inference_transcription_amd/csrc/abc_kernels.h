@@ -228,7 +228,7 @@ __global__ void gemm_screen_kernel(const __bf16* Ah, const __bf16* Al, int64_t n
                                    float* approx_out, float* bound_out, PoolEntry* pool,
                                    int64_t capg, uint32_t* pool_n);
 // per-gene top-k pools (abc_accept.hip)
-__global__ void pool_rescore_kernel(const double* srow, int64_t n, const double* data,
+__global__ void pool_rescore_kernel(const double* scol, int64_t ld, const double* data,
                                     const double* se, int64_t G, ErrSpec E, PoolEntry* pool,
                                     int64_t capg, const uint32_t* pool_n,
                                     const uint32_t* pool_base, int64_t first_index,
