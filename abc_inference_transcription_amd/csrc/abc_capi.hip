@@ -90,7 +90,7 @@ struct abc_ctx {
   int64_t* d_goff = nullptr;
   DevBuf theta, rates, cost, hist, order, bsum, off, cellrec, cu, cl,
       cb, steps, react, queue, stats, moments, superset, edata, ese, err, cand, ccount, ah, al,
-      pflag, bh, bl, cg, gflag, thr, wf, sy, pool, pooln, poolb, presc, srow, apidx, thrlow, spass, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs, selb;
+      pflag, bh, bl, cg, gflag, thr, wf, sy, pool, pooln, poolb, presc, apidx, thrlow, spass, tally, mss, mmap, mom, mth, mst, selc, selo, coff, pitems, mpidx, mk1, mcost, mord, dmeta, dw, dboot, dflag, du, dl, dout, dsort, dpt, dxs, selb;
   AcceptState acc;
   hipEvent_t ev[5]{};
   bool timed[4]{};
@@ -596,7 +596,7 @@ int32_t abc_destroy(abc_ctx* c) {
                     &c->cand, &c->ccount, &c->ah, &c->al, &c->pflag, &c->bh, &c->bl,
                     &c->cg, &c->gflag, &c->thr, &c->wf, &c->sy, &c->tally, &c->mss, &c->mmap, &c->coff, &c->pitems, &c->mpidx, &c->mk1, &c->mcost, &c->mord, &c->dmeta, &c->dw,
                     &c->dboot, &c->dflag, &c->du, &c->dl, &c->dout, &c->dsort, &c->dpt, &c->dxs, &c->selb,
-                    &c->pool, &c->pooln, &c->poolb, &c->presc, &c->srow, &c->apidx, &c->thrlow, &c->spass,
+                    &c->pool, &c->pooln, &c->poolb, &c->presc, &c->apidx, &c->thrlow, &c->spass,
                     &c->mom, &c->mth, &c->mst, &c->selc, &c->selo};
   for (DevBuf* b : bufs)
     if (b->p) hipFree(b->p);
@@ -955,7 +955,7 @@ int screen_batch_pooled(abc_ctx* c, int64_t n, const double* stats_dev, int64_t 
     }
     hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
                        stats_dev + off, nb, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah),
-                       P<__bf16>(c->al), P<uint8_t>(c->pflag), (double*)nullptr);
+                       P<__bf16>(c->al), P<uint8_t>(c->pflag));
     hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),
                        dim3(256), 0, c->stream, P<__bf16>(c->ah), P<__bf16>(c->al), nb, A.Kp,
                        P<__bf16>(c->bh), P<__bf16>(c->bl), A.G, P<double>(c->cg),
@@ -991,7 +991,7 @@ int screen_batch(abc_ctx* c, int64_t n, const double* stats_dev, int64_t first_i
   if ((rc = ensure(c, c->ccount, 64))) return rc;
   hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
                      stats_dev, n, n, A.E.K, A.Kp, npad, P<__bf16>(c->ah), P<__bf16>(c->al),
-                     P<uint8_t>(c->pflag), (double*)nullptr);
+                     P<uint8_t>(c->pflag));
   HIPCHK(c, hipGetLastError());
   unsigned long long cap = std::max<unsigned long long>(A.cand_cap, 1ull << 16);
   for (;;) {
@@ -1341,7 +1341,7 @@ int32_t abc_screen_errors(abc_ctx* c, int64_t n, int32_t K, const double* stats,
   HIPCHK(c, hipMemcpyAsync(c->stats.p, stats, sizeof(double) * n * K, hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(particle_prep_kernel, dim3(grid1(npad, 256)), dim3(256), 0, c->stream,
                      P<double>(c->stats), n, n, K, A.Kp, npad, P<__bf16>(c->ah),
-                     P<__bf16>(c->al), P<uint8_t>(c->pflag), (double*)nullptr);
+                     P<__bf16>(c->al), P<uint8_t>(c->pflag));
   float* ap = P<float>(c->err);
   float* bp = ap + n * G;
   hipLaunchKernelGGL(gemm_screen_kernel, dim3((unsigned)(npad / 64), (unsigned)(A.Gpad / 64)),

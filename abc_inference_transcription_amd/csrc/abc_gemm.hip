@@ -77,16 +77,14 @@ __global__ __launch_bounds__(256) void gene_prep_kernel(const double* __restrict
   gflag[g] = bad ? 1 : 0;
 }
 
-// One thread per particle: A rows [s | s^2] split into bf16 hi/lo, and
-// (srow != NULL) the statistics row-major for the FP64 rescoring.  Row i of
+// One thread per particle: A rows [s | s^2] split into bf16 hi/lo.  Row i of
 // the n rows is stats[i + ld * k] (a block of rows of a larger batch).
 __global__ __launch_bounds__(256) void particle_prep_kernel(const double* __restrict__ stats,
                                                             int64_t n, int64_t ld, int K, int Kp,
                                                             int64_t npad,
                                                             bf16* __restrict__ Ah,
                                                             bf16* __restrict__ Al,
-                                                            uint8_t* __restrict__ pflag,
-                                                            double* __restrict__ srow) {
+                                                            uint8_t* __restrict__ pflag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npad) return;
   bf16* ah = Ah + i * 2 * Kp;
@@ -96,7 +94,6 @@ __global__ __launch_bounds__(256) void particle_prep_kernel(const double* __rest
   for (int k = 0; k < Kp; ++k) {
     if (i < n && k < K) {
       const double s = stats[i + ld * k];
-      if (srow) srow[i * K + k] = s;
       if (!(fabs(s) <= SCREEN_LIMIT)) bad = true;
       const double sc = bad ? 0.0 : s;
       split_bf16(sc, ah + k, al + k);

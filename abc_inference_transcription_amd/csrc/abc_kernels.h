@@ -219,7 +219,7 @@ extern "C" int32_t abc_mlp_synchronize(abc_mlp* p);
 extern "C" int32_t abc_mlp_n_stats(const abc_mlp* p);
 __global__ void particle_prep_kernel(const double* stats, int64_t n, int64_t ld, int K, int Kp,
                                      int64_t npad,
-                                     __bf16* Ah, __bf16* Al, uint8_t* pflag, double* srow);
+                                     __bf16* Ah, __bf16* Al, uint8_t* pflag);
 __global__ void gemm_screen_kernel(const __bf16* Ah, const __bf16* Al, int64_t n, int Kp,
                                    const __bf16* Bh, const __bf16* Bl, int64_t G, const double* cg,
                                    const uint8_t* gflag, const uint8_t* pflag, const double* thr,

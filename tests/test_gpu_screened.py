@@ -43,7 +43,8 @@ def test_screened_abc_recall_and_savings(screened):
     best = max(rules, key=lambda r: r["ssa_work_fraction_avoided"] if r["recall"] >= 0.98 else -1)
     assert best["recall"] >= 0.98, rules
     assert best["ssa_work_fraction_avoided"] > 0.2, rules
-    assert best["speedup_accepted_per_s"] > 1.1, rules
+    # measured about 3.4x at this size (0.62 s plain, 0.18 s screened)
+    assert best["speedup_accepted_per_s"] > 1.8, rules
 
 
 def test_score_rule_reproduces_pass_flags(screened):
