@@ -2016,15 +2016,18 @@ int32_t abc_autotune_ssa(abc_ctx* c, int32_t n_models, const int32_t* models, ui
       bestb = cand[q];
     }
   }
-  // then the work items a lane claims per queue atomic, on that grid
+  // then the work items a lane claims per queue atomic, on that grid: the
+  // better of two trials each (claims 2 and 4 differ by about 1 %, about the
+  // spread of one trial)
   int bestc = 1;
   for (int cl : {2, 4, 8}) {
-    if (rc != ABC_OK) break;
     c->ssa_claim = cl;
-    rc = run(bestb, &ms);
-    if (rc == ABC_OK && ms < best) {
-      best = ms;
-      bestc = cl;
+    for (int trial = 0; rc == ABC_OK && trial < 2; ++trial) {
+      rc = run(bestb, &ms);
+      if (rc == ABC_OK && ms < best) {
+        best = ms;
+        bestc = cl;
+      }
     }
   }
   c->ssa_claim = rc == ABC_OK ? bestc : saved_claim;

@@ -440,7 +440,7 @@ int32_t abc_get_ssa_claim(const abc_ctx* ctx);
  * n prior particles (from first) of each listed model, simulates them (all
  * models in one multi-model call, as abc_simulate_multi_dev) once per
  * candidate grid (1, 2, 3, 4, 6, 8, 12, 16 workgroups per CU, up to
- * occupancy) and then, on the fastest grid, once per queue claim (2, 4 and 8
+ * occupancy) and then, on the fastest grid, twice per queue claim (2, 4 and 8
  * work items per atomic against 1), times ssa_kernel with HIP events and keeps
  * the fastest shape for later calls.  Outputs never depend on the shape, so
  * tuning changes speed only.  best_blocks / best_ms (optional) receive the
