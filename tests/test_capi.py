@@ -41,3 +41,29 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
     with pytest.raises(RuntimeError):
         _lib.load()
+
+
+def prototypes():
+    """Entry point -> number of parameters, from include/*.h."""
+    out = {}
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:int32_t|int64_t|const char\*|void)\s+(abc_\w+)\s*\(([^;]*?)\)\s*;",
+                             src, re.M | re.S):
+            args = [a for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+            out[m.group(1)] = len(args)
+    return out
+
+
+def test_ctypes_signatures_match_the_header():
+    """Every declared entry point gets a ctypes signature of the header's
+    arity (a missing or miscounted one would pass arguments wrongly)."""
+    from abc_inference_transcription_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libabc.so not built")
+    L = _lib.load()
+    protos = prototypes()
+    assert len(protos) == len(declared())
+    wrong = {n: (k, getattr(L, n).argtypes) for n, k in protos.items()
+             if getattr(L, n).argtypes is None or len(getattr(L, n).argtypes) != k}
+    assert not wrong, wrong
