@@ -87,7 +87,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--particles", type=int, default=4096, help="particles per model per step")
+    ap.add_argument("--particles", type=int, default=16384, help="particles per model per step")
     ap.add_argument("--genes", type=int, default=2000)
     ap.add_argument("--topk", type=int, default=1000, help="accepted particles per gene")
     ap.add_argument("--margin", type=float, default=0.02,
