@@ -62,6 +62,7 @@ int main(void) {
                orc_exp10(0.3) + orc_u128_to_f64(1ull << 40, 12345);
     for (uint32_t w = 0; w < 4096; ++w) s += orc_exp1_32(w * 1048573u);
     s += orc_exp1_32(0xFFFFFFFFu) + orc_log(5e-324) + orc_log(1.0 + 1e-15);
+    s += (double)orc_exp_scaling_mismatches(-745.2, 709.0, 10000, 9);
     uint32_t po[64];
     orc_poisson_sample(3.5, 1, 64, po);
     orc_poisson_sample(250.0, 2, 64, po);
